@@ -1,0 +1,58 @@
+/*
+ * storm_gpu.h -- drop-in GPU build of the STORM_contiguous_t half of storm.h.
+ *
+ * Same names, argument meaning and error behaviour as the reference C ABI
+ * (storm.h:224-231, storm.cpp:1159-1285); link libstorm_gpu.so instead of
+ * storm.cpp and the pairwise calls answer from an MI355X through storm_hip.h.
+ *
+ * Differences a caller can observe:
+ *   - a device failure (no GPU, out of device memory) makes the pairwise calls
+ *     print the reason to stderr and return UINT64_MAX; the reason is also
+ *     available from storm_hip_last_error(). The reference has no device and
+ *     so no such failure; it returns 0 only for a NULL handle, as here.
+ */
+#ifndef STORM_GPU_H_
+#define STORM_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct STORM_contiguous_s STORM_contiguous_t;
+
+/* storm.cpp:1159-1170: NULL on allocation failure. */
+STORM_contiguous_t* STORM_contig_new(size_t vector_length);
+/* storm.cpp:1172-1174: NULL-safe. */
+void STORM_contig_free(STORM_contiguous_t* bitmap);
+/* storm.cpp:1176-1220: -1 NULL handle, -2 NULL values, 0 when n_values == 0
+ * (no vector is added), -4 on allocation failure, else n_values. Values should
+ * be sorted (the reference asserts it in debug builds only; order does not
+ * change the result); repeated values are collapsed; values >= vector_length
+ * are out of contract (the reference asserts; this build returns -3 and adds
+ * nothing). */
+int STORM_contig_add(STORM_contiguous_t* bitmap, const uint32_t* values, const uint32_t n_values);
+/* storm.cpp:1222-1230: 1, or -1 for a NULL handle. */
+int STORM_contig_clear(STORM_contiguous_t* bitmap);
+/* storm.cpp:1232-1285: sum over i<j of |Xi n Xj|; 0 for a NULL handle. The
+ * block size of the _blocked forms is a CPU cache knob with no effect on the
+ * result; the device tiles by its own geometry. */
+uint64_t STORM_contig_pairw_intersect_cardinality(STORM_contiguous_t* bitmap);
+uint64_t STORM_contig_pairw_intersect_cardinality_blocked(STORM_contiguous_t* bitmap, uint32_t bsize);
+uint64_t STORM_contig_pairw_intersect_cardinality_list(STORM_contiguous_t* bitmap);
+uint64_t STORM_contig_pairw_intersect_cardinality_blocked_list(STORM_contiguous_t* bitmap, uint32_t bsize);
+
+/* GPU-build extras (no reference counterpart). */
+/* Per-pair counts of the last pairwise call into out[N(N-1)/2], packed upper
+ * triangle, idx(i,j) = i*N - i(i+1)/2 + (j-i-1). Returns 0 or a negative
+ * storm_hip status. */
+int STORM_contig_gpu_pairw_counts(STORM_contiguous_t* bitmap, uint32_t* out);
+/* Number of vectors held. */
+size_t STORM_contig_gpu_size(const STORM_contiguous_t* bitmap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

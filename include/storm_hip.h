@@ -1,0 +1,201 @@
+/*
+ * storm_hip.h -- C ABI of the MI355X (gfx950) exact all-pairs intersection
+ * cardinality engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, integer status codes,
+ * no exceptions and no device-framework types cross this boundary.
+ *
+ * Each entry point names the StormBitmaps interface it stands in for
+ * (paths relative to the StormBitmaps source tree).
+ */
+#ifndef STORM_HIP_H_
+#define STORM_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STORM_HIP_ABI_VERSION 1
+
+/* Status codes. Negative = failure; the reference's C ABI uses the same
+ * convention (storm.cpp:1160-1219: NULL handle -1, NULL values -2, bad_alloc -4). */
+enum {
+    STORM_HIP_OK              = 0,
+    STORM_HIP_ERR_INVALID     = -1,  /* NULL handle or argument out of contract */
+    STORM_HIP_ERR_NULL_VALUES = -2,  /* NULL data pointer with a nonzero size   */
+    STORM_HIP_ERR_DEVICE      = -3,  /* HIP runtime or launch failure           */
+    STORM_HIP_ERR_NOMEM       = -4,  /* device or host allocation failed        */
+    STORM_HIP_ERR_UNSUPPORTED = -5,  /* combination not available               */
+    STORM_HIP_ERR_RANGE       = -6   /* size beyond what the engine supports    */
+};
+
+typedef struct storm_hip_ctx    storm_hip_ctx;
+typedef struct storm_hip_corpus storm_hip_corpus;
+
+const char* storm_hip_strerror(int status);
+/* Library version string, e.g. "storm_hip 0.1 gfx950 abi 1". */
+const char* storm_hip_version(void);
+/* Number of visible HIP devices (0 when none). */
+int storm_hip_device_count(int* n_devices);
+/* Last error message recorded on this thread (never NULL). */
+const char* storm_hip_last_error(void);
+
+/* One engine instance bound to one device and one HIP stream. One in-flight
+ * call per context; calls on one context are stream-ordered. */
+int  storm_hip_ctx_create(int device, uint32_t flags, storm_hip_ctx** out);
+void storm_hip_ctx_free(storm_hip_ctx* ctx);
+int  storm_hip_sync(storm_hip_ctx* ctx);
+
+/* ---- corpora -------------------------------------------------------------
+ *
+ * Dense contiguous corpus: the device form of STORM_contiguous_t
+ * (storm.cpp:1113-1157): n_rows bitmaps of n_words 64-bit words each, row
+ * major. Replaces STORM_contig_new + STORM_contig_add (storm.cpp:1159-1220)
+ * for the bulk-upload case. The words are copied; the caller keeps ownership.
+ */
+int storm_hip_corpus_upload_contig(storm_hip_ctx* ctx, const uint64_t* words,
+                                   uint32_t n_rows, uint32_t n_words,
+                                   storm_hip_corpus** out);
+/* Same corpus shape, generated on the device: bit b of row r is set iff the
+ * (r*n_bits+b)-th draw u of the splitmix64 stream Rng(seed)
+ * (kernels/storm_gen.h:84-102), taken as unit() = (next()>>11)*2^-53,
+ * satisfies u < p. Bits past n_bits in the last word are zero. */
+int storm_hip_corpus_gen_bernoulli(storm_hip_ctx* ctx, uint32_t n_rows, uint32_t n_bits,
+                                   double p, uint64_t seed, storm_hip_corpus** out);
+/* Copy rows [row0, row0+n_rows) of a dense corpus back to the host. */
+int storm_hip_corpus_download_words(storm_hip_ctx* ctx, const storm_hip_corpus* c,
+                                    uint32_t row0, uint32_t n_rows, uint64_t* out);
+
+/* General corpus: per-row arenas in the storm_repr byte layouts
+ * (kernels/storm_repr.h:112-276). All offsets are element offsets into the
+ * matching arena, n_rows+1 entries each (row r spans [off[r], off[r+1])).
+ * A row whose bitmap span is empty has no bitmap on the device; cells that
+ * need one are not eligible for pairs that touch it.
+ *   lists        sorted distinct u32 positions          (ListView)
+ *   runs         [start,end) u32 SoA, maximal           (RunView)
+ *   ewah         EWAH-64 markers + literals             (EwahView)
+ *   bitmap       n_words u64 per bitmap row             (BitmapView::w)
+ *   occ          zone map, 1 bit per 8 words            (BitmapView::occ)
+ *   comp lists   complement positions, rows with 2n > universe (ComplementView)
+ *   meta         8 u32 per row: cardinality, n_runs, n_words, n_nonzero_w,
+ *                first_set, last_set, has_rank, has_occ  (RowMeta)
+ * Pointers for absent arenas may be NULL (their offsets then all zero). */
+typedef struct {
+    uint32_t        n_rows;
+    uint32_t        universe;
+    const uint64_t* list_off;   const uint32_t* list;
+    const uint64_t* run_off;    const uint32_t* run_start;  const uint32_t* run_end;
+    const uint64_t* ewah_off;   const uint64_t* ewah;
+    const uint64_t* bitmap_off; const uint64_t* bitmap;
+    const uint64_t* occ_off;    const uint64_t* occ;
+    const uint64_t* comp_off;   const uint32_t* comp;
+    const uint32_t* meta;       /* 8 per row */
+} storm_hip_arena;
+
+int  storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* arena,
+                             storm_hip_corpus** out);
+/* Materialise a bitmap on the device for every row that lacks one (needed by
+ * the all-bitmap policy on sparse corpora). */
+int  storm_hip_corpus_materialize_bitmaps(storm_hip_ctx* ctx, storm_hip_corpus* c);
+void storm_hip_corpus_free(storm_hip_corpus* c);
+uint32_t storm_hip_corpus_rows(const storm_hip_corpus* c);
+/* Device bytes held by the corpus (arenas + result buffer). */
+uint64_t storm_hip_corpus_device_bytes(const storm_hip_corpus* c);
+
+/* ---- the all-pairs call --------------------------------------------------
+ *
+ * Computes |X_i n X_j| for every i < j exactly. Replaces
+ *   STORM_contig_pairw_intersect_cardinality[_blocked][_list]
+ *     (storm.cpp:1232-1285)                      -> policy ALL_BITMAP, SUM
+ *   storm::allpairs_sum(rows, model, policy, tile)
+ *     (kernels/storm_allpairs.cpp:486-675)       -> SUM, stats
+ *   storm::allpairs_tiles(...)  (:677-732)       -> UPPER_TRI, original order
+ */
+enum {
+    STORM_HIP_POLICY_ALL_BITMAP = 0,  /* every pair B x B, no span test      */
+    STORM_HIP_POLICY_ANALYTIC   = 1,  /* per-tile storm_cost selection       */
+    STORM_HIP_POLICY_LEARNED    = 2,  /* per-tile MLP selection (fallback:
+                                         analytic when features are out of
+                                         the training range)                 */
+    STORM_HIP_POLICY_FIXED      = 3,  /* one cell for every eligible pair     */
+    STORM_HIP_POLICY_EXHAUSTIVE = 4   /* time every eligible cell per tile    */
+};
+enum {
+    STORM_HIP_OUT_SUM       = 0,  /* u64 sum only                            */
+    STORM_HIP_OUT_UPPER_TRI = 1   /* u32 per pair, original row order,
+                                     idx(i,j) = i*N - i(i+1)/2 + (j-i-1)     */
+};
+enum {
+    STORM_HIP_BB_AUTO = 0,       /* MFMA where the tile is dense, else VALU  */
+    STORM_HIP_BB_VALU = 1,       /* AND + v_bcnt popcount                     */
+    STORM_HIP_BB_MFMA = 2        /* 0/1 int8 MFMA with i32 accumulation       */
+};
+/* Cell classes, in storm::Pairing order (kernels/storm_cost.h:47-49) plus the
+ * complement cell (kernels/cell_comp.cpp). */
+enum {
+    STORM_HIP_CELL_BB = 0, STORM_HIP_CELL_BS, STORM_HIP_CELL_BR, STORM_HIP_CELL_BW,
+    STORM_HIP_CELL_SS, STORM_HIP_CELL_SR, STORM_HIP_CELL_SW, STORM_HIP_CELL_RR,
+    STORM_HIP_CELL_RW, STORM_HIP_CELL_WW, STORM_HIP_CELL_EMPTY, STORM_HIP_CELL_CB,
+    STORM_HIP_NCELL
+};
+
+typedef struct {
+    uint32_t policy;      /* STORM_HIP_POLICY_*                               */
+    uint32_t out_mode;    /* STORM_HIP_OUT_*                                  */
+    uint32_t bb_kernel;   /* STORM_HIP_BB_*                                   */
+    uint32_t tile;        /* rows per selection tile (0 -> 64)               */
+    uint32_t fixed_cell;  /* STORM_HIP_CELL_* for POLICY_FIXED                */
+    uint32_t use_graph;   /* capture select->dispatch->reduce in a hipGraph   */
+    uint32_t reserved[2];
+} storm_hip_options;
+
+typedef struct {
+    uint64_t sum;                       /* sum over pairs of |Xi n Xj|        */
+    uint64_t pairs;                     /* N(N-1)/2                           */
+    uint64_t skipped;                   /* settled by the span test           */
+    uint64_t decisions;                 /* selection decisions (tiles)        */
+    uint64_t cell_pairs[STORM_HIP_NCELL];
+    uint64_t cell_tiles[STORM_HIP_NCELL];
+    uint64_t fallback_tiles;            /* learned -> analytic fallbacks      */
+    double   ms_total;                  /* host wall time of the call         */
+    double   ms_device;                 /* HIP-event time of the whole call   */
+    double   ms_bb;                     /* HIP-event time of B x B launches   */
+    double   ms_select;                 /* HIP-event time of selection        */
+    uint64_t bb_launches;
+} storm_hip_stats;
+
+int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c,
+                       const storm_hip_options* opt, storm_hip_stats* stats);
+/* Copy `count` packed upper-triangle counts starting at `offset` from the
+ * last UPPER_TRI call on this corpus. */
+int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c,
+                              uint64_t offset, uint64_t count, uint32_t* out);
+/* Row sums of the last UPPER_TRI result, computed on the device:
+ * row_sum[i] = sum_{j != i} |Xi n Xj|. */
+int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
+                              uint64_t* row_sum);
+
+/* ---- analytic cost model on the device -----------------------------------
+ * CostModel rates (kernels/storm_cost.h:60-92) used by POLICY_ANALYTIC, in
+ * ns per work unit, indexed by STORM_HIP_CELL_* (BB..WW); a rate <= 0 removes
+ * the cell from the candidate set, as select_pairing does (storm_cost.cpp:456). */
+typedef struct {
+    double ns_per_unit[10];
+    double ns_fixed;
+    double ns_per_occ_word;
+    double ns_probe_hot, ns_probe_cold;
+} storm_hip_cost_model;
+int storm_hip_set_cost_model(storm_hip_ctx* ctx, const storm_hip_cost_model* m);
+int storm_hip_get_cost_model(storm_hip_ctx* ctx, storm_hip_cost_model* m);
+/* Per-tile decisions of the last selecting call: `cells[t]` for tile pair t
+ * in row-major upper-triangle tile order. */
+int storm_hip_last_decisions(storm_hip_ctx* ctx, const storm_hip_corpus* c,
+                             uint8_t* cells, uint64_t n_tiles);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

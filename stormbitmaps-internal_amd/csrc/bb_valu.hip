@@ -1,0 +1,143 @@
+// B x B cell, VALU form: exact |Xi n Xj| for a tile of row pairs as
+// AND + v_bcnt_u32_b32 popcount accumulate over LDS-staged row panels.
+//
+// Replaces the reference's bb_dense / simd_and_popcnt (kernels/cell_bb.cpp:402,
+// kernels/storm_simd.h:144-203) and libalgebra's Harley-Seal kernel behind
+// STORM_contig_pairw_intersect_cardinality (storm.cpp:1232-1240) -- but as a
+// Gram-matrix tile: each workgroup owns a BM x BM block of pairs, so a row
+// panel is read once per block instead of once per pair.
+//
+// Layout: rows are `ld` u64 words apart (ld a multiple of 64, zero padded);
+// `rowmap` (optional) maps a tile-row index to a physical row of X; `perm`
+// (optional) maps a tile-row index to the ORIGINAL row id used for output.
+#include "device_common.h"
+#include "internal.h"
+
+namespace stormhip {
+
+template <int BM>
+__global__ __launch_bounds__(256, 2) void bb_valu_kernel(
+    const uint64_t* __restrict__ X, uint32_t ld, uint32_t kwords,
+    const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm, uint32_t n,
+    uint32_t T, const uint32_t* __restrict__ tiles, uint32_t* __restrict__ tri,
+    unsigned long long* __restrict__ sum) {
+    constexpr int KC = 16;               // words per LDS stage
+    constexpr int TM = BM / 16;          // rows (and cols) per thread
+    constexpr int WPT = BM * KC / 256;   // words each thread loads per panel
+    constexpr int TPR = KC / WPT;        // threads per panel row
+    __shared__ uint64_t As[2][KC][BM];
+    __shared__ uint64_t Bs[2][KC][BM];
+
+    uint32_t bi, bj;
+    if (tiles) { bi = tiles[2 * blockIdx.x]; bj = tiles[2 * blockIdx.x + 1]; }
+    else       tri_tile(blockIdx.x, T, bi, bj);
+    const uint32_t i0 = bi * BM, j0 = bj * BM;
+
+    const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    const int lrow = t / TPR, lseg = t % TPR;
+    const uint64_t ra = rowmap ? rowmap[i0 + lrow] : (uint64_t)(i0 + lrow);
+    const uint64_t rb = rowmap ? rowmap[j0 + lrow] : (uint64_t)(j0 + lrow);
+    const uint4* pa = reinterpret_cast<const uint4*>(X + ra * ld + lseg * WPT);
+    const uint4* pb = reinterpret_cast<const uint4*>(X + rb * ld + lseg * WPT);
+
+    uint4 ga[WPT / 2], gb[WPT / 2];
+    auto gload = [&](uint32_t k0) {
+#pragma unroll
+        for (int q = 0; q < WPT / 2; ++q) {
+            ga[q] = pa[k0 / 2 + q];
+            gb[q] = pb[k0 / 2 + q];
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < WPT / 2; ++q) {
+            As[buf][lseg * WPT + 2 * q][lrow]     = ((uint64_t)ga[q].y << 32) | ga[q].x;
+            As[buf][lseg * WPT + 2 * q + 1][lrow] = ((uint64_t)ga[q].w << 32) | ga[q].z;
+            Bs[buf][lseg * WPT + 2 * q][lrow]     = ((uint64_t)gb[q].y << 32) | gb[q].x;
+            Bs[buf][lseg * WPT + 2 * q + 1][lrow] = ((uint64_t)gb[q].w << 32) | gb[q].z;
+        }
+    };
+
+    uint32_t acc[TM][TM];
+#pragma unroll
+    for (int r = 0; r < TM; ++r)
+#pragma unroll
+        for (int c = 0; c < TM; ++c) acc[r][c] = 0;
+
+    const uint32_t nk = kwords / KC;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (uint32_t kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * KC);
+#pragma unroll 4
+        for (int k = 0; k < KC; ++k) {
+            uint64_t a[TM], b[TM];
+#pragma unroll
+            for (int r = 0; r < TM; ++r) a[r] = As[buf][k][ty + 16 * r];
+#pragma unroll
+            for (int c = 0; c < TM; ++c) b[c] = Bs[buf][k][tx + 16 * c];
+#pragma unroll
+            for (int r = 0; r < TM; ++r)
+#pragma unroll
+                for (int c = 0; c < TM; ++c) {
+                    const uint64_t x = a[r] & b[c];
+                    acc[r][c] += __popc((uint32_t)x) + __popc((uint32_t)(x >> 32));
+                }
+        }
+        if (kt + 1 < nk) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    uint64_t local = 0;
+#pragma unroll
+    for (int r = 0; r < TM; ++r) {
+        const uint32_t i = i0 + ty + 16 * r;
+        if (i >= n) continue;
+        const uint32_t oi = perm ? perm[i] : i;
+#pragma unroll
+        for (int c = 0; c < TM; ++c) {
+            const uint32_t j = j0 + tx + 16 * c;
+            if (j >= n || j <= i) continue;
+            const uint32_t v = acc[r][c];
+            local += v;
+            if (tri) {
+                const uint32_t oj = perm ? perm[j] : j;
+                tri[oi < oj ? tri_index(oi, oj, n) : tri_index(oj, oi, n)] = v;
+            }
+        }
+    }
+    local = wave_sum_u64(local);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(sum, (unsigned long long)local);
+}
+
+// Host launcher. `tiles` == nullptr: all T(T+1)/2 upper-triangle tiles of BM
+// rows; otherwise n_tiles explicit (bi, bj) pairs.
+int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32_t n_words,
+                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n,
+                   const uint32_t* tiles, uint64_t n_tiles, uint32_t* tri,
+                   unsigned long long* sum) {
+    const uint32_t kwords = (n_words + 15) / 16 * 16;
+    if (kwords > ld) { set_error("bb_valu: kwords %u > ld %u", kwords, ld); return STORM_HIP_ERR_INVALID; }
+    if (bm == 128) {
+        const uint32_t T = (n + 127) / 128;
+        const uint64_t grid = tiles ? n_tiles : (uint64_t)T * (T + 1) / 2;
+        if (grid == 0) return STORM_HIP_OK;
+        hipLaunchKernelGGL(bb_valu_kernel<128>, dim3((uint32_t)grid), dim3(256), 0, s, X, ld,
+                           kwords, rowmap, perm, n, T, tiles, tri, sum);
+    } else if (bm == 64) {
+        const uint32_t T = (n + 63) / 64;
+        const uint64_t grid = tiles ? n_tiles : (uint64_t)T * (T + 1) / 2;
+        if (grid == 0) return STORM_HIP_OK;
+        hipLaunchKernelGGL(bb_valu_kernel<64>, dim3((uint32_t)grid), dim3(256), 0, s, X, ld,
+                           kwords, rowmap, perm, n, T, tiles, tri, sum);
+    } else {
+        set_error("bb_valu: unsupported tile %d", bm);
+        return STORM_HIP_ERR_INVALID;
+    }
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+}  // namespace stormhip

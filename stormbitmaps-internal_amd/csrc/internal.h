@@ -1,0 +1,86 @@
+// Internal definitions shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "storm_hip.h"
+
+namespace stormhip {
+
+// Row padding of dense corpora: rows are padded to a multiple of this many
+// 64-bit words (zero filled) so every B x B K-stage is a full, aligned load.
+constexpr uint32_t kWordPad = 64;
+// Row-count padding: dense corpora hold a multiple of this many rows (zero
+// rows past n_rows), so a B x B tile never reads outside the allocation.
+constexpr uint32_t kRowPad = 256;
+
+void set_error(const char* fmt, ...);
+
+#define SH_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            ::stormhip::set_error("%s failed: %s (%s:%d)", #call,             \
+                                  hipGetErrorString(e_), __FILE__, __LINE__); \
+            return e_ == hipErrorOutOfMemory ? STORM_HIP_ERR_NOMEM            \
+                                             : STORM_HIP_ERR_DEVICE;          \
+        }                                                                     \
+    } while (0)
+
+#define SH_TRY(expr)                        \
+    do {                                    \
+        int s_ = (expr);                    \
+        if (s_ != STORM_HIP_OK) return s_;  \
+    } while (0)
+
+struct DeviceCostModel {
+    double ns_per_unit[10];
+    double ns_fixed, ns_per_occ_word, ns_probe_hot, ns_probe_cold;
+    double probe_scale, br_margin, rr_scale;
+};
+
+}  // namespace stormhip
+
+struct storm_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    stormhip::DeviceCostModel model{};
+    int cu_count = 0;
+    // MLP selector state lives in mlp.hip (opaque here).
+    void* selector = nullptr;
+};
+
+struct storm_hip_corpus {
+    storm_hip_ctx* ctx = nullptr;
+    uint32_t n_rows = 0;
+    uint32_t n_words = 0;      // logical words per row (ceil(universe/64))
+    uint32_t universe = 0;     // bits
+    bool dense = false;        // contiguous corpus: every row a bitmap
+    // Dense layout: n_rows_pad x ld words.
+    uint64_t* d_words = nullptr;
+    uint32_t ld = 0;
+    uint32_t n_rows_pad = 0;
+    // General layout (arenas). Offsets are u64 element offsets, n_rows+1.
+    uint64_t *d_list_off = nullptr, *d_run_off = nullptr, *d_ewah_off = nullptr;
+    uint64_t *d_bitmap_off = nullptr, *d_occ_off = nullptr, *d_comp_off = nullptr;
+    uint32_t *d_list = nullptr, *d_run_start = nullptr, *d_run_end = nullptr;
+    uint64_t *d_ewah = nullptr, *d_bitmap = nullptr, *d_occ = nullptr;
+    uint32_t* d_comp = nullptr;
+    uint32_t* d_meta = nullptr;      // 8 per row
+    uint64_t* d_ewah_pos = nullptr;  // per EWAH word: decoded word index of its
+                                     // marker's first word (markers) -- see cells
+    uint64_t device_bytes = 0;
+    std::vector<uint32_t> h_meta;    // host copy of meta (8 per row)
+    bool bitmaps_complete = false;
+    // Result of the last UPPER_TRI call.
+    uint32_t* d_tri = nullptr;
+    uint64_t tri_count = 0;
+    bool tri_valid = false;
+    // Decisions of the last selecting call.
+    std::vector<uint8_t> last_cells;
+};

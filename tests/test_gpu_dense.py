@@ -1,0 +1,167 @@
+"""Dense (all-bitmap) B x B on the GPU against the CPU oracle.
+
+The oracle is the plain-C restatement in oracle/ (pinned in test_oracle.py
+against the compiled reference and against closed forms)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+BB_KERNELS = ["valu", "mfma"]
+
+# (n_rows, n_bits, p, seed): ragged rows (not a multiple of any tile), ragged
+# universes (not a multiple of 64 or of a K stage), empty and tiny corpora.
+SHAPES = [
+    (0, 64, 0.5, 1), (1, 64, 0.5, 1), (2, 64, 0.5, 3), (3, 100, 0.3, 4),
+    (17, 1, 0.5, 5), (64, 4033, 0.02, 6), (129, 1000, 0.3, 7), (130, 4033, 0.05, 8),
+    (257, 64 * 17, 0.5, 9), (300, 5000, 0.01, 10), (513, 70001, 0.003, 11),
+    (200, 65536, 0.9, 12), (96, 8192, 1.0, 13), (96, 8192, 0.0, 14),
+]
+
+
+def _print_case(*a):
+    print("case", *a, flush=True)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_device_generator_matches_stream(ctx, shape):
+    n, bits, p, seed = shape
+    _print_case("gen", shape)
+    with ctx.gen_bernoulli(n, bits, p, seed) as c:
+        nw = (bits + 63) // 64
+        got = c.download_words(0, n, nw)
+    want = O.bernoulli_rows(seed, p, bits, 0, n)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("bb", BB_KERNELS)
+@pytest.mark.parametrize("shape", SHAPES)
+def test_allpairs_tri_matches_oracle(ctx, shape, bb):
+    n, bits, p, seed = shape
+    _print_case("tri", shape, bb)
+    words = O.bernoulli_rows(seed, p, bits, 0, n)
+    want_sum, want_tri = O.allpairs_words(words)
+    with ctx.upload_contig(words) as c:
+        st = c.allpairs(out="tri", bb=bb)
+        assert st["pairs"] == n * (n - 1) // 2
+        assert st["sum"] == want_sum
+        if n >= 2:
+            assert np.array_equal(c.tri(), want_tri)
+            _, rs = O.column_closed_forms(words)
+            assert np.array_equal(c.row_sums(), rs)
+        st2 = c.allpairs(out="sum", bb=bb)
+        assert st2["sum"] == want_sum
+
+
+@pytest.mark.parametrize("bb", BB_KERNELS)
+def test_config1_known_sum(ctx, bb):
+    """N=2048 x M=262,144, Bernoulli(0.01), Rng(1): every pair against the
+    oracle, and the total against the reference CPU path's known 54,935,083."""
+    _print_case("config1", bb)
+    n, bits = 2048, 262144
+    with ctx.gen_bernoulli(n, bits, 0.01, 1) as c:
+        st = c.allpairs(out="tri", bb=bb)
+        assert st["sum"] == 54_935_083
+        tri = c.tri()
+        words = c.download_words(0, n, bits // 64)
+    assert np.array_equal(words[:4], O.bernoulli_rows(1, 0.01, bits, 0, 4))
+    assert np.array_equal(words[-2:], O.bernoulli_rows(1, 0.01, bits, n - 2, 2))
+    s, want = O.allpairs_words(words, threads=os.cpu_count() or 8)
+    assert s == 54_935_083
+    assert np.array_equal(tri, want)
+
+
+def _gpu_contig():
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "stormbitmaps-internal_amd", "libstorm_gpu.so")
+    L = C.CDLL(path, mode=C.RTLD_LOCAL)
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    for n, r, a in [("STORM_contig_new", vp, [C.c_size_t]), ("STORM_contig_free", None, [vp]),
+                    ("STORM_contig_add", i32, [vp, vp, u32]), ("STORM_contig_clear", i32, [vp]),
+                    ("STORM_contig_pairw_intersect_cardinality", u64, [vp]),
+                    ("STORM_contig_pairw_intersect_cardinality_blocked", u64, [vp, u32]),
+                    ("STORM_contig_gpu_pairw_counts", i32, [vp, vp])]:
+        f = getattr(L, n)
+        f.restype, f.argtypes = r, a
+    return L
+
+
+class _Xorshift32:
+    """tests/test_storm.c:56 xorshift32, seed 12345."""
+
+    def __init__(self, s=12345):
+        self.s = s
+
+    def __call__(self):
+        x = self.s
+        x ^= (x << 13) & 0xFFFFFFFF
+        x ^= x >> 17
+        x ^= (x << 5) & 0xFFFFFFFF
+        self.s = x
+        return x
+
+
+# tests/test_storm.c:261-270 configurations: (n_vec, universe, max_set, dups)
+CONTIG_CASES = [(40, 1024, 400, 0), (40, 4096, 3, 0), (60, 8192, 200, 0), (60, 8192, 150, 1),
+                (600, 40000, 150, 0), (600, 40000, 150, 1)]
+
+
+@pytest.mark.parametrize("case", CONTIG_CASES)
+def test_contig_dropin_matches_reference_and_oracle(case):
+    n_vec, universe, max_set, dups = case
+    _print_case("contig", case)
+    G = _gpu_contig()
+    rng = _Xorshift32()
+    rows = []
+    for _ in range(n_vec):
+        k = 1 + rng() % max_set
+        v = sorted(rng() % universe for _ in range(k))
+        if not dups:
+            v = sorted(set(v))
+        rows.append(np.array(v, dtype=np.uint32))
+    hg = G.STORM_contig_new(universe)
+    for r in rows:
+        assert G.STORM_contig_add(hg, r.ctypes.data_as(C.c_void_p), len(r)) == len(r)
+    got = G.STORM_contig_pairw_intersect_cardinality(hg)
+    got_b = G.STORM_contig_pairw_intersect_cardinality_blocked(hg, 8)
+    counts = np.zeros(n_vec * (n_vec - 1) // 2, dtype=np.uint32)
+    assert G.STORM_contig_gpu_pairw_counts(hg, counts.ctypes.data_as(C.c_void_p)) == 0
+    G.STORM_contig_free(hg)
+    uniq = [np.unique(r) for r in rows]
+    offs = np.concatenate([[0], np.cumsum([len(u) for u in uniq])]).astype(np.uint32)
+    pos = np.concatenate(uniq).astype(np.uint32)
+    want, want_tri = O.allpairs_lists(offs, pos)
+    assert got == want and got_b == want
+    assert np.array_equal(counts, want_tri)
+    if O.ref_available():
+        R = O.reference()
+        hr = R.STORM_contig_new(universe)
+        for r in rows:
+            R.STORM_contig_add(hr, r.ctypes.data_as(C.c_void_p), len(r))
+        assert R.STORM_contig_pairw_intersect_cardinality(hr) == got
+        R.STORM_contig_free(hr)
+
+
+def test_contig_dropin_edge_behaviour():
+    """storm.cpp:1176-1230 error and edge contract, through the GPU build."""
+    G = _gpu_contig()
+    assert G.STORM_contig_add(None, None, 0) == -1
+    h = G.STORM_contig_new(100)
+    assert G.STORM_contig_add(h, None, 3) == -2
+    one = np.array([5], dtype=np.uint32)
+    assert G.STORM_contig_add(h, one.ctypes.data_as(C.c_void_p), 0) == 0   # nothing added
+    assert G.STORM_contig_pairw_intersect_cardinality(h) == 0
+    assert G.STORM_contig_add(h, one.ctypes.data_as(C.c_void_p), 1) == 1
+    assert G.STORM_contig_pairw_intersect_cardinality(h) == 0            # one vector
+    assert G.STORM_contig_add(h, one.ctypes.data_as(C.c_void_p), 1) == 1
+    assert G.STORM_contig_pairw_intersect_cardinality(h) == 1
+    assert G.STORM_contig_clear(h) == 1
+    assert G.STORM_contig_pairw_intersect_cardinality(h) == 0
+    G.STORM_contig_free(h)
+    assert G.STORM_contig_pairw_intersect_cardinality(None) == 0
+    assert G.STORM_contig_clear(None) == -1
