@@ -95,8 +95,46 @@ typedef struct {
     const uint32_t* meta;       /* 8 per row */
 } storm_hip_arena;
 
+/* Upload from arenas. The lists are authoritative: runs, EWAH and complement
+ * (present when their offset arrays are non-NULL) are rebuilt on the host from
+ * the lists in the same layouts, and `meta` must agree with them (else
+ * STORM_HIP_ERR_INVALID). A bitmap is placed on the device for exactly the rows
+ * whose bitmap span is non-empty. */
 int  storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* arena,
                              storm_hip_corpus** out);
+
+/* Build every representation from sorted, distinct per-row positions (CSR:
+ * row r is pos[offs[r]..offs[r+1])), with the semantics of build_row
+ * (kernels/storm_repr.cpp:102-207) including its rank/occ/complement gates in
+ * RowMeta, and upload. The reference equivalent is building std::vector<Row>
+ * with build_row before allpairs_sum. flags: bits 0-1 bitmap placement
+ * (0: rows with cardinality*64 >= universe, 1: every row, 2: none), plus the
+ * STORM_HIP_BUILD_* representation bits. */
+enum {
+    STORM_HIP_BITMAP_DENSE = 0, STORM_HIP_BITMAP_ALL = 1, STORM_HIP_BITMAP_NONE = 2,
+    STORM_HIP_BUILD_RUNS = 4, STORM_HIP_BUILD_EWAH = 8, STORM_HIP_BUILD_COMP = 16,
+    STORM_HIP_BUILD_ALL = 4 | 8 | 16
+};
+/* Seeded synthetic corpora as host CSR lists (storm::generate's role,
+ * kernels/storm_gen.cpp:173-232, with the spectra the benchmark workloads
+ * need). Row r draws from its own splitmix64 stream seeded by draw r of
+ * Rng(seed) (kernels/storm_gen.h:84-102).
+ *   ZIPF       cardinality Zipf(s=p1) on [1, p2], positions uniform without
+ *              replacement
+ *   RUNS       Poisson(p1) runs, geometric lengths of mean p2, uniform starts,
+ *              plus round(p3 * run bits) uniform singletons
+ *   BERNOULLI  every bit i.i.d. Bernoulli(p1) (geometric skips)
+ * *offs (n_rows+1) and *pos are allocated by the library; release each with
+ * storm_hip_gen_free. */
+enum { STORM_HIP_GEN_ZIPF = 0, STORM_HIP_GEN_RUNS = 1, STORM_HIP_GEN_BERNOULLI = 2 };
+int  storm_hip_gen_corpus(uint32_t kind, uint32_t n_rows, uint32_t universe, double p1,
+                          double p2, double p3, uint64_t seed, uint64_t** offs, uint32_t** pos,
+                          uint64_t* total);
+void storm_hip_gen_free(void* p);
+
+int  storm_hip_corpus_from_lists(storm_hip_ctx* ctx, const uint64_t* offs, const uint32_t* pos,
+                                 uint32_t n_rows, uint32_t universe, uint32_t flags,
+                                 storm_hip_corpus** out);
 /* Materialise a bitmap on the device for every row that lacks one (needed by
  * the all-bitmap policy on sparse corpora). */
 int  storm_hip_corpus_materialize_bitmaps(storm_hip_ctx* ctx, storm_hip_corpus* c);
@@ -129,9 +167,12 @@ enum {
                                      idx(i,j) = i*N - i(i+1)/2 + (j-i-1)     */
 };
 enum {
-    STORM_HIP_BB_AUTO = 0,       /* MFMA where the tile is dense, else VALU  */
+    STORM_HIP_BB_AUTO = 0,       /* fastest exact form for the universe       */
     STORM_HIP_BB_VALU = 1,       /* AND + v_bcnt popcount                     */
-    STORM_HIP_BB_MFMA = 2        /* 0/1 int8 MFMA with i32 accumulation       */
+    STORM_HIP_BB_MFMA = 2,       /* 0/1 int8 MFMA with i32 accumulation       */
+    STORM_HIP_BB_MFMA_FP4 = 3    /* 0/1 e2m1 MFMA with f32 accumulation
+                                    (exact while counts <= 2^24; universes
+                                    up to 2^24 bits)                          */
 };
 /* Cell classes, in storm::Pairing order (kernels/storm_cost.h:47-49) plus the
  * complement cell (kernels/cell_comp.cpp). */
@@ -194,6 +235,42 @@ int storm_hip_get_cost_model(storm_hip_ctx* ctx, storm_hip_cost_model* m);
  * in row-major upper-triangle tile order. */
 int storm_hip_last_decisions(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                              uint8_t* cells, uint64_t n_tiles);
+
+/* ---- learned per-tile selector -------------------------------------------
+ * No reference counterpart: the reference selects with the analytic model
+ * only (kernels/storm_cost.cpp:422-465) and measures candidates at run time in
+ * its Probe policy (kernels/storm_allpairs.cpp:445-482). Here a small MLP
+ * (STORM_HIP_MLP_FEATURES tile features -> 32 -> 32 -> 10 cells, ReLU, fp32)
+ * predicts log2 of each cell's time on a tile and POLICY_LEARNED takes the
+ * cheapest eligible cell; tiles whose features leave the training range use
+ * the analytic model. Parameters are packed W1[32][F] b1 W2[32][32] b2
+ * W3[10][32] b3; `norm` is mean[F], inv_std[F], lo[F], hi[F]. */
+#define STORM_HIP_MLP_FEATURES 16
+int storm_hip_mlp_shape(int* features, int* h1, int* h2, int* outputs, int* n_params);
+/* Time every eligible cell on up to max_tiles sampled non-empty tile pairs
+ * (tile rows each), `repeats` times (min kept). Outputs per sampled tile:
+ * features[F], log2 of the tile's wall time in ns per cell (mask=1 where the
+ * cell was eligible and timed), the tile-pair index and its pair count. */
+int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
+                              uint32_t max_tiles, uint64_t seed, uint32_t repeats,
+                              float* features, float* log2_ns, uint8_t* mask,
+                              uint32_t* tile_index, uint32_t* n_sampled);
+/* Train on the device (He init from `seed`, Adam, minibatches of `batch`);
+ * the trained model then serves POLICY_LEARNED on this context. */
+int storm_hip_selector_train(storm_hip_ctx* ctx, const float* features, const float* log2_ns,
+                             const uint8_t* mask, uint32_t n, uint32_t epochs, uint32_t batch,
+                             float lr, uint64_t seed, float* final_loss);
+/* Network outputs (log2 ns per cell) for n raw feature rows, normalised with
+ * the model's stored statistics. */
+int storm_hip_selector_predict(storm_hip_ctx* ctx, const float* features, uint32_t n, float* out);
+int storm_hip_selector_set_params(storm_hip_ctx* ctx, const float* params, const float* norm);
+int storm_hip_selector_get_params(storm_hip_ctx* ctx, float* params, float* norm);
+/* Gradient of the masked mean-square loss at the current parameters over
+ * (already normalised) inputs; and one Adam step (b1 .9, b2 .999, eps 1e-8)
+ * with the last gradient. Exposed so the device math can be checked. */
+int storm_hip_selector_grad(storm_hip_ctx* ctx, const float* x, const float* t,
+                            const uint8_t* mask, uint32_t n, float* grad, float* loss);
+int storm_hip_selector_adam_step(storm_hip_ctx* ctx, float lr);
 
 #ifdef __cplusplus
 }

@@ -21,18 +21,11 @@ void set_error(const char* fmt, ...) {
     va_end(ap);
 }
 
-int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n,
-                   const uint32_t* tiles, uint64_t n_tiles, uint32_t* tri,
-                   unsigned long long* sum);
-int launch_bb_mfma(hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n,
-                   const uint32_t* tiles, uint64_t n_tiles, uint32_t* tri,
-                   unsigned long long* sum);
 int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_options* o,
                      storm_hip_stats* st, unsigned long long* d_sum);
 void general_free(storm_hip_corpus* c);
 void default_device_model(DeviceCostModel& m);
+void selector_destroy(storm_hip_ctx* ctx);
 
 // Bernoulli(p) bitmap generator: one thread per 64-bit word. Draw k of
 // Rng(seed) is mix(seed + (k+1)*golden); bit b of row r is draw r*n_bits+b.
@@ -171,6 +164,7 @@ void storm_hip_ctx_free(storm_hip_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    selector_destroy(ctx);
     for (auto& e : ctx->ev) if (e) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -255,7 +249,7 @@ int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
     storm_hip_options o{};
     if (opt) o = *opt;
     if (o.policy > STORM_HIP_POLICY_EXHAUSTIVE || o.out_mode > STORM_HIP_OUT_UPPER_TRI ||
-        o.bb_kernel > STORM_HIP_BB_MFMA || o.fixed_cell >= STORM_HIP_NCELL)
+        o.bb_kernel > STORM_HIP_BB_MFMA_FP4 || o.fixed_cell >= STORM_HIP_NCELL)
         return STORM_HIP_ERR_INVALID;
     storm_hip_stats local{};
     storm_hip_stats& s = st ? *st : local;
@@ -284,11 +278,8 @@ int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
         // Every row is a bitmap: the only cell is B x B. All policies agree
         // here; the span test would only skip work (AllBitmap never applies it).
         SH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-        const bool mfma = o.bb_kernel == STORM_HIP_BB_MFMA;
-        rc = mfma ? launch_bb_mfma(ctx->stream, c->d_words, c->ld, c->n_words, nullptr, nullptr,
-                                   c->n_rows, nullptr, 0, tri, d_sum)
-                  : launch_bb_valu(ctx->stream, 128, c->d_words, c->ld, c->n_words, nullptr,
-                                   nullptr, c->n_rows, nullptr, 0, tri, d_sum);
+        rc = launch_bb_dense(o.bb_kernel, ctx->stream, c->d_words, c->ld, c->n_words, nullptr,
+                             c->n_rows, tri, d_sum);
         SH_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
         s.cell_pairs[STORM_HIP_CELL_BB] = s.pairs;
         s.bb_launches = 1;

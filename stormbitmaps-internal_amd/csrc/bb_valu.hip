@@ -10,29 +10,30 @@
 // Layout: rows are `ld` u64 words apart (ld a multiple of 64, zero padded);
 // `rowmap` (optional) maps a tile-row index to a physical row of X; `perm`
 // (optional) maps a tile-row index to the ORIGINAL row id used for output.
+// Tiles come from the grid (upper triangle of the T x T block grid) or, in
+// queue mode, from a device queue drained by persistent workgroups.
 #include "device_common.h"
 #include "internal.h"
 
 namespace stormhip {
 
 template <int BM>
-__global__ __launch_bounds__(256, 2) void bb_valu_kernel(
-    const uint64_t* __restrict__ X, uint32_t ld, uint32_t kwords,
-    const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm, uint32_t n,
-    uint32_t T, const uint32_t* __restrict__ tiles, uint32_t* __restrict__ tri,
-    unsigned long long* __restrict__ sum) {
-    constexpr int KC = 16;               // words per LDS stage
+struct BBValuSmem {
+    static constexpr int KC = 16;   // words per LDS stage
+    uint64_t A[2][KC][BM];
+    uint64_t B[2][KC][BM];
+};
+
+template <int BM>
+__device__ uint64_t bb_valu_tile(BBValuSmem<BM>& sm, const uint64_t* __restrict__ X, uint32_t ld,
+                                 uint32_t kwords, const uint32_t* __restrict__ rowmap,
+                                 const uint32_t* __restrict__ perm, uint32_t n, uint32_t bi,
+                                 uint32_t bj, uint32_t* __restrict__ tri) {
+    constexpr int KC = BBValuSmem<BM>::KC;
     constexpr int TM = BM / 16;          // rows (and cols) per thread
     constexpr int WPT = BM * KC / 256;   // words each thread loads per panel
     constexpr int TPR = KC / WPT;        // threads per panel row
-    __shared__ uint64_t As[2][KC][BM];
-    __shared__ uint64_t Bs[2][KC][BM];
-
-    uint32_t bi, bj;
-    if (tiles) { bi = tiles[2 * blockIdx.x]; bj = tiles[2 * blockIdx.x + 1]; }
-    else       tri_tile(blockIdx.x, T, bi, bj);
     const uint32_t i0 = bi * BM, j0 = bj * BM;
-
     const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
     const int lrow = t / TPR, lseg = t % TPR;
     const uint64_t ra = rowmap ? rowmap[i0 + lrow] : (uint64_t)(i0 + lrow);
@@ -51,10 +52,10 @@ __global__ __launch_bounds__(256, 2) void bb_valu_kernel(
     auto sstore = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < WPT / 2; ++q) {
-            As[buf][lseg * WPT + 2 * q][lrow]     = ((uint64_t)ga[q].y << 32) | ga[q].x;
-            As[buf][lseg * WPT + 2 * q + 1][lrow] = ((uint64_t)ga[q].w << 32) | ga[q].z;
-            Bs[buf][lseg * WPT + 2 * q][lrow]     = ((uint64_t)gb[q].y << 32) | gb[q].x;
-            Bs[buf][lseg * WPT + 2 * q + 1][lrow] = ((uint64_t)gb[q].w << 32) | gb[q].z;
+            sm.A[buf][lseg * WPT + 2 * q][lrow]     = ((uint64_t)ga[q].y << 32) | ga[q].x;
+            sm.A[buf][lseg * WPT + 2 * q + 1][lrow] = ((uint64_t)ga[q].w << 32) | ga[q].z;
+            sm.B[buf][lseg * WPT + 2 * q][lrow]     = ((uint64_t)gb[q].y << 32) | gb[q].x;
+            sm.B[buf][lseg * WPT + 2 * q + 1][lrow] = ((uint64_t)gb[q].w << 32) | gb[q].z;
         }
     };
 
@@ -65,6 +66,7 @@ __global__ __launch_bounds__(256, 2) void bb_valu_kernel(
         for (int c = 0; c < TM; ++c) acc[r][c] = 0;
 
     const uint32_t nk = kwords / KC;
+    __syncthreads();   // LDS free (previous tile of a persistent workgroup)
     gload(0);
     sstore(0);
     __syncthreads();
@@ -75,9 +77,9 @@ __global__ __launch_bounds__(256, 2) void bb_valu_kernel(
         for (int k = 0; k < KC; ++k) {
             uint64_t a[TM], b[TM];
 #pragma unroll
-            for (int r = 0; r < TM; ++r) a[r] = As[buf][k][ty + 16 * r];
+            for (int r = 0; r < TM; ++r) a[r] = sm.A[buf][k][ty + 16 * r];
 #pragma unroll
-            for (int c = 0; c < TM; ++c) b[c] = Bs[buf][k][tx + 16 * c];
+            for (int c = 0; c < TM; ++c) b[c] = sm.B[buf][k][tx + 16 * c];
 #pragma unroll
             for (int r = 0; r < TM; ++r)
 #pragma unroll
@@ -108,30 +110,61 @@ __global__ __launch_bounds__(256, 2) void bb_valu_kernel(
             }
         }
     }
+    return local;
+}
+
+template <int BM>
+__global__ __launch_bounds__(256, 2) void bb_valu_kernel(
+    const uint64_t* __restrict__ X, uint32_t ld, uint32_t kwords,
+    const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm, uint32_t n,
+    uint32_t T, const uint2* __restrict__ queue, const unsigned int* __restrict__ qcount,
+    unsigned int* __restrict__ qhead, uint32_t* __restrict__ tri,
+    unsigned long long* __restrict__ sum, unsigned long long* __restrict__ tile_ticks) {
+    __shared__ BBValuSmem<BM> sm;
+    __shared__ uint32_t s_t;
+    uint64_t local = 0;
+    if (!qcount) {
+        uint32_t bi, bj;
+        tri_tile(blockIdx.x, T, bi, bj);
+        local = bb_valu_tile<BM>(sm, X, ld, kwords, rowmap, perm, n, bi, bj, tri);
+    } else {
+        for (;;) {
+            __syncthreads();
+            if (threadIdx.x == 0) s_t = atomicAdd(qhead, 1u);
+            __syncthreads();
+            const uint32_t t = s_t;
+            if (t >= *qcount) break;
+            const uint2 tp = queue[t];
+            const uint64_t t0 = wall_clock64();
+            local += bb_valu_tile<BM>(sm, X, ld, kwords, rowmap, perm, n, tp.x, tp.y, tri);
+            if (tile_ticks) {
+                __syncthreads();
+                if (threadIdx.x == 0) tile_ticks[t] = wall_clock64() - t0;
+            }
+        }
+    }
     local = wave_sum_u64(local);
     if ((threadIdx.x & 63) == 0 && local) atomicAdd(sum, (unsigned long long)local);
 }
 
-// Host launcher. `tiles` == nullptr: all T(T+1)/2 upper-triangle tiles of BM
-// rows; otherwise n_tiles explicit (bi, bj) pairs.
+// Host launcher. queue == nullptr: all T(T+1)/2 upper-triangle tiles of BM
+// rows, one workgroup each; otherwise `grid` persistent workgroups drain the
+// device queue (qcount/qhead on the device).
 int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n,
-                   const uint32_t* tiles, uint64_t n_tiles, uint32_t* tri,
-                   unsigned long long* sum) {
+                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n, const uint2* queue,
+                   const unsigned int* qcount, unsigned int* qhead, uint32_t grid, uint32_t* tri,
+                   unsigned long long* sum, unsigned long long* tile_ticks) {
     const uint32_t kwords = (n_words + 15) / 16 * 16;
     if (kwords > ld) { set_error("bb_valu: kwords %u > ld %u", kwords, ld); return STORM_HIP_ERR_INVALID; }
+    const uint32_t T = (n + bm - 1) / bm;
+    const uint64_t g = queue ? grid : (uint64_t)T * (T + 1) / 2;
+    if (g == 0) return STORM_HIP_OK;
     if (bm == 128) {
-        const uint32_t T = (n + 127) / 128;
-        const uint64_t grid = tiles ? n_tiles : (uint64_t)T * (T + 1) / 2;
-        if (grid == 0) return STORM_HIP_OK;
-        hipLaunchKernelGGL(bb_valu_kernel<128>, dim3((uint32_t)grid), dim3(256), 0, s, X, ld,
-                           kwords, rowmap, perm, n, T, tiles, tri, sum);
+        hipLaunchKernelGGL(bb_valu_kernel<128>, dim3((uint32_t)g), dim3(256), 0, s, X, ld, kwords,
+                           rowmap, perm, n, T, queue, qcount, qhead, tri, sum, tile_ticks);
     } else if (bm == 64) {
-        const uint32_t T = (n + 63) / 64;
-        const uint64_t grid = tiles ? n_tiles : (uint64_t)T * (T + 1) / 2;
-        if (grid == 0) return STORM_HIP_OK;
-        hipLaunchKernelGGL(bb_valu_kernel<64>, dim3((uint32_t)grid), dim3(256), 0, s, X, ld,
-                           kwords, rowmap, perm, n, T, tiles, tri, sum);
+        hipLaunchKernelGGL(bb_valu_kernel<64>, dim3((uint32_t)g), dim3(256), 0, s, X, ld, kwords,
+                           rowmap, perm, n, T, queue, qcount, qhead, tri, sum, tile_ticks);
     } else {
         set_error("bb_valu: unsupported tile %d", bm);
         return STORM_HIP_ERR_INVALID;

@@ -1,0 +1,398 @@
+// Host-side representation building and device upload of general corpora.
+//
+// From sorted, distinct per-row positions this builds what build_row
+// (kernels/storm_repr.cpp:102-207) builds -- runs, EWAH-64, complement and the
+// RowMeta fields with the same index gates -- in O(card) per row (the
+// reference scans the whole universe per row), multi-threaded over rows, then
+// uploads the arenas. Bitmaps are scattered on the device from the lists.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "device_common.h"
+#include "general.h"
+#include "internal.h"
+
+namespace stormhip {
+
+namespace {
+
+struct HostArena {
+    uint32_t n = 0, universe = 0, nw = 0;
+    bool has_runs = false, has_ewah = false, has_comp = false;
+    std::vector<uint64_t> list_off, run_off, ewah_off, mk_off, comp_off;
+    std::vector<uint32_t> list, run_start, run_end, mk_word, mk_pos, comp, meta;
+    std::vector<uint64_t> ewah;
+    std::vector<uint8_t> want_bitmap;
+};
+
+// Nonzero words of a sorted position list, in order.
+struct WordWalker {
+    const uint32_t* p; uint32_t n, i = 0;
+    WordWalker(const uint32_t* p_, uint32_t n_) : p(p_), n(n_) {}
+    bool next(uint32_t& w, uint64_t& v) {
+        if (i >= n) return false;
+        w = p[i] >> 6;
+        v = 0;
+        while (i < n && (p[i] >> 6) == w) { v |= 1ull << (p[i] & 63); ++i; }
+        return true;
+    }
+};
+
+// EWAH-64 of the row (build_ewah, kernels/storm_repr.cpp:78-100) from its
+// nonzero words; when `out` is null only sizes are returned.
+void ewah_from_list(const uint32_t* p, uint32_t n, uint32_t nw, uint64_t* out, uint32_t* mk_word,
+                    uint32_t* mk_pos, uint32_t& n_out, uint32_t& n_mk) {
+    // Nonzero words of the row.
+    std::vector<uint32_t> wi;
+    std::vector<uint64_t> wv;
+    {
+        WordWalker ww(p, n);
+        uint32_t w; uint64_t v;
+        while (ww.next(w, v)) { wi.push_back(w); wv.push_back(v); }
+    }
+    size_t k = 0;               // next nonzero word
+    uint32_t i = 0, o = 0, m = 0;
+    auto word_at = [&](uint32_t x) -> uint64_t {
+        return (k < wi.size() && wi[k] == x) ? wv[k] : 0ull;
+    };
+    while (i < nw) {
+        uint64_t cur = word_at(i);
+        bool fv = (cur == ~0ull);
+        uint64_t fl = 0;
+        const uint32_t start = i;
+        if (cur == 0) {
+            const uint32_t stop = (k < wi.size()) ? wi[k] : nw;
+            const uint64_t len = std::min<uint64_t>(stop - i, 0x7FFFFFFFull);
+            fl = len; i += (uint32_t)len;
+        } else if (fv) {
+            while (i < nw && k < wi.size() && wi[k] == i && wv[k] == ~0ull && fl < 0x7FFFFFFFull) {
+                ++i; ++k; ++fl;
+            }
+        } else {
+            fv = false;
+        }
+        const uint32_t lit0 = i;
+        const size_t k0 = k;
+        while (i < nw && k < wi.size() && wi[k] == i && wv[k] != ~0ull) { ++i; ++k; }
+        const uint64_t ll = i - lit0;
+        if (out) {
+            mk_word[m] = start;
+            mk_pos[m] = o;
+            out[o] = (uint64_t)fv | (fl << 1) | (ll << 32);
+            for (uint64_t q = 0; q < ll; ++q) out[o + 1 + q] = wv[k0 + q];
+        }
+        o += 1 + (uint32_t)ll;
+        ++m;
+    }
+    if (o == 0) {   // empty universe: one empty marker
+        if (out) { out[0] = 0; mk_word[0] = 0; mk_pos[0] = 0; }
+        o = 1; m = 1;
+    }
+    n_out = o;
+    n_mk = m;
+}
+
+void parallel_rows(uint32_t n, const std::function<void(uint32_t)>& fn) {
+    unsigned th = std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 64) th = 1;
+    std::atomic<uint32_t> next{0};
+    auto work = [&] {
+        for (;;) {
+            const uint32_t lo = next.fetch_add(64);
+            if (lo >= n) break;
+            for (uint32_t r = lo; r < std::min(n, lo + 64); ++r) fn(r);
+        }
+    };
+    std::vector<std::thread> ts;
+    for (unsigned t = 1; t < th; ++t) ts.emplace_back(work);
+    work();
+    for (auto& t : ts) t.join();
+}
+
+int build_host_arena(const uint64_t* offs, const uint32_t* pos, uint32_t n, uint32_t universe,
+                     uint32_t flags, HostArena& A) {
+    const uint32_t bitmap_policy = flags & 3u;
+    A.n = n; A.universe = universe; A.nw = (universe + 63) / 64;
+    A.has_runs = flags & STORM_HIP_BUILD_RUNS;
+    A.has_ewah = flags & STORM_HIP_BUILD_EWAH;
+    A.has_comp = flags & STORM_HIP_BUILD_COMP;
+    // Validate: sorted, distinct, in range.
+    std::atomic<int> bad{0};
+    parallel_rows(n, [&](uint32_t r) {
+        for (uint64_t i = offs[r]; i < offs[r + 1]; ++i)
+            if (pos[i] >= universe || (i > offs[r] && pos[i] <= pos[i - 1])) { bad = 1; return; }
+    });
+    if (bad) { set_error("row positions must be sorted, distinct and < universe"); return STORM_HIP_ERR_INVALID; }
+
+    std::vector<uint32_t> nrun(n), newah(n), nmk(n), ncomp(n);
+    A.meta.assign((size_t)n * 8, 0);
+    A.want_bitmap.assign(n, 0);
+    parallel_rows(n, [&](uint32_t r) {
+        const uint32_t* p = pos + offs[r];
+        const uint32_t c = (uint32_t)(offs[r + 1] - offs[r]);
+        uint32_t runs = 0, nnz = 0, lastw = 0xFFFFFFFFu;
+        for (uint32_t i = 0; i < c; ++i) {
+            if (i == 0 || p[i] != p[i - 1] + 1) ++runs;
+            if ((p[i] >> 6) != lastw) { ++nnz; lastw = p[i] >> 6; }
+        }
+        nrun[r] = runs;
+        uint32_t* m = &A.meta[(size_t)r * 8];
+        m[0] = c; m[1] = runs; m[2] = A.nw; m[3] = nnz;
+        m[4] = c ? p[0] : 0xFFFFFFFFu; m[5] = c ? p[c - 1] : 0;
+        const uint64_t mean_rl = runs ? (uint64_t)c / runs : 0;
+        m[6] = mean_rl >= 512;                                  // rank gate
+        m[7] = (uint64_t)c * 512 > (uint64_t)universe;          // occ gate
+        A.want_bitmap[r] = bitmap_policy == 1 ||
+                           (bitmap_policy == 0 && (uint64_t)c * 64 >= (uint64_t)universe);
+        if (A.has_ewah) {
+            uint32_t o, mk;
+            ewah_from_list(p, c, A.nw, nullptr, nullptr, nullptr, o, mk);
+            newah[r] = o; nmk[r] = mk;
+        }
+        ncomp[r] = (A.has_comp && (uint64_t)c * 2 > universe) ? universe - c : 0;
+    });
+    auto prefix = [n](const std::vector<uint32_t>& cnt, std::vector<uint64_t>& off) {
+        off.assign((size_t)n + 1, 0);
+        for (uint32_t r = 0; r < n; ++r) off[r + 1] = off[r] + cnt[r];
+    };
+    A.list_off.assign(offs, offs + n + 1);
+    A.list.assign(pos + offs[0], pos + offs[n]);
+    if (offs[0] != 0) for (auto& x : A.list_off) x -= offs[0];
+    try {
+        if (A.has_runs) {
+            prefix(nrun, A.run_off);
+            A.run_start.resize(A.run_off[n]);
+            A.run_end.resize(A.run_off[n]);
+        } else {
+            A.run_off.assign((size_t)n + 1, 0);
+        }
+        if (A.has_ewah) {
+            prefix(newah, A.ewah_off);
+            prefix(nmk, A.mk_off);
+            A.ewah.resize(A.ewah_off[n]);
+            A.mk_word.resize(A.mk_off[n]);
+            A.mk_pos.resize(A.mk_off[n]);
+        } else {
+            A.ewah_off.assign((size_t)n + 1, 0);
+            A.mk_off.assign((size_t)n + 1, 0);
+        }
+        prefix(ncomp, A.comp_off);
+        A.comp.resize(A.comp_off[n]);
+    } catch (...) {
+        set_error("host allocation failed building arenas");
+        return STORM_HIP_ERR_NOMEM;
+    }
+    parallel_rows(n, [&](uint32_t r) {
+        const uint32_t* p = pos + offs[r];
+        const uint32_t c = (uint32_t)(offs[r + 1] - offs[r]);
+        if (A.has_runs) {
+            uint64_t k = A.run_off[r];
+            for (uint32_t i = 0; i < c;) {
+                uint32_t j = i;
+                while (j + 1 < c && p[j + 1] == p[j] + 1) ++j;
+                A.run_start[k] = p[i]; A.run_end[k] = p[j] + 1; ++k;
+                i = j + 1;
+            }
+        }
+        if (A.has_ewah) {
+            uint32_t o, mk;
+            ewah_from_list(p, c, A.nw, &A.ewah[A.ewah_off[r]], &A.mk_word[A.mk_off[r]],
+                           &A.mk_pos[A.mk_off[r]], o, mk);
+        }
+        if (A.comp_off[r + 1] > A.comp_off[r]) {
+            uint64_t k = A.comp_off[r];
+            uint32_t prev = 0;
+            for (uint32_t i = 0; i < c; ++i) {
+                for (uint32_t q = prev; q < p[i]; ++q) A.comp[k++] = q;
+                prev = p[i] + 1;
+            }
+            for (uint32_t q = prev; q < universe; ++q) A.comp[k++] = q;
+        }
+    });
+    return STORM_HIP_OK;
+}
+
+template <class T>
+int up(T*& d, const std::vector<T>& h, hipStream_t s, uint64_t& bytes) {
+    const size_t b = std::max<size_t>(h.size(), 1) * sizeof(T);
+    SH_HIP(hipMalloc((void**)&d, b));
+    bytes += b;
+    if (!h.empty()) SH_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return STORM_HIP_OK;
+}
+
+// Scatter the list of every row that has a bitmap into its bitmap row.
+__global__ void scatter_bits_kernel(const uint64_t* __restrict__ list_off,
+                                    const uint32_t* __restrict__ list,
+                                    const uint32_t* __restrict__ bm_row,
+                                    unsigned long long* __restrict__ bitmap, uint32_t ld) {
+    const uint32_t r = blockIdx.x;
+    const uint32_t b = bm_row[r];
+    if (b == kNoBitmap) return;
+    unsigned long long* w = bitmap + (uint64_t)b * ld;
+    for (uint64_t i = list_off[r] + threadIdx.x; i < list_off[r + 1]; i += blockDim.x) {
+        const uint32_t x = list[i];
+        atomicOr(&w[x >> 6], 1ull << (x & 63));
+    }
+}
+
+int upload_bitmaps(storm_hip_corpus* c, hipStream_t s) {
+    // (Re)allocate the store for every row with a bitmap plus the zero row.
+    if (c->d_bitmap) { SH_HIP(hipFree(c->d_bitmap)); c->d_bitmap = nullptr; }
+    const size_t bytes = ((size_t)c->n_bitmap_rows + 1) * c->ld * 8;
+    SH_HIP(hipMalloc(&c->d_bitmap, bytes));
+    c->bm_capacity = c->n_bitmap_rows;
+    SH_HIP(hipMemsetAsync(c->d_bitmap, 0, bytes, s));
+    SH_HIP(hipMemcpyAsync(c->d_bm_row, c->h_bm_row.data(), (size_t)c->n_rows * 4,
+                          hipMemcpyHostToDevice, s));
+    if (c->n_rows) {
+        hipLaunchKernelGGL(scatter_bits_kernel, dim3(c->n_rows), dim3(256), 0, s, c->d_list_off,
+                           c->d_list, c->d_bm_row, (unsigned long long*)c->d_bitmap, c->ld);
+        SH_HIP(hipGetLastError());
+    }
+    return STORM_HIP_OK;
+}
+
+int upload_host_arena(storm_hip_ctx* ctx, const HostArena& A, storm_hip_corpus** out) {
+    storm_hip_corpus* c = new (std::nothrow) storm_hip_corpus();
+    if (!c) return STORM_HIP_ERR_NOMEM;
+    *out = c;   // freed by the caller on failure
+    c->ctx = ctx;
+    c->dense = false;
+    c->n_rows = A.n;
+    c->universe = A.universe;
+    c->n_words = A.nw;
+    c->ld = (std::max(A.nw, 1u) + kWordPad - 1) / kWordPad * kWordPad;
+    c->has_runs = A.has_runs; c->has_ewah = A.has_ewah; c->has_comp = A.has_comp;
+    hipStream_t s = ctx->stream;
+    uint64_t& B = c->device_bytes;
+    SH_TRY(up(c->d_list_off, A.list_off, s, B));
+    SH_TRY(up(c->d_list, A.list, s, B));
+    SH_TRY(up(c->d_run_off, A.run_off, s, B));
+    SH_TRY(up(c->d_run_start, A.run_start, s, B));
+    SH_TRY(up(c->d_run_end, A.run_end, s, B));
+    SH_TRY(up(c->d_ewah_off, A.ewah_off, s, B));
+    SH_TRY(up(c->d_ewah, A.ewah, s, B));
+    SH_TRY(up(c->d_mk_off, A.mk_off, s, B));
+    SH_TRY(up(c->d_mk_word, A.mk_word, s, B));
+    SH_TRY(up(c->d_mk_pos, A.mk_pos, s, B));
+    SH_TRY(up(c->d_comp_off, A.comp_off, s, B));
+    SH_TRY(up(c->d_comp, A.comp, s, B));
+    SH_TRY(up(c->d_meta, A.meta, s, B));
+    c->h_meta = A.meta;
+    // Density order: stable by cardinality (kernels/storm_allpairs.cpp:316-323),
+    // then padding positions map to the sentinel row id n (never read as data).
+    const uint32_t npad = (A.n + kRowPad - 1) / kRowPad * kRowPad + kRowPad;
+    c->h_perm.resize(A.n);
+    std::iota(c->h_perm.begin(), c->h_perm.end(), 0u);
+    std::stable_sort(c->h_perm.begin(), c->h_perm.end(), [&](uint32_t a, uint32_t b) {
+        return A.meta[(size_t)a * 8] < A.meta[(size_t)b * 8];
+    });
+    std::vector<uint32_t> perm(npad, A.n), ident(npad, A.n);
+    std::copy(c->h_perm.begin(), c->h_perm.end(), perm.begin());
+    std::iota(ident.begin(), ident.begin() + A.n, 0u);
+    SH_TRY(up(c->d_perm, perm, s, B));
+    SH_TRY(up(c->d_ident, ident, s, B));
+    // Bitmap rows; the sentinel row id n maps to the zero row.
+    c->h_bm_row.assign((size_t)A.n + 1, kNoBitmap);
+    uint32_t nb = 0;
+    for (uint32_t r = 0; r < A.n; ++r)
+        if (A.want_bitmap[r]) c->h_bm_row[r] = nb++;
+    c->n_bitmap_rows = nb;
+    c->h_bm_row[A.n] = nb;   // zero row
+    c->bitmaps_complete = (nb == A.n);
+    SH_HIP(hipMalloc(&c->d_bm_row, ((size_t)A.n + 1) * 4));
+    B += ((size_t)A.n + 1) * 4;
+    SH_TRY(upload_bitmaps(c, s));
+    SH_HIP(hipMemcpyAsync(c->d_bm_row + A.n, &c->h_bm_row[A.n], 4, hipMemcpyHostToDevice, s));
+    B += ((size_t)nb + 1) * c->ld * 8;
+    SH_HIP(hipStreamSynchronize(s));
+    return STORM_HIP_OK;
+}
+
+}  // namespace
+
+int general_materialize(storm_hip_ctx* ctx, storm_hip_corpus* c) {
+    if (c->dense || c->bitmaps_complete) return STORM_HIP_OK;
+    c->device_bytes -= ((size_t)c->n_bitmap_rows + 1) * c->ld * 8;
+    for (uint32_t r = 0; r < c->n_rows; ++r) c->h_bm_row[r] = r;
+    c->n_bitmap_rows = c->n_rows;
+    c->h_bm_row[c->n_rows] = c->n_rows;
+    SH_TRY(upload_bitmaps(c, ctx->stream));
+    SH_HIP(hipMemcpyAsync(c->d_bm_row + c->n_rows, &c->h_bm_row[c->n_rows], 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    c->device_bytes += ((size_t)c->n_bitmap_rows + 1) * c->ld * 8;
+    c->bitmaps_complete = true;
+    SH_HIP(hipStreamSynchronize(ctx->stream));
+    return STORM_HIP_OK;
+}
+
+void general_free_arenas(storm_hip_corpus* c) {
+    void* ptrs[] = {c->d_list_off, c->d_list, c->d_run_off, c->d_run_start, c->d_run_end,
+                    c->d_ewah_off, c->d_ewah, c->d_mk_off, c->d_mk_word, c->d_mk_pos,
+                    c->d_comp_off, c->d_comp, c->d_meta, c->d_bm_row, c->d_bitmap,
+                    c->d_perm, c->d_ident, c->d_bitmap_off, c->d_occ_off, c->d_occ};
+    for (void* p : ptrs) if (p) (void)hipFree(p);
+}
+
+}  // namespace stormhip
+
+using namespace stormhip;
+
+extern "C" {
+
+int storm_hip_corpus_from_lists(storm_hip_ctx* ctx, const uint64_t* offs, const uint32_t* pos,
+                                uint32_t n_rows, uint32_t universe, uint32_t flags,
+                                storm_hip_corpus** out) {
+    if (!ctx || !out) return STORM_HIP_ERR_INVALID;
+    *out = nullptr;
+    if (!offs || (!pos && n_rows && offs[n_rows] > offs[0])) return STORM_HIP_ERR_NULL_VALUES;
+    if (universe == 0 || (flags & 3u) == 3u) return STORM_HIP_ERR_INVALID;
+    for (uint32_t r = 0; r < n_rows; ++r)
+        if (offs[r + 1] < offs[r]) return STORM_HIP_ERR_INVALID;
+    SH_HIP(hipSetDevice(ctx->device));
+    HostArena A;
+    SH_TRY(build_host_arena(offs, pos, n_rows, universe, flags, A));
+    const int s = upload_host_arena(ctx, A, out);
+    if (s != STORM_HIP_OK && *out) { storm_hip_corpus_free(*out); *out = nullptr; }
+    return s;
+}
+
+int storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* a, storm_hip_corpus** out) {
+    if (!ctx || !a || !out) return STORM_HIP_ERR_INVALID;
+    *out = nullptr;
+    if (!a->list_off || !a->meta) return STORM_HIP_ERR_NULL_VALUES;
+    // The lists carry the row; every other arena is rebuilt from them with the
+    // same layouts, and the given meta must agree with the lists.
+    uint32_t flags = 2;   // bitmaps only where the arena supplies one
+    if (a->run_off) flags |= STORM_HIP_BUILD_RUNS;
+    if (a->ewah_off) flags |= STORM_HIP_BUILD_EWAH;
+    if (a->comp_off) flags |= STORM_HIP_BUILD_COMP;
+    SH_HIP(hipSetDevice(ctx->device));
+    HostArena A;
+    SH_TRY(build_host_arena(a->list_off, a->list, a->n_rows, a->universe, flags, A));
+    for (uint32_t r = 0; r < a->n_rows; ++r) {
+        for (int k = 0; k < 6; ++k)
+            if (A.meta[(size_t)r * 8 + k] != a->meta[(size_t)r * 8 + k]) {
+                set_error("row %u: meta field %d disagrees with its list", r, k);
+                return STORM_HIP_ERR_INVALID;
+            }
+        A.want_bitmap[r] = a->bitmap_off && a->bitmap_off[r + 1] > a->bitmap_off[r];
+    }
+    const int s = upload_host_arena(ctx, A, out);
+    if (s != STORM_HIP_OK && *out) { storm_hip_corpus_free(*out); *out = nullptr; }
+    return s;
+}
+
+int storm_hip_corpus_materialize_bitmaps(storm_hip_ctx* ctx, storm_hip_corpus* c) {
+    if (!ctx || !c || c->ctx != ctx) return STORM_HIP_ERR_INVALID;
+    SH_HIP(hipSetDevice(ctx->device));
+    return general_materialize(ctx, c);
+}
+
+}  // extern "C"

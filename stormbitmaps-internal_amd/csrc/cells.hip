@@ -1,0 +1,377 @@
+// The sparse and compressed cells: exact |Xi n Xj| for every pair of a tile of
+// rows, one kernel per cell class, persistent workgroups pulling tiles from
+// that class's device queue.
+//
+// Cell semantics follow the reference's cell families (kernels/cell_bs.cpp,
+// cell_br.cpp, cell_sparse.cpp, cell_wah.cpp, cell_comp.cpp) and the driver's
+// orientation rules (kernels/storm_allpairs.cpp:192-251):
+//   B x S  probe the bitmap at each list position
+//   B x R  masked popcount of the bitmap over each run
+//   B x W  walk the EWAH markers: one-fills popcount the bitmap, literals AND
+//   S x S  each element of the shorter list searched in the longer one
+//   S x R  each list element located among the runs (the sparser row's list,
+//          the denser row's runs)
+//   S x W  each list element's word looked up in the EWAH marker table
+//   R x R  overlap of each run of the side with fewer runs with the other
+//   R x W  bits of the EWAH row inside each run
+//   W x W  markers of one row against the other's decoded words
+//   C x B  |A n B| = |B| - |A' n B| over the complement list A' (cell_comp.cpp)
+// Unlike the reference driver, S x W and R x W are dispatched (it returns 0
+// for them, storm_allpairs.cpp:245-249); every count here is exact.
+//
+// Work split: a pair's work items (list elements, runs, markers) are spread
+// over the 64 lanes of a wave and wave-reduced ("wave mode"), or, for tiles
+// whose rows are tiny, each lane evaluates whole pairs ("lane mode").
+#include "device_common.h"
+#include "general.h"
+
+namespace stormhip {
+
+namespace {
+
+struct Span32 { const uint32_t* p; uint32_t n; };
+struct RunsV { const uint32_t* s; const uint32_t* e; uint32_t n; };
+struct EwahV { const uint64_t* buf; const uint32_t* mw; const uint32_t* mp; uint32_t nm; };
+
+__device__ inline Span32 list_of(const DevCorpus& D, uint32_t r) {
+    const uint64_t a = D.list_off[r];
+    return {D.list + a, (uint32_t)(D.list_off[r + 1] - a)};
+}
+__device__ inline Span32 comp_of(const DevCorpus& D, uint32_t r) {
+    const uint64_t a = D.comp_off[r];
+    return {D.comp + a, (uint32_t)(D.comp_off[r + 1] - a)};
+}
+__device__ inline RunsV runs_of(const DevCorpus& D, uint32_t r) {
+    const uint64_t a = D.run_off[r];
+    return {D.run_start + a, D.run_end + a, (uint32_t)(D.run_off[r + 1] - a)};
+}
+__device__ inline EwahV ewah_of(const DevCorpus& D, uint32_t r) {
+    const uint64_t m = D.mk_off[r];
+    return {D.ewah + D.ewah_off[r], D.mk_word + m, D.mk_pos + m, (uint32_t)(D.mk_off[r + 1] - m)};
+}
+__device__ inline const uint64_t* bitmap_of(const DevCorpus& D, uint32_t r) {
+    const uint32_t b = D.bm_row[r];
+    return b == kNoBitmap ? nullptr : D.bitmap + (uint64_t)b * D.ld;
+}
+__device__ inline uint32_t meta(const DevCorpus& D, uint32_t r, int k) { return D.meta[(uint64_t)r * 8 + k]; }
+
+// First index with a[idx] >= x.
+__device__ inline uint32_t lower_bound(const uint32_t* a, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+// First index with a[idx] > x.
+__device__ inline uint32_t upper_bound(const uint32_t* a, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint64_t bits_from(uint32_t lo) { return ~0ull << lo; }          // bits >= lo
+__device__ inline uint64_t bits_to(uint32_t hi) { return ~0ull >> (63 - hi); }     // bits <= hi
+
+// Set bits of a bitmap inside [s, e).
+__device__ inline uint32_t bitmap_range_count(const uint64_t* bm, uint32_t s, uint32_t e) {
+    if (s >= e) return 0;
+    const uint32_t ws = s >> 6, we = (e - 1) >> 6;
+    if (ws == we) return __popcll(bm[ws] & bits_from(s & 63) & bits_to((e - 1) & 63));
+    uint32_t c = __popcll(bm[ws] & bits_from(s & 63)) + __popcll(bm[we] & bits_to((e - 1) & 63));
+    for (uint32_t w = ws + 1; w < we; ++w) c += __popcll(bm[w]);
+    return c;
+}
+
+__device__ inline uint32_t ewah_find(const EwahV& E, uint32_t w) {
+    const uint32_t k = upper_bound(E.mw, E.nm, w);
+    return k ? k - 1 : 0;
+}
+__device__ inline uint64_t ewah_word(const EwahV& E, uint32_t m, uint32_t w) {
+    const uint64_t mk = E.buf[E.mp[m]];
+    const uint32_t fl = (uint32_t)((mk >> 1) & 0x7FFFFFFFull), ll = (uint32_t)(mk >> 32);
+    uint32_t off = w - E.mw[m];
+    if (off < fl) return (mk & 1) ? ~0ull : 0ull;
+    off -= fl;
+    return off < ll ? E.buf[E.mp[m] + 1 + off] : 0ull;
+}
+__device__ inline uint32_t ewah_next_start(const EwahV& E, uint32_t m) {
+    return m + 1 < E.nm ? E.mw[m + 1] : 0xFFFFFFFFu;
+}
+
+// Set bits of an EWAH row inside [s, e); fills are counted without a word walk.
+__device__ inline uint32_t ewah_range_count(const EwahV& E, uint32_t s, uint32_t e) {
+    if (s >= e || E.nm == 0) return 0;
+    const uint32_t ws = s >> 6, we = (e - 1) >> 6;
+    uint32_t m = ewah_find(E, ws), w = ws, c = 0;
+    while (w <= we && m < E.nm) {
+        const uint64_t mk = E.buf[E.mp[m]];
+        const uint32_t fl = (uint32_t)((mk >> 1) & 0x7FFFFFFFull), ll = (uint32_t)(mk >> 32);
+        const uint32_t fill_end = E.mw[m] + fl, lit_end = fill_end + ll;
+        if (w < fill_end) {
+            const uint32_t hi = min(fill_end - 1, we);
+            if (mk & 1) {
+                const uint64_t a = max((uint64_t)s, (uint64_t)w * 64);
+                const uint64_t b = min((uint64_t)e, ((uint64_t)hi + 1) * 64);
+                c += (uint32_t)(b - a);
+            }
+            w = hi + 1;
+            continue;
+        }
+        if (w < lit_end) {
+            uint64_t v = E.buf[E.mp[m] + 1 + (w - fill_end)];
+            if (w == ws) v &= bits_from(s & 63);
+            if (w == we) v &= bits_to((e - 1) & 63);
+            c += __popcll(v);
+            ++w;
+            continue;
+        }
+        ++m;
+    }
+    return c;
+}
+
+// One cell evaluated for pair (i, j) (original ids), over work items
+// first, first+stride, ... . Returns this thread's partial count.
+template <int CELL>
+__device__ uint32_t cell_partial(const DevCorpus& D, const DevModel& M, uint32_t i, uint32_t j,
+                                 uint32_t first, uint32_t stride) {
+    const uint32_t ci = meta(D, i, 0), cj = meta(D, j, 0);
+    const bool i_dense = ci >= cj;
+    const uint32_t d = i_dense ? i : j, s = i_dense ? j : i;
+    uint32_t c = 0;
+    if constexpr (CELL == STORM_HIP_CELL_BS) {
+        const uint64_t* bm = bitmap_of(D, d);
+        uint32_t l = s;
+        if (!bm) { bm = bitmap_of(D, s); l = d; }
+        const Span32 L = list_of(D, l);
+        for (uint32_t k = first; k < L.n; k += stride) {
+            const uint32_t x = L.p[k];
+            c += (uint32_t)(bm[x >> 6] >> (x & 63)) & 1u;
+        }
+    } else if constexpr (CELL == STORM_HIP_CELL_BR || CELL == STORM_HIP_CELL_BW) {
+        // Run-count orientation with the reference's margin (storm_cost.cpp:181-188).
+        const double rd = meta(D, d, 1), rs = meta(D, s, 1);
+        bool swap = M.br_margin > 0.0 && rd * M.br_margin <= rs;
+        uint32_t b = swap ? s : d, o = swap ? d : s;
+        if (!bitmap_of(D, b)) { const uint32_t t = b; b = o; o = t; }
+        const uint64_t* bm = bitmap_of(D, b);
+        if constexpr (CELL == STORM_HIP_CELL_BR) {
+            const RunsV R = runs_of(D, o);
+            for (uint32_t k = first; k < R.n; k += stride) c += bitmap_range_count(bm, R.s[k], R.e[k]);
+        } else {
+            const EwahV E = ewah_of(D, o);
+            for (uint32_t k = first; k < E.nm; k += stride) {
+                const uint64_t mk = E.buf[E.mp[k]];
+                const uint32_t fl = (uint32_t)((mk >> 1) & 0x7FFFFFFFull), ll = (uint32_t)(mk >> 32);
+                const uint32_t w0 = E.mw[k];
+                if (mk & 1)
+                    for (uint32_t w = w0; w < w0 + fl; ++w) c += __popcll(bm[w]);
+                for (uint32_t q = 0; q < ll; ++q)
+                    c += __popcll(bm[w0 + fl + q] & E.buf[E.mp[k] + 1 + q]);
+            }
+        }
+    } else if constexpr (CELL == STORM_HIP_CELL_SS) {
+        const Span32 A = list_of(D, s), B = list_of(D, d);   // |A| <= |B|
+        for (uint32_t k = first; k < A.n; k += stride) {
+            const uint32_t x = A.p[k];
+            const uint32_t at = lower_bound(B.p, B.n, x);
+            c += (at < B.n && B.p[at] == x) ? 1u : 0u;
+        }
+    } else if constexpr (CELL == STORM_HIP_CELL_SR) {
+        const Span32 L = list_of(D, s);
+        const RunsV R = runs_of(D, d);
+        for (uint32_t k = first; k < L.n; k += stride) {
+            const uint32_t x = L.p[k];
+            const uint32_t u = upper_bound(R.s, R.n, x);
+            c += (u > 0 && x < R.e[u - 1]) ? 1u : 0u;
+        }
+    } else if constexpr (CELL == STORM_HIP_CELL_SW) {
+        const Span32 L = list_of(D, s);
+        const EwahV E = ewah_of(D, d);
+        for (uint32_t k = first; k < L.n; k += stride) {
+            const uint32_t x = L.p[k];
+            const uint64_t v = ewah_word(E, ewah_find(E, x >> 6), x >> 6);
+            c += (uint32_t)(v >> (x & 63)) & 1u;
+        }
+    } else if constexpr (CELL == STORM_HIP_CELL_RR) {
+        uint32_t a = d, b = s;
+        if (meta(D, s, 1) < meta(D, d, 1)) { a = s; b = d; }
+        const RunsV A = runs_of(D, a), B = runs_of(D, b);
+        for (uint32_t k = first; k < A.n; k += stride) {
+            const uint32_t rs = A.s[k], re = A.e[k];
+            for (uint32_t q = upper_bound(B.e, B.n, rs); q < B.n && B.s[q] < re; ++q)
+                c += min(re, B.e[q]) - max(rs, B.s[q]);
+        }
+    } else if constexpr (CELL == STORM_HIP_CELL_RW) {
+        // min(ra + wb, rb + wa) orientation (storm_cost.cpp:329).
+        const uint64_t ra = meta(D, d, 1), rb = meta(D, s, 1);
+        const uint64_t wa = D.ewah_off[d + 1] - D.ewah_off[d], wb = D.ewah_off[s + 1] - D.ewah_off[s];
+        const uint32_t r_side = (ra + wb <= rb + wa) ? d : s, w_side = (r_side == d) ? s : d;
+        const RunsV R = runs_of(D, r_side);
+        const EwahV E = ewah_of(D, w_side);
+        for (uint32_t k = first; k < R.n; k += stride) c += ewah_range_count(E, R.s[k], R.e[k]);
+    } else if constexpr (CELL == STORM_HIP_CELL_WW) {
+        uint32_t a = d, b = s;
+        if (D.mk_off[s + 1] - D.mk_off[s] < D.mk_off[d + 1] - D.mk_off[d]) { a = s; b = d; }
+        const EwahV A = ewah_of(D, a), B = ewah_of(D, b);
+        for (uint32_t k = first; k < A.nm; k += stride) {
+            const uint64_t mk = A.buf[A.mp[k]];
+            const uint32_t fl = (uint32_t)((mk >> 1) & 0x7FFFFFFFull), ll = (uint32_t)(mk >> 32);
+            const uint32_t w0 = A.mw[k];
+            if ((mk & 1) && fl) c += ewah_range_count(B, w0 * 64u, (w0 + fl) * 64u);
+            if (ll) {
+                uint32_t m = ewah_find(B, w0 + fl), next = ewah_next_start(B, m);
+                for (uint32_t q = 0; q < ll; ++q) {
+                    const uint32_t w = w0 + fl + q;
+                    while (w >= next) { ++m; next = ewah_next_start(B, m); }
+                    c += __popcll(A.buf[A.mp[k] + 1 + q] & ewah_word(B, m, w));
+                }
+            }
+        }
+    } else if constexpr (CELL == STORM_HIP_CELL_CB) {
+        // Complement side a (2n > universe), bitmap side b. Partial = probes of
+        // the complement into b; the caller forms |b| - probes.
+        uint32_t a = d, b = s;
+        if (!(2ull * meta(D, d, 0) > D.universe) || !bitmap_of(D, s)) { a = s; b = d; }
+        const uint64_t* bm = bitmap_of(D, b);
+        const Span32 A = comp_of(D, a);
+        for (uint32_t k = first; k < A.n; k += stride) {
+            const uint32_t x = A.p[k];
+            c += (uint32_t)(bm[x >> 6] >> (x & 63)) & 1u;
+        }
+    }
+    return c;
+}
+
+// The card of the bitmap side the C x B partial was taken against.
+__device__ inline uint32_t cb_base(const DevCorpus& D, uint32_t i, uint32_t j) {
+    const uint32_t ci = meta(D, i, 0), cj = meta(D, j, 0);
+    const uint32_t d = ci >= cj ? i : j, s = ci >= cj ? j : i;
+    uint32_t b = s;
+    if (!(2ull * meta(D, d, 0) > D.universe) || !bitmap_of(D, s)) b = d;
+    return meta(D, b, 0);
+}
+
+// Span test (kernels/storm_allpairs.cpp:627-634): true if provably disjoint.
+__device__ inline bool span_disjoint(const DevCorpus& D, uint32_t i, uint32_t j) {
+    const uint32_t ci = meta(D, i, 0), cj = meta(D, j, 0);
+    return ci == 0 || cj == 0 || meta(D, i, 5) < meta(D, j, 4) || meta(D, j, 5) < meta(D, i, 4);
+}
+
+}  // namespace
+
+template <int CELL>
+__global__ __launch_bounds__(256) void cell_kernel(DevCorpus D, DevModel M,
+                                                   const TileMeta* __restrict__ tm, uint32_t tile,
+                                                   const uint2* __restrict__ queue,
+                                                   DevCounters* __restrict__ ctr,
+                                                   uint32_t* __restrict__ tri,
+                                                   unsigned long long* __restrict__ sum,
+                                                   unsigned long long* __restrict__ tile_cycles) {
+    __shared__ uint32_t s_t;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t local = 0, pairs = 0, skipped = 0;
+    const uint32_t n = D.n;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_t = atomicAdd(&ctr->qhead[CELL], 1u);
+        __syncthreads();
+        const uint32_t t = s_t;
+        if (t >= ctr->qcount[CELL]) break;
+        const uint64_t t0 = wall_clock64();
+        const uint2 tp = queue[t];
+        const uint32_t i0 = tp.x * tile, j0 = tp.y * tile;
+        const uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
+        const bool diag = tp.x == tp.y;
+        const uint32_t np = ni * nj;
+        // Lane mode when the sparser tile's rows are tiny.
+        const TileMeta& A = tm[tp.x];
+        const bool lane_mode = CELL == STORM_HIP_CELL_EMPTY || (A.max_card <= 48 && A.max_runs <= 48);
+        if (lane_mode) {
+            for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
+                const uint32_t a = i0 + p / nj, b = j0 + p % nj;
+                if (diag && b <= a) continue;
+                const uint32_t i = D.perm[a], j = D.perm[b];
+                uint32_t cnt = 0;
+                if (CELL == STORM_HIP_CELL_EMPTY || span_disjoint(D, i, j)) {
+                    ++skipped;
+                } else {
+                    ++pairs;
+                    cnt = cell_partial<CELL>(D, M, i, j, 0, 1);
+                    if (CELL == STORM_HIP_CELL_CB) cnt = cb_base(D, i, j) - cnt;
+                }
+                local += cnt;
+                if (tri) tri[i < j ? tri_index(i, j, n) : tri_index(j, i, n)] = cnt;
+            }
+        } else {
+            for (uint32_t p = wave; p < np; p += blockDim.x / 64) {
+                const uint32_t a = i0 + p / nj, b = j0 + p % nj;
+                if (diag && b <= a) continue;
+                const uint32_t i = D.perm[a], j = D.perm[b];
+                uint32_t cnt = 0;
+                if (span_disjoint(D, i, j)) {
+                    if (lane == 0) ++skipped;
+                } else {
+                    cnt = wave_sum_u32(cell_partial<CELL>(D, M, i, j, lane, 64));
+                    if (CELL == STORM_HIP_CELL_CB) cnt = cb_base(D, i, j) - cnt;
+                    if (lane == 0) ++pairs;
+                }
+                if (lane == 0) {
+                    local += cnt;
+                    if (tri) tri[i < j ? tri_index(i, j, n) : tri_index(j, i, n)] = cnt;
+                }
+            }
+        }
+        if (tile_cycles) {
+            __syncthreads();
+            if (threadIdx.x == 0) tile_cycles[t] = wall_clock64() - t0;
+        }
+    }
+    local = wave_sum_u64(local);
+    pairs = wave_sum_u64(pairs);
+    skipped = wave_sum_u64(skipped);
+    if (lane == 0) {
+        if (local) atomicAdd(sum, (unsigned long long)local);
+        if (pairs) atomicAdd(&ctr->cell_pairs[CELL], (unsigned long long)pairs);
+        if (skipped) {
+            atomicAdd(&ctr->cell_pairs[STORM_HIP_CELL_EMPTY], (unsigned long long)skipped);
+            atomicAdd(&ctr->skipped, (unsigned long long)skipped);
+        }
+    }
+}
+
+int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus& D, const DevModel& M,
+                const TileMeta* tm, uint32_t tile, const uint2* queue, DevCounters* ctr,
+                uint32_t* tri, unsigned long long* sum, unsigned long long* tile_cycles) {
+#define SH_CELL(C)                                                                          \
+    case C:                                                                                 \
+        hipLaunchKernelGGL(cell_kernel<C>, dim3(grid), dim3(256), 0, s, D, M, tm, tile,     \
+                           queue, ctr, tri, sum, tile_cycles);                               \
+        break;
+    switch (cell) {
+        SH_CELL(STORM_HIP_CELL_BS)
+        SH_CELL(STORM_HIP_CELL_BR)
+        SH_CELL(STORM_HIP_CELL_BW)
+        SH_CELL(STORM_HIP_CELL_SS)
+        SH_CELL(STORM_HIP_CELL_SR)
+        SH_CELL(STORM_HIP_CELL_SW)
+        SH_CELL(STORM_HIP_CELL_RR)
+        SH_CELL(STORM_HIP_CELL_RW)
+        SH_CELL(STORM_HIP_CELL_WW)
+        SH_CELL(STORM_HIP_CELL_EMPTY)
+        SH_CELL(STORM_HIP_CELL_CB)
+        default:
+            set_error("launch_cell: cell %d has no sparse kernel", cell);
+            return STORM_HIP_ERR_INVALID;
+    }
+#undef SH_CELL
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+}  // namespace stormhip

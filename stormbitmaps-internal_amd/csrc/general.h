@@ -1,0 +1,83 @@
+// Device-side view of a general (mixed-representation) corpus and the
+// structures the selection and cell kernels share.
+#pragma once
+
+#include <cstdint>
+
+#include "internal.h"
+
+namespace stormhip {
+
+constexpr uint32_t kNoBitmap = 0xFFFFFFFFu;
+constexpr int kNumCells = STORM_HIP_NCELL;   // BB..WW, EMPTY, CB
+
+// Row arenas on the device. Row ids here are ORIGINAL ids; `perm` maps a
+// density-sorted position to its original id.
+struct DevCorpus {
+    uint32_t n, universe, nw, ld;
+    const uint64_t* list_off; const uint32_t* list;
+    const uint64_t* run_off;  const uint32_t* run_start; const uint32_t* run_end;
+    const uint64_t* ewah_off; const uint64_t* ewah;
+    // EWAH marker table: for marker m of row r (m in [mk_off[r], mk_off[r+1])):
+    // mk_word = decoded index of its first word, mk_pos = its offset in the row's
+    // EWAH buffer.
+    const uint64_t* mk_off;   const uint32_t* mk_word; const uint32_t* mk_pos;
+    const uint64_t* comp_off; const uint32_t* comp;
+    const uint32_t* bm_row;   const uint64_t* bitmap;   // bitmap row index, kNoBitmap if none
+    const uint32_t* meta;     // 8 per original row (RowMeta + has_rank, has_occ)
+    const uint32_t* perm;     // sorted position -> original row id
+};
+
+// Aggregate metadata of one row tile (density-sorted rows [lo, hi)). The first
+// block is tile_meta of kernels/storm_allpairs.cpp:281-312; the rest are
+// eligibility flags and spread statistics for the kernels and the selector.
+struct TileMeta {
+    uint32_t card, runs, nnz, nw, first, last;   // means (integer), min first, max last
+    uint32_t has_rank, has_occ;                  // AND over rows
+    uint32_t all_bitmap, all_comp, any_empty;    // AND / AND / OR over rows
+    uint32_t max_card, min_card, max_runs, max_ewah, rows;
+};
+
+// Cells a tile pair can run given the representations resident on the device
+// (bit per STORM_HIP_CELL_*). reps: bit 0 runs, bit 1 EWAH, bit 2 complement.
+__host__ __device__ inline uint32_t eligible_mask(const TileMeta& a, const TileMeta& b,
+                                                  uint32_t reps) {
+    const bool runs = reps & 1, ewah = reps & 2, comp = reps & 4;
+    const bool bb = a.all_bitmap && b.all_bitmap, b1 = a.all_bitmap || b.all_bitmap;
+    uint32_t e = 0;
+    if (bb) e |= 1u << STORM_HIP_CELL_BB;
+    if (b1) e |= 1u << STORM_HIP_CELL_BS;
+    if (b1 && runs) e |= 1u << STORM_HIP_CELL_BR;
+    if (b1 && ewah) e |= 1u << STORM_HIP_CELL_BW;
+    e |= 1u << STORM_HIP_CELL_SS;
+    if (runs) e |= (1u << STORM_HIP_CELL_SR) | (1u << STORM_HIP_CELL_RR);
+    if (ewah) e |= (1u << STORM_HIP_CELL_SW) | (1u << STORM_HIP_CELL_WW);
+    if (runs && ewah) e |= 1u << STORM_HIP_CELL_RW;
+    if (comp && ((a.all_comp && b.all_bitmap) || (b.all_comp && a.all_bitmap)))
+        e |= 1u << STORM_HIP_CELL_CB;
+    return e;
+}
+
+// Sound tile-level emptiness: every pair of the tile fails the span test.
+__host__ __device__ inline bool tile_empty(const TileMeta& a, const TileMeta& b) {
+    return a.max_card == 0 || b.max_card == 0 || a.last < b.first || b.last < a.first;
+}
+
+// Device cost-model knobs passed by value.
+struct DevModel {
+    double ns_per_unit[10];
+    double ns_fixed, ns_per_occ_word, ns_probe_hot, ns_probe_cold;
+    double probe_scale, br_margin, rr_scale;
+};
+
+// Counters written by the selection and cell kernels.
+struct DevCounters {
+    unsigned long long cell_pairs[kNumCells];
+    unsigned long long cell_tiles[kNumCells];
+    unsigned long long skipped;
+    unsigned long long fallback_tiles;
+    unsigned int qcount[kNumCells];   // tiles queued per cell
+    unsigned int qhead[kNumCells];    // next tile to take, per cell
+};
+
+}  // namespace stormhip

@@ -61,6 +61,7 @@ struct storm_hip_corpus {
     uint32_t n_words = 0;      // logical words per row (ceil(universe/64))
     uint32_t universe = 0;     // bits
     bool dense = false;        // contiguous corpus: every row a bitmap
+    bool has_runs = false, has_ewah = false, has_comp = false;
     // Dense layout: n_rows_pad x ld words.
     uint64_t* d_words = nullptr;
     uint32_t ld = 0;
@@ -72,11 +73,22 @@ struct storm_hip_corpus {
     uint64_t *d_ewah = nullptr, *d_bitmap = nullptr, *d_occ = nullptr;
     uint32_t* d_comp = nullptr;
     uint32_t* d_meta = nullptr;      // 8 per row
-    uint64_t* d_ewah_pos = nullptr;  // per EWAH word: decoded word index of its
-                                     // marker's first word (markers) -- see cells
+    // EWAH marker tables (decoded start word and buffer offset per marker).
+    uint64_t* d_mk_off = nullptr;
+    uint32_t *d_mk_word = nullptr, *d_mk_pos = nullptr;
+    // Bitmap store: row r's bitmap is d_bitmap + d_bm_row[r] * ld, or absent
+    // (kNoBitmap). One extra all-zero row sits at index bm_capacity.
+    uint32_t* d_bm_row = nullptr;
+    uint32_t n_bitmap_rows = 0, bm_capacity = 0;
+    std::vector<uint32_t> h_bm_row;
+    // Density order (stable by cardinality) and the identity order, both
+    // padded with the zero-row sentinel mapping.
+    uint32_t *d_perm = nullptr, *d_ident = nullptr;
+    std::vector<uint32_t> h_perm;
     uint64_t device_bytes = 0;
     std::vector<uint32_t> h_meta;    // host copy of meta (8 per row)
     bool bitmaps_complete = false;
+    void* work = nullptr;            // per-corpus scratch of the general path
     // Result of the last UPPER_TRI call.
     uint32_t* d_tri = nullptr;
     uint64_t tri_count = 0;
@@ -84,3 +96,24 @@ struct storm_hip_corpus {
     // Decisions of the last selecting call.
     std::vector<uint8_t> last_cells;
 };
+
+namespace stormhip {
+int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32_t n_words,
+                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n, const uint2* queue,
+                   const unsigned int* qcount, unsigned int* qhead, uint32_t grid, uint32_t* tri,
+                   unsigned long long* sum, unsigned long long* tile_ticks = nullptr);
+int launch_bb_mfma(hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
+                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n, const uint2* queue,
+                   const unsigned int* qcount, unsigned int* qhead, uint32_t grid, uint32_t* tri,
+                   unsigned long long* sum, unsigned long long* tile_ticks = nullptr);
+int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
+                       const uint32_t* rowmap, const uint32_t* perm, uint32_t n,
+                       const uint2* queue, const unsigned int* qcount, unsigned int* qhead,
+                       uint32_t grid, uint32_t* tri, unsigned long long* sum,
+                       unsigned long long* tile_ticks = nullptr);
+// The B x B form for a dense launch: VALU tile of 128, or MFMA (int8 / fp4).
+// AUTO takes fp4 where its exactness bound holds, else int8.
+int launch_bb_dense(uint32_t bb_kernel, hipStream_t s, const uint64_t* X, uint32_t ld,
+                    uint32_t n_words, const uint32_t* rowmap, uint32_t n, uint32_t* tri,
+                    unsigned long long* sum);
+}  // namespace stormhip

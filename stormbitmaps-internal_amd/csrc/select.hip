@@ -1,0 +1,249 @@
+// Device-side selection: per-tile metadata, the analytic cost model of
+// kernels/storm_cost.cpp ported to the device, eligibility against the
+// representations actually resident, and the per-cell work queues.
+#include "device_common.h"
+#include "general.h"
+
+namespace stormhip {
+
+// tile_meta (kernels/storm_allpairs.cpp:281-312) plus eligibility/spread
+// statistics, one workgroup per row tile of density-sorted rows.
+__global__ __launch_bounds__(256) void tile_meta_kernel(DevCorpus D, uint32_t tile,
+                                                        TileMeta* __restrict__ out) {
+    const uint32_t t = blockIdx.x;
+    const uint32_t lo = t * tile, hi = min(D.n, lo + tile);
+    uint64_t card = 0, runs = 0, nnz = 0;
+    uint32_t first = 0xFFFFFFFFu, last = 0, mxc = 0, mnc = 0xFFFFFFFFu, mxr = 0, mxe = 0, nw = 0;
+    uint32_t rank_and = 1, occ_and = 1, bm_and = 1, comp_and = 1, empty_or = 0;
+    for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
+        const uint32_t r = D.perm[k];
+        const uint32_t* m = D.meta + (uint64_t)r * 8;
+        card += m[0]; runs += m[1]; nnz += m[3];
+        nw = max(nw, m[2]);
+        if (m[0]) { first = min(first, m[4]); last = max(last, m[5]); }
+        mxc = max(mxc, m[0]); mnc = min(mnc, m[0]); mxr = max(mxr, m[1]);
+        mxe = max(mxe, (uint32_t)(D.mk_off[r + 1] - D.mk_off[r]));
+        rank_and &= m[6]; occ_and &= m[7];
+        bm_and &= D.bm_row[r] != kNoBitmap;
+        comp_and &= 2ull * m[0] > D.universe;
+        empty_or |= m[0] == 0;
+    }
+    __shared__ uint64_t s64[3][256];
+    __shared__ uint32_t s32[12][256];
+    const int x = threadIdx.x;
+    s64[0][x] = card; s64[1][x] = runs; s64[2][x] = nnz;
+    s32[0][x] = first; s32[1][x] = last; s32[2][x] = mxc; s32[3][x] = mnc; s32[4][x] = mxr;
+    s32[5][x] = mxe; s32[6][x] = nw; s32[7][x] = rank_and; s32[8][x] = occ_and;
+    s32[9][x] = bm_and; s32[10][x] = comp_and; s32[11][x] = empty_or;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (x < st) {
+            for (int q = 0; q < 3; ++q) s64[q][x] += s64[q][x + st];
+            s32[0][x] = min(s32[0][x], s32[0][x + st]);
+            s32[1][x] = max(s32[1][x], s32[1][x + st]);
+            s32[2][x] = max(s32[2][x], s32[2][x + st]);
+            s32[3][x] = min(s32[3][x], s32[3][x + st]);
+            s32[4][x] = max(s32[4][x], s32[4][x + st]);
+            s32[5][x] = max(s32[5][x], s32[5][x + st]);
+            s32[6][x] = max(s32[6][x], s32[6][x + st]);
+            for (int q = 7; q <= 10; ++q) s32[q][x] &= s32[q][x + st];
+            s32[11][x] |= s32[11][x + st];
+        }
+        __syncthreads();
+    }
+    if (x == 0) {
+        const uint32_t cnt = hi > lo ? hi - lo : 1;
+        TileMeta m;
+        m.card = (uint32_t)(s64[0][0] / cnt);
+        m.runs = (uint32_t)(s64[1][0] / cnt);
+        m.nnz = (uint32_t)(s64[2][0] / cnt);
+        m.nw = s32[6][0];
+        m.first = s32[0][0] == 0xFFFFFFFFu ? 0 : s32[0][0];
+        m.last = s32[1][0];
+        m.has_rank = hi > lo && s32[7][0];
+        m.has_occ = hi > lo && s32[8][0];
+        m.all_bitmap = s32[9][0];
+        m.all_comp = s32[10][0];
+        m.any_empty = s32[11][0];
+        m.max_card = s32[2][0];
+        m.min_card = s32[3][0];
+        m.max_runs = s32[4][0];
+        m.max_ewah = s32[5][0];
+        m.rows = hi - lo;
+        out[t] = m;
+    }
+}
+
+// ---- the analytic model (kernels/storm_cost.cpp:195-465) -----------------
+__device__ inline double bb_expected_bins(const TileMeta& a, const TileMeta& b) {
+    const double bins = fmax(1.0, (double)max(a.nw, b.nw) / 8.0);
+    const double fa = fmin(1.0, (double)a.nnz / (double)max(1u, a.nw));
+    const double fb = fmin(1.0, (double)b.nnz / (double)max(1u, b.nw));
+    const double pa = fmin(1.0, 8.0 * fa), pb = fmin(1.0, 8.0 * fb);
+    return fmax(1.0, bins * pa * pb * 8.0);
+}
+
+__device__ double work_units(int p, const TileMeta& a, const TileMeta& b, double margin) {
+    const double sa = a.card, sb = b.card, ra = a.runs, rb = b.runs;
+    const double wa = 2.0 * ra + 1.0 + (double)a.nnz, wb = 2.0 * rb + 1.0 + (double)b.nnz;
+    const bool a_sparse = sa <= sb;
+    const double s_card = a_sparse ? sa : sb;
+    const double d_runs = a_sparse ? rb : ra;
+    const double d_ewah = a_sparse ? wb : wa;
+    switch (p) {
+        case STORM_HIP_CELL_BB: return bb_expected_bins(a, b);
+        case STORM_HIP_CELL_BS: return s_card;
+        case STORM_HIP_CELL_BR: {
+            const double dr = a_sparse ? rb : ra, sr = a_sparse ? ra : rb;
+            return (margin > 0.0 && dr * margin <= sr) ? dr : sr;
+        }
+        case STORM_HIP_CELL_BW: {
+            const double dr = a_sparse ? rb : ra, sr = a_sparse ? ra : rb;
+            const bool sw = margin > 0.0 && dr * margin <= sr;
+            return sw ? (2.0 * dr + 1.0 + (a_sparse ? b.nnz : a.nnz))
+                      : (2.0 * sr + 1.0 + (a_sparse ? a.nnz : b.nnz));
+        }
+        case STORM_HIP_CELL_SS: {
+            const uint32_t mx = (uint32_t)fmax(2.0, fmax(sa, sb));
+            return fmin(sa, sb) * (double)(32 - __clz(mx));
+        }
+        case STORM_HIP_CELL_SR: return s_card + d_runs;
+        case STORM_HIP_CELL_SW: return s_card + d_ewah;
+        case STORM_HIP_CELL_RR: return ra + rb;
+        case STORM_HIP_CELL_RW: return fmin(ra + wb, rb + wa);
+        case STORM_HIP_CELL_WW: return wa + wb;
+        default: return 0.0;
+    }
+}
+
+__device__ double predict(const DevModel& m, int p, const TileMeta& a, const TileMeta& b) {
+    if (p == STORM_HIP_CELL_EMPTY) return 0.0;
+    if (p == STORM_HIP_CELL_BB && !(a.has_occ && b.has_occ))
+        return m.ns_fixed + m.ns_per_unit[p] * (double)max(a.nw, b.nw);
+    double c = m.ns_fixed + m.ns_per_unit[p] * work_units(p, a, b, m.br_margin) *
+                                (p == STORM_HIP_CELL_RR ? m.rr_scale : 1.0);
+    if (p == STORM_HIP_CELL_BS && m.ns_probe_cold > m.ns_probe_hot) {
+        const double touches = a.card <= b.card ? (double)a.nnz : (double)b.nnz;
+        c += (m.ns_probe_cold - m.ns_probe_hot) * m.probe_scale * touches;
+    }
+    if (p == STORM_HIP_CELL_BB) c += m.ns_per_occ_word * fmax(1.0, (double)max(a.nw, b.nw) / 512.0);
+    return c;
+}
+
+// select_pairing over the reference's candidate set (BB seeded, uncalibrated
+// rates excluded), storm_cost.cpp:422-465. Returns EMPTY on its emptiness proof.
+__device__ int select_pairing(const DevModel& m, const TileMeta& a, const TileMeta& b, double* cost) {
+    if (a.card == 0 || b.card == 0 || a.last < b.first || b.last < a.first) {
+        *cost = 0.0;
+        return STORM_HIP_CELL_EMPTY;
+    }
+    int best = STORM_HIP_CELL_BB;
+    double bc = predict(m, best, a, b);
+    for (int p = 0; p < 10; ++p) {
+        if (m.ns_per_unit[p] <= 0.0) continue;
+        const double c = predict(m, p, a, b);
+        if (c < bc) { bc = c; best = p; }
+    }
+    *cost = bc;
+    return best;
+}
+
+__device__ inline int cheapest_eligible(const DevModel& m, const TileMeta& a, const TileMeta& b,
+                                        uint32_t elig) {
+    int best = -1;
+    double bc = 0.0;
+    for (int p = 0; p < 10; ++p) {
+        if (!(elig >> p & 1)) continue;
+        const double c = predict(m, p, a, b);
+        if (best < 0 || c < bc) { bc = c; best = p; }
+    }
+    return best < 0 ? STORM_HIP_CELL_SS : best;
+}
+
+// One thread per tile pair: decide, then append to that cell's queue.
+// policy: 1 analytic, 2 learned (scores given), 3 fixed.
+__global__ __launch_bounds__(256) void select_kernel(
+    const TileMeta* __restrict__ tm, uint32_t T, uint64_t P, DevModel M, uint32_t reps,
+    uint32_t policy, uint32_t fixed_cell, const float* __restrict__ learned_cost,
+    const uint8_t* __restrict__ learned_ok, uint8_t* __restrict__ decisions,
+    uint2* __restrict__ queue, DevCounters* __restrict__ ctr) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P) return;
+    uint32_t bi, bj;
+    tri_tile((uint32_t)t, T, bi, bj);
+    const TileMeta a = tm[bi], b = tm[bj];
+    const uint32_t elig = eligible_mask(a, b, reps);
+    int p;
+    bool fallback = false;
+    if (tile_empty(a, b)) {
+        p = STORM_HIP_CELL_EMPTY;
+    } else if (policy == STORM_HIP_POLICY_FIXED) {
+        p = (int)fixed_cell;
+        if (p == STORM_HIP_CELL_EMPTY || !(elig >> p & 1)) { p = cheapest_eligible(M, a, b, elig); fallback = true; }
+    } else if (policy == STORM_HIP_POLICY_LEARNED && learned_ok && learned_ok[t]) {
+        int best = -1;
+        float bc = 0.f;
+        for (int q = 0; q < 10; ++q) {
+            if (!(elig >> q & 1)) continue;
+            const float c = learned_cost[t * 10 + q];
+            if (best < 0 || c < bc) { bc = c; best = q; }
+        }
+        p = best < 0 ? STORM_HIP_CELL_SS : best;
+    } else {
+        if (policy == STORM_HIP_POLICY_LEARNED) fallback = true;
+        double c;
+        p = select_pairing(M, a, b, &c);
+        // The reference never skips a whole tile on the mean-based proof
+        // (storm_allpairs.cpp:592): it runs B x B and lets the span test act.
+        if (p == STORM_HIP_CELL_EMPTY) p = STORM_HIP_CELL_BB;
+        if (!(elig >> p & 1)) p = cheapest_eligible(M, a, b, elig);
+    }
+    decisions[t] = (uint8_t)p;
+    const uint32_t slot = atomicAdd(&ctr->qcount[p], 1u);
+    queue[(uint64_t)p * P + slot] = make_uint2(bi, bj);
+    atomicAdd(&ctr->cell_tiles[p], 1ull);
+    if (p == STORM_HIP_CELL_BB) {
+        // B x B tiles skip the per-pair span test (same cost either way);
+        // their pairs are counted here.
+        const uint64_t ni = a.rows, nj = b.rows;
+        atomicAdd(&ctr->cell_pairs[p], (unsigned long long)(bi == bj ? ni * (ni - 1) / 2 : ni * nj));
+    }
+    if (fallback) atomicAdd(&ctr->fallback_tiles, 1ull);
+}
+
+// rowmap[pos] = bitmap row of the row at sorted position pos.
+__global__ void compose_rowmap_kernel(const uint32_t* __restrict__ order,
+                                      const uint32_t* __restrict__ bm_row, uint32_t npad,
+                                      uint32_t* __restrict__ rowmap) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < npad) rowmap[k] = bm_row[order[k]];
+}
+
+// Host-side selection entry: tile metas + select into queues.
+int launch_tile_meta(hipStream_t s, const DevCorpus& D, uint32_t tile, uint32_t T, TileMeta* out) {
+    if (T == 0) return STORM_HIP_OK;
+    hipLaunchKernelGGL(tile_meta_kernel, dim3(T), dim3(256), 0, s, D, tile, out);
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel& M, uint32_t reps,
+                  uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
+                  const uint8_t* learned_ok, uint8_t* decisions, uint2* queue, DevCounters* ctr) {
+    const uint64_t P = (uint64_t)T * (T + 1) / 2;
+    if (P == 0) return STORM_HIP_OK;
+    hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, s, tm, T, P,
+                       M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions, queue, ctr);
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+int launch_compose_rowmap(hipStream_t s, const uint32_t* order, const uint32_t* bm_row,
+                          uint32_t npad, uint32_t* rowmap) {
+    hipLaunchKernelGGL(compose_rowmap_kernel, dim3((npad + 255) / 256), dim3(256), 0, s, order,
+                       bm_row, npad, rowmap);
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+}  // namespace stormhip

@@ -15,7 +15,7 @@ GPU_LIB_PATH = os.path.join(HERE, "libstorm_gpu.so")
 
 POLICY = {"all_bitmap": 0, "analytic": 1, "learned": 2, "fixed": 3, "exhaustive": 4}
 OUT = {"sum": 0, "tri": 1}
-BB = {"auto": 0, "valu": 1, "mfma": 2}
+BB = {"auto": 0, "valu": 1, "mfma": 2, "mfma_fp4": 3}
 CELLS = ["BB", "BS", "BR", "BW", "SS", "SR", "SW", "RR", "RW", "WW", "EMPTY", "CB"]
 NCELL = len(CELLS)
 
@@ -103,6 +103,20 @@ def lib():
         "storm_hip_set_cost_model": (i32, [vp, C.POINTER(CostModel)]),
         "storm_hip_get_cost_model": (i32, [vp, C.POINTER(CostModel)]),
         "storm_hip_last_decisions": (i32, [vp, vp, vp, u64]),
+        "storm_hip_gen_corpus": (i32, [u32, u32, u32, C.c_double, C.c_double, C.c_double, u64,
+                                       C.POINTER(C.POINTER(C.c_uint64)),
+                                       C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(u64)]),
+        "storm_hip_gen_free": (None, [vp]),
+        "storm_hip_mlp_shape": (i32, [C.POINTER(i32)] * 5),
+        "storm_hip_selector_sample": (i32, [vp, vp, u32, u32, u64, u32, vp, vp, vp, vp,
+                                            C.POINTER(u32)]),
+        "storm_hip_selector_train": (i32, [vp, vp, vp, vp, u32, u32, u32, C.c_float, u64,
+                                           C.POINTER(C.c_float)]),
+        "storm_hip_selector_predict": (i32, [vp, vp, u32, vp]),
+        "storm_hip_selector_set_params": (i32, [vp, vp, vp]),
+        "storm_hip_selector_get_params": (i32, [vp, vp, vp]),
+        "storm_hip_selector_grad": (i32, [vp, vp, vp, vp, u32, vp, C.POINTER(C.c_float)]),
+        "storm_hip_selector_adam_step": (i32, [vp, C.c_float]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -176,6 +190,60 @@ class Context:
         check(lib().storm_hip_corpus_from_lists(self.h, _ptr(offs), _ptr(pos), n, universe,
                                                 bitmap_policy, C.byref(h)), "from_lists")
         return Corpus(self, h, n)
+
+    # -- learned selector ------------------------------------------------------
+    def selector_sample(self, corpus, tile=64, max_tiles=512, seed=1, repeats=2):
+        F, O = mlp_shape()[0], 10
+        X = np.zeros((max_tiles, F), np.float32)
+        Y = np.zeros((max_tiles, O), np.float32)
+        M = np.zeros((max_tiles, O), np.uint8)
+        idx = np.zeros(max_tiles, np.uint32)
+        n = C.c_uint32(0)
+        check(lib().storm_hip_selector_sample(self.h, corpus.h, tile, max_tiles, seed, repeats,
+                                              _ptr(X), _ptr(Y), _ptr(M), _ptr(idx), C.byref(n)),
+              "selector_sample")
+        k = n.value
+        return X[:k], Y[:k], M[:k], idx[:k]
+
+    def selector_train(self, X, Y, M, epochs=200, batch=256, lr=3e-3, seed=1):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        M = np.ascontiguousarray(M, np.uint8)
+        loss = C.c_float(0)
+        check(lib().storm_hip_selector_train(self.h, _ptr(X), _ptr(Y), _ptr(M), len(X), epochs,
+                                             batch, lr, seed, C.byref(loss)), "selector_train")
+        return loss.value
+
+    def selector_predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros((len(X), 10), np.float32)
+        check(lib().storm_hip_selector_predict(self.h, _ptr(X), len(X), _ptr(out)), "predict")
+        return out
+
+    def selector_set_params(self, params, norm=None):
+        params = np.ascontiguousarray(params, np.float32)
+        norm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        check(lib().storm_hip_selector_set_params(self.h, _ptr(params), _ptr(norm)), "set_params")
+
+    def selector_get_params(self):
+        F, _, _, _, P = mlp_shape()
+        p = np.zeros(P, np.float32)
+        nm = np.zeros(4 * F, np.float32)
+        check(lib().storm_hip_selector_get_params(self.h, _ptr(p), _ptr(nm)), "get_params")
+        return p, nm
+
+    def selector_grad(self, X, Y, M):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        M = np.ascontiguousarray(M, np.uint8)
+        g = np.zeros(mlp_shape()[4], np.float32)
+        loss = C.c_float(0)
+        check(lib().storm_hip_selector_grad(self.h, _ptr(X), _ptr(Y), _ptr(M), len(X), _ptr(g),
+                                            C.byref(loss)), "selector_grad")
+        return g, loss.value
+
+    def selector_adam_step(self, lr):
+        check(lib().storm_hip_selector_adam_step(self.h, lr), "adam_step")
 
     def set_cost_model(self, rates, ns_fixed=2.0, ns_per_occ_word=0.30, hot=0.0, cold=0.0):
         m = CostModel()
@@ -252,3 +320,30 @@ class Corpus:
 
 def tri_index(i, j, n):
     return i * n - i * (i + 1) // 2 + (j - i - 1)
+
+
+def mlp_shape():
+    v = [C.c_int(0) for _ in range(5)]
+    check(lib().storm_hip_mlp_shape(*[C.byref(x) for x in v]), "mlp_shape")
+    return tuple(x.value for x in v)
+
+
+GEN = {"zipf": 0, "runs": 1, "bernoulli": 2}
+
+
+def gen_corpus(kind, n_rows, universe, p1, p2=0.0, p3=0.0, seed=1):
+    """Seeded synthetic rows (storm_hip_gen_corpus) as numpy CSR copies."""
+    offs_p = C.POINTER(C.c_uint64)()
+    pos_p = C.POINTER(C.c_uint32)()
+    total = C.c_uint64(0)
+    check(lib().storm_hip_gen_corpus(GEN[kind], n_rows, universe, p1, p2, p3, seed,
+                                     C.byref(offs_p), C.byref(pos_p), C.byref(total)),
+          "gen_corpus")
+    try:
+        offs = np.ctypeslib.as_array(offs_p, shape=(n_rows + 1,)).copy()
+        pos = (np.ctypeslib.as_array(pos_p, shape=(total.value,)).copy() if total.value
+               else np.zeros(0, np.uint32))
+    finally:
+        lib().storm_hip_gen_free(C.cast(offs_p, C.c_void_p))
+        lib().storm_hip_gen_free(C.cast(pos_p, C.c_void_p))
+    return offs, pos

@@ -1,0 +1,157 @@
+"""Mixed-representation corpora on the GPU: every cell, every selecting policy,
+every tile width, against the CPU oracle's per-pair counts."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import corpora
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+CELLS = ["BB", "BS", "BR", "BW", "SS", "SR", "SW", "RR", "RW", "WW", "EMPTY", "CB"]
+BUILD_ALL = 4 | 8 | 16
+CORPORA = corpora.all_corpora()
+
+
+def _oracle(rows):
+    offs, pos = corpora.csr(rows)
+    s, tri = O.allpairs_lists(offs, pos)
+    return offs, pos, s, tri
+
+
+@pytest.fixture(scope="module")
+def cases():
+    return [(name, u) + _oracle(rows) for name, rows, u in CORPORA]
+
+
+@pytest.mark.parametrize("bitmaps", [0, 1, 2])
+def test_analytic_policy_every_tile_width(ctx, cases, bitmaps):
+    for name, u, offs, pos, want_sum, want in cases:
+        print("case analytic", name, "bitmaps", bitmaps, flush=True)
+        with ctx.from_lists(offs, pos, u, bitmap_policy=bitmaps | BUILD_ALL) as c:
+            for tile in (64, 128, 256):
+                st = c.allpairs(policy="analytic", out="tri", tile=tile)
+                assert st["sum"] == want_sum, (name, tile)
+                assert np.array_equal(c.tri(), want), (name, tile)
+                assert sum(st["cell_pairs"].values()) == st["pairs"], (name, tile, st)
+
+
+def test_fixed_policy_every_cell(ctx, cases):
+    """Every cell forced on every tile it is eligible for (others fall back)."""
+    for name, u, offs, pos, want_sum, want in cases:
+        print("case fixed", name, flush=True)
+        with ctx.from_lists(offs, pos, u, bitmap_policy=1 | BUILD_ALL) as c:
+            for k, cell in enumerate(CELLS):
+                if cell == "EMPTY":
+                    continue
+                st = c.allpairs(policy="fixed", fixed_cell=k, out="tri", tile=64)
+                assert st["sum"] == want_sum, (name, cell)
+                assert np.array_equal(c.tri(), want), (name, cell)
+                if cell != "CB":
+                    # with every representation resident, only tiles proven
+                    # empty avoid the forced cell
+                    used = set(st["cell_tiles"]) - {"EMPTY"}
+                    assert used <= {cell}, (name, cell, st["cell_tiles"])
+
+
+def test_all_bitmap_policy_materialises(ctx, cases):
+    for name, u, offs, pos, want_sum, want in cases[:12]:
+        print("case allbitmap", name, flush=True)
+        with ctx.from_lists(offs, pos, u, bitmap_policy=2) as c:
+            for bb in ("mfma", "valu"):
+                st = c.allpairs(policy="all_bitmap", out="tri", bb=bb)
+                assert st["sum"] == want_sum
+                assert np.array_equal(c.tri(), want)
+                assert st["skipped"] == 0
+
+
+def test_analytic_decisions_match_oracle_cost_model(ctx, cases):
+    """With every representation resident the device decision is the
+    reference's select_pairing on tile_meta (storm_allpairs.cpp:552-596), except
+    that a tile the span test proves empty is settled without a kernel."""
+    L = O.oracle()
+    model = (C.c_double * 16)()
+    L.orc_default_model(model)
+
+    class M(C.Structure):
+        _fields_ = [("a", C.c_uint32 * 6), ("hr", C.c_uint8), ("ho", C.c_uint8)]
+
+    for name, u, offs, pos, _, _ in cases:
+        rows = [pos[int(offs[i]):int(offs[i + 1])] for i in range(len(offs) - 1)]
+        n = len(rows)
+        metas = (M * n)()
+        for i, r in enumerate(rows):
+            nw = (u + 63) // 64
+            k = len(r)
+            runs = int(np.count_nonzero(np.diff(r.astype(np.int64)) != 1) + 1) if k else 0
+            metas[i].a[0], metas[i].a[1], metas[i].a[2] = k, runs, nw
+            metas[i].a[3] = len(np.unique(r >> 6))
+            metas[i].a[4] = int(r[0]) if k else 0xFFFFFFFF
+            metas[i].a[5] = int(r[-1]) if k else 0
+            metas[i].hr = int(runs and k // runs >= 512)
+            metas[i].ho = int(k * 512 > u)
+        order = np.zeros(n, np.uint32)
+        L.orc_density_order(metas, n, order.ctypes.data_as(C.c_void_p))
+        for tile in (64, 128):
+            with ctx.from_lists(offs, pos, u, bitmap_policy=1 | BUILD_ALL) as c:
+                c.allpairs(policy="analytic", tile=tile)
+                T = (n + tile - 1) // tile
+                got = c.last_decisions(T * (T + 1) // 2)
+            t = 0
+            for bi in range(T):
+                for bj in range(bi, T):
+                    ta, tb = M(), M()
+                    lo_a, hi_a = bi * tile, min(n, bi * tile + tile)
+                    lo_b, hi_b = bj * tile, min(n, bj * tile + tile)
+                    L.orc_tile_meta(metas, order.ctypes.data_as(C.c_void_p), lo_a, hi_a, C.byref(ta))
+                    L.orc_tile_meta(metas, order.ctypes.data_as(C.c_void_p), lo_b, hi_b, C.byref(tb))
+                    want = L.orc_select_pairing(model, C.byref(ta), C.byref(tb), None)
+                    rows_a = [rows[order[k]] for k in range(lo_a, hi_a)]
+                    rows_b = [rows[order[k]] for k in range(lo_b, hi_b)]
+                    ka = [r for r in rows_a if len(r)]
+                    kb = [r for r in rows_b if len(r)]
+                    sound_empty = (not ka or not kb or
+                                   max(int(r[-1]) for r in ka) < min(int(r[0]) for r in kb) or
+                                   max(int(r[-1]) for r in kb) < min(int(r[0]) for r in ka))
+                    if sound_empty:
+                        want = 10
+                    elif want == 10:
+                        want = 0      # the reference runs B x B on such a tile
+                    assert got[t] == want, (name, tile, bi, bj)
+                    t += 1
+
+
+def test_graph_replay_matches_eager(ctx, cases):
+    name, u, offs, pos, want_sum, want = cases[20]
+    with ctx.from_lists(offs, pos, u, bitmap_policy=0 | BUILD_ALL) as c:
+        for rep in range(4):
+            st = c.allpairs(policy="analytic", out="tri", use_graph=True)
+            assert st["sum"] == want_sum, rep
+            assert np.array_equal(c.tri(), want), rep
+            st2 = c.allpairs(policy="fixed", fixed_cell=4, out="tri")
+            assert st2["sum"] == want_sum
+            ctx.sync()
+
+
+def test_tiny_and_empty_corpora(ctx):
+    for rows, u in [([], 64), ([np.array([1], np.uint32)], 64),
+                    ([np.zeros(0, np.uint32)] * 5, 100),
+                    ([np.array([3], np.uint32), np.array([3], np.uint32)], 64)]:
+        offs, pos = corpora.csr(rows)
+        with ctx.from_lists(offs, pos, u, bitmap_policy=0 | BUILD_ALL) as c:
+            want = 1 if len(rows) == 2 else 0
+            for pol in ("analytic", "all_bitmap"):
+                st = c.allpairs(policy=pol, out="tri")
+                assert st["sum"] == want
+                if len(rows) >= 2:
+                    assert int(c.tri().sum()) == want
+
+
+def test_upload_rejects_bad_rows(sh, ctx):
+    offs = np.array([0, 2], np.uint64)
+    with pytest.raises(sh.StormHipError):
+        ctx.from_lists(offs, np.array([5, 5], np.uint32), 64)     # not distinct
+    with pytest.raises(sh.StormHipError):
+        ctx.from_lists(offs, np.array([1, 64], np.uint32), 64)    # out of range
