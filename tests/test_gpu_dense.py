@@ -12,7 +12,7 @@ import oracle_py as O
 
 pytestmark = pytest.mark.gpu
 
-BB_KERNELS = ["valu", "mfma"]
+BB_KERNELS = ["valu", "mfma", "mfma_fp4"]
 
 # (n_rows, n_bits, p, seed): ragged rows (not a multiple of any tile), ragged
 # universes (not a multiple of 64 or of a K stage), empty and tiny corpora.
