@@ -134,6 +134,10 @@ def sparse_bench(sh, args):
                 "cell_tiles": st["cell_tiles"], "skipped": st["skipped"],
                 "fallback_tiles": st["fallback_tiles"], "ms_select": st["ms_select"]}
 
+    res["analytic_reference_rates"] = timed("analytic", use_graph=True)
+    t0 = time.perf_counter()
+    model = ctx.calibrate(corpus, tile=args.tile, max_tiles=args.sample_tiles, seed=7, repeats=2)
+    t_cal = time.perf_counter() - t0
     res["analytic"] = timed("analytic", use_graph=True)
     t0 = time.perf_counter()
     X, Y, M, _ = ctx.selector_sample(corpus, tile=args.tile, max_tiles=args.sample_tiles,
@@ -158,6 +162,7 @@ def sparse_bench(sh, args):
         "learned_over_analytic": res["learned"]["pairs_per_s"] / res["analytic"]["pairs_per_s"],
         "selector": {"sampled_tiles": int(len(X)), "train_loss": loss, "sample_s": t_sample,
                      "train_s": t_train},
+        "calibrated_model": model, "calibrate_s": t_cal,
         "setup_s": {"generate": t_gen, "build_upload": t_up},
     }
     if "all_bitmap" in res:

@@ -228,9 +228,19 @@ typedef struct {
     double ns_fixed;
     double ns_per_occ_word;
     double ns_probe_hot, ns_probe_cold;
+    double bb_zonemap;   /* 1: reference pricing of a zone-mapped B x B; 0: Theta(m) */
 } storm_hip_cost_model;
 int storm_hip_set_cost_model(storm_hip_ctx* ctx, const storm_hip_cost_model* m);
 int storm_hip_get_cost_model(storm_hip_ctx* ctx, storm_hip_cost_model* m);
+/* calibrate (kernels/storm_cost.cpp:556-769) on the device, on this corpus's
+ * own tiles (the reference's calibrate_on_rows idea, :779-895): time every
+ * eligible cell on up to max_tiles sampled tile pairs and set each cell's rate
+ * to the median over tiles of time / (pairs x work_units); the device B x B is
+ * priced per word (bb_zonemap = 0); ns_fixed and the occ/probe terms are 0.
+ * The context's model is replaced; `out` (optional) receives it. */
+int storm_hip_calibrate(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
+                        uint32_t max_tiles, uint64_t seed, uint32_t repeats,
+                        storm_hip_cost_model* out);
 /* Per-tile decisions of the last selecting call: `cells[t]` for tile pair t
  * in row-major upper-triangle tile order. */
 int storm_hip_last_decisions(storm_hip_ctx* ctx, const storm_hip_corpus* c,

@@ -8,19 +8,23 @@
 //        unit scales, f32 accumulation: every partial sum is an integer no
 //        larger than the pair's final count, so the result is exact while
 //        counts stay <= 2^24 (the launcher refuses universes above that).
-//        Twice the K per instruction at the same issue rate.
-// The bits -> elements expansion is done ONCE per workgroup stage into LDS and
-// shared by the four waves.
+//        Twice the K per instruction of I8 at the same issue rate.
 //
-// Same contract as the VALU form (bb_valu.hip): a 256 x 256 block of pairs per
-// workgroup, upper triangle of blocks (or a device queue of blocks), optional
-// rowmap/perm indirection.
+// Structure: a workgroup of 8 waves owns a 256 x 256 block of pairs; wave w
+// owns rows 128*(w/4) .. +128 and columns 64*(w%4) .. +64 of it (4 x 2 MFMA
+// blocks of 32 x 32, 128 accumulator registers, two waves per SIMD). Per stage
+// of 4 words, every thread stages one row's packed bits in LDS (double
+// buffered, one barrier per stage), and every wave expands the packed bits of
+// its own fragments into 0/1 elements in registers right before its MFMAs.
 //
-// Operand mapping: lane l of an A (or B) fragment holds 16 bytes of row
-// (l % 32) of its 32-row block, taken from the byte range [16*(l/32), +16) of
-// that row's expanded k-chunk. A and B use the identical mapping, so each MFMA
-// sums products over the same set of bit positions whatever order the matrix
-// core assigns to them.
+// The expansion is cheap because the element ORDER inside a fragment is free:
+// A and B fragments are built by the same function from the same bit
+// positions, so each MFMA sums products over the same set of positions
+// whichever order the matrix core assigns to them. For a 32-bit chunk w:
+//   FP4: dword d (0..3) = (w >> (d-1)) & 0x22222222   (nibble n = bit 4n+d)
+//   I8 : two K=32 steps h = 0, 1 per chunk; dword d (0..3) of step h =
+//        (w >> (4h+d)) & 0x01010101                   (byte j = bit 8j+4h+d)
+// i.e. one shift and one AND per output dword.
 #include "device_common.h"
 #include "internal.h"
 
@@ -32,134 +36,126 @@ typedef int32_t i32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 namespace {
-constexpr int kTile = 256;       // pairs block: kTile x kTile
-constexpr int kWordsStage = 2;   // packed 64-bit words per LDS stage
-constexpr int kRowBytes = 32;    // expanded bytes per row per k-chunk
-
-// I8: 4 bits -> 4 bytes of 0/1. nibble * 0x00204081 puts bit b at bit 8b
-// with no two partial products overlapping; the mask keeps those bits.
-__device__ inline uint32_t nib_to_bytes(uint32_t x) { return (x * 0x00204081u) & 0x01010101u; }
-
-// I8: one 32-bit chunk -> 32 bytes (byte k = bit k) as 8 dwords.
-__device__ inline void expand_i8(uint32_t d, uint4& lo, uint4& hi) {
-    lo.x = nib_to_bytes(d & 0xF);
-    lo.y = nib_to_bytes((d >> 4) & 0xF);
-    lo.z = nib_to_bytes((d >> 8) & 0xF);
-    lo.w = nib_to_bytes((d >> 12) & 0xF);
-    hi.x = nib_to_bytes((d >> 16) & 0xF);
-    hi.y = nib_to_bytes((d >> 20) & 0xF);
-    hi.z = nib_to_bytes((d >> 24) & 0xF);
-    hi.w = nib_to_bytes(d >> 28);
-}
-
-// FP4: 8 bits -> 8 e2m1 nibbles (bit n -> nibble n = 0b0010 or 0).
-__device__ inline uint32_t byte_to_fp4(uint32_t x) {
-    x = (x | (x << 12)) & 0x000F000Fu;
-    x = (x | (x << 6)) & 0x03030303u;
-    x = (x | (x << 3)) & 0x11111111u;
-    return x << 1;
-}
-
-// FP4: one 64-bit chunk (lo, hi dwords) -> 32 bytes as 8 dwords.
-__device__ inline void expand_fp4(uint32_t lo32, uint32_t hi32, uint4& lo, uint4& hi) {
-    lo.x = byte_to_fp4(lo32 & 0xFF);
-    lo.y = byte_to_fp4((lo32 >> 8) & 0xFF);
-    lo.z = byte_to_fp4((lo32 >> 16) & 0xFF);
-    lo.w = byte_to_fp4(lo32 >> 24);
-    hi.x = byte_to_fp4(hi32 & 0xFF);
-    hi.y = byte_to_fp4((hi32 >> 8) & 0xFF);
-    hi.z = byte_to_fp4((hi32 >> 16) & 0xFF);
-    hi.w = byte_to_fp4(hi32 >> 24);
-}
-
+constexpr int kTile = 256;        // pairs block: kTile x kTile
+constexpr int kThreads = 512;     // 8 waves
+constexpr int kWordsStage = 4;    // packed 64-bit words per stage per row
+constexpr int kRowDw = 12;        // LDS dwords per staged row (8 data + 4 pad)
 constexpr int kUnitScale = 127;   // E8M0 exponent of 1.0
+
+__device__ inline i32x8 expand_fp4(uint32_t w) {
+    return i32x8{(int)((w << 1) & 0x22222222u), (int)(w & 0x22222222u),
+                 (int)((w >> 1) & 0x22222222u), (int)((w >> 2) & 0x22222222u), 0, 0, 0, 0};
+}
+__device__ inline i8x16 expand_i8(uint32_t w, int g) {
+    const uint32_t s = 4u * g;
+    const uint32_t d0 = (w >> s) & 0x01010101u, d1 = (w >> (s + 1)) & 0x01010101u;
+    const uint32_t d2 = (w >> (s + 2)) & 0x01010101u, d3 = (w >> (s + 3)) & 0x01010101u;
+    i8x16 r;
+    uint32_t tmp[4] = {d0, d1, d2, d3};
+    __builtin_memcpy(&r, tmp, 16);
+    return r;
+}
 }  // namespace
 
-template <bool FP4>
 struct BBMfmaSmem {
-    static constexpr int kChunks = FP4 ? kWordsStage : kWordsStage * 2;
-    // Expanded operands of one stage: [A|B][chunk][row][32 bytes].
-    uint8_t lds[2][kChunks][kTile][kRowBytes];
+    // Packed bits of one stage: [buffer][row 0..511 (A rows, then B rows)][dw].
+    // Row layout: dwords 0..3 = low halves of words 0..3, 4..7 = high halves.
+    uint32_t st[2][kThreads][kRowDw];
 };
 
+// One 256 x 256 block over stages [s0, s0 + n_stages) of the rows. With
+// `accumulate`, counts are atomically added to `tri` (a K-split block: other
+// blocks add the other stage ranges of the same pairs into a zeroed triangle).
 template <bool FP4>
-__device__ uint64_t bb_mfma_tile(BBMfmaSmem<FP4>& sm, const uint64_t* __restrict__ X, uint32_t ld,
-                                 uint32_t n_stages, const uint32_t* __restrict__ rowmap,
+__device__ uint64_t bb_mfma_tile(BBMfmaSmem& sm, const uint64_t* __restrict__ X, uint32_t ld,
+                                 uint32_t s0, uint32_t n_stages, const uint32_t* __restrict__ rowmap,
                                  const uint32_t* __restrict__ perm, uint32_t n, uint32_t bi,
-                                 uint32_t bj, uint32_t* __restrict__ tri) {
-    constexpr int kChunks = BBMfmaSmem<FP4>::kChunks;
+                                 uint32_t bj, uint32_t* __restrict__ tri, bool accumulate) {
     using Acc = typename std::conditional<FP4, f32x16, i32x16>::type;
     const uint32_t i0 = bi * kTile, j0 = bj * kTile;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    const int wr = wave >> 1, wc = wave & 1;   // wave's 128 x 128 sub-block
+    const int wr = wave >> 2, wc = wave & 3;   // 128 rows x 64 columns
     const int lr = lane & 31, lg = lane >> 5;
 
-    // Thread t stages row t of the A panel and row t of the B panel.
-    const uint64_t ra = rowmap ? rowmap[i0 + t] : (uint64_t)(i0 + t);
-    const uint64_t rb = rowmap ? rowmap[j0 + t] : (uint64_t)(j0 + t);
-    const uint4* pa = reinterpret_cast<const uint4*>(X + ra * ld);
-    const uint4* pb = reinterpret_cast<const uint4*>(X + rb * ld);
+    // Thread t stages row t of the A panel (t < 256) or row t-256 of B.
+    const uint32_t pos = t < kTile ? i0 + t : j0 + (t - kTile);
+    const uint64_t phys = rowmap ? rowmap[pos] : (uint64_t)pos;
+    const uint4* src = reinterpret_cast<const uint4*>(X + phys * ld) + 2 * (size_t)s0;
 
-    Acc acc[4][4];
+    Acc acc[4][2];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[m][q][v] = 0;
 
+    auto stage_store = [&](int buf, const uint4& a, const uint4& b) {
+        // words: a = (w0.lo, w0.hi, w1.lo, w1.hi), b = (w2.lo, w2.hi, w3.lo, w3.hi)
+        *reinterpret_cast<uint4*>(&sm.st[buf][t][0]) = make_uint4(a.x, a.z, b.x, b.z);
+        *reinterpret_cast<uint4*>(&sm.st[buf][t][4]) = make_uint4(a.y, a.w, b.y, b.w);
+    };
+
     __syncthreads();   // LDS free (previous tile of a persistent workgroup)
-    uint4 ga = pa[0], gb = pb[0];
+    uint4 ga = src[0], gb = src[1];
+    stage_store(0, ga, gb);
+    if (n_stages > 1) { ga = src[2]; gb = src[3]; }
+    __syncthreads();
     for (uint32_t s = 0; s < n_stages; ++s) {
-        __syncthreads();   // previous stage's fragments fully read
-        {
-            const uint32_t da[4] = {ga.x, ga.y, ga.z, ga.w};
-            const uint32_t db[4] = {gb.x, gb.y, gb.z, gb.w};
+        const int buf = s & 1;
+        // This wave's packed fragments of the stage: 4 A rows, 2 B rows, each
+        // 4 chunks of this lane group's 32 bits.
+        uint4 pa[4], pb[2];
 #pragma unroll
-            for (int c = 0; c < kChunks; ++c) {
-                uint4 lo, hi;
-                if constexpr (FP4) expand_fp4(da[2 * c], da[2 * c + 1], lo, hi);
-                else               expand_i8(da[c], lo, hi);
-                *reinterpret_cast<uint4*>(&sm.lds[0][c][t][0]) = lo;
-                *reinterpret_cast<uint4*>(&sm.lds[0][c][t][16]) = hi;
-                if constexpr (FP4) expand_fp4(db[2 * c], db[2 * c + 1], lo, hi);
-                else               expand_i8(db[c], lo, hi);
-                *reinterpret_cast<uint4*>(&sm.lds[1][c][t][0]) = lo;
-                *reinterpret_cast<uint4*>(&sm.lds[1][c][t][16]) = hi;
-            }
+        for (int m = 0; m < 4; ++m)
+            pa[m] = *reinterpret_cast<const uint4*>(&sm.st[buf][wr * 128 + m * 32 + lr][lg * 4]);
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            pb[q] = *reinterpret_cast<const uint4*>(&sm.st[buf][kTile + wc * 64 + q * 32 + lr][lg * 4]);
+        // Stage the next words while this stage computes.
+        if (s + 1 < n_stages) {
+            stage_store(buf ^ 1, ga, gb);
+            if (s + 2 < n_stages) { ga = src[2 * (s + 2)]; gb = src[2 * (s + 2) + 1]; }
         }
-        __syncthreads();
-        if (s + 1 < n_stages) { ga = pa[s + 1]; gb = pb[s + 1]; }
 #pragma unroll
-        for (int c = 0; c < kChunks; ++c) {
-            i32x8 fa[4], fb[4];   // I8 uses the low 4 dwords only
+        for (int c = 0; c < 4; ++c) {
+            uint32_t wa[4], wb[2];
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const uint4 v = *reinterpret_cast<const uint4*>(&sm.lds[0][c][wr * 128 + m * 32 + lr][lg * 16]);
-                fa[m] = i32x8{(int)v.x, (int)v.y, (int)v.z, (int)v.w, 0, 0, 0, 0};
-            }
+            for (int m = 0; m < 4; ++m) wa[m] = c == 0 ? pa[m].x : c == 1 ? pa[m].y : c == 2 ? pa[m].z : pa[m].w;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint4 v = *reinterpret_cast<const uint4*>(&sm.lds[1][c][wc * 128 + q * 32 + lr][lg * 16]);
-                fb[q] = i32x8{(int)v.x, (int)v.y, (int)v.z, (int)v.w, 0, 0, 0, 0};
-            }
+            for (int q = 0; q < 2; ++q) wb[q] = c == 0 ? pb[q].x : c == 1 ? pb[q].y : c == 2 ? pb[q].z : pb[q].w;
+            if constexpr (FP4) {
+                i32x8 fb[2];
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
+                for (int q = 0; q < 2; ++q) fb[q] = expand_fp4(wb[q]);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if constexpr (FP4) {
+                for (int m = 0; m < 4; ++m) {
+                    const i32x8 fa = expand_fp4(wa[m]);
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
                         acc[m][q] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                            fa[m], fb[q], acc[m][q], 4, 4, 0, kUnitScale, 0, kUnitScale);
-                    } else {
-                        const i32x16 cur = acc[m][q];
-                        i8x16 a8, b8;
-                        __builtin_memcpy(&a8, &fa[m], 16);
-                        __builtin_memcpy(&b8, &fb[q], 16);
-                        acc[m][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, b8, cur, 0, 0, 0);
+                            fa, fb[q], acc[m][q], 4, 4, 0, kUnitScale, 0, kUnitScale);
+                }
+            } else {
+                // 32 bits per lane group per chunk = two K=32 steps (g = 0, 1
+                // select which nibble half of each byte position the lane takes).
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    i8x16 fb[2];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) fb[q] = expand_i8(wb[q], h);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        const i8x16 fa = expand_i8(wa[m], h);
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            acc[m][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb[q], acc[m][q], 0, 0, 0);
                     }
                 }
+            }
         }
+        __syncthreads();   // next buffer fully staged; this buffer fully read
     }
 
     // C layout of a 32x32 block: lane l, register v -> row 8*(v/4) + 4*(l/32)
@@ -173,14 +169,16 @@ __device__ uint64_t bb_mfma_tile(BBMfmaSmem<FP4>& sm, const uint64_t* __restrict
             if (i >= n) continue;
             const uint32_t oi = perm ? perm[i] : i;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t j = j0 + wc * 128 + q * 32 + lr;
+            for (int q = 0; q < 2; ++q) {
+                const uint32_t j = j0 + wc * 64 + q * 32 + lr;
                 if (j >= n || j <= i) continue;
                 const uint32_t val = (uint32_t)acc[m][q][v];
                 local += val;
                 if (tri) {
                     const uint32_t oj = perm ? perm[j] : j;
-                    tri[oi < oj ? tri_index(oi, oj, n) : tri_index(oj, oi, n)] = val;
+                    uint32_t* dst = &tri[oi < oj ? tri_index(oi, oj, n) : tri_index(oj, oi, n)];
+                    if (accumulate) { if (val) atomicAdd(dst, val); }
+                    else            *dst = val;
                 }
             }
         }
@@ -188,19 +186,25 @@ __device__ uint64_t bb_mfma_tile(BBMfmaSmem<FP4>& sm, const uint64_t* __restrict
 }
 
 template <bool FP4>
-__global__ __launch_bounds__(256, 1) void bb_mfma_kernel(
-    const uint64_t* __restrict__ X, uint32_t ld, uint32_t n_stages,
+__global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
+    const uint64_t* __restrict__ X, uint32_t ld, uint32_t n_stages, uint32_t k_split,
     const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm, uint32_t n,
     uint32_t T, const uint2* __restrict__ queue, const unsigned int* __restrict__ qcount,
     unsigned int* __restrict__ qhead, uint32_t* __restrict__ tri,
     unsigned long long* __restrict__ sum, unsigned long long* __restrict__ tile_ticks) {
-    __shared__ __attribute__((aligned(16))) BBMfmaSmem<FP4> sm;
+    __shared__ __attribute__((aligned(16))) BBMfmaSmem sm;
     __shared__ uint32_t s_t;
     uint64_t local = 0;
     if (!qcount) {
+        // Grid mode: block b is slice b % k_split of upper-triangle tile b / k_split.
         uint32_t bi, bj;
-        tri_tile(blockIdx.x, T, bi, bj);
-        local = bb_mfma_tile<FP4>(sm, X, ld, n_stages, rowmap, perm, n, bi, bj, tri);
+        tri_tile(blockIdx.x / k_split, T, bi, bj);
+        const uint32_t slice = blockIdx.x % k_split;
+        const uint32_t per = (n_stages + k_split - 1) / k_split;
+        const uint32_t s0 = slice * per;
+        const uint32_t cnt = s0 < n_stages ? min(per, n_stages - s0) : 0;
+        if (cnt)
+            local = bb_mfma_tile<FP4>(sm, X, ld, s0, cnt, rowmap, perm, n, bi, bj, tri, k_split > 1);
     } else {
         for (;;) {
             __syncthreads();
@@ -210,7 +214,7 @@ __global__ __launch_bounds__(256, 1) void bb_mfma_kernel(
             if (t >= *qcount) break;
             const uint2 tp = queue[t];
             const uint64_t t0 = wall_clock64();
-            local += bb_mfma_tile<FP4>(sm, X, ld, n_stages, rowmap, perm, n, tp.x, tp.y, tri);
+            local += bb_mfma_tile<FP4>(sm, X, ld, 0, n_stages, rowmap, perm, n, tp.x, tp.y, tri, false);
             if (tile_ticks) {
                 __syncthreads();
                 if (threadIdx.x == 0) tile_ticks[t] = wall_clock64() - t0;
@@ -221,13 +225,16 @@ __global__ __launch_bounds__(256, 1) void bb_mfma_kernel(
     if ((threadIdx.x & 63) == 0 && local) atomicAdd(sum, (unsigned long long)local);
 }
 
+// k_split > 1 (grid mode only): each tile's stages are split over k_split
+// blocks that atomically add into `tri`, which the caller has zeroed.
 int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
                        const uint32_t* rowmap, const uint32_t* perm, uint32_t n,
                        const uint2* queue, const unsigned int* qcount, unsigned int* qhead,
                        uint32_t grid, uint32_t* tri, unsigned long long* sum,
-                       unsigned long long* tile_ticks) {
+                       unsigned long long* tile_ticks, uint32_t k_split) {
     const uint32_t n_stages = (n_words + kWordsStage - 1) / kWordsStage;
-    if ((uint64_t)(n_stages ? n_stages : 1) * kWordsStage > ld) {
+    // Two stages are read before the loop tests its bound: rows must hold them.
+    if ((uint64_t)std::max(n_stages, 2u) * kWordsStage > ld) {
         set_error("bb_mfma: %u stages exceed row stride %u", n_stages, ld);
         return STORM_HIP_ERR_INVALID;
     }
@@ -237,31 +244,44 @@ int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, 
                   fp4 ? "f32" : "i32");
         return STORM_HIP_ERR_RANGE;
     }
+    if (queue || k_split == 0) k_split = 1;
     const uint32_t T = (n + kTile - 1) / kTile;
-    const uint64_t g = queue ? grid : (uint64_t)T * (T + 1) / 2;
+    const uint64_t g = queue ? grid : (uint64_t)T * (T + 1) / 2 * k_split;
     if (g == 0) return STORM_HIP_OK;
+    const uint32_t ns = n_stages ? n_stages : 1;
     if (fp4)
-        hipLaunchKernelGGL(bb_mfma_kernel<true>, dim3((uint32_t)g), dim3(256), 0, s, X, ld,
-                           n_stages ? n_stages : 1, rowmap, perm, n, T, queue, qcount, qhead, tri,
-                           sum, tile_ticks);
+        hipLaunchKernelGGL(bb_mfma_kernel<true>, dim3((uint32_t)g), dim3(kThreads), 0, s, X, ld, ns,
+                           k_split, rowmap, perm, n, T, queue, qcount, qhead, tri, sum, tile_ticks);
     else
-        hipLaunchKernelGGL(bb_mfma_kernel<false>, dim3((uint32_t)g), dim3(256), 0, s, X, ld,
-                           n_stages ? n_stages : 1, rowmap, perm, n, T, queue, qcount, qhead, tri,
-                           sum, tile_ticks);
+        hipLaunchKernelGGL(bb_mfma_kernel<false>, dim3((uint32_t)g), dim3(kThreads), 0, s, X, ld, ns,
+                           k_split, rowmap, perm, n, T, queue, qcount, qhead, tri, sum, tile_ticks);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }
 
 int launch_bb_dense(uint32_t bb_kernel, hipStream_t s, const uint64_t* X, uint32_t ld,
                     uint32_t n_words, const uint32_t* rowmap, uint32_t n, uint32_t* tri,
-                    unsigned long long* sum) {
+                    unsigned long long* sum, uint32_t cu_count) {
     if (bb_kernel == STORM_HIP_BB_VALU)
         return launch_bb_valu(s, 128, X, ld, n_words, rowmap, nullptr, n, nullptr, nullptr,
                               nullptr, 0, tri, sum);
     const bool fp4 = bb_kernel == STORM_HIP_BB_MFMA_FP4 ||
                      (bb_kernel == STORM_HIP_BB_AUTO && n_words < (1u << 18));
+    // Too few 256 x 256 blocks to fill the chip: split K so that about two
+    // blocks per CU run, each stage range at least 8 stages long.
+    const uint32_t T = (n + kTile - 1) / kTile;
+    const uint64_t tiles = (uint64_t)T * (T + 1) / 2;
+    const uint32_t n_stages = (n_words + kWordsStage - 1) / kWordsStage;
+    uint32_t k_split = 1;
+    if (tiles && tiles < 2ull * cu_count) {
+        k_split = (uint32_t)std::min<uint64_t>((2ull * cu_count + tiles - 1) / tiles,
+                                               std::max(1u, n_stages / 8));
+        k_split = std::max(1u, k_split);
+    }
+    if (k_split > 1 && tri)
+        SH_HIP(hipMemsetAsync(tri, 0, (uint64_t)n * (n - 1) / 2 * 4, s));
     return launch_bb_mfma_fmt(fp4, s, X, ld, n_words, rowmap, nullptr, n, nullptr, nullptr,
-                              nullptr, 0, tri, sum);
+                              nullptr, 0, tri, sum, nullptr, k_split);
 }
 
 int launch_bb_mfma(hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
@@ -269,7 +289,7 @@ int launch_bb_mfma(hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_wor
                    const unsigned int* qcount, unsigned int* qhead, uint32_t grid, uint32_t* tri,
                    unsigned long long* sum, unsigned long long* tile_ticks) {
     return launch_bb_mfma_fmt(false, s, X, ld, n_words, rowmap, perm, n, queue, qcount, qhead,
-                              grid, tri, sum, tile_ticks);
+                              grid, tri, sum, tile_ticks, 1);
 }
 
 }  // namespace stormhip

@@ -68,6 +68,9 @@ struct DevModel {
     double ns_per_unit[10];
     double ns_fixed, ns_per_occ_word, ns_probe_hot, ns_probe_cold;
     double probe_scale, br_margin, rr_scale;
+    // 1: B x B priced as the reference's zone-mapped kernel where both sides
+    // carry a zone map; 0: always Theta(m) words, as the device B x B runs.
+    double bb_zonemap;
 };
 
 // Counters written by the selection and cell kernels.

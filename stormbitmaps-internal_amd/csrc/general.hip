@@ -30,6 +30,7 @@ int general_materialize(storm_hip_ctx* ctx, storm_hip_corpus* c);
 void general_free_arenas(storm_hip_corpus* c);
 int learned_scores(storm_hip_ctx* ctx, const TileMeta* tm, uint32_t T, uint64_t P,
                    uint32_t universe, float* cost, uint8_t* ok, bool* have_model);
+double host_work_units(int p, const TileMeta& a, const TileMeta& b, double margin);
 int launch_features(hipStream_t st, const TileMeta* tm, uint32_t T, uint64_t P, uint32_t universe,
                     float* X);
 
@@ -39,7 +40,7 @@ void default_device_model(DeviceCostModel& m) {
     const double r[10] = {0.116, 0.30, 0.60, 0.55, 0.22, 0.35, 0.40, 0.30, 0.45, 0.50};
     for (int i = 0; i < 10; ++i) m.ns_per_unit[i] = r[i];
     m.ns_fixed = 2.0; m.ns_per_occ_word = 0.30; m.ns_probe_hot = 0; m.ns_probe_cold = 0;
-    m.probe_scale = 0.05; m.br_margin = 16.0; m.rr_scale = 1.0;
+    m.probe_scale = 0.05; m.br_margin = 16.0; m.rr_scale = 1.0; m.bb_zonemap = 1.0;
 }
 
 namespace {
@@ -119,6 +120,7 @@ DevModel dev_model(const DeviceCostModel& m) {
     M.ns_fixed = m.ns_fixed; M.ns_per_occ_word = m.ns_per_occ_word;
     M.ns_probe_hot = m.ns_probe_hot; M.ns_probe_cold = m.ns_probe_cold;
     M.probe_scale = m.probe_scale; M.br_margin = m.br_margin; M.rr_scale = m.rr_scale;
+    M.bb_zonemap = m.bb_zonemap;
     return M;
 }
 
@@ -198,7 +200,7 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
         SH_TRY(launch_compose_rowmap(s, c->d_ident, c->d_bm_row, w->npad, w->rowmap));
         SH_HIP(hipEventRecord(ctx->ev[2], s));
         SH_TRY(launch_bb_dense(o.bb_kernel, s, c->d_bitmap, c->ld, c->n_words, w->rowmap,
-                               c->n_rows, tri, d_sum));
+                               c->n_rows, tri, d_sum, (uint32_t)ctx->cu_count));
         SH_HIP(hipEventRecord(ctx->ev[3], s));
         st->cell_pairs[STORM_HIP_CELL_BB] = st->pairs;
         st->cell_tiles[STORM_HIP_CELL_BB] = w->P;
@@ -217,7 +219,7 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
     SH_TRY(ensure_work(c, tile));
     Work* w = static_cast<Work*>(c->work);
     if (o.use_graph) {
-        const uint32_t key[4] = {o.policy, o.fixed_cell, tile, 1u};
+        const uint32_t key[4] = {o.policy, o.fixed_cell, tile, ctx->model_version};
         if (!w->graph || std::memcmp(key, w->graph_key, sizeof key) != 0 || w->graph_tri != tri) {
             if (w->graph) { (void)hipGraphExecDestroy(w->graph); w->graph = nullptr; }
             hipGraph_t g;
@@ -274,6 +276,8 @@ int storm_hip_set_cost_model(storm_hip_ctx* ctx, const storm_hip_cost_model* m) 
     ctx->model.ns_per_occ_word = m->ns_per_occ_word;
     ctx->model.ns_probe_hot = m->ns_probe_hot;
     ctx->model.ns_probe_cold = m->ns_probe_cold;
+    ctx->model.bb_zonemap = m->bb_zonemap;
+    ++ctx->model_version;
     return STORM_HIP_OK;
 }
 
@@ -284,6 +288,7 @@ int storm_hip_get_cost_model(storm_hip_ctx* ctx, storm_hip_cost_model* m) {
     m->ns_per_occ_word = ctx->model.ns_per_occ_word;
     m->ns_probe_hot = ctx->model.ns_probe_hot;
     m->ns_probe_cold = ctx->model.ns_probe_cold;
+    m->bb_zonemap = ctx->model.bb_zonemap;
     return STORM_HIP_OK;
 }
 
@@ -402,6 +407,46 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             log2_ns[i] = has[i] ? (float)std::log2(std::max(best[i], 10.0)) : 0.f;
         }
     *n_sampled = S;
+    return STORM_HIP_OK;
+}
+
+int storm_hip_calibrate(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
+                        uint32_t max_tiles, uint64_t seed, uint32_t repeats,
+                        storm_hip_cost_model* out) {
+    if (!ctx || !c || c->dense) return STORM_HIP_ERR_INVALID;
+    const int F = STORM_HIP_MLP_FEATURES;
+    std::vector<float> X((size_t)max_tiles * F), Y((size_t)max_tiles * 10);
+    std::vector<uint8_t> Mk((size_t)max_tiles * 10);
+    std::vector<uint32_t> idx(max_tiles);
+    uint32_t n = 0;
+    SH_TRY(storm_hip_selector_sample(ctx, c, tile, max_tiles, seed, repeats, X.data(), Y.data(),
+                                     Mk.data(), idx.data(), &n));
+    Work* w = static_cast<Work*>(c->work);
+    if (!w) return STORM_HIP_ERR_INVALID;
+    std::vector<TileMeta> tm(w->T);
+    SH_HIP(hipMemcpy(tm.data(), w->tm, tm.size() * sizeof(TileMeta), hipMemcpyDeviceToHost));
+    DeviceCostModel m = ctx->model;
+    for (int p = 0; p < 10; ++p) {
+        std::vector<double> r;
+        for (uint32_t k = 0; k < n; ++k) {
+            if (!Mk[(size_t)k * 10 + p]) continue;
+            uint32_t bi, bj;
+            tri_tile(idx[k], w->T, bi, bj);
+            const double na = tm[bi].rows, nb = tm[bj].rows;
+            const double pairs = bi == bj ? na * (na - 1) / 2 : na * nb;
+            const double units = std::max(1.0, host_work_units(p, tm[bi], tm[bj], m.br_margin));
+            if (pairs < 1) continue;
+            r.push_back(std::exp2((double)Y[(size_t)k * 10 + p]) / (pairs * units));
+        }
+        if (r.empty()) { m.ns_per_unit[p] = 0.0; continue; }
+        std::nth_element(r.begin(), r.begin() + r.size() / 2, r.end());
+        m.ns_per_unit[p] = r[r.size() / 2];
+    }
+    m.ns_fixed = 0.0; m.ns_per_occ_word = 0.0; m.ns_probe_hot = 0.0; m.ns_probe_cold = 0.0;
+    m.bb_zonemap = 0.0;
+    ctx->model = m;
+    ++ctx->model_version;
+    if (out) SH_TRY(storm_hip_get_cost_model(ctx, out));
     return STORM_HIP_OK;
 }
 

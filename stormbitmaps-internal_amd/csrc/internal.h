@@ -41,6 +41,7 @@ struct DeviceCostModel {
     double ns_per_unit[10];
     double ns_fixed, ns_per_occ_word, ns_probe_hot, ns_probe_cold;
     double probe_scale, br_margin, rr_scale;
+    double bb_zonemap;
 };
 
 }  // namespace stormhip
@@ -53,6 +54,9 @@ struct storm_hip_ctx {
     int cu_count = 0;
     // MLP selector state lives in mlp.hip (opaque here).
     void* selector = nullptr;
+    // Bumped whenever the cost model or the selector changes: captured
+    // graphs bake both in and are rebuilt when it moves.
+    uint32_t model_version = 1;
 };
 
 struct storm_hip_corpus {
@@ -110,10 +114,11 @@ int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, 
                        const uint32_t* rowmap, const uint32_t* perm, uint32_t n,
                        const uint2* queue, const unsigned int* qcount, unsigned int* qhead,
                        uint32_t grid, uint32_t* tri, unsigned long long* sum,
-                       unsigned long long* tile_ticks = nullptr);
+                       unsigned long long* tile_ticks = nullptr, uint32_t k_split = 1);
 // The B x B form for a dense launch: VALU tile of 128, or MFMA (int8 / fp4).
-// AUTO takes fp4 where its exactness bound holds, else int8.
+// AUTO takes fp4 where its exactness bound holds, else int8. Small corpora
+// split K across blocks so the launch fills `cu_count` CUs.
 int launch_bb_dense(uint32_t bb_kernel, hipStream_t s, const uint64_t* X, uint32_t ld,
                     uint32_t n_words, const uint32_t* rowmap, uint32_t n, uint32_t* tri,
-                    unsigned long long* sum);
+                    unsigned long long* sum, uint32_t cu_count);
 }  // namespace stormhip

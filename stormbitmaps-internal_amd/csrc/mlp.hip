@@ -303,6 +303,7 @@ int storm_hip_selector_set_params(storm_hip_ctx* ctx, const float* params, const
         SH_HIP(hipMemcpy(s->norm, norm, 4 * kF * 4, hipMemcpyHostToDevice));
         s->trained = true;
     }
+    ++ctx->model_version;
     return STORM_HIP_OK;
 }
 
@@ -454,6 +455,7 @@ int storm_hip_selector_train(storm_hip_ctx* ctx, const float* x, const float* t,
     SH_HIP(hipStreamSynchronize(st));
     if (final_loss) *final_loss = l;
     s->trained = true;
+    ++ctx->model_version;
     return STORM_HIP_OK;
 }
 

@@ -75,7 +75,13 @@ __global__ __launch_bounds__(256) void tile_meta_kernel(DevCorpus D, uint32_t ti
 }
 
 // ---- the analytic model (kernels/storm_cost.cpp:195-465) -----------------
-__device__ inline double bb_expected_bins(const TileMeta& a, const TileMeta& b) {
+__host__ __device__ inline uint32_t bit_length(uint32_t x) {   // 32 - clz(x)
+    uint32_t b = 0;
+    while (x) { ++b; x >>= 1; }
+    return b;
+}
+
+__host__ __device__ inline double bb_expected_bins(const TileMeta& a, const TileMeta& b) {
     const double bins = fmax(1.0, (double)max(a.nw, b.nw) / 8.0);
     const double fa = fmin(1.0, (double)a.nnz / (double)max(1u, a.nw));
     const double fb = fmin(1.0, (double)b.nnz / (double)max(1u, b.nw));
@@ -83,7 +89,7 @@ __device__ inline double bb_expected_bins(const TileMeta& a, const TileMeta& b) 
     return fmax(1.0, bins * pa * pb * 8.0);
 }
 
-__device__ double work_units(int p, const TileMeta& a, const TileMeta& b, double margin) {
+__host__ __device__ double work_units(int p, const TileMeta& a, const TileMeta& b, double margin) {
     const double sa = a.card, sb = b.card, ra = a.runs, rb = b.runs;
     const double wa = 2.0 * ra + 1.0 + (double)a.nnz, wb = 2.0 * rb + 1.0 + (double)b.nnz;
     const bool a_sparse = sa <= sb;
@@ -105,7 +111,7 @@ __device__ double work_units(int p, const TileMeta& a, const TileMeta& b, double
         }
         case STORM_HIP_CELL_SS: {
             const uint32_t mx = (uint32_t)fmax(2.0, fmax(sa, sb));
-            return fmin(sa, sb) * (double)(32 - __clz(mx));
+            return fmin(sa, sb) * (double)bit_length(mx);
         }
         case STORM_HIP_CELL_SR: return s_card + d_runs;
         case STORM_HIP_CELL_SW: return s_card + d_ewah;
@@ -118,7 +124,7 @@ __device__ double work_units(int p, const TileMeta& a, const TileMeta& b, double
 
 __device__ double predict(const DevModel& m, int p, const TileMeta& a, const TileMeta& b) {
     if (p == STORM_HIP_CELL_EMPTY) return 0.0;
-    if (p == STORM_HIP_CELL_BB && !(a.has_occ && b.has_occ))
+    if (p == STORM_HIP_CELL_BB && (m.bb_zonemap == 0.0 || !(a.has_occ && b.has_occ)))
         return m.ns_fixed + m.ns_per_unit[p] * (double)max(a.nw, b.nw);
     double c = m.ns_fixed + m.ns_per_unit[p] * work_units(p, a, b, m.br_margin) *
                                 (p == STORM_HIP_CELL_RR ? m.rr_scale : 1.0);
@@ -209,6 +215,12 @@ __global__ __launch_bounds__(256) void select_kernel(
         atomicAdd(&ctr->cell_pairs[p], (unsigned long long)(bi == bj ? ni * (ni - 1) / 2 : ni * nj));
     }
     if (fallback) atomicAdd(&ctr->fallback_tiles, 1ull);
+}
+
+// Host view of the work function, for calibration.
+double host_work_units(int p, const TileMeta& a, const TileMeta& b, double margin) {
+    if (p == STORM_HIP_CELL_BB) return (double)std::max(a.nw, b.nw);   // device B x B is Theta(m)
+    return work_units(p, a, b, margin);
 }
 
 // rowmap[pos] = bitmap row of the row at sorted position pos.

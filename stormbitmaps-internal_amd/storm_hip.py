@@ -63,7 +63,12 @@ class Arena(C.Structure):
 class CostModel(C.Structure):
     _fields_ = [("ns_per_unit", C.c_double * 10), ("ns_fixed", C.c_double),
                 ("ns_per_occ_word", C.c_double), ("ns_probe_hot", C.c_double),
-                ("ns_probe_cold", C.c_double)]
+                ("ns_probe_cold", C.c_double), ("bb_zonemap", C.c_double)]
+
+    def as_dict(self):
+        return {"ns_per_unit": {CELLS[i]: self.ns_per_unit[i] for i in range(10)},
+                "ns_fixed": self.ns_fixed, "ns_per_occ_word": self.ns_per_occ_word,
+                "bb_zonemap": self.bb_zonemap}
 
 
 _lib = None
@@ -117,6 +122,7 @@ def lib():
         "storm_hip_selector_get_params": (i32, [vp, vp, vp]),
         "storm_hip_selector_grad": (i32, [vp, vp, vp, vp, u32, vp, C.POINTER(C.c_float)]),
         "storm_hip_selector_adam_step": (i32, [vp, C.c_float]),
+        "storm_hip_calibrate": (i32, [vp, vp, u32, u32, u64, u32, C.POINTER(CostModel)]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -245,11 +251,19 @@ class Context:
     def selector_adam_step(self, lr):
         check(lib().storm_hip_selector_adam_step(self.h, lr), "adam_step")
 
-    def set_cost_model(self, rates, ns_fixed=2.0, ns_per_occ_word=0.30, hot=0.0, cold=0.0):
+    def calibrate(self, corpus, tile=64, max_tiles=512, seed=1, repeats=2):
+        m = CostModel()
+        check(lib().storm_hip_calibrate(self.h, corpus.h, tile, max_tiles, seed, repeats,
+                                        C.byref(m)), "calibrate")
+        return m.as_dict()
+
+    def set_cost_model(self, rates, ns_fixed=2.0, ns_per_occ_word=0.30, hot=0.0, cold=0.0,
+                       bb_zonemap=1.0):
         m = CostModel()
         for i, r in enumerate(rates):
             m.ns_per_unit[i] = r
         m.ns_fixed, m.ns_per_occ_word, m.ns_probe_hot, m.ns_probe_cold = ns_fixed, ns_per_occ_word, hot, cold
+        m.bb_zonemap = bb_zonemap
         check(lib().storm_hip_set_cost_model(self.h, C.byref(m)), "set_cost_model")
 
     def get_cost_model(self):
