@@ -263,6 +263,153 @@ __device__ inline bool span_disjoint(const DevCorpus& D, uint32_t i, uint32_t j)
     return ci == 0 || cj == 0 || meta(D, i, 5) < meta(D, j, 4) || meta(D, j, 5) < meta(D, i, 4);
 }
 
+// ---- S x S at tile granularity: a hash join in LDS -------------------------
+//
+// For a block of up to 64 x 64 pairs of list rows, every pair's count is the
+// number of positions the two rows share. Instead of intersecting 4,096 pairs
+// separately, the I rows' positions go into an LDS hash table keyed by
+// position whose value is a 64-bit mask of the I rows holding it; each J row's
+// positions are then looked up once, and a hit adds 1 to count[i][j] for every
+// i in the mask. Work is |I elements| + |J elements| + matches instead of
+// sum over pairs. Positions are processed in universe chunks small enough that
+// a chunk's I elements fill at most half the table.
+__device__ inline uint32_t bit_len(uint32_t x) { return 32 - __clz(x); }
+
+constexpr int kJoinBits = 12, kJoinSize = 1 << kJoinBits;
+constexpr uint32_t kJoinEmpty = 0xFFFFFFFFu;
+
+struct JoinSmem {
+    uint32_t key[kJoinSize];
+    uint32_t mlo[kJoinSize], mhi[kJoinSize];
+    uint32_t cnt[64][65];
+    uint32_t slot[kJoinSize / 2];   // table slot of each inserted element (for clearing)
+    uint32_t start[2][64];     // first element of each row inside the chunk
+    uint32_t pre[2][65];       // exclusive prefix of the rows' chunk counts
+    uint32_t total_i;
+};
+
+__device__ inline uint32_t join_hash(uint32_t x) { return (x * 2654435761u) >> (32 - kJoinBits); }
+
+// Row index of flattened element f: largest r with pre[r] <= f.
+__device__ inline uint32_t join_row(const uint32_t* pre, uint32_t nrows, uint32_t f) {
+    uint32_t lo = 0, hi = nrows;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= f) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint32_t ni,
+                             uint32_t ja, uint32_t nj, bool diag, uint32_t* __restrict__ tri,
+                             uint64_t& local, uint64_t& pairs, uint64_t& skipped) {
+    const uint32_t tid = threadIdx.x, n = D.n;
+    for (uint32_t k = tid; k < 64 * 65; k += blockDim.x) (&S.cnt[0][0])[k] = 0;
+    for (uint32_t k = tid; k < kJoinSize; k += blockDim.x) {
+        S.key[k] = kJoinEmpty; S.mlo[k] = 0; S.mhi[k] = 0;
+    }
+    if (tid == 0) S.total_i = 0;
+    __syncthreads();
+    if (tid < ni) atomicAdd(&S.total_i, (uint32_t)(D.list_off[D.perm[ia + tid] + 1] - D.list_off[D.perm[ia + tid]]));
+    __syncthreads();
+    const uint64_t U = D.universe;
+    const uint64_t target = kJoinSize / 4;
+    uint64_t width = S.total_i <= target ? U : max<uint64_t>(1, U * target / S.total_i);
+    uint64_t lo = 0;
+    while (lo < U) {
+        const uint64_t hi = min(U, lo + width);
+        // Chunk spans of every row: threads 0..63 take I rows, 64..127 J rows.
+        if (tid < 128) {
+            const uint32_t side = tid >> 6, r = tid & 63;
+            const uint32_t nr = side ? nj : ni;
+            uint32_t st = 0, c = 0;
+            if (r < nr) {
+                const uint32_t row = D.perm[(side ? ja : ia) + r];
+                const Span32 L = list_of(D, row);
+                st = lower_bound(L.p, L.n, (uint32_t)lo);
+                const uint32_t en = hi >= U ? L.n : lower_bound(L.p, L.n, (uint32_t)hi);
+                c = en - st;
+            }
+            S.start[side][r] = st;
+            S.pre[side][r + 1] = c;   // counts first; prefix below
+        }
+        __syncthreads();
+        if (tid < 2) {
+            uint32_t acc = 0;
+            S.pre[tid][0] = 0;
+            for (int r = 1; r <= 64; ++r) { acc += S.pre[tid][r]; S.pre[tid][r] = acc; }
+        }
+        __syncthreads();
+        const uint32_t ei = S.pre[0][ni], ej = S.pre[1][nj];
+        if (ei > kJoinSize / 2 && hi - lo > 1) {   // too full: halve the chunk
+            width = (hi - lo) / 2;
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t f = tid; f < ei; f += blockDim.x) {
+            const uint32_t r = join_row(S.pre[0], ni, f);
+            const Span32 L = list_of(D, D.perm[ia + r]);
+            const uint32_t x = L.p[S.start[0][r] + (f - S.pre[0][r])];
+            uint32_t h = join_hash(x);
+            for (;;) {
+                const uint32_t old = atomicCAS(&S.key[h], kJoinEmpty, x);
+                if (old == kJoinEmpty || old == x) break;
+                h = (h + 1) & (kJoinSize - 1);
+            }
+            if (r < 32) atomicOr(&S.mlo[h], 1u << r);
+            else        atomicOr(&S.mhi[h], 1u << (r - 32));
+            S.slot[f] = h;
+        }
+        __syncthreads();
+        for (uint32_t f = tid; f < ej; f += blockDim.x) {
+            const uint32_t r = join_row(S.pre[1], nj, f);
+            const Span32 L = list_of(D, D.perm[ja + r]);
+            const uint32_t x = L.p[S.start[1][r] + (f - S.pre[1][r])];
+            uint32_t h = join_hash(x);
+            for (;;) {
+                const uint32_t k = S.key[h];
+                if (k == kJoinEmpty) break;
+                if (k == x) {
+                    uint32_t m = S.mlo[h];
+                    while (m) { const int b = __ffs(m) - 1; m &= m - 1; atomicAdd(&S.cnt[b][r], 1u); }
+                    m = S.mhi[h];
+                    while (m) { const int b = __ffs(m) - 1; m &= m - 1; atomicAdd(&S.cnt[32 + b][r], 1u); }
+                    break;
+                }
+                h = (h + 1) & (kJoinSize - 1);
+            }
+        }
+        __syncthreads();
+        // Return the used slots to empty for the next chunk.
+        for (uint32_t f = tid; f < ei; f += blockDim.x) {
+            const uint32_t h = S.slot[f];
+            S.key[h] = kJoinEmpty; S.mlo[h] = 0; S.mhi[h] = 0;
+        }
+        __syncthreads();
+        lo = hi;
+        if (ei < kJoinSize / 8) width = min<uint64_t>(U, width * 2);
+    }
+    for (uint32_t p = tid; p < ni * nj; p += blockDim.x) {
+        const uint32_t a = p / nj, b = p % nj;
+        if (diag && b <= a) continue;
+        const uint32_t i = D.perm[ia + a], j = D.perm[ja + b];
+        const uint32_t v = S.cnt[a][b];
+        if (span_disjoint(D, i, j)) ++skipped; else ++pairs;
+        local += v;
+        if (tri) tri[i < j ? tri_index(i, j, n) : tri_index(j, i, n)] = v;
+    }
+    __syncthreads();
+}
+
+// The join wins over per-pair search unless the J rows dwarf the I rows.
+__device__ inline bool ss_use_join(const TileMeta& A, const TileMeta& B) {
+    const double ci = A.card, cj = B.card;
+    const double rows = (double)A.rows * B.rows;
+    const double per_pair = rows * fmin(ci, cj) * (double)bit_len(max(A.card, B.card) + 2);
+    const double join = 3.0 * (A.rows * ci + B.rows * cj) + A.rows * ci / (kJoinSize / 4.0) * 2200.0;
+    return join < per_pair;
+}
+
 }  // namespace
 
 template <int CELL>
@@ -292,7 +439,19 @@ __global__ __launch_bounds__(256) void cell_kernel(DevCorpus D, DevModel M,
         // Lane mode when the sparser tile's rows are tiny.
         const TileMeta& A = tm[tp.x];
         const bool lane_mode = CELL == STORM_HIP_CELL_EMPTY || (A.max_card <= 48 && A.max_runs <= 48);
-        if (lane_mode) {
+        bool joined = false;
+        if constexpr (CELL == STORM_HIP_CELL_SS) {
+            __shared__ JoinSmem js;
+            if (ss_use_join(A, tm[tp.y])) {
+                joined = true;
+                for (uint32_t si = 0; si < ni; si += 64)
+                    for (uint32_t sj = diag ? si : 0; sj < nj; sj += 64)
+                        ss_tile_join(D, js, i0 + si, min(64u, ni - si), j0 + sj, min(64u, nj - sj),
+                                     diag && si == sj, tri, local, pairs, skipped);
+            }
+        }
+        if (joined) {
+        } else if (lane_mode) {
             for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
                 const uint32_t a = i0 + p / nj, b = j0 + p % nj;
                 if (diag && b <= a) continue;
