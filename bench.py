@@ -1,21 +1,27 @@
 #!/usr/bin/env python3
 """Exact all-pairs intersection cardinality on one MI355X.
 
-Workload (the headline configuration that fits one GPU): dense contiguous
-corpus, N = 32,768 rows x M = 1,048,576 bits (4 GiB packed u64), every bit
-i.i.d. Bernoulli(0.05) from the splitmix64 stream Rng(2), generated on the
-device; one step = every pair i < j, exact |Xi n Xj| written as a packed
+Default workload (the headline configuration that fits one GPU): dense
+contiguous corpus, N = 32,768 rows x M = 1,048,576 bits (4 GiB packed u64),
+every bit i.i.d. Bernoulli(0.05) from the splitmix64 stream Rng(2), generated
+on the device; one step = every pair i < j, exact |Xi n Xj| written as a packed
 upper-triangle u32 (536,854,528 counts, 2 GiB), inputs resident in HBM.
 
-Prints ONE JSON line. `value` = exact pairs/s of the whole step.
-`roofline` prices the dominant kernel (B x B) from HIP events recorded on the
-engine's own stream around its launch. `cpu_baseline` times the reference's
-own all-bitmap B x B kernel (compiled from its sources, oracle/_ref) on the
-host cores over a bounded sample of the same rows.
+Prints ONE JSON line on stdout (progress lines go to stderr). `value` = exact
+pairs/s of the whole step. `roofline` prices the dominant kernel (B x B) from
+HIP events recorded on the engine's own stream around its launch.
+`cpu_baseline` times the reference's own all-bitmap B x B kernel (compiled
+from its sources, oracle/_ref) on the host cores over a bounded sample of the
+same rows.
+
+Other workloads (--config): dense1 (N=2,048 x M=262,144, p=0.01), zipf3 and
+runs4 (mixed representations: analytic vs calibrated vs learned selection vs
+all-bitmap), batch5 (many independent corpora in one set of launches).
 """
 import argparse
 import importlib.util
 import json
+import math
 import os
 import sys
 import time
@@ -24,6 +30,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "stormbitmaps-internal_amd")
+METRIC = "exact pairs/sec on 1 MI355X, learned vs analytic selection; % of roofline"
 
 CONFIGS = {
     # name: (n_rows, n_bits, p, seed, description)
@@ -43,9 +50,19 @@ SPARSE_CONFIGS = {
               "run-heavy rows: N=16,384 over U=36,974,578, Poisson(20) runs of geometric "
               "length (mean 2,000) + 5% singletons, seed 4"),
 }
+BATCH_CONFIGS = {
+    "batch5": (512, 5, "batched uneven instances: 512 corpora, N ~ U[64, 4096], universe "
+                       "log-uniform [2e5, 1.3e8], density log-uniform [1e-6, 0.17], even "
+                       "instances i.i.d. rows, odd instances clustered-run rows, seed 5"),
+}
 MFMA_I8_PEAK_TOPS = 5000.0   # MI355X dense int8, TOP/s
 MFMA_FP4_PEAK_TOPS = 10100.0  # MI355X dense fp4, TFLOP/s
 HBM_PEAK_GBS = 8000.0
+T0 = time.perf_counter()
+
+
+def progress(*msg):
+    print(f"[bench {time.perf_counter() - T0:7.1f}s]", *msg, file=sys.stderr, flush=True)
 
 
 def load_storm_hip():
@@ -88,6 +105,7 @@ def cpu_baseline(sh, corpus, n_bits, target_s=12.0):
     _, t = run(r)
     rate = (r * (r - 1) / 2) / max(t, 1e-6)
     r = int(min(8192, max(256, (2 * rate * target_s) ** 0.5)))
+    progress(f"cpu baseline: {r} rows")
     s, t = run(r)
     pairs = r * (r - 1) // 2
     # The GPU's counts for the same pairs must agree.
@@ -102,6 +120,13 @@ def cpu_baseline(sh, corpus, n_bits, target_s=12.0):
             "sample_sum": int(s), "sample_sum_matches_gpu": bool(gpu == s)}
 
 
+def column_pair_sum(offs, pos, universe):
+    """Sum over pairs of |Xi n Xj| = sum over elements b of C(c_b, 2), c_b =
+    number of rows holding b: an exact check independent of every kernel."""
+    c = np.bincount(pos, minlength=universe).astype(np.uint64)
+    return int((c * (c - np.uint64(1)) // np.uint64(2)).sum(dtype=np.uint64)) if len(c) else 0
+
+
 def sparse_bench(sh, args):
     """Mixed-representation workloads: all-bitmap vs analytic vs learned
     per-tile selection, every policy's sum checked equal."""
@@ -112,23 +137,28 @@ def sparse_bench(sh, args):
     t0 = time.perf_counter()
     offs, pos = sh.gen_corpus(kind, n, u, p1, p2, p3, seed)
     t_gen = time.perf_counter() - t0
+    progress(f"generated {n} rows, {len(pos)} elements in {t_gen:.1f} s")
+    want = column_pair_sum(offs, pos, u)
     ctx = sh.Context(0)
     flags = 0 | 4 | 8   # bitmaps for dense rows, runs, EWAH
     t0 = time.perf_counter()
     corpus = ctx.from_lists(offs, pos, u, bitmap_policy=flags)
     t_up = time.perf_counter() - t0
+    progress(f"built and uploaded in {t_up:.1f} s")
     del pos
     pairs = n * (n - 1) // 2
     res = {}
 
     def timed(policy, **kw):
-        corpus.allpairs(policy=policy, out="tri", tile=args.tile, **kw)   # warm-up
+        progress(f"policy {policy}: warm-up")
+        corpus.allpairs(policy=policy, out="tri", tile=args.tile, **kw)
         ctx.sync()
         ts, st = [], None
-        for _ in range(args.steps):
+        for k in range(args.steps):
             t1 = time.perf_counter()
             st = corpus.allpairs(policy=policy, out="tri", tile=args.tile, **kw)
             ts.append(time.perf_counter() - t1)
+            progress(f"policy {policy}: step {k} {1000 * ts[-1]:.1f} ms")
         ms = 1000 * float(np.median(ts))
         return {"pairs_per_s": pairs / (ms / 1000), "ms": ms, "sum": st["sum"],
                 "cell_tiles": st["cell_tiles"], "skipped": st["skipped"],
@@ -136,14 +166,17 @@ def sparse_bench(sh, args):
 
     res["analytic_reference_rates"] = timed("analytic", use_graph=True)
     t0 = time.perf_counter()
+    progress("calibrating")
     model = ctx.calibrate(corpus, tile=args.tile, max_tiles=args.sample_tiles, seed=7, repeats=2)
     t_cal = time.perf_counter() - t0
     res["analytic"] = timed("analytic", use_graph=True)
     t0 = time.perf_counter()
+    progress("sampling tiles for the selector")
     X, Y, M, _ = ctx.selector_sample(corpus, tile=args.tile, max_tiles=args.sample_tiles,
                                      seed=11, repeats=2)
     t_sample = time.perf_counter() - t0
     t0 = time.perf_counter()
+    progress(f"training on {len(X)} tiles")
     loss = ctx.selector_train(X, Y, M, epochs=300, batch=128, lr=3e-3, seed=5)
     t_train = time.perf_counter() - t0
     res["learned"] = timed("learned", use_graph=True)
@@ -151,13 +184,14 @@ def sparse_bench(sh, args):
         res["all_bitmap"] = timed("all_bitmap")
     sums = {v["sum"] for v in res.values()}
     out = {
-        "metric": "exact pairs/sec on 1 MI355X, learned vs analytic selection",
+        "metric": METRIC,
         "value": res["learned"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": 1, "ms_per_step": res["learned"]["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic (seeded host generator)",
         "config": {"workload": desc, "n_rows": n, "universe": u, "pairs": pairs, "tile": args.tile},
-        "exact_sums_agree": len(sums) == 1, "sum": sums.pop() if len(sums) == 1 else None,
+        "exact_sums_agree": len(sums) == 1 and want in sums,
+        "sum": sums.pop() if len(sums) == 1 else None, "column_closed_form_sum": want,
         "policies": res,
         "learned_over_analytic": res["learned"]["pairs_per_s"] / res["analytic"]["pairs_per_s"],
         "selector": {"sampled_tiles": int(len(X)), "train_loss": loss, "sample_s": t_sample,
@@ -172,49 +206,174 @@ def sparse_bench(sh, args):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="dense2", choices=sorted(CONFIGS) + sorted(SPARSE_CONFIGS))
-    ap.add_argument("--bb", default="mfma", choices=["mfma", "mfma_fp4", "valu"])
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--rows", type=int, default=0, help="sparse configs: first N rows only")
-    ap.add_argument("--tile", type=int, default=64)
-    ap.add_argument("--sample-tiles", type=int, default=2048)
-    ap.add_argument("--no-allbitmap", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the all-pairs triangle would shard by tile rows, see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+class _Rng:
+    """splitmix64 counter stream (the project's generator convention)."""
+    G = 0x9E3779B97F4A7C15
+    M = (1 << 64) - 1
 
-    sh = load_storm_hip()
-    if args.config in SPARSE_CONFIGS:
-        sparse_bench(sh, args)
-        return
+    def __init__(self, seed):
+        self.s = seed & self.M
+
+    def next(self):
+        self.s = (self.s + self.G) & self.M
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.M
+        return z ^ (z >> 31)
+
+    def unit(self):
+        return (self.next() >> 11) * 2.0 ** -53
+
+
+def batch_instances(count, seed):
+    """Instance k: (n_rows, universe, density, kind, seed_k)."""
+    g = _Rng(seed)
+    out = []
+    for k in range(count):
+        n = 64 + int(g.unit() * 4033)
+        u = int(math.exp(math.log(2e5) + g.unit() * (math.log(1.3e8) - math.log(2e5))))
+        d = math.exp(math.log(1e-6) + g.unit() * (math.log(0.17) - math.log(1e-6)))
+        out.append((n, u, d, "bernoulli" if k % 2 == 0 else "runs", g.next() & 0xFFFFFFFF))
+    return out
+
+
+def batch_bench(sh, args):
+    """Many independent corpora through one set of launches per policy.
+    The full workload does not fit one GPU (about 1.2e12 list elements), so a
+    bounded sample runs: the instances, in seeded order, whose estimated
+    device size is at most --instance-mib, up to --budget-gib in total."""
+    count, seed, desc = BATCH_CONFIGS[args.config]
+    inst = batch_instances(count, seed)
+    cap, budget = args.instance_mib << 20, args.budget_gib << 30
+    chosen, used, skipped = [], 0, 0
+    for k, (n, u, d, kind, sk) in enumerate(inst):
+        elems = n * u * d
+        est = 4 * elems * 2 + (n * u / 8 if d >= 1 / 64 else 0)    # lists+runs, bitmaps
+        if est > cap or used + est > budget:
+            skipped += 1
+            continue
+        chosen.append(k)
+        used += est
+    progress(f"{len(chosen)} of {count} instances chosen (~{used / 2**30:.1f} GiB)")
+    ctx = sh.Context(0)
+    corpora, wants, t_gen = [], [], 0.0
+    for k in chosen:
+        n, u, d, kind, sk = inst[k]
+        t0 = time.perf_counter()
+        if kind == "bernoulli":
+            offs, pos = sh.gen_corpus("bernoulli", n, u, d, 0.0, 0.0, sk)
+        else:
+            # clustered runs: Poisson(r) runs of geometric length L with r*L = d*U
+            L = max(1.0, min(2000.0, d * u / 20.0))
+            offs, pos = sh.gen_corpus("runs", n, u, max(d * u / L, 1e-3), L, 0.05, sk)
+        t_gen += time.perf_counter() - t0
+        wants.append(column_pair_sum(offs, pos, u))
+        corpora.append(ctx.from_lists(offs, pos, u, bitmap_policy=0 | 4 | 8))
+        del offs, pos
+        if len(corpora) % 32 == 0:
+            progress(f"built {len(corpora)} of {len(chosen)} corpora")
+    pairs = sum(c.n_pairs for c in corpora)
+    progress(f"built {len(corpora)} corpora, {pairs} pairs")
+    res = {}
+    batch = ctx.batch(corpora)
+
+    def timed(policy):
+        batch.allpairs(policy=policy, tile=args.tile, use_graph=True)
+        ctx.sync()
+        ts, lats, st, sums = [], [], None, None
+        for k in range(args.steps):
+            t1 = time.perf_counter()
+            st, sums, lat = batch.allpairs(policy=policy, tile=args.tile, use_graph=True)
+            ts.append(time.perf_counter() - t1)
+            lats.append(lat)
+            progress(f"batch {policy}: step {k} {1000 * ts[-1]:.1f} ms")
+        ms = 1000 * float(np.median(ts))
+        lat = np.median(np.stack(lats), axis=0)
+        return {"pairs_per_s": pairs / (ms / 1000), "ms": ms, "ms_device": st["ms_device"],
+                "cell_tiles": st["cell_tiles"],
+                "latency_ms_p50": float(np.percentile(lat, 50)),
+                "latency_ms_p99": float(np.percentile(lat, 99)),
+                "exact": [int(s) for s in sums] == wants}
+
+    res["analytic_reference_rates"] = timed("analytic")
+    big = max(range(len(corpora)), key=lambda i: corpora[i].n_pairs)
+    progress("calibrating on the largest instance")
+    model = ctx.calibrate(corpora[big], tile=args.tile, max_tiles=args.sample_tiles, seed=7,
+                          repeats=2)
+    res["analytic"] = timed("analytic")
+    progress("sampling tiles for the selector")
+    Xs, Ys, Ms = [], [], []
+    order = sorted(range(len(corpora)), key=lambda i: -corpora[i].n_pairs)[:8]
+    for i in order:
+        X, Y, M, _ = ctx.selector_sample(corpora[i], tile=args.tile,
+                                         max_tiles=max(64, args.sample_tiles // 8), seed=11 + i,
+                                         repeats=2)
+        Xs.append(X), Ys.append(Y), Ms.append(M)
+    X, Y, M = np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
+    progress(f"training on {len(X)} tiles")
+    loss = ctx.selector_train(X, Y, M, epochs=300, batch=128, lr=3e-3, seed=5)
+    res["learned"] = timed("learned")
+    # The same corpora one call at a time (no batching) for comparison.
+    ts = []
+    for k in range(max(1, args.steps // 2)):
+        t1 = time.perf_counter()
+        for c in corpora:
+            c.allpairs(policy="learned", tile=args.tile, use_graph=True)
+        ts.append(time.perf_counter() - t1)
+    seq_ms = 1000 * float(np.median(ts))
+    out = {
+        "metric": METRIC,
+        "value": res["learned"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": res["learned"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded host generator)",
+        "config": {"workload": desc + f" (bounded sample: {len(chosen)} instances of at most "
+                                      f"{args.instance_mib} MiB estimated, {args.budget_gib} GiB "
+                                      f"total; {skipped} larger instances not run)",
+                   "instances_run": len(chosen), "instances_skipped": skipped,
+                   "pairs": pairs, "tile": args.tile},
+        "exact_all": all(v["exact"] for v in res.values()),
+        "policies": res,
+        "learned_over_analytic": res["learned"]["pairs_per_s"] / res["analytic"]["pairs_per_s"],
+        "sequential_learned_ms": seq_ms,
+        "batched_over_sequential": seq_ms / res["learned"]["ms"],
+        "selector": {"sampled_tiles": int(len(X)), "train_loss": loss},
+        "calibrated_model": model, "setup_s": {"generate": t_gen},
+    }
+    batch.free()
+    for c in corpora:
+        c.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def dense_bench(sh, args):
     n, bits, p, seed, desc = CONFIGS[args.config]
+    nw = (bits + 63) // 64
+    bb = args.bb
+    if bb == "auto":
+        bb = "mfma_fp4" if nw < (1 << 18) else "mfma"
     ctx = sh.Context(0)
     corpus = ctx.gen_bernoulli(n, bits, p, seed)
+    progress(f"generated {n} x {bits} bits on the device; B x B kernel {bb}")
     for _ in range(args.warmup):
-        corpus.allpairs(out="tri", bb=args.bb)
+        corpus.allpairs(out="tri", bb=bb)
     ctx.sync()
     t0 = time.perf_counter()
     bb_ms, dev_ms, sums = [], [], set()
-    for _ in range(args.steps):
-        st = corpus.allpairs(out="tri", bb=args.bb)
+    for k in range(args.steps):
+        st = corpus.allpairs(out="tri", bb=bb)
         bb_ms.append(st["ms_bb"])
         dev_ms.append(st["ms_device"])
         sums.add(st["sum"])
+        progress(f"step {k}: B x B {st['ms_bb']:.1f} ms")
     ctx.sync()
     wall = time.perf_counter() - t0
     ms_step = wall * 1000.0 / args.steps
     pairs = n * (n - 1) // 2
     kern_ms = float(np.mean(bb_ms))
-    kernel = "bb_mfma" if args.bb == "mfma" else "bb_valu"
-    if args.bb in ("mfma", "mfma_fp4"):
-        fp4 = args.bb == "mfma_fp4"
+    if bb in ("mfma", "mfma_fp4"):
+        fp4 = bb == "mfma_fp4"
         kernel = "bb_mfma_fp4" if fp4 else "bb_mfma_i8"
         peak = MFMA_FP4_PEAK_TOPS if fp4 else MFMA_I8_PEAK_TOPS
         ops = 2.0 * pairs * bits                      # MACs x 2, exact pairs only
@@ -226,7 +385,8 @@ def main():
                 "algorithmic": "2*M ops per exact pair i<j (0/1 operands, "
                                + ("e2m1 with f32" if fp4 else "int8 with i32") + " accumulate)"}
     else:
-        wp = pairs * ((bits + 63) // 64)
+        kernel = "bb_valu"
+        wp = pairs * nw
         valu_peak = 256 * 64 * 2.4e9 / 4            # 64-bit word-pairs/s (4 VALU ops each)
         achieved = wp / (kern_ms * 1e-3)
         roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": valu_peak / 1e12,
@@ -235,16 +395,16 @@ def main():
                 "kernel": kernel, "kernel_ms": kern_ms,
                 "algorithmic": "ceil(M/64) 64-bit AND+popcount per exact pair i<j"}
     out = {
-        "metric": "exact pairs/sec on 1 MI355X",
+        "metric": METRIC,
         "value": pairs / (ms_step / 1000.0),
         "unit": "pairs/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"mfma": "int8", "mfma_fp4": "fp4 (e2m1 0/1, f32 exact accumulate)"}.get(args.bb, "u32"),
+        "dtype": {"mfma": "int8", "mfma_fp4": "fp4 (e2m1 0/1, f32 exact accumulate)"}.get(bb, "u32"),
         "data": "synthetic (device splitmix64 Bernoulli stream)",
         "config": {"workload": desc, "n_rows": n, "n_bits": bits, "density": p, "seed": seed,
-                   "pairs": pairs, "bb_kernel": args.bb},
+                   "pairs": pairs, "bb_kernel": bb},
         "exact_sum": sums.pop() if len(sums) == 1 else None,
         "ms_device_per_step": float(np.mean(dev_ms)),
         "roofline": roof,
@@ -253,6 +413,38 @@ def main():
     corpus.free()
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="dense2",
+                    choices=sorted(CONFIGS) + sorted(SPARSE_CONFIGS) + sorted(BATCH_CONFIGS))
+    ap.add_argument("--bb", default="auto", choices=["auto", "mfma", "mfma_fp4", "valu"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rows", type=int, default=0, help="sparse configs: first N rows only")
+    ap.add_argument("--tile", type=int, default=64)
+    ap.add_argument("--sample-tiles", type=int, default=2048)
+    ap.add_argument("--no-allbitmap", action="store_true")
+    ap.add_argument("--instance-mib", type=int, default=256,
+                    help="batch configs: largest estimated instance size run")
+    ap.add_argument("--budget-gib", type=int, default=16,
+                    help="batch configs: total estimated size of the instances run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the all-pairs triangle would shard by tile rows, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    sh = load_storm_hip()
+    if args.config in SPARSE_CONFIGS:
+        sparse_bench(sh, args)
+    elif args.config in BATCH_CONFIGS:
+        batch_bench(sh, args)
+    else:
+        dense_bench(sh, args)
 
 
 if __name__ == "__main__":

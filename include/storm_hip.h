@@ -219,6 +219,26 @@ int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c,
 int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                               uint64_t* row_sum);
 
+/* ---- batches of independent corpora --------------------------------------
+ * One call over many general corpora: every corpus is selected per tile
+ * (analytic, learned or fixed policy), the tiles of all corpora share one
+ * device queue per cell class, and each class runs as ONE persistent launch
+ * for the whole batch (with use_graph, the whole sequence is one hipGraph).
+ * The reference runs independent problems one call at a time
+ * (kernels/storm_allpairs.h:196-214, storm.h:218); this has no single
+ * counterpart there. `sums[k]` (optional) receives corpus k's sum;
+ * `latency_ms[k]` (optional) the time from the start of the call's launches
+ * to the end of corpus k's last tile (0 when it has no tile). With
+ * OUT_UPPER_TRI every corpus keeps its own triangle (result_copy_tri). The
+ * corpora must stay alive while the batch exists. */
+typedef struct storm_hip_batch storm_hip_batch;
+int storm_hip_batch_create(storm_hip_ctx* ctx, storm_hip_corpus* const* corpora, uint32_t n,
+                           storm_hip_batch** out);
+int storm_hip_batch_allpairs(storm_hip_ctx* ctx, storm_hip_batch* b,
+                             const storm_hip_options* opt, storm_hip_stats* stats,
+                             uint64_t* sums, double* latency_ms);
+void storm_hip_batch_free(storm_hip_batch* b);
+
 /* ---- analytic cost model on the device -----------------------------------
  * CostModel rates (kernels/storm_cost.h:60-92) used by POLICY_ANALYTIC, in
  * ns per work unit, indexed by STORM_HIP_CELL_* (BB..WW); a rate <= 0 removes

@@ -26,7 +26,7 @@
 //        (w >> (4h+d)) & 0x01010101                   (byte j = bit 8j+4h+d)
 // i.e. one shift and one AND per output dword.
 #include "device_common.h"
-#include "internal.h"
+#include "general.h"
 
 namespace stormhip {
 
@@ -67,7 +67,7 @@ struct BBMfmaSmem {
 // `accumulate`, counts are atomically added to `tri` (a K-split block: other
 // blocks add the other stage ranges of the same pairs into a zeroed triangle).
 template <bool FP4>
-__device__ uint64_t bb_mfma_tile(BBMfmaSmem& sm, const uint64_t* __restrict__ X, uint32_t ld,
+__device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem& sm, const uint64_t* __restrict__ X, uint32_t ld,
                                  uint32_t s0, uint32_t n_stages, const uint32_t* __restrict__ rowmap,
                                  const uint32_t* __restrict__ perm, uint32_t n, uint32_t bi,
                                  uint32_t bj, uint32_t* __restrict__ tri, bool accumulate) {
@@ -189,9 +189,10 @@ template <bool FP4>
 __global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
     const uint64_t* __restrict__ X, uint32_t ld, uint32_t n_stages, uint32_t k_split,
     const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm, uint32_t n,
-    uint32_t T, const uint2* __restrict__ queue, const unsigned int* __restrict__ qcount,
-    unsigned int* __restrict__ qhead, uint32_t* __restrict__ tri,
-    unsigned long long* __restrict__ sum, unsigned long long* __restrict__ tile_ticks) {
+    uint32_t T, const DevCorpus* __restrict__ Ds, const QItem* __restrict__ queue,
+    const unsigned int* __restrict__ qcount, unsigned int* __restrict__ qhead,
+    uint32_t* __restrict__ tri, unsigned long long* __restrict__ sum,
+    unsigned long long* __restrict__ tile_ticks) {
     __shared__ __attribute__((aligned(16))) BBMfmaSmem sm;
     __shared__ uint32_t s_t;
     uint64_t local = 0;
@@ -201,8 +202,9 @@ __global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
         tri_tile(blockIdx.x / k_split, T, bi, bj);
         const uint32_t slice = blockIdx.x % k_split;
         const uint32_t per = (n_stages + k_split - 1) / k_split;
-        const uint32_t s0 = slice * per;
-        const uint32_t cnt = s0 < n_stages ? min(per, n_stages - s0) : 0;
+        // Clamp first so the stage count never goes through a wrapped difference.
+        const uint32_t s0 = min(slice * per, n_stages);
+        const uint32_t cnt = min(per, n_stages - s0);
         if (cnt)
             local = bb_mfma_tile<FP4>(sm, X, ld, s0, cnt, rowmap, perm, n, bi, bj, tri, k_split > 1);
     } else {
@@ -212,14 +214,22 @@ __global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
             __syncthreads();
             const uint32_t t = s_t;
             if (t >= *qcount) break;
-            const uint2 tp = queue[t];
+            const QItem e = queue[t];
+            const DevCorpus D = Ds[e.inst];
             const uint64_t t0 = wall_clock64();
-            local += bb_mfma_tile<FP4>(sm, X, ld, 0, n_stages, rowmap, perm, n, tp.x, tp.y, tri, false);
-            if (tile_ticks) {
-                __syncthreads();
-                if (threadIdx.x == 0) tile_ticks[t] = wall_clock64() - t0;
+            const uint32_t ns = max(1u, (D.nw + kWordsStage - 1) / kWordsStage);
+            uint64_t part = bb_mfma_tile<FP4>(sm, D.bitmap, D.ld, 0, ns, D.rowmap, D.perm, D.n,
+                                              e.bi, e.bj, D.tri, false);
+            part = wave_sum_u64(part);
+            if ((threadIdx.x & 63) == 0 && part) atomicAdd(D.sum, (unsigned long long)part);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint64_t now = wall_clock64();
+                if (tile_ticks) tile_ticks[t] = now - t0;
+                if (D.finish) atomicMax(D.finish, (unsigned long long)now);
             }
         }
+        return;
     }
     local = wave_sum_u64(local);
     if ((threadIdx.x & 63) == 0 && local) atomicAdd(sum, (unsigned long long)local);
@@ -227,11 +237,8 @@ __global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
 
 // k_split > 1 (grid mode only): each tile's stages are split over k_split
 // blocks that atomically add into `tri`, which the caller has zeroed.
-int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                       const uint32_t* rowmap, const uint32_t* perm, uint32_t n,
-                       const uint2* queue, const unsigned int* qcount, unsigned int* qhead,
-                       uint32_t grid, uint32_t* tri, unsigned long long* sum,
-                       unsigned long long* tile_ticks, uint32_t k_split) {
+// Exactness and layout checks shared by both launch modes.
+static int mfma_checks(bool fp4, uint32_t n_words, uint32_t ld) {
     const uint32_t n_stages = (n_words + kWordsStage - 1) / kWordsStage;
     // Two stages are read before the loop tests its bound: rows must hold them.
     if ((uint64_t)std::max(n_stages, 2u) * kWordsStage > ld) {
@@ -244,17 +251,46 @@ int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, 
                   fp4 ? "f32" : "i32");
         return STORM_HIP_ERR_RANGE;
     }
-    if (queue || k_split == 0) k_split = 1;
+    return STORM_HIP_OK;
+}
+
+int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
+                       const uint32_t* rowmap, const uint32_t* perm, uint32_t n, uint32_t* tri,
+                       unsigned long long* sum, uint32_t k_split) {
+    SH_TRY(mfma_checks(fp4, n_words, ld));
+    if (k_split == 0) k_split = 1;
+    const uint32_t n_stages = (n_words + kWordsStage - 1) / kWordsStage;
     const uint32_t T = (n + kTile - 1) / kTile;
-    const uint64_t g = queue ? grid : (uint64_t)T * (T + 1) / 2 * k_split;
+    const uint64_t g = (uint64_t)T * (T + 1) / 2 * k_split;
     if (g == 0) return STORM_HIP_OK;
     const uint32_t ns = n_stages ? n_stages : 1;
     if (fp4)
         hipLaunchKernelGGL(bb_mfma_kernel<true>, dim3((uint32_t)g), dim3(kThreads), 0, s, X, ld, ns,
-                           k_split, rowmap, perm, n, T, queue, qcount, qhead, tri, sum, tile_ticks);
+                           k_split, rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr, tri,
+                           sum, nullptr);
     else
         hipLaunchKernelGGL(bb_mfma_kernel<false>, dim3((uint32_t)g), dim3(kThreads), 0, s, X, ld, ns,
-                           k_split, rowmap, perm, n, T, queue, qcount, qhead, tri, sum, tile_ticks);
+                           k_split, rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr, tri,
+                           sum, nullptr);
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+// Queue mode over the corpora in Ds; `max_words`/`min_ld` bound every corpus
+// of the call (the caller picks fp4 only when all of them allow it).
+int launch_bb_mfma_queue(bool fp4, hipStream_t s, const DevCorpus* Ds, uint32_t max_words,
+                         uint32_t min_ld, const QItem* queue, const unsigned int* qcount,
+                         unsigned int* qhead, uint32_t grid, unsigned long long* tile_ticks) {
+    SH_TRY(mfma_checks(fp4, max_words, min_ld));
+    if (grid == 0) return STORM_HIP_OK;
+    if (fp4)
+        hipLaunchKernelGGL(bb_mfma_kernel<true>, dim3(grid), dim3(kThreads), 0, s, nullptr, 0u, 1u,
+                           1u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
+                           tile_ticks);
+    else
+        hipLaunchKernelGGL(bb_mfma_kernel<false>, dim3(grid), dim3(kThreads), 0, s, nullptr, 0u, 1u,
+                           1u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
+                           tile_ticks);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }
@@ -263,8 +299,7 @@ int launch_bb_dense(uint32_t bb_kernel, hipStream_t s, const uint64_t* X, uint32
                     uint32_t n_words, const uint32_t* rowmap, uint32_t n, uint32_t* tri,
                     unsigned long long* sum, uint32_t cu_count) {
     if (bb_kernel == STORM_HIP_BB_VALU)
-        return launch_bb_valu(s, 128, X, ld, n_words, rowmap, nullptr, n, nullptr, nullptr,
-                              nullptr, 0, tri, sum);
+        return launch_bb_valu(s, 128, X, ld, n_words, rowmap, nullptr, n, tri, sum);
     const bool fp4 = bb_kernel == STORM_HIP_BB_MFMA_FP4 ||
                      (bb_kernel == STORM_HIP_BB_AUTO && n_words < (1u << 18));
     // Too few 256 x 256 blocks to fill the chip: split K so that about two
@@ -273,23 +308,17 @@ int launch_bb_dense(uint32_t bb_kernel, hipStream_t s, const uint64_t* X, uint32
     const uint64_t tiles = (uint64_t)T * (T + 1) / 2;
     const uint32_t n_stages = (n_words + kWordsStage - 1) / kWordsStage;
     uint32_t k_split = 1;
-    if (tiles && tiles < 2ull * cu_count) {
+    if (tiles && n_stages && tiles < 2ull * cu_count) {
         k_split = (uint32_t)std::min<uint64_t>((2ull * cu_count + tiles - 1) / tiles,
                                                std::max(1u, n_stages / 8));
         k_split = std::max(1u, k_split);
+        // No empty slices: as many slices as ceil(n_stages / per) needs.
+        const uint32_t per = (n_stages + k_split - 1) / k_split;
+        k_split = (n_stages + per - 1) / per;
     }
     if (k_split > 1 && tri)
         SH_HIP(hipMemsetAsync(tri, 0, (uint64_t)n * (n - 1) / 2 * 4, s));
-    return launch_bb_mfma_fmt(fp4, s, X, ld, n_words, rowmap, nullptr, n, nullptr, nullptr,
-                              nullptr, 0, tri, sum, nullptr, k_split);
-}
-
-int launch_bb_mfma(hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n, const uint2* queue,
-                   const unsigned int* qcount, unsigned int* qhead, uint32_t grid, uint32_t* tri,
-                   unsigned long long* sum, unsigned long long* tile_ticks) {
-    return launch_bb_mfma_fmt(false, s, X, ld, n_words, rowmap, perm, n, queue, qcount, qhead,
-                              grid, tri, sum, tile_ticks, 1);
+    return launch_bb_mfma_fmt(fp4, s, X, ld, n_words, rowmap, nullptr, n, tri, sum, k_split);
 }
 
 }  // namespace stormhip

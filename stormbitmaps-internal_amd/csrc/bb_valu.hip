@@ -13,7 +13,7 @@
 // Tiles come from the grid (upper triangle of the T x T block grid) or, in
 // queue mode, from a device queue drained by persistent workgroups.
 #include "device_common.h"
-#include "internal.h"
+#include "general.h"
 
 namespace stormhip {
 
@@ -117,54 +117,82 @@ template <int BM>
 __global__ __launch_bounds__(256, 2) void bb_valu_kernel(
     const uint64_t* __restrict__ X, uint32_t ld, uint32_t kwords,
     const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm, uint32_t n,
-    uint32_t T, const uint2* __restrict__ queue, const unsigned int* __restrict__ qcount,
-    unsigned int* __restrict__ qhead, uint32_t* __restrict__ tri,
-    unsigned long long* __restrict__ sum, unsigned long long* __restrict__ tile_ticks) {
+    uint32_t T, const DevCorpus* __restrict__ Ds, const QItem* __restrict__ queue,
+    const unsigned int* __restrict__ qcount, unsigned int* __restrict__ qhead,
+    uint32_t* __restrict__ tri, unsigned long long* __restrict__ sum,
+    unsigned long long* __restrict__ tile_ticks) {
     __shared__ BBValuSmem<BM> sm;
     __shared__ uint32_t s_t;
-    uint64_t local = 0;
     if (!qcount) {
         uint32_t bi, bj;
         tri_tile(blockIdx.x, T, bi, bj);
-        local = bb_valu_tile<BM>(sm, X, ld, kwords, rowmap, perm, n, bi, bj, tri);
-    } else {
-        for (;;) {
-            __syncthreads();
-            if (threadIdx.x == 0) s_t = atomicAdd(qhead, 1u);
-            __syncthreads();
-            const uint32_t t = s_t;
-            if (t >= *qcount) break;
-            const uint2 tp = queue[t];
-            const uint64_t t0 = wall_clock64();
-            local += bb_valu_tile<BM>(sm, X, ld, kwords, rowmap, perm, n, tp.x, tp.y, tri);
-            if (tile_ticks) {
-                __syncthreads();
-                if (threadIdx.x == 0) tile_ticks[t] = wall_clock64() - t0;
-            }
+        uint64_t local = bb_valu_tile<BM>(sm, X, ld, kwords, rowmap, perm, n, bi, bj, tri);
+        local = wave_sum_u64(local);
+        if ((threadIdx.x & 63) == 0 && local) atomicAdd(sum, (unsigned long long)local);
+        return;
+    }
+    // Queue mode: tiles of any corpus of the call, each with its own arrays.
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_t = atomicAdd(qhead, 1u);
+        __syncthreads();
+        const uint32_t t = s_t;
+        if (t >= *qcount) break;
+        const QItem e = queue[t];
+        const DevCorpus D = Ds[e.inst];
+        const uint64_t t0 = wall_clock64();
+        uint64_t local = bb_valu_tile<BM>(sm, D.bitmap, D.ld, (D.nw + 15) / 16 * 16, D.rowmap,
+                                          D.perm, D.n, e.bi, e.bj, D.tri);
+        local = wave_sum_u64(local);
+        if ((threadIdx.x & 63) == 0 && local) atomicAdd(D.sum, (unsigned long long)local);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint64_t now = wall_clock64();
+            if (tile_ticks) tile_ticks[t] = now - t0;
+            if (D.finish) atomicMax(D.finish, (unsigned long long)now);
         }
     }
-    local = wave_sum_u64(local);
-    if ((threadIdx.x & 63) == 0 && local) atomicAdd(sum, (unsigned long long)local);
 }
 
 // Host launcher. queue == nullptr: all T(T+1)/2 upper-triangle tiles of BM
 // rows, one workgroup each; otherwise `grid` persistent workgroups drain the
 // device queue (qcount/qhead on the device).
 int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n, const uint2* queue,
-                   const unsigned int* qcount, unsigned int* qhead, uint32_t grid, uint32_t* tri,
-                   unsigned long long* sum, unsigned long long* tile_ticks) {
+                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n, uint32_t* tri,
+                   unsigned long long* sum) {
     const uint32_t kwords = (n_words + 15) / 16 * 16;
     if (kwords > ld) { set_error("bb_valu: kwords %u > ld %u", kwords, ld); return STORM_HIP_ERR_INVALID; }
     const uint32_t T = (n + bm - 1) / bm;
-    const uint64_t g = queue ? grid : (uint64_t)T * (T + 1) / 2;
+    const uint64_t g = (uint64_t)T * (T + 1) / 2;
     if (g == 0) return STORM_HIP_OK;
     if (bm == 128) {
         hipLaunchKernelGGL(bb_valu_kernel<128>, dim3((uint32_t)g), dim3(256), 0, s, X, ld, kwords,
-                           rowmap, perm, n, T, queue, qcount, qhead, tri, sum, tile_ticks);
+                           rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr, tri, sum, nullptr);
     } else if (bm == 64) {
         hipLaunchKernelGGL(bb_valu_kernel<64>, dim3((uint32_t)g), dim3(256), 0, s, X, ld, kwords,
-                           rowmap, perm, n, T, queue, qcount, qhead, tri, sum, tile_ticks);
+                           rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr, tri, sum, nullptr);
+    } else {
+        set_error("bb_valu: unsupported tile %d", bm);
+        return STORM_HIP_ERR_INVALID;
+    }
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+// Queue mode: `grid` persistent workgroups drain a device queue of tiles of
+// the corpora in Ds (every corpus's rows padded so ld covers its words).
+int launch_bb_valu_queue(hipStream_t s, int bm, const DevCorpus* Ds, const QItem* queue,
+                         const unsigned int* qcount, unsigned int* qhead, uint32_t grid,
+                         unsigned long long* tile_ticks) {
+    if (grid == 0) return STORM_HIP_OK;
+    if (bm == 128) {
+        hipLaunchKernelGGL(bb_valu_kernel<128>, dim3(grid), dim3(256), 0, s, nullptr, 0u, 0u,
+                           nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
+                           tile_ticks);
+    } else if (bm == 64) {
+        hipLaunchKernelGGL(bb_valu_kernel<64>, dim3(grid), dim3(256), 0, s, nullptr, 0u, 0u,
+                           nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
+                           tile_ticks);
     } else {
         set_error("bb_valu: unsupported tile %d", bm);
         return STORM_HIP_ERR_INVALID;

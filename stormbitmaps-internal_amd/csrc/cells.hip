@@ -413,17 +413,13 @@ __device__ inline bool ss_use_join(const TileMeta& A, const TileMeta& B) {
 }  // namespace
 
 template <int CELL>
-__global__ __launch_bounds__(256) void cell_kernel(DevCorpus D, DevModel M,
-                                                   const TileMeta* __restrict__ tm, uint32_t tile,
-                                                   const uint2* __restrict__ queue,
+__global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__ Ds, DevModel M,
+                                                   const QItem* __restrict__ queue,
                                                    DevCounters* __restrict__ ctr,
-                                                   uint32_t* __restrict__ tri,
-                                                   unsigned long long* __restrict__ sum,
                                                    unsigned long long* __restrict__ tile_cycles) {
     __shared__ uint32_t s_t;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint64_t local = 0, pairs = 0, skipped = 0;
-    const uint32_t n = D.n;
+    uint64_t pairs = 0, skipped = 0;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_t = atomicAdd(&ctr->qhead[CELL], 1u);
@@ -431,18 +427,22 @@ __global__ __launch_bounds__(256) void cell_kernel(DevCorpus D, DevModel M,
         const uint32_t t = s_t;
         if (t >= ctr->qcount[CELL]) break;
         const uint64_t t0 = wall_clock64();
-        const uint2 tp = queue[t];
-        const uint32_t i0 = tp.x * tile, j0 = tp.y * tile;
+        const QItem e = queue[t];
+        const DevCorpus D = Ds[e.inst];
+        const uint32_t n = D.n, tile = D.tile;
+        uint32_t* const tri = D.tri;
+        uint64_t local = 0;
+        const uint32_t i0 = e.bi * tile, j0 = e.bj * tile;
         const uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
-        const bool diag = tp.x == tp.y;
+        const bool diag = e.bi == e.bj;
         const uint32_t np = ni * nj;
         // Lane mode when the sparser tile's rows are tiny.
-        const TileMeta& A = tm[tp.x];
+        const TileMeta& A = D.tm[e.bi];
         const bool lane_mode = CELL == STORM_HIP_CELL_EMPTY || (A.max_card <= 48 && A.max_runs <= 48);
         bool joined = false;
         if constexpr (CELL == STORM_HIP_CELL_SS) {
             __shared__ JoinSmem js;
-            if (ss_use_join(A, tm[tp.y])) {
+            if (ss_use_join(A, D.tm[e.bj])) {
                 joined = true;
                 for (uint32_t si = 0; si < ni; si += 64)
                     for (uint32_t sj = diag ? si : 0; sj < nj; sj += 64)
@@ -486,16 +486,18 @@ __global__ __launch_bounds__(256) void cell_kernel(DevCorpus D, DevModel M,
                 }
             }
         }
-        if (tile_cycles) {
-            __syncthreads();
-            if (threadIdx.x == 0) tile_cycles[t] = wall_clock64() - t0;
+        local = wave_sum_u64(local);
+        if (lane == 0 && local) atomicAdd(D.sum, (unsigned long long)local);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint64_t now = wall_clock64();
+            if (tile_cycles) tile_cycles[t] = now - t0;
+            if (D.finish) atomicMax(D.finish, (unsigned long long)now);
         }
     }
-    local = wave_sum_u64(local);
     pairs = wave_sum_u64(pairs);
     skipped = wave_sum_u64(skipped);
     if (lane == 0) {
-        if (local) atomicAdd(sum, (unsigned long long)local);
         if (pairs) atomicAdd(&ctr->cell_pairs[CELL], (unsigned long long)pairs);
         if (skipped) {
             atomicAdd(&ctr->cell_pairs[STORM_HIP_CELL_EMPTY], (unsigned long long)skipped);
@@ -504,13 +506,12 @@ __global__ __launch_bounds__(256) void cell_kernel(DevCorpus D, DevModel M,
     }
 }
 
-int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus& D, const DevModel& M,
-                const TileMeta* tm, uint32_t tile, const uint2* queue, DevCounters* ctr,
-                uint32_t* tri, unsigned long long* sum, unsigned long long* tile_cycles) {
+int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
+                const QItem* queue, DevCounters* ctr, unsigned long long* tile_cycles) {
 #define SH_CELL(C)                                                                          \
     case C:                                                                                 \
-        hipLaunchKernelGGL(cell_kernel<C>, dim3(grid), dim3(256), 0, s, D, M, tm, tile,     \
-                           queue, ctr, tri, sum, tile_cycles);                               \
+        hipLaunchKernelGGL(cell_kernel<C>, dim3(grid), dim3(256), 0, s, Ds, M, queue, ctr,  \
+                           tile_cycles);                                                     \
         break;
     switch (cell) {
         SH_CELL(STORM_HIP_CELL_BS)

@@ -26,7 +26,24 @@ struct DevCorpus {
     const uint32_t* bm_row;   const uint64_t* bitmap;   // bitmap row index, kNoBitmap if none
     const uint32_t* meta;     // 8 per original row (RowMeta + has_rank, has_occ)
     const uint32_t* perm;     // sorted position -> original row id
+    // Per-call state of this corpus inside a (possibly multi-corpus) call.
+    const struct TileMeta* tm;        // its row tiles' metadata
+    const uint32_t* rowmap;           // sorted position -> bitmap row
+    uint32_t* tri;                    // per-pair output or null
+    unsigned long long* sum;          // its sum
+    unsigned long long* finish;       // wall clock of its last finished tile (max)
+    uint32_t tile;                    // rows per tile
 };
+
+// One queued tile pair: corpus index, tile row, tile column.
+struct QItem { uint32_t inst, bi, bj, pad; };
+
+int launch_bb_valu_queue(hipStream_t s, int bm, const DevCorpus* Ds, const QItem* queue,
+                         const unsigned int* qcount, unsigned int* qhead, uint32_t grid,
+                         unsigned long long* tile_ticks);
+int launch_bb_mfma_queue(bool fp4, hipStream_t s, const DevCorpus* Ds, uint32_t max_words,
+                         uint32_t min_ld, const QItem* queue, const unsigned int* qcount,
+                         unsigned int* qhead, uint32_t grid, unsigned long long* tile_ticks);
 
 // Aggregate metadata of one row tile (density-sorted rows [lo, hi)). The first
 // block is tile_meta of kernels/storm_allpairs.cpp:281-312; the rest are

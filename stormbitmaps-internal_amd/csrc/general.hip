@@ -1,11 +1,14 @@
-// The all-pairs call on general (mixed-representation) corpora:
-//   select   per-tile metadata -> analytic / learned / fixed decision -> one
-//            device queue per cell class            (select.hip)
-//   dispatch one persistent launch per cell class draining its queue
-//            (cells.hip, bb_valu.hip, bb_mfma.hip)
-//   reduce   the u64 sum and the per-cell counters by atomics in the launches
-// All counts of work live on the device, so the sequence needs no host round
-// trip and can be captured in a hipGraph and replayed.
+// The all-pairs call on general (mixed-representation) corpora, for one corpus
+// or a batch of independent corpora in the same launches:
+//   select   per corpus: tile metadata -> analytic / learned / fixed decision
+//            per tile pair -> appended to ONE device queue per cell class,
+//            entries tagged with their corpus             (select.hip)
+//   dispatch one persistent launch per cell class drains its queue across all
+//            corpora of the call (cells.hip, bb_valu.hip, bb_mfma.hip)
+//   reduce   per-corpus sums, per-cell counters and per-corpus finish times
+//            by atomics inside those launches
+// Every count of work lives on the device, so the whole sequence is captured
+// once in a hipGraph and replayed.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -20,12 +23,12 @@ namespace stormhip {
 int launch_tile_meta(hipStream_t s, const DevCorpus& D, uint32_t tile, uint32_t T, TileMeta* out);
 int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel& M, uint32_t reps,
                   uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
-                  const uint8_t* learned_ok, uint8_t* decisions, uint2* queue, DevCounters* ctr);
+                  const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
+                  uint32_t inst, DevCounters* ctr);
 int launch_compose_rowmap(hipStream_t s, const uint32_t* order, const uint32_t* bm_row,
                           uint32_t npad, uint32_t* rowmap);
-int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus& D, const DevModel& M,
-                const TileMeta* tm, uint32_t tile, const uint2* queue, DevCounters* ctr,
-                uint32_t* tri, unsigned long long* sum, unsigned long long* tile_cycles);
+int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
+                const QItem* queue, DevCounters* ctr, unsigned long long* tile_cycles);
 int general_materialize(storm_hip_ctx* ctx, storm_hip_corpus* c);
 void general_free_arenas(storm_hip_corpus* c);
 int learned_scores(storm_hip_ctx* ctx, const TileMeta* tm, uint32_t T, uint64_t P,
@@ -45,39 +48,28 @@ void default_device_model(DeviceCostModel& m) {
 
 namespace {
 
+// Per-corpus selection state for one tile width.
 struct Work {
     uint32_t tile = 0, T = 0, npad = 0;
     uint64_t P = 0;
     TileMeta* tm = nullptr;
     uint8_t* decisions = nullptr;
-    uint2* queue = nullptr;
-    DevCounters* ctr = nullptr;
     uint32_t* rowmap = nullptr;
     float* learned_cost = nullptr;
     uint8_t* learned_ok = nullptr;
-    unsigned long long* sum = nullptr;
-    // Captured select -> dispatch -> reduce graph and what it was built for.
-    hipGraphExec_t graph = nullptr;
-    uint32_t graph_key[4] = {0, 0, 0, 0};
-    uint32_t* graph_tri = nullptr;
 };
 
 void free_work(Work* w) {
     if (!w) return;
-    if (w->graph) (void)hipGraphExecDestroy(w->graph);
-    void* p[] = {w->tm, w->decisions, w->queue, w->ctr, w->rowmap, w->learned_cost,
-                 w->learned_ok, w->sum};
+    void* p[] = {w->tm, w->decisions, w->rowmap, w->learned_cost, w->learned_ok};
     for (void* x : p) if (x) (void)hipFree(x);
     delete w;
 }
 
-int ensure_work(storm_hip_corpus* c, uint32_t tile) {
-    Work* w = static_cast<Work*>(c->work);
-    if (w && w->tile == tile) return STORM_HIP_OK;
-    free_work(w);
-    c->work = nullptr;
-    w = new Work();
-    c->work = w;
+// Selection state of corpus c at this tile width (owned by the caller).
+int make_work(const storm_hip_corpus* c, uint32_t tile, Work** out) {
+    Work* w = new Work();
+    *out = w;
     w->tile = tile;
     w->T = (c->n_rows + tile - 1) / tile;
     w->P = (uint64_t)w->T * (w->T + 1) / 2;
@@ -85,23 +77,44 @@ int ensure_work(storm_hip_corpus* c, uint32_t tile) {
     const uint64_t P1 = w->P ? w->P : 1;
     SH_HIP(hipMalloc(&w->tm, sizeof(TileMeta) * (w->T ? w->T : 1)));
     SH_HIP(hipMalloc(&w->decisions, P1));
-    SH_HIP(hipMalloc(&w->queue, sizeof(uint2) * P1 * kNumCells));
-    SH_HIP(hipMalloc(&w->ctr, sizeof(DevCounters)));
     SH_HIP(hipMalloc(&w->rowmap, sizeof(uint32_t) * w->npad));
     SH_HIP(hipMalloc(&w->learned_cost, sizeof(float) * P1 * 10));
     SH_HIP(hipMalloc(&w->learned_ok, P1));
-    SH_HIP(hipMalloc(&w->sum, 8));
     return STORM_HIP_OK;
 }
 
-// Zero fill as a kernel (graph replays must not depend on memset nodes).
-__global__ void zero_kernel(uint32_t* p, uint32_t n_words) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n_words) p[k] = 0;
+// Device state of one call over a list of corpora: per-corpus selection
+// state, the device copy of every corpus's view, the queues and the results.
+// Cached and reused (graph included) while the corpora, their device arrays,
+// the tile width and the outputs stay the same.
+struct Plan {
+    std::vector<storm_hip_corpus*> cs;
+    std::vector<uint32_t*> tris;
+    std::vector<Work*> works;               // one per corpus, owned
+    std::vector<DevCorpus> h_corpora;       // host copy of d_corpora
+    uint32_t tile = 0;
+    uint64_t totalP = 0;
+    DevCorpus* d_corpora = nullptr;
+    QItem* queue = nullptr;                 // kNumCells x totalP
+    DevCounters* ctr = nullptr;
+    unsigned long long* sums = nullptr;     // one per corpus
+    unsigned long long* finish = nullptr;   // one per corpus, then the start stamp
+    hipGraphExec_t graph = nullptr;
+    uint32_t graph_key[4] = {0, 0, 0, 0};
+};
+
+void free_plan(Plan* p) {
+    if (!p) return;
+    if (p->graph) (void)hipGraphExecDestroy(p->graph);
+    for (Work* w : p->works) free_work(w);
+    void* q[] = {p->d_corpora, p->queue, p->ctr, p->sums, p->finish};
+    for (void* x : q) if (x) (void)hipFree(x);
+    delete p;
 }
 
-DevCorpus dev_corpus(const storm_hip_corpus* c, const uint32_t* order) {
-    DevCorpus D;
+DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
+                     unsigned long long* sum, unsigned long long* finish) {
+    DevCorpus D{};
     D.n = c->n_rows; D.universe = c->universe; D.nw = c->n_words; D.ld = c->ld;
     D.list_off = c->d_list_off; D.list = c->d_list;
     D.run_off = c->d_run_off; D.run_start = c->d_run_start; D.run_end = c->d_run_end;
@@ -110,7 +123,9 @@ DevCorpus dev_corpus(const storm_hip_corpus* c, const uint32_t* order) {
     D.comp_off = c->d_comp_off; D.comp = c->d_comp;
     D.bm_row = c->d_bm_row; D.bitmap = c->d_bitmap;
     D.meta = c->d_meta;
-    D.perm = order;
+    D.perm = c->d_perm;
+    D.tm = w->tm; D.rowmap = w->rowmap; D.tile = w->tile;
+    D.tri = tri; D.sum = sum; D.finish = finish;
     return D;
 }
 
@@ -124,56 +139,234 @@ DevModel dev_model(const DeviceCostModel& m) {
     return M;
 }
 
-// The kernel sequence of one selecting call, on `s`. Timing events (if any)
-// bracket selection and the B x B launch.
-int enqueue_selecting(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_options& o,
-                      Work* w, uint32_t* tri, bool timed) {
+uint32_t reps_of(const storm_hip_corpus* c) {
+    return (c->has_runs ? 1u : 0u) | (c->has_ewah ? 2u : 0u) | (c->has_comp ? 4u : 0u);
+}
+
+// (Re)build the plan for this list of corpora, tile width and outputs.
+// A plan is rebuilt when anything it baked in moved: the corpus list, the
+// outputs, the tile width, or any device array of a corpus (materialising
+// bitmaps replaces the bitmap store, for instance).
+bool plan_current(const Plan* p, const std::vector<storm_hip_corpus*>& cs,
+                  const std::vector<uint32_t*>& tris, uint32_t tile) {
+    if (!p || p->cs != cs || p->tris != tris || p->tile != tile) return false;
+    for (size_t k = 0; k < cs.size(); ++k) {
+        const DevCorpus D = dev_corpus(cs[k], p->works[k], tris[k], p->sums + k, p->finish + k);
+        if (std::memcmp(&D, &p->h_corpora[k], sizeof D) != 0) return false;
+    }
+    return true;
+}
+
+int build_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
+               const std::vector<uint32_t*>& tris, uint32_t tile) {
+    free_plan(p);
+    p = new Plan();
+    p->cs = cs;
+    p->tris = tris;
+    p->tile = tile;
+    const size_t n = cs.size();
+    for (storm_hip_corpus* c : cs) {
+        Work* w = nullptr;
+        const int rc = make_work(c, tile, &w);
+        p->works.push_back(w);
+        SH_TRY(rc);
+        p->totalP += w->P;
+    }
+    const uint64_t P1 = p->totalP ? p->totalP : 1;
+    SH_HIP(hipMalloc(&p->d_corpora, sizeof(DevCorpus) * std::max<size_t>(n, 1)));
+    SH_HIP(hipMalloc(&p->queue, sizeof(QItem) * P1 * kNumCells));
+    SH_HIP(hipMalloc(&p->ctr, sizeof(DevCounters)));
+    SH_HIP(hipMalloc(&p->sums, 8 * std::max<size_t>(n, 1)));
+    SH_HIP(hipMalloc(&p->finish, 8 * (n + 1)));
+    p->h_corpora.resize(n);
+    for (size_t k = 0; k < n; ++k)
+        p->h_corpora[k] = dev_corpus(cs[k], p->works[k], tris[k], p->sums + k, p->finish + k);
+    if (n)
+        SH_HIP(hipMemcpy(p->d_corpora, p->h_corpora.data(), sizeof(DevCorpus) * n,
+                         hipMemcpyHostToDevice));
+    return STORM_HIP_OK;
+}
+
+// On failure the plan is freed and left null (never half built).
+int ensure_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
+                const std::vector<uint32_t*>& tris, uint32_t tile) {
+    if (plan_current(p, cs, tris, tile)) return STORM_HIP_OK;
+    const int rc = build_plan(p, cs, tris, tile);
+    if (rc != STORM_HIP_OK) { free_plan(p); p = nullptr; }
+    return rc;
+}
+
+// The all-bitmap path's row map (identity order through the bitmap store).
+int ensure_bitmap_rowmap(storm_hip_corpus* c, Work** out) {
+    Work* w = static_cast<Work*>(c->work);
+    if (!w) {
+        const int rc = make_work(c, 256, &w);
+        if (rc != STORM_HIP_OK) { free_work(w); return rc; }
+        c->work = w;
+    }
+    *out = w;
+    return STORM_HIP_OK;
+}
+
+// Zero fill as a kernel (graph replays must not depend on memset nodes).
+__global__ void zero_kernel(uint32_t* p, uint32_t n_words) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_words) p[k] = 0;
+}
+__global__ void stamp_kernel(unsigned long long* out) { *out = wall_clock64(); }
+
+int zero_words(hipStream_t s, void* p, uint64_t bytes) {
+    const uint32_t n = (uint32_t)(bytes / 4);
+    if (!n) return STORM_HIP_OK;
+    hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t*>(p), n);
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+// The kernel sequence of one selecting call on the context's stream.
+int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, bool timed) {
     hipStream_t s = ctx->stream;
-    const DevCorpus D = dev_corpus(c, c->d_perm);
     const DevModel M = dev_model(ctx->model);
-    const uint32_t reps = (c->has_runs ? 1u : 0u) | (c->has_ewah ? 2u : 0u) | (c->has_comp ? 4u : 0u);
-    const uint32_t zero_words = (uint32_t)(sizeof(DevCounters) / 4);
-    hipLaunchKernelGGL(zero_kernel, dim3((zero_words + 255) / 256), dim3(256), 0, s,
-                       reinterpret_cast<uint32_t*>(w->ctr), zero_words);
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(w->sum), 2u);
+    const size_t n = p->cs.size();
+    SH_TRY(zero_words(s, p->ctr, sizeof(DevCounters)));
+    SH_TRY(zero_words(s, p->sums, 8 * n));
+    SH_TRY(zero_words(s, p->finish, 8 * n));
+    hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, p->finish + n);
     SH_HIP(hipGetLastError());
     if (timed) SH_HIP(hipEventRecord(ctx->ev[4], s));
-    SH_TRY(launch_tile_meta(s, D, w->tile, w->T, w->tm));
-    const float* lc = nullptr;
-    const uint8_t* lok = nullptr;
-    if (o.policy == STORM_HIP_POLICY_LEARNED) {
-        bool have = false;
-        SH_TRY(learned_scores(ctx, w->tm, w->T, w->P, c->universe, w->learned_cost, w->learned_ok, &have));
-        if (have) { lc = w->learned_cost; lok = w->learned_ok; }
+    uint32_t max_words = 0, min_ld = 0xFFFFFFFFu;
+    for (size_t k = 0; k < n; ++k) {
+        storm_hip_corpus* c = p->cs[k];
+        Work* w = p->works[k];
+        SH_TRY(launch_tile_meta(s, p->h_corpora[k], w->tile, w->T, w->tm));
+        const float* lc = nullptr;
+        const uint8_t* lok = nullptr;
+        if (o.policy == STORM_HIP_POLICY_LEARNED) {
+            bool have = false;
+            SH_TRY(learned_scores(ctx, w->tm, w->T, w->P, c->universe, w->learned_cost,
+                                  w->learned_ok, &have));
+            if (have) { lc = w->learned_cost; lok = w->learned_ok; }
+        }
+        SH_TRY(launch_select(s, w->tm, w->T, M, reps_of(c), o.policy, o.fixed_cell, lc, lok,
+                             w->decisions, p->queue, p->totalP, (uint32_t)k, p->ctr));
+        SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
+        // Only corpora with bitmap rows can queue B x B tiles.
+        if (c->n_bitmap_rows) {
+            max_words = std::max(max_words, c->n_words);
+            min_ld = std::min(min_ld, c->ld);
+        }
     }
-    SH_TRY(launch_select(s, w->tm, w->T, M, reps, o.policy, o.fixed_cell, lc, lok, w->decisions,
-                         w->queue, w->ctr));
-    SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
+    // fp4 with f32 accumulation is exact below 2^18 words (counts < 2^24).
+    const bool fp4_ok = max_words < (1u << 18);
+    if (min_ld == 0xFFFFFFFFu) min_ld = 4 * 2;
     if (timed) SH_HIP(hipEventRecord(ctx->ev[5], s));
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(w->P ? w->P : 1, (uint64_t)ctx->cu_count * 4);
+    const uint64_t P1 = p->totalP ? p->totalP : 1;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(P1, (uint64_t)ctx->cu_count * 4);
     for (int cell = 0; cell < kNumCells; ++cell) {
         if (cell == STORM_HIP_CELL_BB) continue;
-        SH_TRY(launch_cell(cell, s, grid, D, M, w->tm, w->tile, w->queue + (uint64_t)cell * w->P,
-                           w->ctr, tri, w->sum, nullptr));
+        SH_TRY(launch_cell(cell, s, grid, p->d_corpora, M, p->queue + (uint64_t)cell * p->totalP,
+                           p->ctr, nullptr));
     }
     if (timed) SH_HIP(hipEventRecord(ctx->ev[2], s));
-    const uint2* qbb = w->queue + (uint64_t)STORM_HIP_CELL_BB * w->P;
-    const uint32_t gbb = (uint32_t)std::min<uint64_t>(w->P ? w->P : 1, (uint64_t)ctx->cu_count * 2);
-    if (w->tile == 256)
-        SH_TRY(launch_bb_mfma(s, c->d_bitmap, c->ld, c->n_words, w->rowmap, c->d_perm, c->n_rows,
-                              qbb, &w->ctr->qcount[STORM_HIP_CELL_BB],
-                              &w->ctr->qhead[STORM_HIP_CELL_BB], gbb, tri, w->sum));
-    else
-        SH_TRY(launch_bb_valu(s, (int)w->tile, c->d_bitmap, c->ld, c->n_words, w->rowmap,
-                              c->d_perm, c->n_rows, qbb, &w->ctr->qcount[STORM_HIP_CELL_BB],
-                              &w->ctr->qhead[STORM_HIP_CELL_BB], gbb, tri, w->sum));
+    const QItem* qbb = p->queue + (uint64_t)STORM_HIP_CELL_BB * p->totalP;
+    if (p->tile == 256) {
+        // The MFMA kernel runs one 512-thread workgroup per CU.
+        const uint32_t gbb = (uint32_t)std::min<uint64_t>(P1, (uint64_t)ctx->cu_count);
+        // 256-row B x B tiles run on the matrix cores: fp4 where exact unless
+        // int8 is asked for.
+        SH_TRY(launch_bb_mfma_queue(fp4_ok && o.bb_kernel != STORM_HIP_BB_MFMA, s, p->d_corpora, max_words, min_ld, qbb,
+                                    &p->ctr->qcount[STORM_HIP_CELL_BB],
+                                    &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr));
+    } else {
+        const uint32_t gbb = (uint32_t)std::min<uint64_t>(P1, (uint64_t)ctx->cu_count * 2);
+        SH_TRY(launch_bb_valu_queue(s, (int)p->tile, p->d_corpora, qbb,
+                                    &p->ctr->qcount[STORM_HIP_CELL_BB],
+                                    &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr));
+    }
     if (timed) SH_HIP(hipEventRecord(ctx->ev[3], s));
+    return STORM_HIP_OK;
+}
+
+// Run the plan (graph or eager), then read sums, counters, decisions, latencies.
+int run_plan(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, storm_hip_stats* st,
+             uint64_t* sums, double* latency_ms) {
+    hipStream_t s = ctx->stream;
+    if (o.use_graph) {
+        const uint32_t key[4] = {o.policy, o.fixed_cell, o.bb_kernel, ctx->model_version};
+        if (!p->graph || std::memcmp(key, p->graph_key, sizeof key) != 0) {
+            if (p->graph) { (void)hipGraphExecDestroy(p->graph); p->graph = nullptr; }
+            hipGraph_t g;
+            SH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_selecting(ctx, p, o, false);
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (rc != STORM_HIP_OK) {
+                if (e == hipSuccess) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) { set_error("graph capture: %s", hipGetErrorString(e)); return STORM_HIP_ERR_DEVICE; }
+            e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { set_error("graph instantiate: %s", hipGetErrorString(e)); return STORM_HIP_ERR_DEVICE; }
+            std::memcpy(p->graph_key, key, sizeof key);
+        }
+        SH_HIP(hipGraphLaunch(p->graph, s));
+    } else {
+        SH_TRY(enqueue_selecting(ctx, p, o, true));
+    }
+    const size_t n = p->cs.size();
+    std::vector<unsigned long long> hs(n), hf(n + 1);
+    DevCounters h;
+    SH_HIP(hipMemcpyAsync(&h, p->ctr, sizeof h, hipMemcpyDeviceToHost, s));
+    if (n) SH_HIP(hipMemcpyAsync(hs.data(), p->sums, 8 * n, hipMemcpyDeviceToHost, s));
+    SH_HIP(hipMemcpyAsync(hf.data(), p->finish, 8 * (n + 1), hipMemcpyDeviceToHost, s));
+    for (size_t k = 0; k < n; ++k) {
+        storm_hip_corpus* c = p->cs[k];
+        const Work* w = p->works[k];
+        c->last_cells.resize(w->P);
+        if (w->P) SH_HIP(hipMemcpyAsync(c->last_cells.data(), w->decisions, w->P, hipMemcpyDeviceToHost, s));
+    }
+    SH_HIP(hipStreamSynchronize(s));
+    uint64_t total = 0, decisions = 0;
+    for (size_t k = 0; k < n; ++k) {
+        total += hs[k];
+        if (sums) sums[k] = hs[k];
+        // wall_clock64 runs at 100 MHz; a corpus with no tile keeps finish 0.
+        if (latency_ms) latency_ms[k] = hf[k] > hf[n] ? (double)(hf[k] - hf[n]) * 1e-5 : 0.0;
+        decisions += p->works[k]->P;
+    }
+    st->sum = total;
+    for (int k = 0; k < kNumCells; ++k) {
+        st->cell_pairs[k] = h.cell_pairs[k];
+        st->cell_tiles[k] = h.cell_tiles[k];
+    }
+    st->skipped = h.skipped;
+    st->fallback_tiles = h.fallback_tiles;
+    st->decisions = decisions;
+    st->bb_launches = h.qcount[STORM_HIP_CELL_BB] ? 1 : 0;
+    if (!o.use_graph) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
+        st->ms_select = ms;
+        hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+        st->ms_bb = ms;
+    }
+    return STORM_HIP_OK;
+}
+
+int check_tile(uint32_t tile) {
+    if (tile != 64 && tile != 128 && tile != 256) {
+        set_error("tile must be 64, 128 or 256 rows (got %u)", tile);
+        return STORM_HIP_ERR_INVALID;
+    }
     return STORM_HIP_OK;
 }
 
 }  // namespace
 
 void general_free(storm_hip_corpus* c) {
+    free_plan(static_cast<Plan*>(c->plan));
+    c->plan = nullptr;
     free_work(static_cast<Work*>(c->work));
     c->work = nullptr;
     if (!c->dense) general_free_arenas(c);
@@ -183,10 +376,7 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
                      storm_hip_stats* st, unsigned long long* d_sum) {
     const storm_hip_options& o = *opt;
     const uint32_t tile = o.tile ? o.tile : 64;
-    if (tile != 64 && tile != 128 && tile != 256) {
-        set_error("tile must be 64, 128 or 256 rows (got %u)", tile);
-        return STORM_HIP_ERR_INVALID;
-    }
+    SH_TRY(check_tile(tile));
     hipStream_t s = ctx->stream;
     uint32_t* tri = (o.out_mode == STORM_HIP_OUT_UPPER_TRI) ? c->d_tri : nullptr;
     if (c->n_rows < 2) return STORM_HIP_OK;
@@ -195,8 +385,8 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
         // Every pair B x B in identity order, no span test (storm_allpairs.cpp:
         // AllBitmap), bitmaps materialised on the device where missing.
         SH_TRY(general_materialize(ctx, c));
-        SH_TRY(ensure_work(c, 256));
-        Work* w = static_cast<Work*>(c->work);
+        Work* w = nullptr;
+        SH_TRY(ensure_bitmap_rowmap(c, &w));
         SH_TRY(launch_compose_rowmap(s, c->d_ident, c->d_bm_row, w->npad, w->rowmap));
         SH_HIP(hipEventRecord(ctx->ev[2], s));
         SH_TRY(launch_bb_dense(o.bb_kernel, s, c->d_bitmap, c->ld, c->n_words, w->rowmap,
@@ -215,51 +405,14 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
         set_error("the exhaustive policy is driven through storm_hip_selector_sample");
         return STORM_HIP_ERR_UNSUPPORTED;
     }
-
-    SH_TRY(ensure_work(c, tile));
-    Work* w = static_cast<Work*>(c->work);
-    if (o.use_graph) {
-        const uint32_t key[4] = {o.policy, o.fixed_cell, tile, ctx->model_version};
-        if (!w->graph || std::memcmp(key, w->graph_key, sizeof key) != 0 || w->graph_tri != tri) {
-            if (w->graph) { (void)hipGraphExecDestroy(w->graph); w->graph = nullptr; }
-            hipGraph_t g;
-            SH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            const int rc = enqueue_selecting(ctx, c, o, w, tri, false);
-            hipError_t e = hipStreamEndCapture(s, &g);
-            if (rc != STORM_HIP_OK) return rc;
-            if (e != hipSuccess) { set_error("graph capture: %s", hipGetErrorString(e)); return STORM_HIP_ERR_DEVICE; }
-            e = hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) { set_error("graph instantiate: %s", hipGetErrorString(e)); return STORM_HIP_ERR_DEVICE; }
-            std::memcpy(w->graph_key, key, sizeof key);
-            w->graph_tri = tri;
-        }
-        SH_HIP(hipGraphLaunch(w->graph, s));
-    } else {
-        SH_TRY(enqueue_selecting(ctx, c, o, w, tri, true));
-    }
-    // Fold the path's sum into the caller's accumulator and read the counters.
-    SH_HIP(hipMemcpyAsync(d_sum, w->sum, 8, hipMemcpyDeviceToDevice, s));
-    DevCounters h;
-    SH_HIP(hipMemcpyAsync(&h, w->ctr, sizeof h, hipMemcpyDeviceToHost, s));
-    c->last_cells.resize(w->P);
-    if (w->P) SH_HIP(hipMemcpyAsync(c->last_cells.data(), w->decisions, w->P, hipMemcpyDeviceToHost, s));
-    SH_HIP(hipStreamSynchronize(s));
-    for (int k = 0; k < kNumCells; ++k) {
-        st->cell_pairs[k] = h.cell_pairs[k];
-        st->cell_tiles[k] = h.cell_tiles[k];
-    }
-    st->skipped = h.skipped;
-    st->fallback_tiles = h.fallback_tiles;
-    st->decisions = w->P;
-    st->bb_launches = h.qcount[STORM_HIP_CELL_BB] ? 1 : 0;
-    if (!o.use_graph) {
-        float ms = 0;
-        hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
-        st->ms_select = ms;
-        hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
-        st->ms_bb = ms;
-    }
+    Plan* p = static_cast<Plan*>(c->plan);
+    const int rc = ensure_plan(p, {c}, {tri}, tile);
+    c->plan = p;
+    SH_TRY(rc);
+    uint64_t sum = 0;
+    SH_TRY(run_plan(ctx, p, o, st, &sum, nullptr));
+    // Fold into the caller's accumulator.
+    SH_HIP(hipMemcpyAsync(d_sum, p->sums, 8, hipMemcpyDeviceToDevice, s));
     return STORM_HIP_OK;
 }
 
@@ -267,7 +420,77 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
 
 using namespace stormhip;
 
+struct storm_hip_batch {
+    storm_hip_ctx* ctx = nullptr;
+    std::vector<storm_hip_corpus*> cs;
+    Plan* plan = nullptr;
+};
+
 extern "C" {
+
+int storm_hip_batch_create(storm_hip_ctx* ctx, storm_hip_corpus* const* corpora, uint32_t n,
+                           storm_hip_batch** out) {
+    if (!ctx || !out || (!corpora && n)) return STORM_HIP_ERR_INVALID;
+    for (uint32_t k = 0; k < n; ++k)
+        if (!corpora[k] || corpora[k]->ctx != ctx || corpora[k]->dense) {
+            set_error("batch member %u is not a general corpus of this context", k);
+            return STORM_HIP_ERR_INVALID;
+        }
+    storm_hip_batch* b = new (std::nothrow) storm_hip_batch();
+    if (!b) return STORM_HIP_ERR_NOMEM;
+    b->ctx = ctx;
+    b->cs.assign(corpora, corpora + n);
+    *out = b;
+    return STORM_HIP_OK;
+}
+
+void storm_hip_batch_free(storm_hip_batch* b) {
+    if (!b) return;
+    if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    free_plan(b->plan);
+    delete b;
+}
+
+int storm_hip_batch_allpairs(storm_hip_ctx* ctx, storm_hip_batch* b, const storm_hip_options* opt,
+                             storm_hip_stats* st, uint64_t* sums, double* latency_ms) {
+    if (!ctx || !b || b->ctx != ctx || !opt) return STORM_HIP_ERR_INVALID;
+    const storm_hip_options& o = *opt;
+    if (o.policy == STORM_HIP_POLICY_ALL_BITMAP || o.policy == STORM_HIP_POLICY_EXHAUSTIVE) {
+        set_error("batched calls select per tile (analytic, learned or fixed)");
+        return STORM_HIP_ERR_UNSUPPORTED;
+    }
+    const uint32_t tile = o.tile ? o.tile : 64;
+    SH_TRY(check_tile(tile));
+    SH_HIP(hipSetDevice(ctx->device));
+    storm_hip_stats local{};
+    storm_hip_stats& s = st ? *st : local;
+    std::memset(&s, 0, sizeof s);
+    std::vector<uint32_t*> tris(b->cs.size(), nullptr);
+    for (size_t k = 0; k < b->cs.size(); ++k) {
+        storm_hip_corpus* c = b->cs[k];
+        const uint64_t n = c->n_rows, pairs = n ? n * (n - 1) / 2 : 0;
+        s.pairs += pairs;
+        c->tri_valid = false;
+        if (o.out_mode == STORM_HIP_OUT_UPPER_TRI) {
+            if (c->tri_count != pairs) {
+                if (c->d_tri) { SH_HIP(hipFree(c->d_tri)); c->d_tri = nullptr; }
+                if (pairs) SH_HIP(hipMalloc(&c->d_tri, pairs * 4));
+                c->tri_count = pairs;
+            }
+            tris[k] = c->d_tri;
+        }
+    }
+    SH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    SH_TRY(ensure_plan(b->plan, b->cs, tris, tile));
+    SH_TRY(run_plan(ctx, b->plan, o, &s, sums, latency_ms));
+    SH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    SH_HIP(hipEventSynchronize(ctx->ev[1]));
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    s.ms_device = ms;
+    for (size_t k = 0; k < b->cs.size(); ++k) b->cs[k]->tri_valid = tris[k] != nullptr;
+    return STORM_HIP_OK;
+}
 
 int storm_hip_set_cost_model(storm_hip_ctx* ctx, const storm_hip_cost_model* m) {
     if (!ctx || !m) return STORM_HIP_ERR_INVALID;
@@ -298,40 +521,36 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
                               uint32_t* n_sampled) {
     if (!ctx || !c || c->ctx != ctx || c->dense || !n_sampled) return STORM_HIP_ERR_INVALID;
     if (max_tiles && (!features || !log2_ns || !mask || !tile_index)) return STORM_HIP_ERR_NULL_VALUES;
-    if (tile != 64 && tile != 128 && tile != 256) return STORM_HIP_ERR_INVALID;
+    SH_TRY(check_tile(tile));
     *n_sampled = 0;
     if (c->n_rows < 2 || max_tiles == 0) return STORM_HIP_OK;
     SH_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    SH_TRY(ensure_work(c, tile));
-    Work* w = static_cast<Work*>(c->work);
-    const int F = STORM_HIP_MLP_FEATURES;
-    // Tile metas, features and the analytic pass (for the proven-empty tiles).
+    // One analytic pass finds the proven-empty tiles and fills the tile metas.
+    Plan* p = static_cast<Plan*>(c->plan);
+    int rc = ensure_plan(p, {c}, {nullptr}, tile);
+    c->plan = p;
+    SH_TRY(rc);
     storm_hip_options o{};
     o.policy = STORM_HIP_POLICY_ANALYTIC;
     o.tile = tile;
-    const DevCorpus D = dev_corpus(c, c->d_perm);
-    const DevModel M = dev_model(ctx->model);
-    const uint32_t reps = (c->has_runs ? 1u : 0u) | (c->has_ewah ? 2u : 0u) | (c->has_comp ? 4u : 0u);
-    SH_HIP(hipMemsetAsync(w->ctr, 0, sizeof(DevCounters), s));
-    SH_TRY(launch_tile_meta(s, D, w->tile, w->T, w->tm));
-    SH_TRY(launch_select(s, w->tm, w->T, M, reps, o.policy, 0, nullptr, nullptr, w->decisions,
-                         w->queue, w->ctr));
-    SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
+    storm_hip_stats st{};
+    SH_TRY(run_plan(ctx, p, o, &st, nullptr, nullptr));
+    const Work* w = p->works[0];
+    const int F = STORM_HIP_MLP_FEATURES;
     float* dX = nullptr;
     SH_HIP(hipMalloc(&dX, sizeof(float) * F * w->P));
     std::vector<float> X((size_t)F * w->P);
     std::vector<TileMeta> tm(w->T);
-    std::vector<uint8_t> dec(w->P);
-    int rc = launch_features(s, w->tm, w->T, w->P, c->universe, dX);
+    rc = launch_features(s, w->tm, w->T, w->P, c->universe, dX);
     if (rc == STORM_HIP_OK) {
         (void)hipMemcpyAsync(X.data(), dX, X.size() * 4, hipMemcpyDeviceToHost, s);
         (void)hipMemcpyAsync(tm.data(), w->tm, tm.size() * sizeof(TileMeta), hipMemcpyDeviceToHost, s);
-        (void)hipMemcpyAsync(dec.data(), w->decisions, w->P, hipMemcpyDeviceToHost, s);
         if (hipStreamSynchronize(s) != hipSuccess) rc = STORM_HIP_ERR_DEVICE;
     }
     (void)hipFree(dX);
     if (rc != STORM_HIP_OK) return rc;
+    const std::vector<uint8_t>& dec = c->last_cells;
     // Seeded sample of the non-empty tile pairs.
     std::vector<uint32_t> cand;
     for (uint64_t t = 0; t < w->P; ++t)
@@ -344,47 +563,48 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
     if (cand.size() > max_tiles) cand.resize(max_tiles);
     std::sort(cand.begin(), cand.end());
     const uint32_t S = (uint32_t)cand.size();
-    std::vector<uint2> pairs(S);
+    std::vector<QItem> items(S);
     std::vector<uint32_t> elig(S);
+    const uint32_t reps = reps_of(c);
     for (uint32_t k = 0; k < S; ++k) {
         uint32_t bi, bj;
         tri_tile(cand[k], w->T, bi, bj);
-        pairs[k] = make_uint2(bi, bj);
+        items[k] = QItem{0, bi, bj, 0};
         elig[k] = eligible_mask(tm[bi], tm[bj], reps);
         tile_index[k] = cand[k];
         std::memcpy(features + (size_t)k * F, &X[(size_t)cand[k] * F], F * 4);
     }
-    // Time every eligible cell on its sampled tiles, each launch alone.
+    // Time every eligible cell on its sampled tiles, each class alone.
     unsigned long long* dticks = nullptr;
     SH_HIP(hipMalloc(&dticks, sizeof(unsigned long long) * std::max<uint32_t>(S, 1)));
     std::vector<unsigned long long> ticks(S);
     std::vector<double> best((size_t)S * 10, 0.0);
     std::vector<uint8_t> has((size_t)S * 10, 0);
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(S, 1), (uint64_t)ctx->cu_count * 4);
+    const DevModel M = dev_model(ctx->model);
+    const bool fp4 = c->n_words < (1u << 18);   // counts < 2^24: exact in f32
     for (int cell = 0; cell < 10 && rc == STORM_HIP_OK; ++cell) {
-        std::vector<uint2> q;
+        std::vector<QItem> q;
         std::vector<uint32_t> slot_of;
         for (uint32_t k = 0; k < S; ++k)
-            if (elig[k] >> cell & 1) { q.push_back(pairs[k]); slot_of.push_back(k); }
+            if (elig[k] >> cell & 1) { q.push_back(items[k]); slot_of.push_back(k); }
         if (q.empty()) continue;
-        uint2* dq = w->queue + (uint64_t)cell * w->P;
-        (void)hipMemcpyAsync(dq, q.data(), q.size() * sizeof(uint2), hipMemcpyHostToDevice, s);
+        QItem* dq = p->queue + (uint64_t)cell * p->totalP;
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(q.size(), (uint64_t)ctx->cu_count * (cell == 0 && tile == 256 ? 1 : 4));
+        (void)hipMemcpyAsync(dq, q.data(), q.size() * sizeof(QItem), hipMemcpyHostToDevice, s);
         for (uint32_t r = 0; r < std::max(1u, repeats) && rc == STORM_HIP_OK; ++r) {
             DevCounters h{};
             h.qcount[cell] = (uint32_t)q.size();
-            (void)hipMemcpyAsync(w->ctr, &h, sizeof h, hipMemcpyHostToDevice, s);
+            (void)hipMemcpyAsync(p->ctr, &h, sizeof h, hipMemcpyHostToDevice, s);
             if (cell == STORM_HIP_CELL_BB) {
-                if (w->tile == 256)
-                    rc = launch_bb_mfma(s, c->d_bitmap, c->ld, c->n_words, w->rowmap, c->d_perm,
-                                        c->n_rows, dq, &w->ctr->qcount[cell], &w->ctr->qhead[cell],
-                                        grid, nullptr, w->sum, dticks);
+                if (tile == 256)
+                    rc = launch_bb_mfma_queue(fp4, s, p->d_corpora, c->n_words, c->ld, dq,
+                                              &p->ctr->qcount[cell], &p->ctr->qhead[cell], grid,
+                                              dticks);
                 else
-                    rc = launch_bb_valu(s, (int)w->tile, c->d_bitmap, c->ld, c->n_words, w->rowmap,
-                                        c->d_perm, c->n_rows, dq, &w->ctr->qcount[cell],
-                                        &w->ctr->qhead[cell], grid, nullptr, w->sum, dticks);
+                    rc = launch_bb_valu_queue(s, (int)tile, p->d_corpora, dq, &p->ctr->qcount[cell],
+                                              &p->ctr->qhead[cell], grid, dticks);
             } else {
-                rc = launch_cell(cell, s, grid, D, M, w->tm, w->tile, dq, w->ctr, nullptr, w->sum,
-                                 dticks);
+                rc = launch_cell(cell, s, grid, p->d_corpora, M, dq, p->ctr, dticks);
             }
             if (rc != STORM_HIP_OK) break;
             (void)hipMemcpyAsync(ticks.data(), dticks, q.size() * 8, hipMemcpyDeviceToHost, s);
@@ -421,8 +641,12 @@ int storm_hip_calibrate(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
     uint32_t n = 0;
     SH_TRY(storm_hip_selector_sample(ctx, c, tile, max_tiles, seed, repeats, X.data(), Y.data(),
                                      Mk.data(), idx.data(), &n));
-    Work* w = static_cast<Work*>(c->work);
-    if (!w) return STORM_HIP_ERR_INVALID;
+    const Plan* plan = static_cast<const Plan*>(c->plan);
+    if (n == 0 || !plan) {
+        set_error("calibrate: no non-empty tile pair to time");
+        return STORM_HIP_ERR_INVALID;
+    }
+    const Work* w = plan->works[0];
     std::vector<TileMeta> tm(w->T);
     SH_HIP(hipMemcpy(tm.data(), w->tm, tm.size() * sizeof(TileMeta), hipMemcpyDeviceToHost));
     DeviceCostModel m = ctx->model;

@@ -92,7 +92,8 @@ struct storm_hip_corpus {
     uint64_t device_bytes = 0;
     std::vector<uint32_t> h_meta;    // host copy of meta (8 per row)
     bool bitmaps_complete = false;
-    void* work = nullptr;            // per-corpus scratch of the general path
+    void* work = nullptr;            // per-corpus selection state of the general path
+    void* plan = nullptr;            // cached launch plan of its single-corpus calls
     // Result of the last UPPER_TRI call.
     uint32_t* d_tri = nullptr;
     uint64_t tri_count = 0;
@@ -102,19 +103,15 @@ struct storm_hip_corpus {
 };
 
 namespace stormhip {
+// Grid launches over all upper-triangle tiles of one bitmap matrix.
 int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n, const uint2* queue,
-                   const unsigned int* qcount, unsigned int* qhead, uint32_t grid, uint32_t* tri,
-                   unsigned long long* sum, unsigned long long* tile_ticks = nullptr);
-int launch_bb_mfma(hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n, const uint2* queue,
-                   const unsigned int* qcount, unsigned int* qhead, uint32_t grid, uint32_t* tri,
-                   unsigned long long* sum, unsigned long long* tile_ticks = nullptr);
+                   const uint32_t* rowmap, const uint32_t* perm, uint32_t n, uint32_t* tri,
+                   unsigned long long* sum);
 int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                       const uint32_t* rowmap, const uint32_t* perm, uint32_t n,
-                       const uint2* queue, const unsigned int* qcount, unsigned int* qhead,
-                       uint32_t grid, uint32_t* tri, unsigned long long* sum,
-                       unsigned long long* tile_ticks = nullptr, uint32_t k_split = 1);
+                       const uint32_t* rowmap, const uint32_t* perm, uint32_t n, uint32_t* tri,
+                       unsigned long long* sum, uint32_t k_split = 1);
+// Queue launches over tiles of several corpora (declared with the device
+// corpus view in general.h).
 // The B x B form for a dense launch: VALU tile of 128, or MFMA (int8 / fp4).
 // AUTO takes fp4 where its exactness bound holds, else int8. Small corpora
 // split K across blocks so the launch fills `cu_count` CUs.

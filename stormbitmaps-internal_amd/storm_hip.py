@@ -123,6 +123,10 @@ def lib():
         "storm_hip_selector_grad": (i32, [vp, vp, vp, vp, u32, vp, C.POINTER(C.c_float)]),
         "storm_hip_selector_adam_step": (i32, [vp, C.c_float]),
         "storm_hip_calibrate": (i32, [vp, vp, u32, u32, u64, u32, C.POINTER(CostModel)]),
+        "storm_hip_batch_create": (i32, [vp, vp, u32, pvp]),
+        "storm_hip_batch_allpairs": (i32, [vp, vp, C.POINTER(Options), C.POINTER(Stats), vp,
+                                           vp]),
+        "storm_hip_batch_free": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -196,6 +200,9 @@ class Context:
         check(lib().storm_hip_corpus_from_lists(self.h, _ptr(offs), _ptr(pos), n, universe,
                                                 bitmap_policy, C.byref(h)), "from_lists")
         return Corpus(self, h, n)
+
+    def batch(self, corpora):
+        return Batch(self, corpora)
 
     # -- learned selector ------------------------------------------------------
     def selector_sample(self, corpus, tile=64, max_tiles=512, seed=1, repeats=2):
@@ -330,6 +337,43 @@ class Corpus:
         check(lib().storm_hip_last_decisions(self.ctx.h, self.h, _ptr(out), n_tiles),
               "last_decisions")
         return out
+
+
+class Batch:
+    """Independent general corpora run through one set of launches
+    (storm_hip_batch_*). The corpora must outlive the batch."""
+
+    def __init__(self, ctx, corpora):
+        self.ctx, self.corpora = ctx, list(corpora)
+        hs = (C.c_void_p * max(1, len(self.corpora)))(*[c.h for c in self.corpora])
+        self.h = C.c_void_p()
+        check(lib().storm_hip_batch_create(ctx.h, hs, len(self.corpora), C.byref(self.h)),
+              "batch_create")
+
+    def free(self):
+        if self.h:
+            lib().storm_hip_batch_free(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+    def allpairs(self, policy="analytic", out="sum", bb="auto", tile=64, fixed_cell=0,
+                 use_graph=False):
+        """Returns (stats dict, per-corpus sums, per-corpus latency in ms)."""
+        o = Options()
+        o.policy, o.out_mode, o.bb_kernel = POLICY[policy], OUT[out], BB[bb]
+        o.tile, o.fixed_cell, o.use_graph = tile, fixed_cell, int(use_graph)
+        st = Stats()
+        k = len(self.corpora)
+        sums = np.zeros(max(1, k), dtype=np.uint64)
+        lat = np.zeros(max(1, k), dtype=np.float64)
+        check(lib().storm_hip_batch_allpairs(self.ctx.h, self.h, C.byref(o), C.byref(st),
+                                             _ptr(sums), _ptr(lat)), "batch_allpairs")
+        return st.as_dict(), sums[:k], lat[:k]
 
 
 def tri_index(i, j, n):

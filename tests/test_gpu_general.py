@@ -155,3 +155,40 @@ def test_upload_rejects_bad_rows(sh, ctx):
         ctx.from_lists(offs, np.array([5, 5], np.uint32), 64)     # not distinct
     with pytest.raises(sh.StormHipError):
         ctx.from_lists(offs, np.array([1, 64], np.uint32), 64)    # out of range
+
+
+@pytest.mark.parametrize("tile", [64, 256])
+def test_batch_matches_single_calls(ctx, cases, tile):
+    """Many corpora (different universes and representations) through one set
+    of launches: each corpus's sum and triangle equal the oracle's, the stats
+    add up, and graph replays agree with eager runs."""
+    picks = [cases[k] for k in (0, 3, 5, 9, 14, 20, 27, 33, 41, 52, 60, 75)]
+    print("case batch tile", tile, [p[0] for p in picks], flush=True)
+    cs = [ctx.from_lists(offs, pos, u, bitmap_policy=(k % 3) | BUILD_ALL)
+          for k, (name, u, offs, pos, _, _) in enumerate(picks)]
+    try:
+        with ctx.batch(cs) as b:
+            for pol, graph in (("analytic", False), ("analytic", True), ("fixed", False),
+                               ("analytic", True), ("learned", False)):
+                st, sums, lat = b.allpairs(policy=pol, fixed_cell=4, out="tri", tile=tile,
+                                           use_graph=graph)
+                for c, (name, u, offs, pos, want_sum, want), s, l in zip(cs, picks, sums, lat):
+                    assert int(s) == want_sum, (name, pol, graph)
+                    assert np.array_equal(c.tri(), want), (name, pol, graph)
+                    assert l >= 0.0
+                assert st["sum"] == sum(p[4] for p in picks)
+                assert st["pairs"] == sum(c.n_pairs for c in cs)
+                assert sum(st["cell_pairs"].values()) == st["pairs"]
+                ctx.sync()
+            st, sums, _ = b.allpairs(policy="analytic", out="sum", tile=tile)
+            assert [int(s) for s in sums] == [p[4] for p in picks]
+    finally:
+        for c in cs:
+            c.free()
+
+
+def test_batch_rejects_dense_members(sh, ctx):
+    words = O.bernoulli_rows(1, 0.5, 128, 0, 4)
+    with ctx.upload_contig(words) as d:
+        with pytest.raises(sh.StormHipError):
+            ctx.batch([d])
