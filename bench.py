@@ -140,11 +140,16 @@ def sparse_bench(sh, args):
     progress(f"generated {n} rows, {len(pos)} elements in {t_gen:.1f} s")
     want = column_pair_sum(offs, pos, u)
     ctx = sh.Context(0)
-    flags = 0 | 4 | 8   # bitmaps for dense rows, runs, EWAH
+    # Every representation resident where HBM allows: bitmaps for every row
+    # (the all-bitmap store), runs, EWAH and complements; the selectors then
+    # choose among all cells per tile. Over the budget, bitmaps for dense rows.
+    bitmap_bytes = n * ((u + 63) // 64) * 8
+    all_bitmaps = bitmap_bytes <= args.bitmap_budget_gib << 30
+    flags = (1 if all_bitmaps else 0) | 4 | 8 | 16
     t0 = time.perf_counter()
     corpus = ctx.from_lists(offs, pos, u, bitmap_policy=flags)
     t_up = time.perf_counter() - t0
-    progress(f"built and uploaded in {t_up:.1f} s")
+    progress(f"built and uploaded in {t_up:.1f} s (bitmaps for every row: {all_bitmaps})")
     del pos
     pairs = n * (n - 1) // 2
     res = {}
@@ -165,16 +170,17 @@ def sparse_bench(sh, args):
                 "fallback_tiles": st["fallback_tiles"], "ms_select": st["ms_select"]}
 
     res["analytic_reference_rates"] = timed("analytic", use_graph=True)
+    # One sampling pass (every candidate cell timed on sampled tiles) feeds
+    # both the calibrated analytic model and the learned selector.
     t0 = time.perf_counter()
-    progress("calibrating")
-    model = ctx.calibrate(corpus, tile=args.tile, max_tiles=args.sample_tiles, seed=7, repeats=2)
+    progress("sampling tiles")
+    X, Y, M, idx = ctx.selector_sample(corpus, tile=args.tile, max_tiles=args.sample_tiles,
+                                       seed=11, repeats=2)
+    t_sample = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    model = ctx.calibrate_samples(corpus, args.tile, Y, M, idx)
     t_cal = time.perf_counter() - t0
     res["analytic"] = timed("analytic", use_graph=True)
-    t0 = time.perf_counter()
-    progress("sampling tiles for the selector")
-    X, Y, M, _ = ctx.selector_sample(corpus, tile=args.tile, max_tiles=args.sample_tiles,
-                                     seed=11, repeats=2)
-    t_sample = time.perf_counter() - t0
     t0 = time.perf_counter()
     progress(f"training on {len(X)} tiles")
     loss = ctx.selector_train(X, Y, M, epochs=300, batch=128, lr=3e-3, seed=5)
@@ -415,17 +421,36 @@ def dense_bench(sh, args):
     print(json.dumps(out), flush=True)
 
 
+def copy_probe(args):
+    """A plain device copy of known size (1 GiB read + 1 GiB written per
+    launch), for scaling the HBM byte counters a profiler reads on the B x B
+    kernel. Not a bench line: prints the bytes it moved."""
+    import torch
+    n = 1 << 28                                    # 2^28 int32 = 1 GiB
+    a = torch.ones(n, dtype=torch.int32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(max(1, args.steps)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": 4 * n, "copy_bytes_written_per_launch": 4 * n,
+                      "launches": max(1, args.steps)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="dense2",
-                    choices=sorted(CONFIGS) + sorted(SPARSE_CONFIGS) + sorted(BATCH_CONFIGS))
+                    choices=sorted(CONFIGS) + sorted(SPARSE_CONFIGS) + sorted(BATCH_CONFIGS)
+                    + ["copy"])
     ap.add_argument("--bb", default="auto", choices=["auto", "mfma", "mfma_fp4", "valu"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rows", type=int, default=0, help="sparse configs: first N rows only")
-    ap.add_argument("--tile", type=int, default=64)
+    ap.add_argument("--tile", type=int, default=256,
+                    help="rows per selection tile (sparse and batch configs)")
+    ap.add_argument("--bitmap-budget-gib", type=int, default=160,
+                    help="sparse configs: keep a bitmap of every row when they fit this")
     ap.add_argument("--sample-tiles", type=int, default=2048)
     ap.add_argument("--no-allbitmap", action="store_true")
     ap.add_argument("--instance-mib", type=int, default=256,
@@ -438,6 +463,9 @@ def main():
               "(the all-pairs triangle would shard by tile rows, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "copy":
+        copy_probe(args)
+        return
     sh = load_storm_hip()
     if args.config in SPARSE_CONFIGS:
         sparse_bench(sh, args)

@@ -219,6 +219,39 @@ int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c,
 int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                               uint64_t* row_sum);
 
+/* ---- launch shapes and the autotuner -------------------------------------
+ * No reference counterpart (the reference's kernels have fixed shapes).
+ * The matrix-core B x B kernel's launch shape: packed words per LDS stage
+ * (4 or 8), tile order (0 = row-major upper triangle, g > 0 = bands of g tile
+ * rows walked column by column, each XCD taking a contiguous range) and the
+ * K-split target (blocks per CU when a corpus has fewer 256 x 256 tiles than
+ * that). A context starts from {8, 4, 2}. */
+typedef struct {
+    uint32_t stage_words;
+    uint32_t band;
+    uint32_t blocks_per_cu;
+    uint32_t reserved;
+} storm_hip_bb_config;
+int storm_hip_set_bb_config(storm_hip_ctx* ctx, const storm_hip_bb_config* cfg);
+int storm_hip_get_bb_config(storm_hip_ctx* ctx, storm_hip_bb_config* cfg);
+
+typedef struct {
+    uint32_t tile;                 /* best selection tile (general corpora; 0 for dense) */
+    uint32_t n_tried;              /* candidates timed                                    */
+    storm_hip_bb_config bb;        /* best B x B launch shape (now the context's)         */
+    double ms_best;                /* device time of one call with the choice             */
+    double ms_default;             /* device time with the starting shape and tile        */
+} storm_hip_tune_result;
+/* Time candidate shapes on this corpus with `opt`'s policy and B x B kernel
+ * (each candidate runs the whole call once, sums only, after one warm-up)
+ * and keep the fastest: general corpora try selection tiles 64, 128 and 256,
+ * then, when the best tile puts B x B on the matrix cores (tile 256) or the
+ * corpus is dense, every stage width x tile order x K-split target. The
+ * context's B x B shape becomes the best one; pass result.tile as the tile
+ * of later calls. Counts never depend on the shape. */
+int storm_hip_autotune(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_options* opt,
+                       storm_hip_tune_result* result);
+
 /* ---- batches of independent corpora --------------------------------------
  * One call over many general corpora: every corpus is selected per tile
  * (analytic, learned or fixed policy), the tiles of all corpora share one
@@ -261,6 +294,12 @@ int storm_hip_get_cost_model(storm_hip_ctx* ctx, storm_hip_cost_model* m);
 int storm_hip_calibrate(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
                         uint32_t max_tiles, uint64_t seed, uint32_t repeats,
                         storm_hip_cost_model* out);
+/* The same fit from timings already drawn by storm_hip_selector_sample on
+ * this corpus at this tile width (n tiles: log2_ns, mask, tile_index as that
+ * call returned them), so one sampling pass serves both models. */
+int storm_hip_calibrate_samples(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
+                                uint32_t n, const float* log2_ns, const uint8_t* mask,
+                                const uint32_t* tile_index, storm_hip_cost_model* out);
 /* Per-tile decisions of the last selecting call: `cells[t]` for tile pair t
  * in row-major upper-triangle tile order. */
 int storm_hip_last_decisions(storm_hip_ctx* ctx, const storm_hip_corpus* c,

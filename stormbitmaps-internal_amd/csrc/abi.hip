@@ -278,8 +278,8 @@ int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
         // Every row is a bitmap: the only cell is B x B. All policies agree
         // here; the span test would only skip work (AllBitmap never applies it).
         SH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-        rc = launch_bb_dense(o.bb_kernel, ctx->stream, c->d_words, c->ld, c->n_words, nullptr,
-                             c->n_rows, tri, d_sum, (uint32_t)ctx->cu_count);
+        rc = launch_bb_dense(o.bb_kernel, ctx->mfma, ctx->stream, c->d_words, c->ld, c->n_words,
+                             nullptr, c->n_rows, tri, d_sum, (uint32_t)ctx->cu_count);
         SH_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
         s.cell_pairs[STORM_HIP_CELL_BB] = s.pairs;
         s.bb_launches = 1;

@@ -13,14 +13,19 @@
 // Structure: a workgroup of 8 waves owns a 256 x 256 block of pairs; wave w
 // owns rows 128*(w/4) .. +128 and columns 64*(w%4) .. +64 of it (4 x 2 MFMA
 // blocks of 32 x 32, 128 accumulator registers, two waves per SIMD). Per stage
-// of 4 words, every thread stages one row's packed bits in LDS (double
-// buffered, one barrier per stage), and every wave expands the packed bits of
-// its own fragments into 0/1 elements in registers right before its MFMAs.
+// of SW words (SW = 4 or 8), every thread stages one row's packed bits in LDS
+// (double buffered, one barrier per stage) exactly as loaded, and every wave
+// expands the packed bits of its own fragments into 0/1 elements in registers
+// right before its MFMAs. The next stage's global loads are issued before the
+// current stage's MFMAs and consumed one stage later, so their latency hides
+// behind a stage of matrix work.
 //
-// The expansion is cheap because the element ORDER inside a fragment is free:
-// A and B fragments are built by the same function from the same bit
-// positions, so each MFMA sums products over the same set of positions
-// whichever order the matrix core assigns to them. For a 32-bit chunk w:
+// Bits are assigned to matrix K positions freely: A and B fragments are built
+// by the same function from the same bit positions, so each MFMA sums products
+// over the same set of positions whichever order the matrix core assigns to
+// them. Lane group g (lanes 32g..32g+31) takes dwords g*SW .. g*SW+SW-1 of the
+// stage's 2*SW dwords (natural memory order), one 32-bit chunk per MFMA step;
+// for a chunk w:
 //   FP4: dword d (0..3) = (w >> (d-1)) & 0x22222222   (nibble n = bit 4n+d)
 //   I8 : two K=32 steps h = 0, 1 per chunk; dword d (0..3) of step h =
 //        (w >> (4h+d)) & 0x01010101                   (byte j = bit 8j+4h+d)
@@ -34,13 +39,14 @@ typedef int8_t i8x16 __attribute__((ext_vector_type(16)));
 typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
 typedef int32_t i32x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4 global_u32x4;
 
 namespace {
 constexpr int kTile = 256;        // pairs block: kTile x kTile
 constexpr int kThreads = 512;     // 8 waves
-constexpr int kWordsStage = 4;    // packed 64-bit words per stage per row
-constexpr int kRowDw = 12;        // LDS dwords per staged row (8 data + 4 pad)
 constexpr int kUnitScale = 127;   // E8M0 exponent of 1.0
+constexpr int kStageWords = 8;    // packed 64-bit words per stage per row (SW)
 
 __device__ inline i32x8 expand_fp4(uint32_t w) {
     return i32x8{(int)((w << 1) & 0x22222222u), (int)(w & 0x22222222u),
@@ -55,23 +61,34 @@ __device__ inline i8x16 expand_i8(uint32_t w, int g) {
     __builtin_memcpy(&r, tmp, 16);
     return r;
 }
+__device__ inline uint32_t comp(const uint4& v, int c) {
+    return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
+}
+
 }  // namespace
 
+template <int SW>
 struct BBMfmaSmem {
+    // LDS dwords per staged row: 2*SW data + 4 pad (conflict-free 16-byte
+    // reads down a column of rows).
+    static constexpr int kRowDw = 2 * SW + 4;
     // Packed bits of one stage: [buffer][row 0..511 (A rows, then B rows)][dw].
-    // Row layout: dwords 0..3 = low halves of words 0..3, 4..7 = high halves.
     uint32_t st[2][kThreads][kRowDw];
 };
 
-// One 256 x 256 block over stages [s0, s0 + n_stages) of the rows. With
+// One 256 x 256 block over stages [s0, s0 + n_stages) of SW words. With
 // `accumulate`, counts are atomically added to `tri` (a K-split block: other
 // blocks add the other stage ranges of the same pairs into a zeroed triangle).
-template <bool FP4>
-__device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem& sm, const uint64_t* __restrict__ X, uint32_t ld,
-                                 uint32_t s0, uint32_t n_stages, const uint32_t* __restrict__ rowmap,
-                                 const uint32_t* __restrict__ perm, uint32_t n, uint32_t bi,
-                                 uint32_t bj, uint32_t* __restrict__ tri, bool accumulate) {
+template <bool FP4, int SW>
+__device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint64_t* X, uint32_t ld,
+                                                 uint32_t s0, uint32_t n_stages,
+                                                 const uint32_t* __restrict__ rowmap,
+                                                 const uint32_t* __restrict__ perm, uint32_t n,
+                                                 uint32_t bi, uint32_t bj,
+                                                 uint32_t* __restrict__ tri, bool accumulate) {
     using Acc = typename std::conditional<FP4, f32x16, i32x16>::type;
+    constexpr int L = SW / 2;        // 16-byte loads per row per stage
+    constexpr int R = SW / 4;        // 16-byte LDS reads per fragment row per stage
     const uint32_t i0 = bi * kTile, j0 = bj * kTile;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
@@ -81,7 +98,8 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem& sm, const uint64_t*
     // Thread t stages row t of the A panel (t < 256) or row t-256 of B.
     const uint32_t pos = t < kTile ? i0 + t : j0 + (t - kTile);
     const uint64_t phys = rowmap ? rowmap[pos] : (uint64_t)pos;
-    const uint4* src = reinterpret_cast<const uint4*>(X + phys * ld) + 2 * (size_t)s0;
+    // Rows live in global memory: loads through this pointer are global loads.
+    global_u32x4* src = (global_u32x4*)(X + phys * ld) + (size_t)L * s0;
 
     Acc acc[4][2];
 #pragma unroll
@@ -91,40 +109,56 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem& sm, const uint64_t*
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[m][q][v] = 0;
 
-    auto stage_store = [&](int buf, const uint4& a, const uint4& b) {
-        // words: a = (w0.lo, w0.hi, w1.lo, w1.hi), b = (w2.lo, w2.hi, w3.lo, w3.hi)
-        *reinterpret_cast<uint4*>(&sm.st[buf][t][0]) = make_uint4(a.x, a.z, b.x, b.z);
-        *reinterpret_cast<uint4*>(&sm.st[buf][t][4]) = make_uint4(a.y, a.w, b.y, b.w);
+    // The next stage's words, in named registers (an array here would be
+    // placed in memory by the compiler).
+    u32x4 g0, g1, g2, g3;
+    auto gload = [&](uint32_t s) {
+        global_u32x4* p = src + (size_t)L * s;
+        g0 = p[0];
+        g1 = p[1];
+        if constexpr (L == 4) { g2 = p[2]; g3 = p[3]; }
+    };
+    auto stage_store = [&](int buf) {
+        u32x4* d = reinterpret_cast<u32x4*>(&sm.st[buf][t][0]);
+        d[0] = g0;
+        d[1] = g1;
+        if constexpr (L == 4) { d[2] = g2; d[3] = g3; }
     };
 
     __syncthreads();   // LDS free (previous tile of a persistent workgroup)
-    uint4 ga = src[0], gb = src[1];
-    stage_store(0, ga, gb);
-    if (n_stages > 1) { ga = src[2]; gb = src[3]; }
+    gload(0);
+    stage_store(0);
+    if (n_stages > 1) gload(1);
     __syncthreads();
     for (uint32_t s = 0; s < n_stages; ++s) {
         const int buf = s & 1;
         // This wave's packed fragments of the stage: 4 A rows, 2 B rows, each
-        // 4 chunks of this lane group's 32 bits.
-        uint4 pa[4], pb[2];
+        // SW chunks of this lane group's 32 bits.
+        uint4 pa[4][R], pb[2][R];
 #pragma unroll
         for (int m = 0; m < 4; ++m)
-            pa[m] = *reinterpret_cast<const uint4*>(&sm.st[buf][wr * 128 + m * 32 + lr][lg * 4]);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                pa[m][r] = *reinterpret_cast<const uint4*>(
+                    &sm.st[buf][wr * 128 + m * 32 + lr][lg * SW + 4 * r]);
 #pragma unroll
         for (int q = 0; q < 2; ++q)
-            pb[q] = *reinterpret_cast<const uint4*>(&sm.st[buf][kTile + wc * 64 + q * 32 + lr][lg * 4]);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                pb[q][r] = *reinterpret_cast<const uint4*>(
+                    &sm.st[buf][kTile + wc * 64 + q * 32 + lr][lg * SW + 4 * r]);
         // Stage the next words while this stage computes.
         if (s + 1 < n_stages) {
-            stage_store(buf ^ 1, ga, gb);
-            if (s + 2 < n_stages) { ga = src[2 * (s + 2)]; gb = src[2 * (s + 2) + 1]; }
+            stage_store(buf ^ 1);
+            if (s + 2 < n_stages) gload(s + 2);
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < SW; ++c) {
             uint32_t wa[4], wb[2];
 #pragma unroll
-            for (int m = 0; m < 4; ++m) wa[m] = c == 0 ? pa[m].x : c == 1 ? pa[m].y : c == 2 ? pa[m].z : pa[m].w;
+            for (int m = 0; m < 4; ++m) wa[m] = comp(pa[m][c >> 2], c & 3);
 #pragma unroll
-            for (int q = 0; q < 2; ++q) wb[q] = c == 0 ? pb[q].x : c == 1 ? pb[q].y : c == 2 ? pb[q].z : pb[q].w;
+            for (int q = 0; q < 2; ++q) wb[q] = comp(pb[q][c >> 2], c & 3);
             if constexpr (FP4) {
                 i32x8 fb[2];
 #pragma unroll
@@ -138,7 +172,7 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem& sm, const uint64_t*
                             fa, fb[q], acc[m][q], 4, 4, 0, kUnitScale, 0, kUnitScale);
                 }
             } else {
-                // 32 bits per lane group per chunk = two K=32 steps (g = 0, 1
+                // 32 bits per lane group per chunk = two K=32 steps (h = 0, 1
                 // select which nibble half of each byte position the lane takes).
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
@@ -185,28 +219,32 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem& sm, const uint64_t*
     return local;
 }
 
-template <bool FP4>
+template <bool FP4, int SW>
 __global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
     const uint64_t* __restrict__ X, uint32_t ld, uint32_t n_stages, uint32_t k_split,
-    const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm, uint32_t n,
-    uint32_t T, const DevCorpus* __restrict__ Ds, const QItem* __restrict__ queue,
+    uint32_t band, const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm,
+    uint32_t n, uint32_t T, const DevCorpus* __restrict__ Ds, const QItem* __restrict__ queue,
     const unsigned int* __restrict__ qcount, unsigned int* __restrict__ qhead,
     uint32_t* __restrict__ tri, unsigned long long* __restrict__ sum,
     unsigned long long* __restrict__ tile_ticks) {
-    __shared__ __attribute__((aligned(16))) BBMfmaSmem sm;
+    __shared__ __attribute__((aligned(16))) BBMfmaSmem<SW> sm;
     __shared__ uint32_t s_t;
     uint64_t local = 0;
     if (!qcount) {
-        // Grid mode: block b is slice b % k_split of upper-triangle tile b / k_split.
+        // Grid mode: position p = slice p % k_split of tile p / k_split, tiles
+        // in row-major upper-triangle order (band 0) or the banded order.
+        const uint32_t p = band ? xcd_position(blockIdx.x, gridDim.x) : blockIdx.x;
         uint32_t bi, bj;
-        tri_tile(blockIdx.x / k_split, T, bi, bj);
-        const uint32_t slice = blockIdx.x % k_split;
+        if (band) banded_tile(p / k_split, T, band, bi, bj);
+        else      tri_tile(p / k_split, T, bi, bj);
+        const uint32_t slice = p % k_split;
         const uint32_t per = (n_stages + k_split - 1) / k_split;
         // Clamp first so the stage count never goes through a wrapped difference.
         const uint32_t s0 = min(slice * per, n_stages);
         const uint32_t cnt = min(per, n_stages - s0);
         if (cnt)
-            local = bb_mfma_tile<FP4>(sm, X, ld, s0, cnt, rowmap, perm, n, bi, bj, tri, k_split > 1);
+            local = bb_mfma_tile<FP4, SW>(sm, X, ld, s0, cnt, rowmap, perm, n, bi, bj, tri,
+                                          k_split > 1);
     } else {
         for (;;) {
             __syncthreads();
@@ -217,9 +255,9 @@ __global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
             const QItem e = queue[t];
             const DevCorpus D = Ds[e.inst];
             const uint64_t t0 = wall_clock64();
-            const uint32_t ns = max(1u, (D.nw + kWordsStage - 1) / kWordsStage);
-            uint64_t part = bb_mfma_tile<FP4>(sm, D.bitmap, D.ld, 0, ns, D.rowmap, D.perm, D.n,
-                                              e.bi, e.bj, D.tri, false);
+            const uint32_t ns = max(1u, (D.nw + SW - 1) / SW);
+            uint64_t part = bb_mfma_tile<FP4, SW>(sm, D.bitmap, D.ld, 0, ns, D.rowmap, D.perm,
+                                                  D.n, e.bi, e.bj, D.tri, false);
             part = wave_sum_u64(part);
             if ((threadIdx.x & 63) == 0 && part) atomicAdd(D.sum, (unsigned long long)part);
             __syncthreads();
@@ -235,14 +273,17 @@ __global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
     if ((threadIdx.x & 63) == 0 && local) atomicAdd(sum, (unsigned long long)local);
 }
 
-// k_split > 1 (grid mode only): each tile's stages are split over k_split
-// blocks that atomically add into `tri`, which the caller has zeroed.
-// Exactness and layout checks shared by both launch modes.
-static int mfma_checks(bool fp4, uint32_t n_words, uint32_t ld) {
-    const uint32_t n_stages = (n_words + kWordsStage - 1) / kWordsStage;
+// Exactness and layout checks of one bitmap matrix (n_words per row, rows
+// ld words apart) for the matrix-core kernel with this stage width.
+int bb_mfma_check(bool fp4, uint32_t sw, uint32_t n_words, uint32_t ld) {
+    if (sw != 4 && sw != 8) {
+        set_error("bb_mfma: stage width %u words (4 or 8)", sw);
+        return STORM_HIP_ERR_INVALID;
+    }
+    const uint32_t n_stages = (n_words + sw - 1) / sw;
     // Two stages are read before the loop tests its bound: rows must hold them.
-    if ((uint64_t)std::max(n_stages, 2u) * kWordsStage > ld) {
-        set_error("bb_mfma: %u stages exceed row stride %u", n_stages, ld);
+    if ((uint64_t)std::max(n_stages, 2u) * sw > ld) {
+        set_error("bb_mfma: %u stages of %u words exceed row stride %u", n_stages, sw, ld);
         return STORM_HIP_ERR_INVALID;
     }
     // Exactness bounds: i32 accumulators below 2^31 bits; f32 below 2^24.
@@ -254,71 +295,119 @@ static int mfma_checks(bool fp4, uint32_t n_words, uint32_t ld) {
     return STORM_HIP_OK;
 }
 
-int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                       const uint32_t* rowmap, const uint32_t* perm, uint32_t n, uint32_t* tri,
-                       unsigned long long* sum, uint32_t k_split) {
-    SH_TRY(mfma_checks(fp4, n_words, ld));
+namespace {
+
+template <bool FP4, int SW>
+void launch_grid(hipStream_t s, uint32_t g, const uint64_t* X, uint32_t ld, uint32_t ns,
+                 uint32_t k_split, uint32_t band, const uint32_t* rowmap, const uint32_t* perm,
+                 uint32_t n, uint32_t T, uint32_t* tri, unsigned long long* sum) {
+    hipLaunchKernelGGL((bb_mfma_kernel<FP4, SW>), dim3(g), dim3(kThreads), 0, s, X, ld, ns, k_split,
+                       band, rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr, tri, sum,
+                       nullptr);
+}
+
+template <bool FP4, int SW>
+void launch_queue(hipStream_t s, uint32_t grid, const DevCorpus* Ds, const QItem* queue,
+                  const unsigned int* qcount, unsigned int* qhead, unsigned long long* tile_ticks) {
+    hipLaunchKernelGGL((bb_mfma_kernel<FP4, SW>), dim3(grid), dim3(kThreads), 0, s, nullptr, 0u, 1u,
+                       1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
+                       tile_ticks);
+}
+
+// Zero fill of a u32 array as a kernel.
+__global__ void zero_u32_kernel(uint32_t* p, uint64_t count) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count;
+         k += (uint64_t)gridDim.x * blockDim.x)
+        p[k] = 0;
+}
+
+}  // namespace
+
+// k_split > 1 (grid mode only): each tile's stages are split over k_split
+// blocks that atomically add into `tri`, which the caller has zeroed.
+int launch_bb_mfma_fmt(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
+                       uint32_t ld, uint32_t n_words, const uint32_t* rowmap, const uint32_t* perm,
+                       uint32_t n, uint32_t* tri, unsigned long long* sum, uint32_t k_split) {
+    const uint32_t sw = cfg.stage_words;
+    SH_TRY(bb_mfma_check(fp4, sw, n_words, ld));
     if (k_split == 0) k_split = 1;
-    const uint32_t n_stages = (n_words + kWordsStage - 1) / kWordsStage;
+    const uint32_t n_stages = (n_words + sw - 1) / sw;
     const uint32_t T = (n + kTile - 1) / kTile;
     const uint64_t g = (uint64_t)T * (T + 1) / 2 * k_split;
     if (g == 0) return STORM_HIP_OK;
+    if (g > 0xFFFFFFFFull) {
+        set_error("bb_mfma: %llu blocks exceed one launch", (unsigned long long)g);
+        return STORM_HIP_ERR_RANGE;
+    }
     const uint32_t ns = n_stages ? n_stages : 1;
-    if (fp4)
-        hipLaunchKernelGGL(bb_mfma_kernel<true>, dim3((uint32_t)g), dim3(kThreads), 0, s, X, ld, ns,
-                           k_split, rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr, tri,
-                           sum, nullptr);
-    else
-        hipLaunchKernelGGL(bb_mfma_kernel<false>, dim3((uint32_t)g), dim3(kThreads), 0, s, X, ld, ns,
-                           k_split, rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr, tri,
-                           sum, nullptr);
+    const uint32_t G = (uint32_t)g;
+    if (fp4 && sw == 8)  launch_grid<true, 8>(s, G, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum);
+    if (fp4 && sw == 4)  launch_grid<true, 4>(s, G, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum);
+    if (!fp4 && sw == 8) launch_grid<false, 8>(s, G, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum);
+    if (!fp4 && sw == 4) launch_grid<false, 4>(s, G, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }
 
-// Queue mode over the corpora in Ds; `max_words`/`min_ld` bound every corpus
-// of the call (the caller picks fp4 only when all of them allow it).
-int launch_bb_mfma_queue(bool fp4, hipStream_t s, const DevCorpus* Ds, uint32_t max_words,
-                         uint32_t min_ld, const QItem* queue, const unsigned int* qcount,
-                         unsigned int* qhead, uint32_t grid, unsigned long long* tile_ticks) {
-    SH_TRY(mfma_checks(fp4, max_words, min_ld));
+uint32_t bb_valu_blocks_per_cu(int bm);
+
+uint32_t bb_blocks_per_cu(uint32_t tile, const MfmaConfig& cfg) {
+    if (tile != 256) return bb_valu_blocks_per_cu((int)tile);
+    int nb = 0;
+    const hipError_t e = cfg.stage_words == 4
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bb_mfma_kernel<true, 4>, kThreads, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bb_mfma_kernel<true, 8>, kThreads, 0);
+    return (e == hipSuccess && nb > 0) ? (uint32_t)nb : 1;
+}
+
+// Queue mode over the corpora in Ds. The caller has checked every corpus that
+// can queue a tile with bb_mfma_check (and picks fp4 only when all allow it).
+int launch_bb_mfma_queue(bool fp4, const MfmaConfig& cfg, hipStream_t s, const DevCorpus* Ds,
+                         const QItem* queue, const unsigned int* qcount, unsigned int* qhead,
+                         uint32_t grid, unsigned long long* tile_ticks) {
+    const uint32_t sw = cfg.stage_words;
+    if (sw != 4 && sw != 8) {
+        set_error("bb_mfma: stage width %u words (4 or 8)", sw);
+        return STORM_HIP_ERR_INVALID;
+    }
     if (grid == 0) return STORM_HIP_OK;
-    if (fp4)
-        hipLaunchKernelGGL(bb_mfma_kernel<true>, dim3(grid), dim3(kThreads), 0, s, nullptr, 0u, 1u,
-                           1u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
-                           tile_ticks);
-    else
-        hipLaunchKernelGGL(bb_mfma_kernel<false>, dim3(grid), dim3(kThreads), 0, s, nullptr, 0u, 1u,
-                           1u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
-                           tile_ticks);
+    if (fp4 && sw == 8)  launch_queue<true, 8>(s, grid, Ds, queue, qcount, qhead, tile_ticks);
+    if (fp4 && sw == 4)  launch_queue<true, 4>(s, grid, Ds, queue, qcount, qhead, tile_ticks);
+    if (!fp4 && sw == 8) launch_queue<false, 8>(s, grid, Ds, queue, qcount, qhead, tile_ticks);
+    if (!fp4 && sw == 4) launch_queue<false, 4>(s, grid, Ds, queue, qcount, qhead, tile_ticks);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }
 
-int launch_bb_dense(uint32_t bb_kernel, hipStream_t s, const uint64_t* X, uint32_t ld,
-                    uint32_t n_words, const uint32_t* rowmap, uint32_t n, uint32_t* tri,
-                    unsigned long long* sum, uint32_t cu_count) {
+int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
+                    uint32_t ld, uint32_t n_words, const uint32_t* rowmap, uint32_t n,
+                    uint32_t* tri, unsigned long long* sum, uint32_t cu_count) {
     if (bb_kernel == STORM_HIP_BB_VALU)
         return launch_bb_valu(s, 128, X, ld, n_words, rowmap, nullptr, n, tri, sum);
     const bool fp4 = bb_kernel == STORM_HIP_BB_MFMA_FP4 ||
                      (bb_kernel == STORM_HIP_BB_AUTO && n_words < (1u << 18));
-    // Too few 256 x 256 blocks to fill the chip: split K so that about two
-    // blocks per CU run, each stage range at least 8 stages long.
+    // Too few 256 x 256 blocks to fill the chip: split K so that about
+    // blocks_per_cu blocks per CU run, each stage range at least 4 stages long.
     const uint32_t T = (n + kTile - 1) / kTile;
     const uint64_t tiles = (uint64_t)T * (T + 1) / 2;
-    const uint32_t n_stages = (n_words + kWordsStage - 1) / kWordsStage;
+    const uint32_t n_stages = (n_words + cfg.stage_words - 1) / cfg.stage_words;
+    const uint64_t want = (uint64_t)std::max(1u, cfg.blocks_per_cu) * cu_count;
     uint32_t k_split = 1;
-    if (tiles && n_stages && tiles < 2ull * cu_count) {
-        k_split = (uint32_t)std::min<uint64_t>((2ull * cu_count + tiles - 1) / tiles,
-                                               std::max(1u, n_stages / 8));
+    if (tiles && n_stages && tiles < want) {
+        k_split = (uint32_t)std::min<uint64_t>((want + tiles - 1) / tiles,
+                                               std::max(1u, n_stages / 4));
         k_split = std::max(1u, k_split);
         // No empty slices: as many slices as ceil(n_stages / per) needs.
         const uint32_t per = (n_stages + k_split - 1) / k_split;
         k_split = (n_stages + per - 1) / per;
     }
-    if (k_split > 1 && tri)
-        SH_HIP(hipMemsetAsync(tri, 0, (uint64_t)n * (n - 1) / 2 * 4, s));
-    return launch_bb_mfma_fmt(fp4, s, X, ld, n_words, rowmap, nullptr, n, tri, sum, k_split);
+    if (k_split > 1 && tri) {
+        const uint64_t count = (uint64_t)n * (n - 1) / 2;
+        hipLaunchKernelGGL(zero_u32_kernel, dim3((uint32_t)std::min<uint64_t>((count + 255) / 256, 65536)),
+                           dim3(256), 0, s, tri, count);
+        SH_HIP(hipGetLastError());
+    }
+    return launch_bb_mfma_fmt(fp4, cfg, s, X, ld, n_words, rowmap, nullptr, n, tri, sum, k_split);
 }
 
 }  // namespace stormhip

@@ -179,6 +179,15 @@ int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32
     return STORM_HIP_OK;
 }
 
+// Resident workgroups per CU of the VALU kernel for a tile width.
+uint32_t bb_valu_blocks_per_cu(int bm) {
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (bm == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bb_valu_kernel<128>, 256, 0);
+    if (bm == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bb_valu_kernel<64>, 256, 0);
+    return (e == hipSuccess && nb > 0) ? (uint32_t)nb : 2;
+}
+
 // Queue mode: `grid` persistent workgroups drain a device queue of tiles of
 // the corpora in Ds (every corpus's rows padded so ld covers its words).
 int launch_bb_valu_queue(hipStream_t s, int bm, const DevCorpus* Ds, const QItem* queue,

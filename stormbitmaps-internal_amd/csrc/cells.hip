@@ -22,6 +22,8 @@
 // Work split: a pair's work items (list elements, runs, markers) are spread
 // over the 64 lanes of a wave and wave-reduced ("wave mode"), or, for tiles
 // whose rows are tiny, each lane evaluates whole pairs ("lane mode").
+#include <atomic>
+
 #include "device_common.h"
 #include "general.h"
 
@@ -401,6 +403,104 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
     __syncthreads();
 }
 
+// ---- R x R and S x S with one row staged in LDS ------------------------------
+//
+// Per-pair search in global memory is a chain of dependent loads (a binary
+// search per run or element). Here the workgroup stages one row of the tile's
+// "staged" side in LDS (its run starts and ends, or its sorted positions) and
+// its waves take the other side's rows one by one: every lane streams that
+// row's runs or positions from global memory (coalesced) and searches the
+// LDS copy. The staged side is the one with more runs (elements) per row, so
+// the streamed side is the shorter one; a staged row must fit kStageItems.
+constexpr uint32_t kStageItems = 4096;
+
+struct StageSmem {
+    uint32_t a[kStageItems];   // run starts / sorted positions
+    uint32_t b[kStageItems];   // run ends (R x R)
+};
+
+template <int CELL>
+__device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint32_t ni,
+                            uint32_t ja, uint32_t nj, bool diag, bool stage_j,
+                            uint32_t* __restrict__ tri, uint64_t& local, uint64_t& pairs,
+                            uint64_t& skipped) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const uint32_t n = D.n;
+    const uint32_t xs0 = stage_j ? ja : ia, nx = stage_j ? nj : ni;
+    const uint32_t ys0 = stage_j ? ia : ja, ny = stage_j ? ni : nj;
+    for (uint32_t u = 0; u < nx; ++u) {
+        const uint32_t x = D.perm[xs0 + u];
+        __syncthreads();   // the previous staged row is no longer read
+        uint32_t nx_items;
+        if constexpr (CELL == STORM_HIP_CELL_RR) {
+            const RunsV R = runs_of(D, x);
+            nx_items = R.n;
+            for (uint32_t k = tid; k < R.n; k += blockDim.x) { S.a[k] = R.s[k]; S.b[k] = R.e[k]; }
+        } else {
+            const Span32 L = list_of(D, x);
+            nx_items = L.n;
+            for (uint32_t k = tid; k < L.n; k += blockDim.x) S.a[k] = L.p[k];
+        }
+        __syncthreads();
+        for (uint32_t v = wave; v < ny; v += nwaves) {
+            // Tile coordinates of the pair: (row in I, row in J).
+            const uint32_t ti = stage_j ? v : u, tj = stage_j ? u : v;
+            if (diag && tj <= ti) continue;
+            const uint32_t y = D.perm[ys0 + v];
+            uint32_t cnt = 0;
+            if (span_disjoint(D, x, y)) {
+                if (lane == 0) ++skipped;
+            } else {
+                uint32_t c = 0;
+                if constexpr (CELL == STORM_HIP_CELL_RR) {
+                    const RunsV Q = runs_of(D, y);
+                    for (uint32_t k = lane; k < Q.n; k += 64) {
+                        const uint32_t rs = Q.s[k], re = Q.e[k];
+                        uint32_t lo = 0, hi = nx_items;   // first staged run ending after rs
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (S.b[mid] <= rs) lo = mid + 1; else hi = mid;
+                        }
+                        for (uint32_t q = lo; q < nx_items && S.a[q] < re; ++q)
+                            c += min(re, S.b[q]) - max(rs, S.a[q]);
+                    }
+                } else {
+                    const Span32 Q = list_of(D, y);
+                    for (uint32_t k = lane; k < Q.n; k += 64) {
+                        const uint32_t e = Q.p[k];
+                        uint32_t lo = 0, hi = nx_items;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (S.a[mid] < e) lo = mid + 1; else hi = mid;
+                        }
+                        c += (lo < nx_items && S.a[lo] == e) ? 1u : 0u;
+                    }
+                }
+                cnt = wave_sum_u32(c);
+                if (lane == 0) ++pairs;
+            }
+            if (lane == 0) {
+                local += cnt;
+                if (tri) tri[x < y ? tri_index(x, y, n) : tri_index(y, x, n)] = cnt;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Which side of a tile pair to stage in LDS (-1: neither fits): the side with
+// more items per row, if its largest row fits.
+template <int CELL>
+__device__ inline int stage_side(const TileMeta& A, const TileMeta& B) {
+    const uint32_t ma = CELL == STORM_HIP_CELL_RR ? A.max_runs : A.max_card;
+    const uint32_t mb = CELL == STORM_HIP_CELL_RR ? B.max_runs : B.max_card;
+    const uint32_t ua = CELL == STORM_HIP_CELL_RR ? A.runs : A.card;
+    const uint32_t ub = CELL == STORM_HIP_CELL_RR ? B.runs : B.card;
+    const bool fa = ma <= kStageItems, fb = mb <= kStageItems;
+    if (fa && fb) return ub > ua ? 1 : 0;
+    return fa ? 0 : fb ? 1 : -1;
+}
+
 // The join wins over per-pair search unless the J rows dwarf the I rows.
 __device__ inline bool ss_use_join(const TileMeta& A, const TileMeta& B) {
     const double ci = A.card, cj = B.card;
@@ -441,13 +541,31 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         const bool lane_mode = CELL == STORM_HIP_CELL_EMPTY || (A.max_card <= 48 && A.max_runs <= 48);
         bool joined = false;
         if constexpr (CELL == STORM_HIP_CELL_SS) {
-            __shared__ JoinSmem js;
+            // The join and the staged search never run together: one LDS area.
+            __shared__ union { JoinSmem js; StageSmem ss; } sm;
             if (ss_use_join(A, D.tm[e.bj])) {
                 joined = true;
                 for (uint32_t si = 0; si < ni; si += 64)
                     for (uint32_t sj = diag ? si : 0; sj < nj; sj += 64)
-                        ss_tile_join(D, js, i0 + si, min(64u, ni - si), j0 + sj, min(64u, nj - sj),
-                                     diag && si == sj, tri, local, pairs, skipped);
+                        ss_tile_join(D, sm.js, i0 + si, min(64u, ni - si), j0 + sj,
+                                     min(64u, nj - sj), diag && si == sj, tri, local, pairs,
+                                     skipped);
+            } else if (!lane_mode) {
+                const int side = stage_side<CELL>(A, D.tm[e.bj]);
+                if (side >= 0) {
+                    joined = true;
+                    tile_staged<CELL>(D, sm.ss, i0, ni, j0, nj, diag, side == 1, tri, local,
+                                      pairs, skipped);
+                }
+            }
+        }
+        if constexpr (CELL == STORM_HIP_CELL_RR) {
+            __shared__ StageSmem ss;
+            const int side = stage_side<CELL>(A, D.tm[e.bj]);
+            if (!lane_mode && side >= 0) {
+                joined = true;
+                tile_staged<CELL>(D, ss, i0, ni, j0, nj, diag, side == 1, tri, local, pairs,
+                                  skipped);
             }
         }
         if (joined) {
@@ -504,6 +622,30 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
             atomicAdd(&ctr->skipped, (unsigned long long)skipped);
         }
     }
+}
+
+// Workgroups of a cell kernel resident on one CU at once (registers, LDS and
+// waves permitting), queried once per cell.
+uint32_t cell_blocks_per_cu(int cell) {
+    static std::atomic<int> cache[kNumCells];
+    if (cell < 0 || cell >= kNumCells) return 1;
+    int v = cache[cell].load(std::memory_order_relaxed);
+    if (v > 0) return (uint32_t)v;
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define SH_OCC(C) \
+    case C: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, cell_kernel<C>, 256, 0); break;
+    switch (cell) {
+        SH_OCC(STORM_HIP_CELL_BS) SH_OCC(STORM_HIP_CELL_BR) SH_OCC(STORM_HIP_CELL_BW)
+        SH_OCC(STORM_HIP_CELL_SS) SH_OCC(STORM_HIP_CELL_SR) SH_OCC(STORM_HIP_CELL_SW)
+        SH_OCC(STORM_HIP_CELL_RR) SH_OCC(STORM_HIP_CELL_RW) SH_OCC(STORM_HIP_CELL_WW)
+        SH_OCC(STORM_HIP_CELL_EMPTY) SH_OCC(STORM_HIP_CELL_CB)
+        default: break;
+    }
+#undef SH_OCC
+    v = (e == hipSuccess && nb > 0) ? nb : 4;
+    cache[cell].store(v, std::memory_order_relaxed);
+    return (uint32_t)v;
 }
 
 int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,

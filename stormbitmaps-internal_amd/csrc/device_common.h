@@ -29,6 +29,47 @@ __host__ __device__ inline void tri_tile(uint32_t t, uint32_t T, uint32_t& bi, u
     bj = (uint32_t)(b + (t - start(b)));
 }
 
+// Grid block -> position in a tile order. Workgroups are dispatched to the
+// 8 XCDs round-robin (block b on XCD b % 8), so block b takes position
+// (b % 8) * (G8 / 8) + b / 8 of the first G8 = 8 * floor(G / 8) positions:
+// each XCD walks a contiguous range of the order, and tiles running together
+// on one XCD are neighbours in it. The tail keeps its own order. A bijection
+// on [0, G).
+__host__ __device__ inline uint32_t xcd_position(uint32_t b, uint32_t G) {
+    const uint32_t G8 = G / 8 * 8;
+    if (b >= G8) return b;
+    return (b % 8) * (G8 / 8) + b / 8;
+}
+
+// Banded order of the upper-triangle tiles of a T x T grid: bands of `gh`
+// tile rows; inside a band, column by column (bj ascending), the band's rows
+// bi <= bj. Neighbours in this order share row panels both ways (a run of 32
+// tiles touches about gh + 32/gh panels instead of 33).
+__host__ __device__ inline void banded_tile(uint32_t v, uint32_t T, uint32_t gh, uint32_t& bi,
+                                            uint32_t& bj) {
+    uint64_t base = 0;
+    uint32_t lo = 0, h = 0;
+    for (lo = 0; lo < T; lo += gh) {
+        h = gh < T - lo ? gh : T - lo;
+        const uint64_t cnt = (uint64_t)h * (h + 1) / 2 + (uint64_t)h * (T - lo - h);
+        if (v < base + cnt) break;
+        base += cnt;
+    }
+    uint64_t off = v - base;
+    const uint64_t dsz = (uint64_t)h * (h + 1) / 2;
+    if (off < dsz) {
+        // Diagonal block: column c holds rows lo .. lo + c.
+        uint32_t c = 0;
+        while ((uint64_t)(c + 1) * (c + 2) / 2 <= off) ++c;
+        bj = lo + c;
+        bi = lo + (uint32_t)(off - (uint64_t)c * (c + 1) / 2);
+    } else {
+        off -= dsz;
+        bj = lo + h + (uint32_t)(off / h);
+        bi = lo + (uint32_t)(off % h);
+    }
+}
+
 __device__ inline uint64_t wave_sum_u64(uint64_t v) {
     for (int off = 32; off > 0; off >>= 1) {
         const uint32_t lo = __shfl_xor((uint32_t)v, off, 64);

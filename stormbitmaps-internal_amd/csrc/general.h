@@ -41,9 +41,14 @@ struct QItem { uint32_t inst, bi, bj, pad; };
 int launch_bb_valu_queue(hipStream_t s, int bm, const DevCorpus* Ds, const QItem* queue,
                          const unsigned int* qcount, unsigned int* qhead, uint32_t grid,
                          unsigned long long* tile_ticks);
-int launch_bb_mfma_queue(bool fp4, hipStream_t s, const DevCorpus* Ds, uint32_t max_words,
-                         uint32_t min_ld, const QItem* queue, const unsigned int* qcount,
-                         unsigned int* qhead, uint32_t grid, unsigned long long* tile_ticks);
+int launch_bb_mfma_queue(bool fp4, const MfmaConfig& cfg, hipStream_t s, const DevCorpus* Ds,
+                         const QItem* queue, const unsigned int* qcount, unsigned int* qhead,
+                         uint32_t grid, unsigned long long* tile_ticks);
+int bb_mfma_check(bool fp4, uint32_t stage_words, uint32_t n_words, uint32_t ld);
+// Workgroups resident per CU: a cell kernel; the B x B kernel of a tile width
+// (256 = matrix cores with this launch shape, else VALU).
+uint32_t cell_blocks_per_cu(int cell);
+uint32_t bb_blocks_per_cu(uint32_t tile, const MfmaConfig& cfg);
 
 // Aggregate metadata of one row tile (density-sorted rows [lo, hi)). The first
 // block is tile_meta of kernels/storm_allpairs.cpp:281-312; the rest are
@@ -89,6 +94,13 @@ struct DevModel {
     // carry a zone map; 0: always Theta(m) words, as the device B x B runs.
     double bb_zonemap;
 };
+
+// Learned selection ranks only cells predicted (analytically) within this
+// factor of the cheapest eligible cell on the tile.
+constexpr double kLearnedPrune = 32.0;
+__host__ __device__ double predict(const DevModel& m, int p, const TileMeta& a, const TileMeta& b);
+__host__ __device__ uint32_t learned_candidates(const DevModel& m, const TileMeta& a,
+                                                const TileMeta& b, uint32_t elig);
 
 // Counters written by the selection and cell kernels.
 struct DevCounters {

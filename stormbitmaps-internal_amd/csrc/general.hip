@@ -235,7 +235,7 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, p->finish + n);
     SH_HIP(hipGetLastError());
     if (timed) SH_HIP(hipEventRecord(ctx->ev[4], s));
-    uint32_t max_words = 0, min_ld = 0xFFFFFFFFu;
+    uint32_t max_words = 0;
     for (size_t k = 0; k < n; ++k) {
         storm_hip_corpus* c = p->cs[k];
         Work* w = p->works[k];
@@ -252,34 +252,33 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
                              w->decisions, p->queue, p->totalP, (uint32_t)k, p->ctr));
         SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
         // Only corpora with bitmap rows can queue B x B tiles.
-        if (c->n_bitmap_rows) {
-            max_words = std::max(max_words, c->n_words);
-            min_ld = std::min(min_ld, c->ld);
-        }
+        if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
     }
-    // fp4 with f32 accumulation is exact below 2^18 words (counts < 2^24).
-    const bool fp4_ok = max_words < (1u << 18);
-    if (min_ld == 0xFFFFFFFFu) min_ld = 4 * 2;
+    // fp4 with f32 accumulation is exact below 2^18 words (counts < 2^24);
+    // 256-row B x B tiles run on the matrix cores, fp4 unless int8 is asked for.
+    const bool fp4 = max_words < (1u << 18) && o.bb_kernel != STORM_HIP_BB_MFMA;
+    if (p->tile == 256)
+        for (storm_hip_corpus* c : p->cs)
+            if (c->n_bitmap_rows) SH_TRY(bb_mfma_check(fp4, ctx->mfma.stage_words, c->n_words, c->ld));
     if (timed) SH_HIP(hipEventRecord(ctx->ev[5], s));
     const uint64_t P1 = p->totalP ? p->totalP : 1;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(P1, (uint64_t)ctx->cu_count * 4);
+    // Persistent grids: as many workgroups as fit on the chip at once.
     for (int cell = 0; cell < kNumCells; ++cell) {
         if (cell == STORM_HIP_CELL_BB) continue;
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(
+            P1, (uint64_t)ctx->cu_count * cell_blocks_per_cu(cell));
         SH_TRY(launch_cell(cell, s, grid, p->d_corpora, M, p->queue + (uint64_t)cell * p->totalP,
                            p->ctr, nullptr));
     }
     if (timed) SH_HIP(hipEventRecord(ctx->ev[2], s));
     const QItem* qbb = p->queue + (uint64_t)STORM_HIP_CELL_BB * p->totalP;
+    const uint32_t gbb = (uint32_t)std::min<uint64_t>(
+        P1, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
     if (p->tile == 256) {
-        // The MFMA kernel runs one 512-thread workgroup per CU.
-        const uint32_t gbb = (uint32_t)std::min<uint64_t>(P1, (uint64_t)ctx->cu_count);
-        // 256-row B x B tiles run on the matrix cores: fp4 where exact unless
-        // int8 is asked for.
-        SH_TRY(launch_bb_mfma_queue(fp4_ok && o.bb_kernel != STORM_HIP_BB_MFMA, s, p->d_corpora, max_words, min_ld, qbb,
+        SH_TRY(launch_bb_mfma_queue(fp4, ctx->mfma, s, p->d_corpora, qbb,
                                     &p->ctr->qcount[STORM_HIP_CELL_BB],
                                     &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr));
     } else {
-        const uint32_t gbb = (uint32_t)std::min<uint64_t>(P1, (uint64_t)ctx->cu_count * 2);
         SH_TRY(launch_bb_valu_queue(s, (int)p->tile, p->d_corpora, qbb,
                                     &p->ctr->qcount[STORM_HIP_CELL_BB],
                                     &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr));
@@ -389,7 +388,7 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
         SH_TRY(ensure_bitmap_rowmap(c, &w));
         SH_TRY(launch_compose_rowmap(s, c->d_ident, c->d_bm_row, w->npad, w->rowmap));
         SH_HIP(hipEventRecord(ctx->ev[2], s));
-        SH_TRY(launch_bb_dense(o.bb_kernel, s, c->d_bitmap, c->ld, c->n_words, w->rowmap,
+        SH_TRY(launch_bb_dense(o.bb_kernel, ctx->mfma, s, c->d_bitmap, c->ld, c->n_words, w->rowmap,
                                c->n_rows, tri, d_sum, (uint32_t)ctx->cu_count));
         SH_HIP(hipEventRecord(ctx->ev[3], s));
         st->cell_pairs[STORM_HIP_CELL_BB] = st->pairs;
@@ -566,22 +565,30 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
     std::vector<QItem> items(S);
     std::vector<uint32_t> elig(S);
     const uint32_t reps = reps_of(c);
+    const DevModel M = dev_model(ctx->model);
     for (uint32_t k = 0; k < S; ++k) {
         uint32_t bi, bj;
         tri_tile(cand[k], w->T, bi, bj);
         items[k] = QItem{0, bi, bj, 0};
-        elig[k] = eligible_mask(tm[bi], tm[bj], reps);
+        // The cells the learned selector may rank on this tile.
+        elig[k] = learned_candidates(M, tm[bi], tm[bj], eligible_mask(tm[bi], tm[bj], reps));
         tile_index[k] = cand[k];
         std::memcpy(features + (size_t)k * F, &X[(size_t)cand[k] * F], F * 4);
     }
-    // Time every eligible cell on its sampled tiles, each class alone.
+    // Time every eligible cell on its sampled tiles, each class alone, as a
+    // persistent launch like the production one. A tile's cost is its CU time:
+    // its wall-clock latency divided by the number of workgroups sharing its
+    // CU during the measurement (cells that fit several workgroups per CU
+    // overlap them; the matrix-core B x B holds a CU alone).
     unsigned long long* dticks = nullptr;
     SH_HIP(hipMalloc(&dticks, sizeof(unsigned long long) * std::max<uint32_t>(S, 1)));
     std::vector<unsigned long long> ticks(S);
     std::vector<double> best((size_t)S * 10, 0.0);
     std::vector<uint8_t> has((size_t)S * 10, 0);
-    const DevModel M = dev_model(ctx->model);
     const bool fp4 = c->n_words < (1u << 18);   // counts < 2^24: exact in f32
+    if (tile == 256 && c->n_bitmap_rows)
+        SH_TRY(bb_mfma_check(fp4, ctx->mfma.stage_words, c->n_words, c->ld));
+    const uint32_t cu = (uint32_t)std::max(1, ctx->cu_count);
     for (int cell = 0; cell < 10 && rc == STORM_HIP_OK; ++cell) {
         std::vector<QItem> q;
         std::vector<uint32_t> slot_of;
@@ -589,7 +596,10 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             if (elig[k] >> cell & 1) { q.push_back(items[k]); slot_of.push_back(k); }
         if (q.empty()) continue;
         QItem* dq = p->queue + (uint64_t)cell * p->totalP;
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(q.size(), (uint64_t)ctx->cu_count * (cell == 0 && tile == 256 ? 1 : 4));
+        const uint32_t per_cu = cell == STORM_HIP_CELL_BB ? bb_blocks_per_cu(tile, ctx->mfma)
+                                                          : cell_blocks_per_cu(cell);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(q.size(), (uint64_t)cu * per_cu);
+        const double sharing = std::max(1.0, std::min((double)per_cu, (double)grid / cu));
         (void)hipMemcpyAsync(dq, q.data(), q.size() * sizeof(QItem), hipMemcpyHostToDevice, s);
         for (uint32_t r = 0; r < std::max(1u, repeats) && rc == STORM_HIP_OK; ++r) {
             DevCounters h{};
@@ -597,7 +607,7 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             (void)hipMemcpyAsync(p->ctr, &h, sizeof h, hipMemcpyHostToDevice, s);
             if (cell == STORM_HIP_CELL_BB) {
                 if (tile == 256)
-                    rc = launch_bb_mfma_queue(fp4, s, p->d_corpora, c->n_words, c->ld, dq,
+                    rc = launch_bb_mfma_queue(fp4, ctx->mfma, s, p->d_corpora, dq,
                                               &p->ctr->qcount[cell], &p->ctr->qhead[cell], grid,
                                               dticks);
                 else
@@ -611,7 +621,7 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             if (hipStreamSynchronize(s) != hipSuccess) { rc = STORM_HIP_ERR_DEVICE; break; }
             for (size_t e = 0; e < q.size(); ++e) {
                 const uint32_t k = slot_of[e];
-                const double ns = (double)ticks[e] * 10.0;   // 100 MHz wall clock
+                const double ns = (double)ticks[e] * 10.0 / sharing;   // 100 MHz wall clock
                 if (!has[(size_t)k * 10 + cell] || ns < best[(size_t)k * 10 + cell])
                     best[(size_t)k * 10 + cell] = ns;
                 has[(size_t)k * 10 + cell] = 1;
@@ -641,15 +651,42 @@ int storm_hip_calibrate(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
     uint32_t n = 0;
     SH_TRY(storm_hip_selector_sample(ctx, c, tile, max_tiles, seed, repeats, X.data(), Y.data(),
                                      Mk.data(), idx.data(), &n));
-    const Plan* plan = static_cast<const Plan*>(c->plan);
-    if (n == 0 || !plan) {
-        set_error("calibrate: no non-empty tile pair to time");
+    return storm_hip_calibrate_samples(ctx, c, tile, n, Y.data(), Mk.data(), idx.data(), out);
+}
+
+int storm_hip_calibrate_samples(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
+                                uint32_t n, const float* log2_ns, const uint8_t* mask,
+                                const uint32_t* tile_index, storm_hip_cost_model* out) {
+    if (!ctx || !c || c->ctx != ctx || c->dense) return STORM_HIP_ERR_INVALID;
+    if (n && (!log2_ns || !mask || !tile_index)) return STORM_HIP_ERR_NULL_VALUES;
+    SH_TRY(check_tile(tile));
+    if (n == 0) {
+        set_error("calibrate: no timed tile pair");
         return STORM_HIP_ERR_INVALID;
     }
+    SH_HIP(hipSetDevice(ctx->device));
+    // This corpus's tile metadata at this tile width.
+    Plan* plan = static_cast<Plan*>(c->plan);
+    const int prc = ensure_plan(plan, {c}, {nullptr}, tile);
+    c->plan = plan;
+    SH_TRY(prc);
     const Work* w = plan->works[0];
+    SH_TRY(launch_tile_meta(ctx->stream, plan->h_corpora[0], w->tile, w->T, w->tm));
     std::vector<TileMeta> tm(w->T);
-    SH_HIP(hipMemcpy(tm.data(), w->tm, tm.size() * sizeof(TileMeta), hipMemcpyDeviceToHost));
+    SH_HIP(hipMemcpyAsync(tm.data(), w->tm, tm.size() * sizeof(TileMeta), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    SH_HIP(hipStreamSynchronize(ctx->stream));
+    const float* Y = log2_ns;
+    const uint8_t* Mk = mask;
+    const uint32_t* idx = tile_index;
+    for (uint32_t k = 0; k < n; ++k)
+        if (idx[k] >= w->P) {
+            set_error("calibrate: tile index %u out of range (%llu tile pairs)", idx[k],
+                      (unsigned long long)w->P);
+            return STORM_HIP_ERR_RANGE;
+        }
     DeviceCostModel m = ctx->model;
+    bool sampled[10] = {};
     for (int p = 0; p < 10; ++p) {
         std::vector<double> r;
         for (uint32_t k = 0; k < n; ++k) {
@@ -662,10 +699,26 @@ int storm_hip_calibrate(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
             if (pairs < 1) continue;
             r.push_back(std::exp2((double)Y[(size_t)k * 10 + p]) / (pairs * units));
         }
-        if (r.empty()) { m.ns_per_unit[p] = 0.0; continue; }
+        if (r.empty()) continue;
         std::nth_element(r.begin(), r.begin() + r.size() / 2, r.end());
         m.ns_per_unit[p] = r[r.size() / 2];
+        sampled[p] = true;
     }
+    // Cells this corpus never ran keep their previous rate, rescaled by the
+    // median ratio new/old over the cells it did run (so every rate is on the
+    // device's scale and none is dropped from the candidate set).
+    std::vector<double> ratio;
+    for (int p = 0; p < 10; ++p)
+        if (sampled[p] && ctx->model.ns_per_unit[p] > 0)
+            ratio.push_back(m.ns_per_unit[p] / ctx->model.ns_per_unit[p]);
+    if (ratio.empty()) {
+        set_error("calibrate: no cell could be timed on this corpus");
+        return STORM_HIP_ERR_INVALID;
+    }
+    std::nth_element(ratio.begin(), ratio.begin() + ratio.size() / 2, ratio.end());
+    const double scale = ratio[ratio.size() / 2];
+    for (int p = 0; p < 10; ++p)
+        if (!sampled[p]) m.ns_per_unit[p] = ctx->model.ns_per_unit[p] * scale;
     m.ns_fixed = 0.0; m.ns_per_occ_word = 0.0; m.ns_probe_hot = 0.0; m.ns_probe_cold = 0.0;
     m.bb_zonemap = 0.0;
     ctx->model = m;

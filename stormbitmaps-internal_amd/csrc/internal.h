@@ -44,6 +44,13 @@ struct DeviceCostModel {
     double bb_zonemap;
 };
 
+// Launch shape of the matrix-core B x B kernel (chosen by the autotuner).
+struct MfmaConfig {
+    uint32_t stage_words = 8;     // packed words per LDS stage: 4 or 8
+    uint32_t band = 4;            // tile order: 0 row-major, else bands of this many tile rows
+    uint32_t blocks_per_cu = 2;   // K-split target when there are fewer tiles than this per CU
+};
+
 }  // namespace stormhip
 
 struct storm_hip_ctx {
@@ -51,6 +58,7 @@ struct storm_hip_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     stormhip::DeviceCostModel model{};
+    stormhip::MfmaConfig mfma{};     // B x B launch shape (defaults or tuned)
     int cu_count = 0;
     // MLP selector state lives in mlp.hip (opaque here).
     void* selector = nullptr;
@@ -107,15 +115,15 @@ namespace stormhip {
 int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32_t n_words,
                    const uint32_t* rowmap, const uint32_t* perm, uint32_t n, uint32_t* tri,
                    unsigned long long* sum);
-int launch_bb_mfma_fmt(bool fp4, hipStream_t s, const uint64_t* X, uint32_t ld, uint32_t n_words,
-                       const uint32_t* rowmap, const uint32_t* perm, uint32_t n, uint32_t* tri,
-                       unsigned long long* sum, uint32_t k_split = 1);
+int launch_bb_mfma_fmt(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
+                       uint32_t ld, uint32_t n_words, const uint32_t* rowmap, const uint32_t* perm,
+                       uint32_t n, uint32_t* tri, unsigned long long* sum, uint32_t k_split = 1);
 // Queue launches over tiles of several corpora (declared with the device
 // corpus view in general.h).
 // The B x B form for a dense launch: VALU tile of 128, or MFMA (int8 / fp4).
 // AUTO takes fp4 where its exactness bound holds, else int8. Small corpora
 // split K across blocks so the launch fills `cu_count` CUs.
-int launch_bb_dense(uint32_t bb_kernel, hipStream_t s, const uint64_t* X, uint32_t ld,
-                    uint32_t n_words, const uint32_t* rowmap, uint32_t n, uint32_t* tri,
-                    unsigned long long* sum, uint32_t cu_count);
+int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
+                    uint32_t ld, uint32_t n_words, const uint32_t* rowmap, uint32_t n,
+                    uint32_t* tri, unsigned long long* sum, uint32_t cu_count);
 }  // namespace stormhip

@@ -122,7 +122,7 @@ __host__ __device__ double work_units(int p, const TileMeta& a, const TileMeta& 
     }
 }
 
-__device__ double predict(const DevModel& m, int p, const TileMeta& a, const TileMeta& b) {
+__host__ __device__ double predict(const DevModel& m, int p, const TileMeta& a, const TileMeta& b) {
     if (p == STORM_HIP_CELL_EMPTY) return 0.0;
     if (p == STORM_HIP_CELL_BB && (m.bb_zonemap == 0.0 || !(a.has_occ && b.has_occ)))
         return m.ns_fixed + m.ns_per_unit[p] * (double)max(a.nw, b.nw);
@@ -166,6 +166,25 @@ __device__ inline int cheapest_eligible(const DevModel& m, const TileMeta& a, co
     return best < 0 ? STORM_HIP_CELL_SS : best;
 }
 
+// Cells the learned selector may rank on a tile: eligible cells whose
+// analytic prediction is within kLearnedPrune times the cheapest eligible
+// one. The same set is timed when training samples are drawn, so the network
+// never ranks a cell it had no chance to see on such a tile.
+__host__ __device__ uint32_t learned_candidates(const DevModel& m, const TileMeta& a,
+                                                const TileMeta& b, uint32_t elig) {
+    double c[10];
+    double lo = -1.0;
+    for (int p = 0; p < 10; ++p) {
+        if (!(elig >> p & 1)) continue;
+        c[p] = predict(m, p, a, b);
+        if (lo < 0.0 || c[p] < lo) lo = c[p];
+    }
+    uint32_t mask = 0;
+    for (int p = 0; p < 10; ++p)
+        if ((elig >> p & 1) && c[p] <= kLearnedPrune * lo + 1.0) mask |= 1u << p;
+    return mask;
+}
+
 // One thread per tile pair: decide, then append to that cell's queue.
 // policy: 1 analytic, 2 learned (scores given), 3 fixed.
 __global__ __launch_bounds__(256) void select_kernel(
@@ -187,10 +206,11 @@ __global__ __launch_bounds__(256) void select_kernel(
         p = (int)fixed_cell;
         if (p == STORM_HIP_CELL_EMPTY || !(elig >> p & 1)) { p = cheapest_eligible(M, a, b, elig); fallback = true; }
     } else if (policy == STORM_HIP_POLICY_LEARNED && learned_ok && learned_ok[t]) {
+        const uint32_t cand = learned_candidates(M, a, b, elig);
         int best = -1;
         float bc = 0.f;
         for (int q = 0; q < 10; ++q) {
-            if (!(elig >> q & 1)) continue;
+            if (!(cand >> q & 1)) continue;
             const float c = learned_cost[t * 10 + q];
             if (best < 0 || c < bc) { bc = c; best = q; }
         }

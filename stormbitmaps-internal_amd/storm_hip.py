@@ -71,6 +71,24 @@ class CostModel(C.Structure):
                 "bb_zonemap": self.bb_zonemap}
 
 
+class BBConfig(C.Structure):
+    _fields_ = [("stage_words", C.c_uint32), ("band", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
+
+    def as_dict(self):
+        return {"stage_words": self.stage_words, "band": self.band,
+                "blocks_per_cu": self.blocks_per_cu}
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("tile", C.c_uint32), ("n_tried", C.c_uint32), ("bb", BBConfig),
+                ("ms_best", C.c_double), ("ms_default", C.c_double)]
+
+    def as_dict(self):
+        return {"tile": self.tile, "n_tried": self.n_tried, "bb": self.bb.as_dict(),
+                "ms_best": self.ms_best, "ms_default": self.ms_default}
+
+
 _lib = None
 
 
@@ -123,10 +141,15 @@ def lib():
         "storm_hip_selector_grad": (i32, [vp, vp, vp, vp, u32, vp, C.POINTER(C.c_float)]),
         "storm_hip_selector_adam_step": (i32, [vp, C.c_float]),
         "storm_hip_calibrate": (i32, [vp, vp, u32, u32, u64, u32, C.POINTER(CostModel)]),
+        "storm_hip_calibrate_samples": (i32, [vp, vp, u32, u32, vp, vp, vp,
+                                              C.POINTER(CostModel)]),
         "storm_hip_batch_create": (i32, [vp, vp, u32, pvp]),
         "storm_hip_batch_allpairs": (i32, [vp, vp, C.POINTER(Options), C.POINTER(Stats), vp,
                                            vp]),
         "storm_hip_batch_free": (None, [vp]),
+        "storm_hip_set_bb_config": (i32, [vp, C.POINTER(BBConfig)]),
+        "storm_hip_get_bb_config": (i32, [vp, C.POINTER(BBConfig)]),
+        "storm_hip_autotune": (i32, [vp, vp, C.POINTER(Options), C.POINTER(TuneResult)]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -264,6 +287,17 @@ class Context:
                                         C.byref(m)), "calibrate")
         return m.as_dict()
 
+    def calibrate_samples(self, corpus, tile, Y, M, idx):
+        """Fit the analytic rates from selector_sample's output (Y, M, idx)."""
+        Y = np.ascontiguousarray(Y, np.float32)
+        M = np.ascontiguousarray(M, np.uint8)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        m = CostModel()
+        check(lib().storm_hip_calibrate_samples(self.h, corpus.h, tile, len(idx), _ptr(Y),
+                                                _ptr(M), _ptr(idx), C.byref(m)),
+              "calibrate_samples")
+        return m.as_dict()
+
     def set_cost_model(self, rates, ns_fixed=2.0, ns_per_occ_word=0.30, hot=0.0, cold=0.0,
                        bb_zonemap=1.0):
         m = CostModel()
@@ -277,6 +311,24 @@ class Context:
         m = CostModel()
         check(lib().storm_hip_get_cost_model(self.h, C.byref(m)), "get_cost_model")
         return m
+
+    # -- launch shapes -----------------------------------------------------------
+    def set_bb_config(self, stage_words=8, band=4, blocks_per_cu=2):
+        c = BBConfig(stage_words, band, blocks_per_cu, 0)
+        check(lib().storm_hip_set_bb_config(self.h, C.byref(c)), "set_bb_config")
+
+    def get_bb_config(self):
+        c = BBConfig()
+        check(lib().storm_hip_get_bb_config(self.h, C.byref(c)), "get_bb_config")
+        return c.as_dict()
+
+    def autotune(self, corpus, policy="analytic", bb="auto", tile=64, use_graph=False):
+        o = Options()
+        o.policy, o.out_mode, o.bb_kernel = POLICY[policy], OUT["sum"], BB[bb]
+        o.tile, o.use_graph = tile, int(use_graph)
+        r = TuneResult()
+        check(lib().storm_hip_autotune(self.h, corpus.h, C.byref(o), C.byref(r)), "autotune")
+        return r.as_dict()
 
 
 class Corpus:

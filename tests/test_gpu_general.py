@@ -192,3 +192,16 @@ def test_batch_rejects_dense_members(sh, ctx):
     with ctx.upload_contig(words) as d:
         with pytest.raises(sh.StormHipError):
             ctx.batch([d])
+
+
+def test_autotune_general_keeps_counts(ctx, cases):
+    name, u, offs, pos, want_sum, want = cases[20]
+    print("case autotune", name, flush=True)
+    with ctx.from_lists(offs, pos, u, bitmap_policy=1 | BUILD_ALL) as c:
+        r = ctx.autotune(c, policy="analytic", tile=64)
+        assert r["tile"] in (64, 128, 256)
+        assert r["n_tried"] in (3, 27)
+        st = c.allpairs(policy="analytic", out="tri", tile=r["tile"], use_graph=True)
+        assert st["sum"] == want_sum
+        assert np.array_equal(c.tri(), want)
+    ctx.set_bb_config(8, 4, 2)
