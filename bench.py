@@ -156,16 +156,22 @@ def sparse_bench(sh, args):
 
     def timed(policy, **kw):
         progress(f"policy {policy}: warm-up")
-        corpus.allpairs(policy=policy, out="tri", tile=args.tile, **kw)
+        t1 = time.perf_counter()
+        st = corpus.allpairs(policy=policy, out="tri", tile=args.tile, **kw)
         ctx.sync()
-        ts, st = [], None
-        for k in range(args.steps):
+        warm = time.perf_counter() - t1
+        ts = []
+        # A policy slower than --slow-s per call is timed on its warm-up call
+        # alone (graph capture included), and says so.
+        for k in range(args.steps if warm < args.slow_s else 0):
             t1 = time.perf_counter()
             st = corpus.allpairs(policy=policy, out="tri", tile=args.tile, **kw)
             ts.append(time.perf_counter() - t1)
             progress(f"policy {policy}: step {k} {1000 * ts[-1]:.1f} ms")
-        ms = 1000 * float(np.median(ts))
+        single = not ts
+        ms = 1000 * float(np.median(ts if ts else [warm]))
         return {"pairs_per_s": pairs / (ms / 1000), "ms": ms, "sum": st["sum"],
+                "timed_calls": len(ts) if ts else 1, "single_untimed_warmup": single,
                 "cell_tiles": st["cell_tiles"], "skipped": st["skipped"],
                 "fallback_tiles": st["fallback_tiles"], "ms_select": st["ms_select"]}
 
@@ -173,9 +179,14 @@ def sparse_bench(sh, args):
     # One sampling pass (every candidate cell timed on sampled tiles) feeds
     # both the calibrated analytic model and the learned selector.
     t0 = time.perf_counter()
-    progress("sampling tiles")
-    X, Y, M, idx = ctx.selector_sample(corpus, tile=args.tile, max_tiles=args.sample_tiles,
-                                       seed=11, repeats=2)
+    parts = []
+    chunks = max(1, args.sample_chunks)
+    for ch in range(chunks):
+        parts.append(ctx.selector_sample(corpus, tile=args.tile,
+                                         max_tiles=max(1, args.sample_tiles // chunks),
+                                         seed=11 + ch, repeats=2))
+        progress(f"sampled chunk {ch + 1}/{chunks}: {len(parts[-1][0])} tiles")
+    X, Y, M, idx = (np.concatenate([p[k] for p in parts]) for k in range(4))
     t_sample = time.perf_counter() - t0
     t0 = time.perf_counter()
     model = ctx.calibrate_samples(corpus, args.tile, Y, M, idx)
@@ -312,9 +323,10 @@ def batch_bench(sh, args):
     order = sorted(range(len(corpora)), key=lambda i: -corpora[i].n_pairs)[:8]
     for i in order:
         X, Y, M, _ = ctx.selector_sample(corpora[i], tile=args.tile,
-                                         max_tiles=max(64, args.sample_tiles // 8), seed=11 + i,
+                                         max_tiles=max(32, args.sample_tiles // 8), seed=11 + i,
                                          repeats=2)
         Xs.append(X), Ys.append(Y), Ms.append(M)
+        progress(f"sampled instance {i}: {len(X)} tiles")
     X, Y, M = np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
     progress(f"training on {len(X)} tiles")
     loss = ctx.selector_train(X, Y, M, epochs=300, batch=128, lr=3e-3, seed=5)
@@ -449,9 +461,13 @@ def main():
     ap.add_argument("--rows", type=int, default=0, help="sparse configs: first N rows only")
     ap.add_argument("--tile", type=int, default=256,
                     help="rows per selection tile (sparse and batch configs)")
+    ap.add_argument("--slow-s", type=float, default=30.0,
+                    help="sparse configs: a policy slower than this per call runs once")
     ap.add_argument("--bitmap-budget-gib", type=int, default=160,
                     help="sparse configs: keep a bitmap of every row when they fit this")
-    ap.add_argument("--sample-tiles", type=int, default=2048)
+    ap.add_argument("--sample-tiles", type=int, default=256)
+    ap.add_argument("--sample-chunks", type=int, default=8,
+                    help="sampling passes (a progress line after each)")
     ap.add_argument("--no-allbitmap", action="store_true")
     ap.add_argument("--instance-mib", type=int, default=256,
                     help="batch configs: largest estimated instance size run")

@@ -52,6 +52,45 @@ int STORM_contig_gpu_pairw_counts(STORM_contiguous_t* bitmap, uint32_t* out);
 /* Number of vectors held. */
 size_t STORM_contig_gpu_size(const STORM_contiguous_t* bitmap);
 
+/* ---- STORM_t: the 65,536-value block container (storm.h:150-221) ---------
+ * Same names and results as storm.cpp:872-1046 for sorted input: one vector
+ * per STORM_add, the pairwise calls sum |Ci n Cj| over every pair, the square
+ * form over every (vector of the first, vector of the second). The host keeps
+ * each vector as a sorted position list and the pairwise calls run the
+ * engine's per-tile selection on the device (the corpus is uploaded once per
+ * change). Serialized sizes follow the reference's block layout (blocks of
+ * 65,536 values; fewer than 4,096 values in a block are u16 scalars).
+ *
+ * Differences a caller can observe:
+ *   - repeated values count once (set semantics, as the reference's bitmap
+ *     blocks do; its scalar blocks assume distinct values);
+ *   - unsorted input is sorted (the reference requires sorted input);
+ *   - the value 4,294,967,295 is out of range (the device universe is u32):
+ *     STORM_add returns -3 and adds nothing;
+ *   - a device failure makes the pairwise calls print the reason to stderr
+ *     and return UINT64_MAX (the reference returns UINT64_MAX only for a NULL
+ *     handle), and the square form return 0; the reason is also in
+ *     storm_hip_last_error(). */
+typedef struct STORM_s STORM_t;
+
+/* storm.cpp:872-879: NULL on allocation failure. */
+STORM_t* STORM_new(void);
+/* storm.cpp:881-892: NULL-safe. */
+void STORM_free(STORM_t* bitmap);
+/* storm.cpp:894-916: -1 NULL handle; otherwise one vector is appended (empty
+ * when values is NULL or n_values is 0) and 1 is returned. */
+int STORM_add(STORM_t* bitmap, const uint32_t* values, const uint32_t n_values);
+/* storm.cpp:918-925: 1, or -1 for a NULL handle. */
+int STORM_clear(STORM_t* bitmap);
+/* storm.cpp:927-1011: UINT64_MAX for a NULL handle. bsize is a CPU cache knob
+ * with no effect on the result. */
+uint64_t STORM_pairw_intersect_cardinality(STORM_t* bitmap);
+uint64_t STORM_pairw_intersect_cardinality_blocked(STORM_t* bitmap, uint32_t bsize);
+/* storm.cpp:1027-1046: 0 when either handle is NULL. */
+uint64_t STORM_intersect_cardinality_square(const STORM_t* bitmap1, const STORM_t* bitmap2);
+/* storm.cpp:1013-1023 with storm.cpp:396-418: 0 for a NULL handle. */
+uint64_t STORM_serialized_size(const STORM_t* bitmap);
+
 #ifdef __cplusplus
 }
 #endif

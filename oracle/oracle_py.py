@@ -110,6 +110,14 @@ def reference():
         "STORM_contig_clear": (i32, [vp]),
         "STORM_contig_pairw_intersect_cardinality": (u64, [vp]),
         "STORM_contig_pairw_intersect_cardinality_blocked": (u64, [vp, u32]),
+        "STORM_new": (vp, []),
+        "STORM_free": (None, [vp]),
+        "STORM_add": (i32, [vp, vp, u32]),
+        "STORM_clear": (i32, [vp]),
+        "STORM_pairw_intersect_cardinality": (u64, [vp]),
+        "STORM_pairw_intersect_cardinality_blocked": (u64, [vp, u32]),
+        "STORM_intersect_cardinality_square": (u64, [vp, vp]),
+        "STORM_serialized_size": (u64, [vp]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)

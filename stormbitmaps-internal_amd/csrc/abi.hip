@@ -149,12 +149,18 @@ int storm_hip_ctx_create(int device, uint32_t flags, storm_hip_ctx** out) {
     if (!c) return STORM_HIP_ERR_NOMEM;
     c->device = device;
     c->cu_count = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux[1], hipStreamNonBlocking) != hipSuccess) {
+        for (hipStream_t s : {c->stream, c->aux[0], c->aux[1]})
+            if (s) (void)hipStreamDestroy(s);
         delete c;
         set_error("hipStreamCreate failed");
         return STORM_HIP_ERR_DEVICE;
     }
     for (auto& e : c->ev) hipEventCreate(&e);
+    hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
+    for (auto& e : c->join_ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
     default_device_model(c->model);
     *out = c;
     return STORM_HIP_OK;
@@ -164,8 +170,12 @@ void storm_hip_ctx_free(storm_hip_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    for (hipStream_t s : ctx->aux) if (s) hipStreamSynchronize(s);
     selector_destroy(ctx);
     for (auto& e : ctx->ev) if (e) hipEventDestroy(e);
+    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
+    for (auto& e : ctx->join_ev) if (e) hipEventDestroy(e);
+    for (hipStream_t s : ctx->aux) if (s) hipStreamDestroy(s);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }

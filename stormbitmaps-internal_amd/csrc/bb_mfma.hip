@@ -46,7 +46,6 @@ namespace {
 constexpr int kTile = 256;        // pairs block: kTile x kTile
 constexpr int kThreads = 512;     // 8 waves
 constexpr int kUnitScale = 127;   // E8M0 exponent of 1.0
-constexpr int kStageWords = 8;    // packed 64-bit words per stage per row (SW)
 
 __device__ inline i32x8 expand_fp4(uint32_t w) {
     return i32x8{(int)((w << 1) & 0x22222222u), (int)(w & 0x22222222u),

@@ -262,29 +262,47 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
             if (c->n_bitmap_rows) SH_TRY(bb_mfma_check(fp4, ctx->mfma.stage_words, c->n_words, c->ld));
     if (timed) SH_HIP(hipEventRecord(ctx->ev[5], s));
     const uint64_t P1 = p->totalP ? p->totalP : 1;
-    // Persistent grids: as many workgroups as fit on the chip at once.
-    for (int cell = 0; cell < kNumCells; ++cell) {
-        if (cell == STORM_HIP_CELL_BB) continue;
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(
-            P1, (uint64_t)ctx->cu_count * cell_blocks_per_cu(cell));
-        SH_TRY(launch_cell(cell, s, grid, p->d_corpora, M, p->queue + (uint64_t)cell * p->totalP,
-                           p->ctr, nullptr));
-    }
-    if (timed) SH_HIP(hipEventRecord(ctx->ev[2], s));
+    // Dispatch on three streams so the classes overlap: B x B first on side
+    // stream 0 (its tiles are the longest, and one launch holds at most one
+    // workgroup per CU), the cell classes alternating between the main stream
+    // and side stream 1. A class whose persistent workgroups cannot all be
+    // resident waits for room; none waits for another class to finish.
+    SH_HIP(hipEventRecord(ctx->fork_ev, s));
+    for (hipStream_t a : ctx->aux) SH_HIP(hipStreamWaitEvent(a, ctx->fork_ev, 0));
+    int rc = STORM_HIP_OK;
+    hipStream_t sb = ctx->aux[0];
+    if (timed) (void)hipEventRecord(ctx->ev[2], sb);
     const QItem* qbb = p->queue + (uint64_t)STORM_HIP_CELL_BB * p->totalP;
     const uint32_t gbb = (uint32_t)std::min<uint64_t>(
         P1, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
-    if (p->tile == 256) {
-        SH_TRY(launch_bb_mfma_queue(fp4, ctx->mfma, s, p->d_corpora, qbb,
-                                    &p->ctr->qcount[STORM_HIP_CELL_BB],
-                                    &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr));
-    } else {
-        SH_TRY(launch_bb_valu_queue(s, (int)p->tile, p->d_corpora, qbb,
-                                    &p->ctr->qcount[STORM_HIP_CELL_BB],
-                                    &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr));
+    if (p->tile == 256)
+        rc = launch_bb_mfma_queue(fp4, ctx->mfma, sb, p->d_corpora, qbb,
+                                  &p->ctr->qcount[STORM_HIP_CELL_BB],
+                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr);
+    else
+        rc = launch_bb_valu_queue(sb, (int)p->tile, p->d_corpora, qbb,
+                                  &p->ctr->qcount[STORM_HIP_CELL_BB],
+                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr);
+    if (timed) (void)hipEventRecord(ctx->ev[3], sb);
+    // Persistent grids: as many workgroups as fit on the chip at once.
+    int k = 0;
+    for (int cell = 0; cell < kNumCells && rc == STORM_HIP_OK; ++cell) {
+        if (cell == STORM_HIP_CELL_BB) continue;
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(
+            P1, (uint64_t)ctx->cu_count * cell_blocks_per_cu(cell));
+        rc = launch_cell(cell, (k++ & 1) ? ctx->aux[1] : s, grid, p->d_corpora, M,
+                         p->queue + (uint64_t)cell * p->totalP, p->ctr, nullptr);
     }
-    if (timed) SH_HIP(hipEventRecord(ctx->ev[3], s));
-    return STORM_HIP_OK;
+    // Join the side streams back, whatever happened (a capture must close).
+    for (int a = 0; a < 2; ++a) {
+        const hipError_t e1 = hipEventRecord(ctx->join_ev[a], ctx->aux[a]);
+        const hipError_t e2 = hipStreamWaitEvent(s, ctx->join_ev[a], 0);
+        if (rc == STORM_HIP_OK && (e1 != hipSuccess || e2 != hipSuccess)) {
+            set_error("stream join failed: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+            rc = STORM_HIP_ERR_DEVICE;
+        }
+    }
+    return rc;
 }
 
 // Run the plan (graph or eager), then read sums, counters, decisions, latencies.

@@ -56,6 +56,10 @@ struct MfmaConfig {
 struct storm_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Side streams (default priority) so cell classes run concurrently:
+    // forked from and joined back into `stream` with the fork/join events.
+    hipStream_t aux[2] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[2] = {};
     hipEvent_t ev[8] = {};
     stormhip::DeviceCostModel model{};
     stormhip::MfmaConfig mfma{};     // B x B launch shape (defaults or tuned)
