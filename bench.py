@@ -374,6 +374,13 @@ def dense_bench(sh, args):
     ctx = sh.Context(0)
     corpus = ctx.gen_bernoulli(n, bits, p, seed)
     progress(f"generated {n} x {bits} bits on the device; B x B kernel {bb}")
+    tuned = None
+    if bb != "valu" and not args.no_tune:
+        # The autotuner picks the launch shape on this corpus (outside the
+        # timed steps); the timed steps run with it.
+        tuned = ctx.autotune(corpus, bb=bb)
+        progress(f"autotuned: {tuned['bb']} {tuned['ms_best']:.1f} ms "
+                 f"(starting shape {tuned['ms_default']:.1f} ms, {tuned['n_tried']} shapes)")
     for _ in range(args.warmup):
         corpus.allpairs(out="tri", bb=bb)
     ctx.sync()
@@ -426,6 +433,7 @@ def dense_bench(sh, args):
         "exact_sum": sums.pop() if len(sums) == 1 else None,
         "ms_device_per_step": float(np.mean(dev_ms)),
         "roofline": roof,
+        "launch_shape": tuned,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(sh, corpus, bits),
     }
     corpus.free()
@@ -458,6 +466,8 @@ def main():
                     + ["copy"])
     ap.add_argument("--bb", default="auto", choices=["auto", "mfma", "mfma_fp4", "valu"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="dense configs: keep the default B x B launch shape")
     ap.add_argument("--rows", type=int, default=0, help="sparse configs: first N rows only")
     ap.add_argument("--tile", type=int, default=256,
                     help="rows per selection tile (sparse and batch configs)")

@@ -225,12 +225,13 @@ int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
  * (4 or 8), tile order (0 = row-major upper triangle, g > 0 = bands of g tile
  * rows walked column by column, each XCD taking a contiguous range) and the
  * K-split target (blocks per CU when a corpus has fewer 256 x 256 tiles than
- * that). A context starts from {8, 4, 2}. */
+ * that) and the waves per workgroup (8: two per SIMD, 128 x 64 pairs each;
+ * 4: one per SIMD, 128 x 128 pairs each). A context starts from {8, 4, 2, 8}. */
 typedef struct {
     uint32_t stage_words;
     uint32_t band;
     uint32_t blocks_per_cu;
-    uint32_t reserved;
+    uint32_t waves;
 } storm_hip_bb_config;
 int storm_hip_set_bb_config(storm_hip_ctx* ctx, const storm_hip_bb_config* cfg);
 int storm_hip_get_bb_config(storm_hip_ctx* ctx, storm_hip_bb_config* cfg);

@@ -44,7 +44,6 @@ typedef __attribute__((address_space(1))) const u32x4 global_u32x4;
 
 namespace {
 constexpr int kTile = 256;        // pairs block: kTile x kTile
-constexpr int kThreads = 512;     // 8 waves
 constexpr int kUnitScale = 127;   // E8M0 exponent of 1.0
 
 __device__ inline i32x8 expand_fp4(uint32_t w) {
@@ -72,13 +71,13 @@ struct BBMfmaSmem {
     // reads down a column of rows).
     static constexpr int kRowDw = 2 * SW + 4;
     // Packed bits of one stage: [buffer][row 0..511 (A rows, then B rows)][dw].
-    uint32_t st[2][kThreads][kRowDw];
+    uint32_t st[2][2 * kTile][kRowDw];
 };
 
 // One 256 x 256 block over stages [s0, s0 + n_stages) of SW words. With
 // `accumulate`, counts are atomically added to `tri` (a K-split block: other
 // blocks add the other stage ranges of the same pairs into a zeroed triangle).
-template <bool FP4, int SW>
+template <bool FP4, int SW, int NW>
 __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint64_t* X, uint32_t ld,
                                                  uint32_t s0, uint32_t n_stages,
                                                  const uint32_t* __restrict__ rowmap,
@@ -88,40 +87,61 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
     using Acc = typename std::conditional<FP4, f32x16, i32x16>::type;
     constexpr int L = SW / 2;        // 16-byte loads per row per stage
     constexpr int R = SW / 4;        // 16-byte LDS reads per fragment row per stage
+    constexpr int NB = 16 / NW;      // 32-column MFMA blocks per wave: 2 (8 waves) or 4 (4 waves)
+    constexpr int WC = kTile / 32 / NB;   // waves across the columns: 4 or 2
+    constexpr int THREADS = 64 * NW;
     const uint32_t i0 = bi * kTile, j0 = bj * kTile;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    const int wr = wave >> 2, wc = wave & 3;   // 128 rows x 64 columns
+    const int wr = wave / WC, wc = wave % WC;   // 128 rows x 32*NB columns
     const int lr = lane & 31, lg = lane >> 5;
 
-    // Thread t stages row t of the A panel (t < 256) or row t-256 of B.
-    const uint32_t pos = t < kTile ? i0 + t : j0 + (t - kTile);
-    const uint64_t phys = rowmap ? rowmap[pos] : (uint64_t)pos;
-    // Rows live in global memory: loads through this pointer are global loads.
-    global_u32x4* src = (global_u32x4*)(X + phys * ld) + (size_t)L * s0;
+    // Thread t stages row t of the 512 staged rows (A panel, then B) and, with
+    // 4 waves, row t + 256 as well.
+    auto src_of = [&](uint32_t r) {
+        const uint32_t pos = r < (uint32_t)kTile ? i0 + r : j0 + (r - kTile);
+        const uint64_t phys = rowmap ? rowmap[pos] : (uint64_t)pos;
+        // Rows live in global memory: loads through this pointer are global loads.
+        return (global_u32x4*)(X + phys * ld) + (size_t)L * s0;
+    };
+    global_u32x4* src = src_of(t);
+    global_u32x4* src2 = THREADS < 512 ? src_of(t + THREADS) : src;
 
-    Acc acc[4][2];
+    // Accumulators as two arrays of 4 x 2 blocks (one array of 16 blocks is
+    // too large for the compiler to keep in registers; it goes to scratch).
+    Acc acc_lo[4][2], acc_hi[4][NB > 2 ? NB - 2 : 1];
+    auto acc = [&](int m, int q) -> Acc& { return q < 2 ? acc_lo[m][q] : acc_hi[m][q - 2]; };
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[m][q][v] = 0;
+        for (int q = 0; q < NB; ++q) acc(m, q) = Acc{};
 
     // The next stage's words, in named registers (an array here would be
     // placed in memory by the compiler).
-    u32x4 g0, g1, g2, g3;
+    u32x4 g0, g1, g2, g3, h0, h1, h2, h3;
     auto gload = [&](uint32_t s) {
         global_u32x4* p = src + (size_t)L * s;
         g0 = p[0];
         g1 = p[1];
         if constexpr (L == 4) { g2 = p[2]; g3 = p[3]; }
+        if constexpr (THREADS < 512) {
+            global_u32x4* p2 = src2 + (size_t)L * s;
+            h0 = p2[0];
+            h1 = p2[1];
+            if constexpr (L == 4) { h2 = p2[2]; h3 = p2[3]; }
+        }
     };
     auto stage_store = [&](int buf) {
         u32x4* d = reinterpret_cast<u32x4*>(&sm.st[buf][t][0]);
         d[0] = g0;
         d[1] = g1;
         if constexpr (L == 4) { d[2] = g2; d[3] = g3; }
+        if constexpr (THREADS < 512) {
+            u32x4* d2 = reinterpret_cast<u32x4*>(&sm.st[buf][t + THREADS][0]);
+            d2[0] = h0;
+            d2[1] = h1;
+            if constexpr (L == 4) { d2[2] = h2; d2[3] = h3; }
+        }
     };
 
     __syncthreads();   // LDS free (previous tile of a persistent workgroup)
@@ -131,9 +151,9 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
     __syncthreads();
     for (uint32_t s = 0; s < n_stages; ++s) {
         const int buf = s & 1;
-        // This wave's packed fragments of the stage: 4 A rows, 2 B rows, each
+        // This wave's packed fragments of the stage: 4 A rows, NB B rows, each
         // SW chunks of this lane group's 32 bits.
-        uint4 pa[4][R], pb[2][R];
+        uint4 pa[4][R], pb[NB][R];
 #pragma unroll
         for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -141,11 +161,11 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
                 pa[m][r] = *reinterpret_cast<const uint4*>(
                     &sm.st[buf][wr * 128 + m * 32 + lr][lg * SW + 4 * r]);
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+        for (int q = 0; q < NB; ++q)
 #pragma unroll
             for (int r = 0; r < R; ++r)
                 pb[q][r] = *reinterpret_cast<const uint4*>(
-                    &sm.st[buf][kTile + wc * 64 + q * 32 + lr][lg * SW + 4 * r]);
+                    &sm.st[buf][kTile + wc * 32 * NB + q * 32 + lr][lg * SW + 4 * r]);
         // Stage the next words while this stage computes.
         if (s + 1 < n_stages) {
             stage_store(buf ^ 1);
@@ -153,37 +173,37 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
         }
 #pragma unroll
         for (int c = 0; c < SW; ++c) {
-            uint32_t wa[4], wb[2];
+            uint32_t wa[4], wb[NB];
 #pragma unroll
             for (int m = 0; m < 4; ++m) wa[m] = comp(pa[m][c >> 2], c & 3);
 #pragma unroll
-            for (int q = 0; q < 2; ++q) wb[q] = comp(pb[q][c >> 2], c & 3);
+            for (int q = 0; q < NB; ++q) wb[q] = comp(pb[q][c >> 2], c & 3);
             if constexpr (FP4) {
-                i32x8 fb[2];
+                i32x8 fb[NB];
 #pragma unroll
-                for (int q = 0; q < 2; ++q) fb[q] = expand_fp4(wb[q]);
+                for (int q = 0; q < NB; ++q) fb[q] = expand_fp4(wb[q]);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     const i32x8 fa = expand_fp4(wa[m]);
 #pragma unroll
-                    for (int q = 0; q < 2; ++q)
-                        acc[m][q] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                            fa, fb[q], acc[m][q], 4, 4, 0, kUnitScale, 0, kUnitScale);
+                    for (int q = 0; q < NB; ++q)
+                        acc(m, q) = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                            fa, fb[q], acc(m, q), 4, 4, 0, kUnitScale, 0, kUnitScale);
                 }
             } else {
                 // 32 bits per lane group per chunk = two K=32 steps (h = 0, 1
                 // select which nibble half of each byte position the lane takes).
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    i8x16 fb[2];
+                    i8x16 fb[NB];
 #pragma unroll
-                    for (int q = 0; q < 2; ++q) fb[q] = expand_i8(wb[q], h);
+                    for (int q = 0; q < NB; ++q) fb[q] = expand_i8(wb[q], h);
 #pragma unroll
                     for (int m = 0; m < 4; ++m) {
                         const i8x16 fa = expand_i8(wa[m], h);
 #pragma unroll
-                        for (int q = 0; q < 2; ++q)
-                            acc[m][q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb[q], acc[m][q], 0, 0, 0);
+                        for (int q = 0; q < NB; ++q)
+                            acc(m, q) = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb[q], acc(m, q), 0, 0, 0);
                     }
                 }
             }
@@ -202,10 +222,10 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
             if (i >= n) continue;
             const uint32_t oi = perm ? perm[i] : i;
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const uint32_t j = j0 + wc * 64 + q * 32 + lr;
+            for (int q = 0; q < NB; ++q) {
+                const uint32_t j = j0 + wc * 32 * NB + q * 32 + lr;
                 if (j >= n || j <= i) continue;
-                const uint32_t val = (uint32_t)acc[m][q][v];
+                const uint32_t val = (uint32_t)acc(m, q)[v];
                 local += val;
                 if (tri) {
                     const uint32_t oj = perm ? perm[j] : j;
@@ -218,8 +238,8 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
     return local;
 }
 
-template <bool FP4, int SW>
-__global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
+template <bool FP4, int SW, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
     const uint64_t* __restrict__ X, uint32_t ld, uint32_t n_stages, uint32_t k_split,
     uint32_t band, const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm,
     uint32_t n, uint32_t T, const DevCorpus* __restrict__ Ds, const QItem* __restrict__ queue,
@@ -242,7 +262,7 @@ __global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
         const uint32_t s0 = min(slice * per, n_stages);
         const uint32_t cnt = min(per, n_stages - s0);
         if (cnt)
-            local = bb_mfma_tile<FP4, SW>(sm, X, ld, s0, cnt, rowmap, perm, n, bi, bj, tri,
+            local = bb_mfma_tile<FP4, SW, NW>(sm, X, ld, s0, cnt, rowmap, perm, n, bi, bj, tri,
                                           k_split > 1);
     } else {
         for (;;) {
@@ -255,7 +275,7 @@ __global__ __launch_bounds__(kThreads, 1) void bb_mfma_kernel(
             const DevCorpus D = Ds[e.inst];
             const uint64_t t0 = wall_clock64();
             const uint32_t ns = max(1u, (D.nw + SW - 1) / SW);
-            uint64_t part = bb_mfma_tile<FP4, SW>(sm, D.bitmap, D.ld, 0, ns, D.rowmap, D.perm,
+            uint64_t part = bb_mfma_tile<FP4, SW, NW>(sm, D.bitmap, D.ld, 0, ns, D.rowmap, D.perm,
                                                   D.n, e.bi, e.bj, D.tri, false);
             part = wave_sum_u64(part);
             if ((threadIdx.x & 63) == 0 && part) atomicAdd(D.sum, (unsigned long long)part);
@@ -296,21 +316,62 @@ int bb_mfma_check(bool fp4, uint32_t sw, uint32_t n_words, uint32_t ld) {
 
 namespace {
 
-template <bool FP4, int SW>
-void launch_grid(hipStream_t s, uint32_t g, const uint64_t* X, uint32_t ld, uint32_t ns,
-                 uint32_t k_split, uint32_t band, const uint32_t* rowmap, const uint32_t* perm,
-                 uint32_t n, uint32_t T, uint32_t* tri, unsigned long long* sum) {
-    hipLaunchKernelGGL((bb_mfma_kernel<FP4, SW>), dim3(g), dim3(kThreads), 0, s, X, ld, ns, k_split,
-                       band, rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr, tri, sum,
-                       nullptr);
-}
+struct GridLaunch {
+    hipStream_t s;
+    uint32_t g;
+    const uint64_t* X;
+    uint32_t ld, ns, k_split, band;
+    const uint32_t *rowmap, *perm;
+    uint32_t n, T;
+    uint32_t* tri;
+    unsigned long long* sum;
+    template <bool FP4, int SW, int NW>
+    void operator()() const {
+        hipLaunchKernelGGL((bb_mfma_kernel<FP4, SW, NW>), dim3(g), dim3(64 * NW), 0, s, X, ld, ns,
+                           k_split, band, rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr,
+                           tri, sum, nullptr);
+    }
+};
 
-template <bool FP4, int SW>
-void launch_queue(hipStream_t s, uint32_t grid, const DevCorpus* Ds, const QItem* queue,
-                  const unsigned int* qcount, unsigned int* qhead, unsigned long long* tile_ticks) {
-    hipLaunchKernelGGL((bb_mfma_kernel<FP4, SW>), dim3(grid), dim3(kThreads), 0, s, nullptr, 0u, 1u,
-                       1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
-                       tile_ticks);
+struct QueueLaunch {
+    hipStream_t s;
+    uint32_t grid;
+    const DevCorpus* Ds;
+    const QItem* queue;
+    const unsigned int* qcount;
+    unsigned int* qhead;
+    unsigned long long* tile_ticks;
+    template <bool FP4, int SW, int NW>
+    void operator()() const {
+        hipLaunchKernelGGL((bb_mfma_kernel<FP4, SW, NW>), dim3(grid), dim3(64 * NW), 0, s, nullptr,
+                           0u, 1u, 1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead,
+                           nullptr, nullptr, tile_ticks);
+    }
+};
+
+struct Occupancy {
+    int* nb;
+    hipError_t* e;
+    template <bool FP4, int SW, int NW>
+    void operator()() const {
+        *e = hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, bb_mfma_kernel<FP4, SW, NW>, 64 * NW, 0);
+    }
+};
+
+// Shape dispatch: (format, stage words, waves) -> one kernel instance.
+template <class F>
+int with_shape(bool fp4, const MfmaConfig& cfg, F&& f) {
+    const uint32_t sw = cfg.stage_words, nw = cfg.waves;
+    if (fp4 && sw == 8 && nw == 8)  { f.template operator()<true, 8, 8>(); return STORM_HIP_OK; }
+    if (fp4 && sw == 4 && nw == 8)  { f.template operator()<true, 4, 8>(); return STORM_HIP_OK; }
+    if (fp4 && sw == 8 && nw == 4)  { f.template operator()<true, 8, 4>(); return STORM_HIP_OK; }
+    if (fp4 && sw == 4 && nw == 4)  { f.template operator()<true, 4, 4>(); return STORM_HIP_OK; }
+    if (!fp4 && sw == 8 && nw == 8) { f.template operator()<false, 8, 8>(); return STORM_HIP_OK; }
+    if (!fp4 && sw == 4 && nw == 8) { f.template operator()<false, 4, 8>(); return STORM_HIP_OK; }
+    if (!fp4 && sw == 8 && nw == 4) { f.template operator()<false, 8, 4>(); return STORM_HIP_OK; }
+    if (!fp4 && sw == 4 && nw == 4) { f.template operator()<false, 4, 4>(); return STORM_HIP_OK; }
+    set_error("bb_mfma: no kernel for %u-word stages and %u waves", sw, nw);
+    return STORM_HIP_ERR_INVALID;
 }
 
 // Zero fill of a u32 array as a kernel.
@@ -339,11 +400,8 @@ int launch_bb_mfma_fmt(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uin
         return STORM_HIP_ERR_RANGE;
     }
     const uint32_t ns = n_stages ? n_stages : 1;
-    const uint32_t G = (uint32_t)g;
-    if (fp4 && sw == 8)  launch_grid<true, 8>(s, G, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum);
-    if (fp4 && sw == 4)  launch_grid<true, 4>(s, G, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum);
-    if (!fp4 && sw == 8) launch_grid<false, 8>(s, G, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum);
-    if (!fp4 && sw == 4) launch_grid<false, 4>(s, G, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum);
+    const GridLaunch L{s, (uint32_t)g, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum};
+    SH_TRY(with_shape(fp4, cfg, L));
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }
@@ -353,9 +411,8 @@ uint32_t bb_valu_blocks_per_cu(int bm);
 uint32_t bb_blocks_per_cu(uint32_t tile, const MfmaConfig& cfg) {
     if (tile != 256) return bb_valu_blocks_per_cu((int)tile);
     int nb = 0;
-    const hipError_t e = cfg.stage_words == 4
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bb_mfma_kernel<true, 4>, kThreads, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bb_mfma_kernel<true, 8>, kThreads, 0);
+    hipError_t e = hipErrorInvalidValue;
+    if (with_shape(true, cfg, Occupancy{&nb, &e}) != STORM_HIP_OK) return 1;
     return (e == hipSuccess && nb > 0) ? (uint32_t)nb : 1;
 }
 
@@ -364,16 +421,8 @@ uint32_t bb_blocks_per_cu(uint32_t tile, const MfmaConfig& cfg) {
 int launch_bb_mfma_queue(bool fp4, const MfmaConfig& cfg, hipStream_t s, const DevCorpus* Ds,
                          const QItem* queue, const unsigned int* qcount, unsigned int* qhead,
                          uint32_t grid, unsigned long long* tile_ticks) {
-    const uint32_t sw = cfg.stage_words;
-    if (sw != 4 && sw != 8) {
-        set_error("bb_mfma: stage width %u words (4 or 8)", sw);
-        return STORM_HIP_ERR_INVALID;
-    }
     if (grid == 0) return STORM_HIP_OK;
-    if (fp4 && sw == 8)  launch_queue<true, 8>(s, grid, Ds, queue, qcount, qhead, tile_ticks);
-    if (fp4 && sw == 4)  launch_queue<true, 4>(s, grid, Ds, queue, qcount, qhead, tile_ticks);
-    if (!fp4 && sw == 8) launch_queue<false, 8>(s, grid, Ds, queue, qcount, qhead, tile_ticks);
-    if (!fp4 && sw == 4) launch_queue<false, 4>(s, grid, Ds, queue, qcount, qhead, tile_ticks);
+    SH_TRY(with_shape(fp4, cfg, QueueLaunch{s, grid, Ds, queue, qcount, qhead, tile_ticks}));
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

@@ -532,8 +532,17 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         const uint32_t n = D.n, tile = D.tile;
         uint32_t* const tri = D.tri;
         uint64_t local = 0;
-        const uint32_t i0 = e.bi * tile, j0 = e.bj * tile;
-        const uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
+        uint32_t i0 = e.bi * tile, j0 = e.bj * tile;
+        uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
+        if (e.sub) {
+            // Sub-block: `sb` rows per side from `off` rows in (the same rows
+            // on both sides of a diagonal tile, so it stays diagonal).
+            const uint32_t sb = e.sub & 0xFFFFu, off = e.sub >> 16;
+            uint32_t oi = min(off, ni > sb ? ni - sb : 0u), oj = min(off, nj > sb ? nj - sb : 0u);
+            if (e.bi == e.bj) oi = oj = min(oi, oj);
+            i0 += oi; j0 += oj;
+            ni = min(ni - oi, sb); nj = min(nj - oj, sb);
+        }
         const bool diag = e.bi == e.bj;
         const uint32_t np = ni * nj;
         // Lane mode when the sparser tile's rows are tiny.

@@ -35,8 +35,11 @@ struct DevCorpus {
     uint32_t tile;                    // rows per tile
 };
 
-// One queued tile pair: corpus index, tile row, tile column.
-struct QItem { uint32_t inst, bi, bj, pad; };
+// One queued tile pair: corpus index, tile row, tile column, and `sub`: 0 for
+// the whole tile, else a sub-block of (sub & 0xFFFF) rows on each side
+// starting (sub >> 16) rows into the tile (the selector's sampling runs time
+// a sub-block of median-density rows and scale by the pair ratio).
+struct QItem { uint32_t inst, bi, bj, sub; };
 
 int launch_bb_valu_queue(hipStream_t s, int bm, const DevCorpus* Ds, const QItem* queue,
                          const unsigned int* qcount, unsigned int* qhead, uint32_t grid,

@@ -582,16 +582,29 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
     const uint32_t S = (uint32_t)cand.size();
     std::vector<QItem> items(S);
     std::vector<uint32_t> elig(S);
+    std::vector<double> scale(S, 1.0);
     const uint32_t reps = reps_of(c);
     const DevModel M = dev_model(ctx->model);
+    // Cells other than B x B are timed on a kSampleRows x kSampleRows block
+    // of median-density rows of each tile (their cost is per pair) and the
+    // time is scaled by the tile's pair count over the block's.
+    constexpr uint32_t kSampleRows = 32;
+    const uint32_t sub = tile > kSampleRows ? ((tile - kSampleRows) / 2) << 16 | kSampleRows : 0;
     for (uint32_t k = 0; k < S; ++k) {
         uint32_t bi, bj;
         tri_tile(cand[k], w->T, bi, bj);
-        items[k] = QItem{0, bi, bj, 0};
+        items[k] = QItem{0, bi, bj, sub};
         // The cells the learned selector may rank on this tile.
         elig[k] = learned_candidates(M, tm[bi], tm[bj], eligible_mask(tm[bi], tm[bj], reps));
         tile_index[k] = cand[k];
         std::memcpy(features + (size_t)k * F, &X[(size_t)cand[k] * F], F * 4);
+        if (sub) {
+            const double ni = tm[bi].rows, nj = tm[bj].rows;
+            const double si = std::min<double>(ni, kSampleRows), sj = std::min<double>(nj, kSampleRows);
+            const double full = bi == bj ? ni * (ni - 1) / 2 : ni * nj;
+            const double part = bi == bj ? si * (si - 1) / 2 : si * sj;
+            scale[k] = part > 0 ? full / part : 1.0;
+        }
     }
     // Time every eligible cell on its sampled tiles, each class alone, as a
     // persistent launch like the production one. A tile's cost is its CU time:
@@ -611,7 +624,11 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
         std::vector<QItem> q;
         std::vector<uint32_t> slot_of;
         for (uint32_t k = 0; k < S; ++k)
-            if (elig[k] >> cell & 1) { q.push_back(items[k]); slot_of.push_back(k); }
+            if (elig[k] >> cell & 1) {
+                q.push_back(items[k]);
+                if (cell == STORM_HIP_CELL_BB) q.back().sub = 0;   // whole tiles
+                slot_of.push_back(k);
+            }
         if (q.empty()) continue;
         QItem* dq = p->queue + (uint64_t)cell * p->totalP;
         const uint32_t per_cu = cell == STORM_HIP_CELL_BB ? bb_blocks_per_cu(tile, ctx->mfma)
@@ -639,7 +656,9 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             if (hipStreamSynchronize(s) != hipSuccess) { rc = STORM_HIP_ERR_DEVICE; break; }
             for (size_t e = 0; e < q.size(); ++e) {
                 const uint32_t k = slot_of[e];
-                const double ns = (double)ticks[e] * 10.0 / sharing;   // 100 MHz wall clock
+                // 100 MHz wall clock -> ns of CU time for the whole tile.
+                const double ns = (double)ticks[e] * 10.0 / sharing *
+                                  (cell == STORM_HIP_CELL_BB ? 1.0 : scale[k]);
                 if (!has[(size_t)k * 10 + cell] || ns < best[(size_t)k * 10 + cell])
                     best[(size_t)k * 10 + cell] = ns;
                 has[(size_t)k * 10 + cell] = 1;

@@ -11,7 +11,7 @@ namespace {
 
 bool valid(const storm_hip_bb_config& c) {
     return (c.stage_words == 4 || c.stage_words == 8) && c.band <= 64 && c.blocks_per_cu >= 1 &&
-           c.blocks_per_cu <= 16;
+           c.blocks_per_cu <= 16 && (c.waves == 4 || c.waves == 8);
 }
 
 MfmaConfig to_internal(const storm_hip_bb_config& c) {
@@ -19,6 +19,7 @@ MfmaConfig to_internal(const storm_hip_bb_config& c) {
     m.stage_words = c.stage_words;
     m.band = c.band;
     m.blocks_per_cu = c.blocks_per_cu;
+    m.waves = c.waves;
     return m;
 }
 
@@ -27,7 +28,13 @@ storm_hip_bb_config to_public(const MfmaConfig& m) {
     c.stage_words = m.stage_words;
     c.band = m.band;
     c.blocks_per_cu = m.blocks_per_cu;
+    c.waves = m.waves;
     return c;
+}
+
+bool same(const MfmaConfig& a, const MfmaConfig& b) {
+    return a.stage_words == b.stage_words && a.band == b.band &&
+           a.blocks_per_cu == b.blocks_per_cu && a.waves == b.waves;
 }
 
 // One call with sums only; its device time in ms.
@@ -44,7 +51,7 @@ extern "C" {
 
 int storm_hip_set_bb_config(storm_hip_ctx* ctx, const storm_hip_bb_config* cfg) {
     if (!ctx || !cfg || !valid(*cfg)) {
-        set_error("bb config: stage_words 4 or 8, band <= 64, blocks_per_cu in [1, 16]");
+        set_error("bb config: stage_words 4 or 8, band <= 64, blocks_per_cu in [1, 16], waves 4 or 8");
         return STORM_HIP_ERR_INVALID;
     }
     ctx->mfma = to_internal(*cfg);
@@ -101,19 +108,25 @@ int storm_hip_autotune(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
     } else {
         o.tile = start_tile;
     }
-    // Matrix-core B x B shape.
+    // Matrix-core B x B shape. The K-split target only matters for corpora
+    // with fewer than a few 256-row tiles per CU; elsewhere it is not varied.
+    const uint64_t T = (c->n_rows + 255) / 256, tiles = T * (T + 1) / 2;
+    std::vector<uint32_t> bpcs = {start.blocks_per_cu};
+    if (tiles < 4ull * (uint64_t)ctx->cu_count) bpcs = {1u, 2u, 4u};
     MfmaConfig best_cfg = start;
     bool have = false;
     std::vector<MfmaConfig> cands;
-    for (uint32_t sw : {8u, 4u})
-        for (uint32_t band : {0u, 2u, 4u, 8u})
-            for (uint32_t bpc : {1u, 2u, 4u}) {
-                MfmaConfig m;
-                m.stage_words = sw;
-                m.band = band;
-                m.blocks_per_cu = bpc;
-                cands.push_back(m);
-            }
+    for (uint32_t waves : {8u, 4u})
+        for (uint32_t sw : {8u, 4u})
+            for (uint32_t band : {0u, 2u, 4u, 8u})
+                for (uint32_t bpc : bpcs) {
+                    MfmaConfig m;
+                    m.stage_words = sw;
+                    m.band = band;
+                    m.blocks_per_cu = bpc;
+                    m.waves = waves;
+                    cands.push_back(m);
+                }
     for (const MfmaConfig& m : cands) {
         ctx->mfma = m;
         ++ctx->model_version;
@@ -122,9 +135,7 @@ int storm_hip_autotune(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
         if (rc == STORM_HIP_OK) rc = time_call(ctx, c, o, &ms);
         if (rc != STORM_HIP_OK) break;
         ++result->n_tried;
-        if (m.stage_words == start.stage_words && m.band == start.band &&
-            m.blocks_per_cu == start.blocks_per_cu && c->dense)
-            result->ms_default = ms;
+        if (same(m, start) && c->dense) result->ms_default = ms;
         if (!have || ms < best) { best = ms; best_cfg = m; have = true; }
     }
     ctx->mfma = rc == STORM_HIP_OK ? best_cfg : start;

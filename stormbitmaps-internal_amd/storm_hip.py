@@ -73,11 +73,11 @@ class CostModel(C.Structure):
 
 class BBConfig(C.Structure):
     _fields_ = [("stage_words", C.c_uint32), ("band", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
+                ("blocks_per_cu", C.c_uint32), ("waves", C.c_uint32)]
 
     def as_dict(self):
         return {"stage_words": self.stage_words, "band": self.band,
-                "blocks_per_cu": self.blocks_per_cu}
+                "blocks_per_cu": self.blocks_per_cu, "waves": self.waves}
 
 
 class TuneResult(C.Structure):
@@ -313,8 +313,8 @@ class Context:
         return m
 
     # -- launch shapes -----------------------------------------------------------
-    def set_bb_config(self, stage_words=8, band=4, blocks_per_cu=2):
-        c = BBConfig(stage_words, band, blocks_per_cu, 0)
+    def set_bb_config(self, stage_words=8, band=4, blocks_per_cu=2, waves=8):
+        c = BBConfig(stage_words, band, blocks_per_cu, waves)
         check(lib().storm_hip_set_bb_config(self.h, C.byref(c)), "set_bb_config")
 
     def get_bb_config(self):

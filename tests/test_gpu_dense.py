@@ -76,19 +76,22 @@ def test_config1_known_sum(ctx, bb):
     assert np.array_equal(tri, want)
 
 
-SHAPE_SET = [(8, 0, 2), (4, 0, 1), (8, 4, 2), (4, 2, 4), (8, 3, 1), (8, 8, 16), (4, 64, 2)]
+SHAPE_SET = [(8, 0, 2, 8), (4, 0, 1, 8), (8, 4, 2, 8), (4, 2, 4, 8), (8, 3, 1, 8), (8, 8, 16, 8),
+             (4, 64, 2, 8), (8, 4, 2, 4), (4, 0, 1, 4), (8, 2, 4, 4)]
+DEFAULT_SHAPE = {"stage_words": 8, "band": 4, "blocks_per_cu": 2, "waves": 8}
 
 
 @pytest.mark.parametrize("cfg", SHAPE_SET)
 def test_every_launch_shape_exact(sh, ctx, cfg):
-    """Stage width x tile order x K-split target: the per-pair counts never
-    change (ragged rows over several 256-row tiles, ragged universe, K-split
-    and whole-K launches)."""
-    sw, band, bpc = cfg
+    """Stage width x tile order x K-split target x waves: the per-pair counts
+    never change (ragged rows over several 256-row tiles, ragged universe,
+    K-split and whole-K launches)."""
+    sw, band, bpc, waves = cfg
     _print_case("shape", cfg)
-    ctx.set_bb_config(sw, band, bpc)
+    ctx.set_bb_config(sw, band, bpc, waves)
     try:
-        assert ctx.get_bb_config() == {"stage_words": sw, "band": band, "blocks_per_cu": bpc}
+        assert ctx.get_bb_config() == {"stage_words": sw, "band": band, "blocks_per_cu": bpc,
+                                       "waves": waves}
         for n, bits, p, seed in [(1500, 9000, 0.05, 21), (777, 70001, 0.01, 22), (300, 64, 0.5, 23)]:
             words = O.bernoulli_rows(seed, p, bits, 0, n)
             want_sum, want = O.allpairs_words(words)
@@ -98,14 +101,14 @@ def test_every_launch_shape_exact(sh, ctx, cfg):
                     assert st["sum"] == want_sum, (cfg, n, bits, bb)
                     assert np.array_equal(c.tri(), want), (cfg, n, bits, bb)
     finally:
-        ctx.set_bb_config(8, 4, 2)
+        ctx.set_bb_config(**DEFAULT_SHAPE)
 
 
 def test_bb_config_rejects_bad_shapes(sh, ctx):
-    for bad in [(6, 0, 2), (8, 65, 2), (8, 4, 0), (8, 4, 17)]:
+    for bad in [(6, 0, 2, 8), (8, 65, 2, 8), (8, 4, 0, 8), (8, 4, 17, 8), (8, 4, 2, 6)]:
         with pytest.raises(sh.StormHipError):
             ctx.set_bb_config(*bad)
-    assert ctx.get_bb_config() == {"stage_words": 8, "band": 4, "blocks_per_cu": 2}
+    assert ctx.get_bb_config() == DEFAULT_SHAPE
 
 
 def test_autotune_dense_keeps_counts(ctx):
@@ -115,13 +118,14 @@ def test_autotune_dense_keeps_counts(ctx):
     want_sum, want = O.allpairs_words(words)
     with ctx.upload_contig(words) as c:
         r = ctx.autotune(c, bb="auto")
-        assert r["n_tried"] == 24 and r["tile"] == 0
+        # 2 wave counts x 2 stage widths x 4 tile orders x 3 K-split targets
+        assert r["n_tried"] == 48 and r["tile"] == 0
         assert r["bb"]["stage_words"] in (4, 8) and 0 < r["ms_best"] <= r["ms_default"] * 1.0001
         assert ctx.get_bb_config() == r["bb"]
         st = c.allpairs(out="tri", bb="auto")
         assert st["sum"] == want_sum
         assert np.array_equal(c.tri(), want)
-    ctx.set_bb_config(8, 4, 2)
+    ctx.set_bb_config(**DEFAULT_SHAPE)
 
 
 def _gpu_contig():

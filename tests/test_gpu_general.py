@@ -204,4 +204,4 @@ def test_autotune_general_keeps_counts(ctx, cases):
         st = c.allpairs(policy="analytic", out="tri", tile=r["tile"], use_graph=True)
         assert st["sum"] == want_sum
         assert np.array_equal(c.tri(), want)
-    ctx.set_bb_config(8, 4, 2)
+    ctx.set_bb_config(8, 4, 2, 8)

@@ -124,6 +124,10 @@ def test_learned_policy_is_exact_and_beats_untrained_analytic(sh, ctx):
         c.allpairs(policy="analytic", tile=64)
         T = (n + 63) // 64
         analytic = c.last_decisions(T * (T + 1) // 2)[idx]
+        # Training and held-out tiles drawn from the same distribution (the
+        # sample comes back in tile order, which runs from sparse to dense).
+        order = np.random.default_rng(11).permutation(len(X))
+        X, Y, M, idx, analytic = X[order], Y[order], M[order], idx[order], analytic[order]
         half = len(X) // 2
         loss = ctx.selector_train(X[:half], Y[:half], M[:half], epochs=400, batch=64, lr=3e-3,
                                   seed=7)
