@@ -535,15 +535,15 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         uint32_t i0 = e.bi * tile, j0 = e.bj * tile;
         uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
         if (e.sub) {
-            // Sub-block: `sb` rows per side from `off` rows in (the same rows
-            // on both sides of a diagonal tile, so it stays diagonal).
-            const uint32_t sb = e.sub & 0xFFFFu, off = e.sub >> 16;
-            uint32_t oi = min(off, ni > sb ? ni - sb : 0u), oj = min(off, nj > sb ? nj - sb : 0u);
-            if (e.bi == e.bj) oi = oj = min(oi, oj);
+            // One block of the tile (empty when it lies past the last row).
+            const uint32_t sb = e.sub & 0xFFFFu;
+            const uint32_t oi = ((e.sub >> 16) & 0xFFu) * sb, oj = (e.sub >> 24) * sb;
+            ni = oi < ni ? min(ni - oi, sb) : 0u;
+            nj = oj < nj ? min(nj - oj, sb) : 0u;
             i0 += oi; j0 += oj;
-            ni = min(ni - oi, sb); nj = min(nj - oj, sb);
         }
-        const bool diag = e.bi == e.bj;
+        // Pairs i < j only where the block straddles the diagonal.
+        const bool diag = i0 == j0;
         const uint32_t np = ni * nj;
         // Lane mode when the sparser tile's rows are tiny.
         const TileMeta& A = D.tm[e.bi];

@@ -36,10 +36,19 @@ struct DevCorpus {
 };
 
 // One queued tile pair: corpus index, tile row, tile column, and `sub`: 0 for
-// the whole tile, else a sub-block of (sub & 0xFFFF) rows on each side
-// starting (sub >> 16) rows into the tile (the selector's sampling runs time
-// a sub-block of median-density rows and scale by the pair ratio).
+// the whole tile, else one block of it: (sub & 0xFFFF) rows per side, the
+// ((sub >> 16) & 0xFF)-th such block of the tile's rows and the (sub >> 24)-th
+// of its columns. Per-pair cells take 256-row tiles as 64 x 64 blocks (more
+// work items than CUs several times over); the selector's sampling runs time
+// one block of median-density rows and scale by the pair ratio.
 struct QItem { uint32_t inst, bi, bj, sub; };
+
+__host__ __device__ inline uint32_t make_sub(uint32_t rows, uint32_t bi_blk, uint32_t bj_blk) {
+    return rows | (bi_blk << 16) | (bj_blk << 24);
+}
+
+// Rows per block when a tile is split for the per-pair cells.
+constexpr uint32_t kSplitRows = 64;
 
 int launch_bb_valu_queue(hipStream_t s, int bm, const DevCorpus* Ds, const QItem* queue,
                          const unsigned int* qcount, unsigned int* qhead, uint32_t grid,

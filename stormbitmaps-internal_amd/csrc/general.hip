@@ -24,7 +24,7 @@ int launch_tile_meta(hipStream_t s, const DevCorpus& D, uint32_t tile, uint32_t 
 int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel& M, uint32_t reps,
                   uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
-                  uint32_t inst, DevCounters* ctr);
+                  uint32_t inst, uint32_t tile, DevCounters* ctr);
 int launch_compose_rowmap(hipStream_t s, const uint32_t* order, const uint32_t* bm_row,
                           uint32_t npad, uint32_t* rowmap);
 int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
@@ -94,8 +94,9 @@ struct Plan {
     std::vector<DevCorpus> h_corpora;       // host copy of d_corpora
     uint32_t tile = 0;
     uint64_t totalP = 0;
+    uint64_t qstride = 0;                   // queue slots per cell class
     DevCorpus* d_corpora = nullptr;
-    QItem* queue = nullptr;                 // kNumCells x totalP
+    QItem* queue = nullptr;                 // kNumCells x qstride
     DevCounters* ctr = nullptr;
     unsigned long long* sums = nullptr;     // one per corpus
     unsigned long long* finish = nullptr;   // one per corpus, then the start stamp
@@ -173,8 +174,11 @@ int build_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
         p->totalP += w->P;
     }
     const uint64_t P1 = p->totalP ? p->totalP : 1;
+    // A tile of a per-pair cell may be queued as (tile / kSplitRows)^2 blocks.
+    const uint64_t split = tile > kSplitRows ? (uint64_t)(tile / kSplitRows) * (tile / kSplitRows) : 1;
+    p->qstride = P1 * split;
     SH_HIP(hipMalloc(&p->d_corpora, sizeof(DevCorpus) * std::max<size_t>(n, 1)));
-    SH_HIP(hipMalloc(&p->queue, sizeof(QItem) * P1 * kNumCells));
+    SH_HIP(hipMalloc(&p->queue, sizeof(QItem) * p->qstride * kNumCells));
     SH_HIP(hipMalloc(&p->ctr, sizeof(DevCounters)));
     SH_HIP(hipMalloc(&p->sums, 8 * std::max<size_t>(n, 1)));
     SH_HIP(hipMalloc(&p->finish, 8 * (n + 1)));
@@ -249,7 +253,8 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
             if (have) { lc = w->learned_cost; lok = w->learned_ok; }
         }
         SH_TRY(launch_select(s, w->tm, w->T, M, reps_of(c), o.policy, o.fixed_cell, lc, lok,
-                             w->decisions, p->queue, p->totalP, (uint32_t)k, p->ctr));
+                             w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
+                             p->ctr));
         SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
         // Only corpora with bitmap rows can queue B x B tiles.
         if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
@@ -272,7 +277,7 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     int rc = STORM_HIP_OK;
     hipStream_t sb = ctx->aux[0];
     if (timed) (void)hipEventRecord(ctx->ev[2], sb);
-    const QItem* qbb = p->queue + (uint64_t)STORM_HIP_CELL_BB * p->totalP;
+    const QItem* qbb = p->queue + (uint64_t)STORM_HIP_CELL_BB * p->qstride;
     const uint32_t gbb = (uint32_t)std::min<uint64_t>(
         P1, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
     if (p->tile == 256)
@@ -289,9 +294,9 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     for (int cell = 0; cell < kNumCells && rc == STORM_HIP_OK; ++cell) {
         if (cell == STORM_HIP_CELL_BB) continue;
         const uint32_t grid = (uint32_t)std::min<uint64_t>(
-            P1, (uint64_t)ctx->cu_count * cell_blocks_per_cu(cell));
+            p->qstride, (uint64_t)ctx->cu_count * cell_blocks_per_cu(cell));
         rc = launch_cell(cell, (k++ & 1) ? ctx->aux[1] : s, grid, p->d_corpora, M,
-                         p->queue + (uint64_t)cell * p->totalP, p->ctr, nullptr);
+                         p->queue + (uint64_t)cell * p->qstride, p->ctr, nullptr);
     }
     // Join the side streams back, whatever happened (a capture must close).
     for (int a = 0; a < 2; ++a) {
@@ -589,19 +594,24 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
     // of median-density rows of each tile (their cost is per pair) and the
     // time is scaled by the tile's pair count over the block's.
     constexpr uint32_t kSampleRows = 32;
-    const uint32_t sub = tile > kSampleRows ? ((tile - kSampleRows) / 2) << 16 | kSampleRows : 0;
     for (uint32_t k = 0; k < S; ++k) {
         uint32_t bi, bj;
         tri_tile(cand[k], w->T, bi, bj);
-        items[k] = QItem{0, bi, bj, sub};
         // The cells the learned selector may rank on this tile.
         elig[k] = learned_candidates(M, tm[bi], tm[bj], eligible_mask(tm[bi], tm[bj], reps));
         tile_index[k] = cand[k];
         std::memcpy(features + (size_t)k * F, &X[(size_t)cand[k] * F], F * 4);
-        if (sub) {
-            const double ni = tm[bi].rows, nj = tm[bj].rows;
-            const double si = std::min<double>(ni, kSampleRows), sj = std::min<double>(nj, kSampleRows);
-            const double full = bi == bj ? ni * (ni - 1) / 2 : ni * nj;
+        items[k] = QItem{0, bi, bj, 0};
+        if (tile > kSampleRows) {
+            // The middle block of each side (the same block on a diagonal tile).
+            const uint32_t ri = tm[bi].rows, rj = tm[bj].rows, mid = tile / kSampleRows / 2;
+            uint32_t xi = std::min(mid, ri ? (ri - 1) / kSampleRows : 0);
+            uint32_t xj = std::min(mid, rj ? (rj - 1) / kSampleRows : 0);
+            if (bi == bj) xi = xj = std::min(xi, xj);
+            items[k].sub = make_sub(kSampleRows, xi, xj);
+            const double si = std::min<double>(kSampleRows, ri - xi * kSampleRows);
+            const double sj = std::min<double>(kSampleRows, rj - xj * kSampleRows);
+            const double full = bi == bj ? (double)ri * (ri - 1) / 2 : (double)ri * rj;
             const double part = bi == bj ? si * (si - 1) / 2 : si * sj;
             scale[k] = part > 0 ? full / part : 1.0;
         }
@@ -630,7 +640,7 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
                 slot_of.push_back(k);
             }
         if (q.empty()) continue;
-        QItem* dq = p->queue + (uint64_t)cell * p->totalP;
+        QItem* dq = p->queue + (uint64_t)cell * p->qstride;
         const uint32_t per_cu = cell == STORM_HIP_CELL_BB ? bb_blocks_per_cu(tile, ctx->mfma)
                                                           : cell_blocks_per_cu(cell);
         const uint32_t grid = (uint32_t)std::min<uint64_t>(q.size(), (uint64_t)cu * per_cu);

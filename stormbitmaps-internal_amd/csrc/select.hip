@@ -191,7 +191,8 @@ __global__ __launch_bounds__(256) void select_kernel(
     const TileMeta* __restrict__ tm, uint32_t T, uint64_t P, DevModel M, uint32_t reps,
     uint32_t policy, uint32_t fixed_cell, const float* __restrict__ learned_cost,
     const uint8_t* __restrict__ learned_ok, uint8_t* __restrict__ decisions,
-    QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, DevCounters* __restrict__ ctr) {
+    QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, bool split,
+    DevCounters* __restrict__ ctr) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P) return;
     uint32_t bi, bj;
@@ -225,8 +226,21 @@ __global__ __launch_bounds__(256) void select_kernel(
         if (!(elig >> p & 1)) p = cheapest_eligible(M, a, b, elig);
     }
     decisions[t] = (uint8_t)p;
-    const uint32_t slot = atomicAdd(&ctr->qcount[p], 1u);
-    queue[(uint64_t)p * qstride + slot] = QItem{inst, bi, bj, 0};
+    if (split && p != STORM_HIP_CELL_BB && p != STORM_HIP_CELL_EMPTY) {
+        // Per-pair cells: the tile as kSplitRows x kSplitRows blocks (upper
+        // triangle of blocks on a diagonal tile), each its own work item.
+        const uint32_t ni = (a.rows + kSplitRows - 1) / kSplitRows;
+        const uint32_t nj = (b.rows + kSplitRows - 1) / kSplitRows;
+        const uint32_t count = bi == bj ? ni * (ni + 1) / 2 : ni * nj;
+        uint32_t slot = atomicAdd(&ctr->qcount[p], count);
+        for (uint32_t x = 0; x < ni; ++x)
+            for (uint32_t y = bi == bj ? x : 0; y < nj; ++y)
+                queue[(uint64_t)p * qstride + slot++] =
+                    QItem{inst, bi, bj, make_sub(kSplitRows, x, y)};
+    } else {
+        const uint32_t slot = atomicAdd(&ctr->qcount[p], 1u);
+        queue[(uint64_t)p * qstride + slot] = QItem{inst, bi, bj, 0};
+    }
     atomicAdd(&ctr->cell_tiles[p], 1ull);
     if (p == STORM_HIP_CELL_BB) {
         // B x B tiles skip the per-pair span test (same cost either way);
@@ -262,12 +276,12 @@ int launch_tile_meta(hipStream_t s, const DevCorpus& D, uint32_t tile, uint32_t 
 int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel& M, uint32_t reps,
                   uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
-                  uint32_t inst, DevCounters* ctr) {
+                  uint32_t inst, uint32_t tile, DevCounters* ctr) {
     const uint64_t P = (uint64_t)T * (T + 1) / 2;
     if (P == 0) return STORM_HIP_OK;
     hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, s, tm, T, P,
                        M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions, queue,
-                       qstride, inst, ctr);
+                       qstride, inst, tile > kSplitRows, ctr);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }
