@@ -375,7 +375,11 @@ def dense_bench(sh, args):
     corpus = ctx.gen_bernoulli(n, bits, p, seed)
     progress(f"generated {n} x {bits} bits on the device; B x B kernel {bb}")
     tuned = None
-    if bb != "valu" and not args.no_tune:
+    if args.shape:
+        sw, band, bpc, waves = (int(x) for x in args.shape.split(","))
+        ctx.set_bb_config(sw, band, bpc, waves)
+        tuned = {"pinned": ctx.get_bb_config()}
+    elif bb != "valu" and not args.no_tune:
         # The autotuner picks the launch shape on this corpus (outside the
         # timed steps); the timed steps run with it.
         tuned = ctx.autotune(corpus, bb=bb)
@@ -468,6 +472,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="dense configs: keep the default B x B launch shape")
+    ap.add_argument("--shape", default="",
+                    help="dense configs: pin the B x B launch shape "
+                         "stage_words,band,blocks_per_cu,waves (no tuning)")
     ap.add_argument("--rows", type=int, default=0, help="sparse configs: first N rows only")
     ap.add_argument("--tile", type=int, default=256,
                     help="rows per selection tile (sparse and batch configs)")
