@@ -150,6 +150,7 @@ def sparse_bench(sh, args):
     corpus = ctx.from_lists(offs, pos, u, bitmap_policy=flags)
     t_up = time.perf_counter() - t0
     progress(f"built and uploaded in {t_up:.1f} s (bitmaps for every row: {all_bitmaps})")
+    build = None if args.no_build_timing else build_timing(sh, ctx, offs, pos, u)
     del pos
     pairs = n * (n - 1) // 2
     res = {}
@@ -197,6 +198,22 @@ def sparse_bench(sh, args):
     loss = ctx.selector_train(X, Y, M, epochs=300, batch=128, lr=3e-3, seed=5)
     t_train = time.perf_counter() - t0
     res["learned"] = timed("learned", use_graph=True)
+    # Thresholded mode: only pairs with Jaccard >= t, against the exact call.
+    thresh = {}
+    for t in (float(x) for x in args.threshold.split(",") if x):
+        ts, n_hits, st = [], 0, None
+        for k in range(1 + args.steps):
+            t1 = time.perf_counter()
+            st, hits = corpus.allpairs_threshold(t, policy="analytic", tile=args.tile,
+                                                 use_graph=True)
+            if k:
+                ts.append(time.perf_counter() - t1)
+            n_hits = len(hits)
+        ms = 1000 * float(np.median(ts))
+        thresh[str(t)] = {"pairs_reported": n_hits, "ms": ms, "ms_device": st["ms_device"],
+                          "tiles_settled_empty": st["cell_tiles"].get("EMPTY", 0),
+                          "over_exact_analytic": res["analytic"]["ms"] / ms}
+        progress(f"threshold {t}: {n_hits} pairs in {ms:.1f} ms")
     if not args.no_allbitmap:
         res["all_bitmap"] = timed("all_bitmap")
     sums = {v["sum"] for v in res.values()}
@@ -215,12 +232,41 @@ def sparse_bench(sh, args):
                      "train_s": t_train},
         "calibrated_model": model, "calibrate_s": t_cal,
         "setup_s": {"generate": t_gen, "build_upload": t_up},
+        "representation_build": build,
+        "thresholded": thresh,
     }
     if "all_bitmap" in res:
         out["learned_over_all_bitmap"] = res["learned"]["pairs_per_s"] / res["all_bitmap"]["pairs_per_s"]
     corpus.free()
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def build_timing(sh, ctx, offs, pos, u):
+    """Representation building (runs, EWAH + markers, complements, meta,
+    bitmaps of dense rows) three ways: from lists already in HBM on the device,
+    from host lists (upload + device build), and on host threads + upload."""
+    flags = 0 | 4 | 8 | 16
+    to = sh.DeviceArray(np.asarray(offs, dtype=np.uint64))
+    tp = sh.DeviceArray(np.asarray(pos, dtype=np.uint32))
+    out = {"elements": int(len(pos)), "rows": int(len(offs) - 1), "flags": flags}
+    for name, make in (("device_resident_s", lambda: ctx.from_device_lists(to, tp, u, flags)),
+                       ("host_lists_device_build_s", lambda: ctx.from_lists(offs, pos, u, flags)),
+                       ("host_build_s", lambda: ctx.from_lists(offs, pos, u, flags,
+                                                               on_host=True))):
+        ts = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            c = make()
+            ts.append(time.perf_counter() - t0)
+            c.free()
+        out[name] = min(ts)
+        progress(f"representation build {name}: {out[name]:.3f} s")
+    out["device_elements_per_s"] = len(pos) / out["device_resident_s"]
+    out["device_over_host"] = out["host_build_s"] / out["device_resident_s"]
+    to.free()
+    tp.free()
+    return out
 
 
 class _Rng:
@@ -486,6 +532,11 @@ def main():
     ap.add_argument("--sample-chunks", type=int, default=8,
                     help="sampling passes (a progress line after each)")
     ap.add_argument("--no-allbitmap", action="store_true")
+    ap.add_argument("--no-build-timing", action="store_true",
+                    help="sparse configs: skip timing the representation builders")
+    ap.add_argument("--threshold", default="0.01,0.1,0.5",
+                    help="sparse configs: Jaccard thresholds to time in thresholded mode "
+                         "(comma list, empty for none)")
     ap.add_argument("--instance-mib", type=int, default=256,
                     help="batch configs: largest estimated instance size run")
     ap.add_argument("--budget-gib", type=int, default=16,

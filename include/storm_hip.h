@@ -109,11 +109,15 @@ int  storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* arena,
  * RowMeta, and upload. The reference equivalent is building std::vector<Row>
  * with build_row before allpairs_sum. flags: bits 0-1 bitmap placement
  * (0: rows with cardinality*64 >= universe, 1: every row, 2: none), plus the
- * STORM_HIP_BUILD_* representation bits. */
+ * STORM_HIP_BUILD_* representation bits. The lists are uploaded and the
+ * representations built on the device, one wavefront per row; with
+ * STORM_HIP_BUILD_ON_HOST they are built by host threads and uploaded (same
+ * arenas byte for byte). */
 enum {
     STORM_HIP_BITMAP_DENSE = 0, STORM_HIP_BITMAP_ALL = 1, STORM_HIP_BITMAP_NONE = 2,
     STORM_HIP_BUILD_RUNS = 4, STORM_HIP_BUILD_EWAH = 8, STORM_HIP_BUILD_COMP = 16,
-    STORM_HIP_BUILD_ALL = 4 | 8 | 16
+    STORM_HIP_BUILD_ALL = 4 | 8 | 16,
+    STORM_HIP_BUILD_ON_HOST = 32
 };
 /* Seeded synthetic corpora as host CSR lists (storm::generate's role,
  * kernels/storm_gen.cpp:173-232, with the spectra the benchmark workloads
@@ -135,6 +139,49 @@ void storm_hip_gen_free(void* p);
 int  storm_hip_corpus_from_lists(storm_hip_ctx* ctx, const uint64_t* offs, const uint32_t* pos,
                                  uint32_t n_rows, uint32_t universe, uint32_t flags,
                                  storm_hip_corpus** out);
+/* As storm_hip_corpus_from_lists for lists already in device memory (d_offs:
+ * n_rows + 1 u64 offsets, d_pos: u32 positions); the corpus keeps copies and
+ * builds on the device (STORM_HIP_BUILD_ON_HOST is ignored). */
+int  storm_hip_corpus_from_device_lists(storm_hip_ctx* ctx, const uint64_t* d_offs,
+                                        const uint32_t* d_pos, uint32_t n_rows,
+                                        uint32_t universe, uint32_t flags,
+                                        storm_hip_corpus** out);
+
+/* STORMBIN ingest. Format (tools/sets2bin.py:23-25): b"STORMBIN", then u32
+ * version, n_rows, n_bits, pad, then per row a u32 count followed by count
+ * ascending u32 positions. As the reference loader (load_stormbin,
+ * bench/bench_allpairs.cpp:44-72): rows are read to the end of the file or the
+ * first truncated record; every `stride`-th row (stride 0 means 1) is kept
+ * unless it is empty; at most max_rows rows are kept (0: no limit); the
+ * universe is n_bits; fewer than 2 kept rows is STORM_HIP_ERR_INVALID.
+ * storm_hip_stormbin_read returns host CSR lists (release with
+ * storm_hip_gen_free); storm_hip_corpus_from_stormbin uploads the row records
+ * as they lie in the file, gathers them into the list on the device and
+ * builds every representation there (flags as storm_hip_corpus_from_lists).
+ * Corpus row k is the k-th kept row. */
+int  storm_hip_stormbin_read(const char* path, uint32_t max_rows, uint32_t stride,
+                             uint64_t** offs, uint32_t** pos, uint32_t* n_rows,
+                             uint32_t* universe);
+int  storm_hip_corpus_from_stormbin(storm_hip_ctx* ctx, const char* path, uint32_t max_rows,
+                                    uint32_t stride, uint32_t flags, storm_hip_corpus** out);
+
+/* Arenas of a general corpus back to the host (layouts as storm_hip_arena;
+ * EWAH marker tables: per marker its first decoded word and its offset in the
+ * row's EWAH buffer; BM_ROW: bitmap slot of each row or 0xFFFFFFFF; BITMAP:
+ * one row of `ld` words per slot, ld a multiple of 64). Sizes are in bytes;
+ * a download must ask for exactly the arena's size. */
+enum {
+    STORM_HIP_ARENA_LIST_OFF = 0, STORM_HIP_ARENA_LIST, STORM_HIP_ARENA_RUN_OFF,
+    STORM_HIP_ARENA_RUN_START, STORM_HIP_ARENA_RUN_END, STORM_HIP_ARENA_EWAH_OFF,
+    STORM_HIP_ARENA_EWAH, STORM_HIP_ARENA_MK_OFF, STORM_HIP_ARENA_MK_WORD,
+    STORM_HIP_ARENA_MK_POS, STORM_HIP_ARENA_COMP_OFF, STORM_HIP_ARENA_COMP,
+    STORM_HIP_ARENA_META, STORM_HIP_ARENA_BM_ROW, STORM_HIP_ARENA_BITMAP,
+    STORM_HIP_ARENA_COUNT
+};
+int  storm_hip_corpus_arena_sizes(storm_hip_ctx* ctx, const storm_hip_corpus* c,
+                                  uint64_t bytes[STORM_HIP_ARENA_COUNT]);
+int  storm_hip_corpus_download_arena(storm_hip_ctx* ctx, const storm_hip_corpus* c,
+                                     uint32_t which, void* dst, uint64_t bytes);
 /* Materialise a bitmap on the device for every row that lacks one (needed by
  * the all-bitmap policy on sparse corpora). */
 int  storm_hip_corpus_materialize_bitmaps(storm_hip_ctx* ctx, storm_hip_corpus* c);
@@ -218,6 +265,32 @@ int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c,
  * row_sum[i] = sum_{j != i} |Xi n Xj|. */
 int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                               uint64_t* row_sum);
+
+/* ---- thresholded mode -----------------------------------------------------
+ *
+ * Exact |Xi n Xj| for exactly the pairs i < j whose Jaccard similarity reaches
+ * t, the opt-in contract of the reference's thresholded mode
+ * (bench/bench_threshold.cpp, README.md:114-161): a pair is reported when
+ * |Xi n Xj| > 0 and |Xi n Xj| / (|Xi| + |Xj| - |Xi n Xj|) >= t, evaluated in
+ * double precision as there. The reference generates candidates with a
+ * prefix-filter index; here the selection kernel drops every tile pair whose
+ * cardinality ranges bound the Jaccard below t (min/max length filter, exact
+ * in the same double arithmetic), the cells skip pairs failing the same bound,
+ * and surviving pairs are counted exactly and appended on the device.
+ * General corpora and the ANALYTIC / LEARNED / FIXED policies (else
+ * STORM_HIP_ERR_UNSUPPORTED); t outside (0, 1] is STORM_HIP_ERR_INVALID.
+ * *n_pairs receives the number of reported pairs; they are kept on the corpus
+ * sorted by (i, j) in original row ids until the next thresholded call.
+ * stats->sum is the sum over the pairs the call counted (not every pair). */
+typedef struct {
+    uint32_t i, j;      /* original row ids, i < j */
+    uint32_t count;     /* |Xi n Xj| */
+} storm_hip_pair;
+int storm_hip_allpairs_threshold(storm_hip_ctx* ctx, storm_hip_corpus* c,
+                                 const storm_hip_options* opt, double t,
+                                 storm_hip_stats* stats, uint64_t* n_pairs);
+int storm_hip_result_copy_pairs(storm_hip_ctx* ctx, const storm_hip_corpus* c,
+                                uint64_t offset, uint64_t count, storm_hip_pair* out);
 
 /* ---- launch shapes and the autotuner -------------------------------------
  * No reference counterpart (the reference's kernels have fixed shapes).

@@ -177,6 +177,45 @@ def lists_closed_forms(offs, pos, universe):
     return total, rs
 
 
+class OrcRow(C.Structure):
+    """orc_row (storm_oracle.h)."""
+    _fields_ = [("bitmap", C.POINTER(C.c_uint64)), ("n_words", C.c_uint32),
+                ("rank", C.POINTER(C.c_uint64)), ("n_rank", C.c_uint32),
+                ("occ", C.POINTER(C.c_uint64)), ("n_occ", C.c_uint32),
+                ("list", C.POINTER(C.c_uint32)), ("n_list", C.c_uint32),
+                ("run_start", C.POINTER(C.c_uint32)), ("run_end", C.POINTER(C.c_uint32)),
+                ("n_runs", C.c_uint32),
+                ("ewah", C.POINTER(C.c_uint64)), ("n_ewah", C.c_uint32),
+                ("comp_list", C.POINTER(C.c_uint32)), ("n_comp", C.c_uint32),
+                ("comp_start", C.POINTER(C.c_uint32)), ("comp_end", C.POINTER(C.c_uint32)),
+                ("n_comp_runs", C.c_uint32),
+                ("meta", C.c_uint32 * 6), ("has_rank", C.c_uint8), ("has_occ", C.c_uint8)]
+
+
+def build_row(pos, u):
+    """orc_build_row (build_row, storm_repr.cpp:102-207) as numpy arrays."""
+    L = oracle()
+    L.orc_build_row.argtypes = [C.POINTER(OrcRow), C.c_void_p, C.c_uint32, C.c_uint32]
+    L.orc_build_row.restype = C.c_int
+    L.orc_free_row.argtypes = [C.POINTER(OrcRow)]
+    pos = np.ascontiguousarray(pos, dtype=np.uint32)
+    r = OrcRow()
+    if L.orc_build_row(C.byref(r), _p(pos), len(pos), u) != 0:
+        raise ValueError("orc_build_row failed")
+
+    def arr(p, n, dt):
+        return np.ctypeslib.as_array(p, shape=(n,)).astype(dt).copy() if n else np.zeros(0, dt)
+
+    out = dict(meta=np.array(list(r.meta), np.uint32), bitmap=arr(r.bitmap, r.n_words, np.uint64),
+               rank=arr(r.rank, r.n_rank, np.uint64), occ=arr(r.occ, r.n_occ, np.uint64),
+               run_start=arr(r.run_start, r.n_runs, np.uint32),
+               run_end=arr(r.run_end, r.n_runs, np.uint32),
+               ewah=arr(r.ewah, r.n_ewah, np.uint64), comp=arr(r.comp_list, r.n_comp, np.uint32),
+               has_rank=r.has_rank, has_occ=r.has_occ)
+    L.orc_free_row(C.byref(r))
+    return out
+
+
 def ref_generate(n_rows, universe, density, structure, spectrum, seed):
     """Corpus from the reference generator (kernels/storm_gen.cpp:173-232)."""
     L = reference()

@@ -20,16 +20,16 @@
 // current stage's MFMAs and consumed one stage later, so their latency hides
 // behind a stage of matrix work.
 //
-// Bits are assigned to matrix K positions freely: A and B fragments are built
-// by the same function from the same bit positions, so each MFMA sums products
-// over the same set of positions whichever order the matrix core assigns to
-// them. Lane group g (lanes 32g..32g+31) takes dwords g*SW .. g*SW+SW-1 of the
-// stage's 2*SW dwords (natural memory order), one 32-bit chunk per MFMA step;
-// for a chunk w:
-//   FP4: dword d (0..3) = (w >> (d-1)) & 0x22222222   (nibble n = bit 4n+d)
+// Bits are assigned to matrix K positions freely: A and B fragments place the
+// same bit positions in the same K slots, so each MFMA sums products over the
+// same set of positions whichever order the matrix core assigns to them. Lane
+// group g (lanes 32g..32g+31) takes dwords g*SW .. g*SW+SW-1 of the stage's
+// 2*SW dwords (natural memory order), one 32-bit chunk per MFMA step; for a
+// chunk w:
+//   FP4: nibble n of dword d (0..3) holds bit 4n+d, encoded 0.5/1/2/2 in A and
+//        2/1/0.5/0.5 in B (a mask, or a shift and a mask, per dword)
 //   I8 : two K=32 steps h = 0, 1 per chunk; dword d (0..3) of step h =
 //        (w >> (4h+d)) & 0x01010101                   (byte j = bit 8j+4h+d)
-// i.e. one shift and one AND per output dword.
 #include "device_common.h"
 #include "general.h"
 
@@ -46,9 +46,20 @@ namespace {
 constexpr int kTile = 256;        // pairs block: kTile x kTile
 constexpr int kUnitScale = 127;   // E8M0 exponent of 1.0
 
-__device__ inline i32x8 expand_fp4(uint32_t w) {
-    return i32x8{(int)((w << 1) & 0x22222222u), (int)(w & 0x22222222u),
-                 (int)((w >> 1) & 0x22222222u), (int)((w >> 2) & 0x22222222u), 0, 0, 0, 0};
+// FP4 operands: A and B give bit 4n+d of a chunk the same K slot (nibble n of
+// dword d) but may encode it differently, as long as a set bit in both
+// multiplies to exactly 1.0. A keeps bits where they are whenever a mask
+// alone yields a usable value (d = 0: 0.5, d = 1: 1.0, d = 2: 2.0; the sign
+// bit d = 3 moves to 2.0); B carries the reciprocal (2.0, 1.0, 0.5, 0.5).
+// A costs 5 VALU operations per chunk, B 7; A fragments are expanded twice as
+// often as B fragments per wave.
+__device__ inline i32x8 expand_fp4_a(uint32_t w) {
+    return i32x8{(int)(w & 0x11111111u), (int)(w & 0x22222222u), (int)(w & 0x44444444u),
+                 (int)((w >> 1) & 0x44444444u), 0, 0, 0, 0};
+}
+__device__ inline i32x8 expand_fp4_b(uint32_t w) {
+    return i32x8{(int)((w << 2) & 0x44444444u), (int)(w & 0x22222222u),
+                 (int)((w >> 2) & 0x11111111u), (int)((w >> 3) & 0x11111111u), 0, 0, 0, 0};
 }
 __device__ inline i8x16 expand_i8(uint32_t w, int g) {
     const uint32_t s = 4u * g;
@@ -77,13 +88,16 @@ struct BBMfmaSmem {
 // One 256 x 256 block over stages [s0, s0 + n_stages) of SW words. With
 // `accumulate`, counts are atomically added to `tri` (a K-split block: other
 // blocks add the other stage ranges of the same pairs into a zeroed triangle).
-template <bool FP4, int SW, int NW>
+// THR (thresholded queue launches): every pair goes through the output of the
+// corpus view `Dp`; a separate instance, so the common epilogue stays small.
+template <bool FP4, int SW, int NW, bool THR = false>
 __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint64_t* X, uint32_t ld,
                                                  uint32_t s0, uint32_t n_stages,
                                                  const uint32_t* __restrict__ rowmap,
                                                  const uint32_t* __restrict__ perm, uint32_t n,
                                                  uint32_t bi, uint32_t bj,
-                                                 uint32_t* __restrict__ tri, bool accumulate) {
+                                                 uint32_t* __restrict__ tri, bool accumulate,
+                                                 const DevCorpus* __restrict__ Dp = nullptr) {
     using Acc = typename std::conditional<FP4, f32x16, i32x16>::type;
     constexpr int L = SW / 2;        // 16-byte loads per row per stage
     constexpr int R = SW / 4;        // 16-byte LDS reads per fragment row per stage
@@ -181,10 +195,10 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
             if constexpr (FP4) {
                 i32x8 fb[NB];
 #pragma unroll
-                for (int q = 0; q < NB; ++q) fb[q] = expand_fp4(wb[q]);
+                for (int q = 0; q < NB; ++q) fb[q] = expand_fp4_b(wb[q]);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    const i32x8 fa = expand_fp4(wa[m]);
+                    const i32x8 fa = expand_fp4_a(wa[m]);
 #pragma unroll
                     for (int q = 0; q < NB; ++q)
                         acc(m, q) = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
@@ -227,7 +241,9 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
                 if (j >= n || j <= i) continue;
                 const uint32_t val = (uint32_t)acc(m, q)[v];
                 local += val;
-                if (tri) {
+                if constexpr (THR) {
+                    emit_pair(*Dp, oi, perm ? perm[j] : j, val);
+                } else if (tri) {
                     const uint32_t oj = perm ? perm[j] : j;
                     uint32_t* dst = &tri[oi < oj ? tri_index(oi, oj, n) : tri_index(oj, oi, n)];
                     if (accumulate) { if (val) atomicAdd(dst, val); }
@@ -238,7 +254,7 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
     return local;
 }
 
-template <bool FP4, int SW, int NW>
+template <bool FP4, int SW, int NW, bool THR = false>
 __global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
     const uint64_t* __restrict__ X, uint32_t ld, uint32_t n_stages, uint32_t k_split,
     uint32_t band, const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm,
@@ -249,6 +265,7 @@ __global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
     __shared__ __attribute__((aligned(16))) BBMfmaSmem<SW> sm;
     __shared__ uint32_t s_t;
     uint64_t local = 0;
+    if (THR && !qcount) return;   // thresholded instances run in queue mode only
     if (!qcount) {
         // Grid mode: position p = slice p % k_split of tile p / k_split, tiles
         // in row-major upper-triangle order (band 0) or the banded order.
@@ -275,8 +292,9 @@ __global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
             const DevCorpus D = Ds[e.inst];
             const uint64_t t0 = wall_clock64();
             const uint32_t ns = max(1u, (D.nw + SW - 1) / SW);
-            uint64_t part = bb_mfma_tile<FP4, SW, NW>(sm, D.bitmap, D.ld, 0, ns, D.rowmap, D.perm,
-                                                  D.n, e.bi, e.bj, D.tri, false);
+            uint64_t part = bb_mfma_tile<FP4, SW, NW, THR>(sm, D.bitmap, D.ld, 0, ns, D.rowmap,
+                                                       D.perm, D.n, e.bi, e.bj, D.tri, false,
+                                                       &Ds[e.inst]);
             part = wave_sum_u64(part);
             if ((threadIdx.x & 63) == 0 && part) atomicAdd(D.sum, (unsigned long long)part);
             __syncthreads();
@@ -418,11 +436,23 @@ uint32_t bb_blocks_per_cu(uint32_t tile, const MfmaConfig& cfg) {
 
 // Queue mode over the corpora in Ds. The caller has checked every corpus that
 // can queue a tile with bb_mfma_check (and picks fp4 only when all allow it).
+// Thresholded calls run the 8-word, 8-wave shape's thresholded instance.
 int launch_bb_mfma_queue(bool fp4, const MfmaConfig& cfg, hipStream_t s, const DevCorpus* Ds,
                          const QItem* queue, const unsigned int* qcount, unsigned int* qhead,
-                         uint32_t grid, unsigned long long* tile_ticks) {
+                         uint32_t grid, unsigned long long* tile_ticks, bool thresholded) {
     if (grid == 0) return STORM_HIP_OK;
-    SH_TRY(with_shape(fp4, cfg, QueueLaunch{s, grid, Ds, queue, qcount, qhead, tile_ticks}));
+    if (thresholded) {
+        if (fp4)
+            hipLaunchKernelGGL((bb_mfma_kernel<true, 8, 8, true>), dim3(grid), dim3(512), 0, s,
+                               nullptr, 0u, 1u, 1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue,
+                               qcount, qhead, nullptr, nullptr, tile_ticks);
+        else
+            hipLaunchKernelGGL((bb_mfma_kernel<false, 8, 8, true>), dim3(grid), dim3(512), 0, s,
+                               nullptr, 0u, 1u, 1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue,
+                               qcount, qhead, nullptr, nullptr, tile_ticks);
+    } else {
+        SH_TRY(with_shape(fp4, cfg, QueueLaunch{s, grid, Ds, queue, qcount, qhead, tile_ticks}));
+    }
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

@@ -24,11 +24,14 @@ struct BBValuSmem {
     uint64_t B[2][KC][BM];
 };
 
-template <int BM>
+// THR (thresholded queue launches): pairs go through the output of the corpus
+// view `Dp` instead of `tri`.
+template <int BM, bool THR = false>
 __device__ uint64_t bb_valu_tile(BBValuSmem<BM>& sm, const uint64_t* __restrict__ X, uint32_t ld,
                                  uint32_t kwords, const uint32_t* __restrict__ rowmap,
                                  const uint32_t* __restrict__ perm, uint32_t n, uint32_t bi,
-                                 uint32_t bj, uint32_t* __restrict__ tri) {
+                                 uint32_t bj, uint32_t* __restrict__ tri,
+                                 const DevCorpus* __restrict__ Dp = nullptr) {
     constexpr int KC = BBValuSmem<BM>::KC;
     constexpr int TM = BM / 16;          // rows (and cols) per thread
     constexpr int WPT = BM * KC / 256;   // words each thread loads per panel
@@ -104,7 +107,9 @@ __device__ uint64_t bb_valu_tile(BBValuSmem<BM>& sm, const uint64_t* __restrict_
             if (j >= n || j <= i) continue;
             const uint32_t v = acc[r][c];
             local += v;
-            if (tri) {
+            if constexpr (THR) {
+                emit_pair(*Dp, oi, perm ? perm[j] : j, v);
+            } else if (tri) {
                 const uint32_t oj = perm ? perm[j] : j;
                 tri[oi < oj ? tri_index(oi, oj, n) : tri_index(oj, oi, n)] = v;
             }
@@ -113,7 +118,7 @@ __device__ uint64_t bb_valu_tile(BBValuSmem<BM>& sm, const uint64_t* __restrict_
     return local;
 }
 
-template <int BM>
+template <int BM, bool THR = false>
 __global__ __launch_bounds__(256, 2) void bb_valu_kernel(
     const uint64_t* __restrict__ X, uint32_t ld, uint32_t kwords,
     const uint32_t* __restrict__ rowmap, const uint32_t* __restrict__ perm, uint32_t n,
@@ -123,6 +128,7 @@ __global__ __launch_bounds__(256, 2) void bb_valu_kernel(
     unsigned long long* __restrict__ tile_ticks) {
     __shared__ BBValuSmem<BM> sm;
     __shared__ uint32_t s_t;
+    if (THR && !qcount) return;   // thresholded instances run in queue mode only
     if (!qcount) {
         uint32_t bi, bj;
         tri_tile(blockIdx.x, T, bi, bj);
@@ -141,8 +147,9 @@ __global__ __launch_bounds__(256, 2) void bb_valu_kernel(
         const QItem e = queue[t];
         const DevCorpus D = Ds[e.inst];
         const uint64_t t0 = wall_clock64();
-        uint64_t local = bb_valu_tile<BM>(sm, D.bitmap, D.ld, (D.nw + 15) / 16 * 16, D.rowmap,
-                                          D.perm, D.n, e.bi, e.bj, D.tri);
+        uint64_t local = bb_valu_tile<BM, THR>(sm, D.bitmap, D.ld, (D.nw + 15) / 16 * 16,
+                                               D.rowmap, D.perm, D.n, e.bi, e.bj, D.tri,
+                                               &Ds[e.inst]);
         local = wave_sum_u64(local);
         if ((threadIdx.x & 63) == 0 && local) atomicAdd(D.sum, (unsigned long long)local);
         __syncthreads();
@@ -192,16 +199,26 @@ uint32_t bb_valu_blocks_per_cu(int bm) {
 // the corpora in Ds (every corpus's rows padded so ld covers its words).
 int launch_bb_valu_queue(hipStream_t s, int bm, const DevCorpus* Ds, const QItem* queue,
                          const unsigned int* qcount, unsigned int* qhead, uint32_t grid,
-                         unsigned long long* tile_ticks) {
+                         unsigned long long* tile_ticks, bool thresholded) {
     if (grid == 0) return STORM_HIP_OK;
     if (bm == 128) {
-        hipLaunchKernelGGL(bb_valu_kernel<128>, dim3(grid), dim3(256), 0, s, nullptr, 0u, 0u,
-                           nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
-                           tile_ticks);
+        if (thresholded)
+            hipLaunchKernelGGL((bb_valu_kernel<128, true>), dim3(grid), dim3(256), 0, s, nullptr,
+                               0u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead,
+                               nullptr, nullptr, tile_ticks);
+        else
+            hipLaunchKernelGGL(bb_valu_kernel<128>, dim3(grid), dim3(256), 0, s, nullptr, 0u, 0u,
+                               nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr,
+                               nullptr, tile_ticks);
     } else if (bm == 64) {
-        hipLaunchKernelGGL(bb_valu_kernel<64>, dim3(grid), dim3(256), 0, s, nullptr, 0u, 0u,
-                           nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr, nullptr,
-                           tile_ticks);
+        if (thresholded)
+            hipLaunchKernelGGL((bb_valu_kernel<64, true>), dim3(grid), dim3(256), 0, s, nullptr,
+                               0u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead,
+                               nullptr, nullptr, tile_ticks);
+        else
+            hipLaunchKernelGGL(bb_valu_kernel<64>, dim3(grid), dim3(256), 0, s, nullptr, 0u, 0u,
+                               nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead, nullptr,
+                               nullptr, tile_ticks);
     } else {
         set_error("bb_valu: unsupported tile %d", bm);
         return STORM_HIP_ERR_INVALID;

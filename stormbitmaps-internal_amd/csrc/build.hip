@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "device_common.h"
+#include "build.h"
 #include "general.h"
 #include "internal.h"
 
@@ -217,15 +218,6 @@ int build_host_arena(const uint64_t* offs, const uint32_t* pos, uint32_t n, uint
     return STORM_HIP_OK;
 }
 
-template <class T>
-int up(T*& d, const std::vector<T>& h, hipStream_t s, uint64_t& bytes) {
-    const size_t b = std::max<size_t>(h.size(), 1) * sizeof(T);
-    SH_HIP(hipMalloc((void**)&d, b));
-    bytes += b;
-    if (!h.empty()) SH_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    return STORM_HIP_OK;
-}
-
 // Scatter the list of every row that has a bitmap into its bitmap row.
 __global__ void scatter_bits_kernel(const uint64_t* __restrict__ list_off,
                                     const uint32_t* __restrict__ list,
@@ -258,64 +250,90 @@ int upload_bitmaps(storm_hip_corpus* c, hipStream_t s) {
     return STORM_HIP_OK;
 }
 
-int upload_host_arena(storm_hip_ctx* ctx, const HostArena& A, storm_hip_corpus** out) {
-    storm_hip_corpus* c = new (std::nothrow) storm_hip_corpus();
+int upload_host_arena(storm_hip_ctx* ctx, const HostArena& A, uint32_t flags,
+                      storm_hip_corpus** out) {
+    storm_hip_corpus* c = corpus_new(ctx, A.n, A.universe, flags);
     if (!c) return STORM_HIP_ERR_NOMEM;
     *out = c;   // freed by the caller on failure
-    c->ctx = ctx;
-    c->dense = false;
-    c->n_rows = A.n;
-    c->universe = A.universe;
-    c->n_words = A.nw;
-    c->ld = (std::max(A.nw, 1u) + kWordPad - 1) / kWordPad * kWordPad;
-    c->has_runs = A.has_runs; c->has_ewah = A.has_ewah; c->has_comp = A.has_comp;
     hipStream_t s = ctx->stream;
     uint64_t& B = c->device_bytes;
-    SH_TRY(up(c->d_list_off, A.list_off, s, B));
-    SH_TRY(up(c->d_list, A.list, s, B));
-    SH_TRY(up(c->d_run_off, A.run_off, s, B));
-    SH_TRY(up(c->d_run_start, A.run_start, s, B));
-    SH_TRY(up(c->d_run_end, A.run_end, s, B));
-    SH_TRY(up(c->d_ewah_off, A.ewah_off, s, B));
-    SH_TRY(up(c->d_ewah, A.ewah, s, B));
-    SH_TRY(up(c->d_mk_off, A.mk_off, s, B));
-    SH_TRY(up(c->d_mk_word, A.mk_word, s, B));
-    SH_TRY(up(c->d_mk_pos, A.mk_pos, s, B));
-    SH_TRY(up(c->d_comp_off, A.comp_off, s, B));
-    SH_TRY(up(c->d_comp, A.comp, s, B));
-    SH_TRY(up(c->d_meta, A.meta, s, B));
+    SH_TRY(upload_vec(c->d_list_off, A.list_off, s, B));
+    SH_TRY(upload_vec(c->d_list, A.list, s, B));
+    SH_TRY(upload_vec(c->d_run_off, A.run_off, s, B));
+    SH_TRY(upload_vec(c->d_run_start, A.run_start, s, B));
+    SH_TRY(upload_vec(c->d_run_end, A.run_end, s, B));
+    SH_TRY(upload_vec(c->d_ewah_off, A.ewah_off, s, B));
+    SH_TRY(upload_vec(c->d_ewah, A.ewah, s, B));
+    SH_TRY(upload_vec(c->d_mk_off, A.mk_off, s, B));
+    SH_TRY(upload_vec(c->d_mk_word, A.mk_word, s, B));
+    SH_TRY(upload_vec(c->d_mk_pos, A.mk_pos, s, B));
+    SH_TRY(upload_vec(c->d_comp_off, A.comp_off, s, B));
+    SH_TRY(upload_vec(c->d_comp, A.comp, s, B));
+    SH_TRY(upload_vec(c->d_meta, A.meta, s, B));
     c->h_meta = A.meta;
+    return corpus_finish(ctx, c, A.want_bitmap);
+}
+
+}  // namespace
+
+storm_hip_corpus* corpus_new(storm_hip_ctx* ctx, uint32_t n, uint32_t universe, uint32_t flags) {
+    storm_hip_corpus* c = new (std::nothrow) storm_hip_corpus();
+    if (!c) return nullptr;
+    c->ctx = ctx;
+    c->dense = false;
+    c->n_rows = n;
+    c->universe = universe;
+    c->n_words = (universe + 63) / 64;
+    c->ld = (std::max(c->n_words, 1u) + kWordPad - 1) / kWordPad * kWordPad;
+    c->has_runs = flags & STORM_HIP_BUILD_RUNS;
+    c->has_ewah = flags & STORM_HIP_BUILD_EWAH;
+    c->has_comp = flags & STORM_HIP_BUILD_COMP;
+    return c;
+}
+
+std::vector<uint8_t> bitmap_rows(const std::vector<uint32_t>& meta, uint32_t n, uint32_t universe,
+                                 uint32_t policy) {
+    std::vector<uint8_t> want(n, 0);
+    for (uint32_t r = 0; r < n; ++r)
+        want[r] = policy == STORM_HIP_BITMAP_ALL ||
+                  (policy == STORM_HIP_BITMAP_DENSE &&
+                   (uint64_t)meta[(size_t)r * 8] * 64 >= (uint64_t)universe);
+    return want;
+}
+
+int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uint8_t>& want_bitmap) {
+    const uint32_t n = c->n_rows;
+    hipStream_t s = ctx->stream;
+    uint64_t& B = c->device_bytes;
     // Density order: stable by cardinality (kernels/storm_allpairs.cpp:316-323),
     // then padding positions map to the sentinel row id n (never read as data).
-    const uint32_t npad = (A.n + kRowPad - 1) / kRowPad * kRowPad + kRowPad;
-    c->h_perm.resize(A.n);
+    const uint32_t npad = (n + kRowPad - 1) / kRowPad * kRowPad + kRowPad;
+    c->h_perm.resize(n);
     std::iota(c->h_perm.begin(), c->h_perm.end(), 0u);
     std::stable_sort(c->h_perm.begin(), c->h_perm.end(), [&](uint32_t a, uint32_t b) {
-        return A.meta[(size_t)a * 8] < A.meta[(size_t)b * 8];
+        return c->h_meta[(size_t)a * 8] < c->h_meta[(size_t)b * 8];
     });
-    std::vector<uint32_t> perm(npad, A.n), ident(npad, A.n);
+    std::vector<uint32_t> perm(npad, n), ident(npad, n);
     std::copy(c->h_perm.begin(), c->h_perm.end(), perm.begin());
-    std::iota(ident.begin(), ident.begin() + A.n, 0u);
-    SH_TRY(up(c->d_perm, perm, s, B));
-    SH_TRY(up(c->d_ident, ident, s, B));
+    std::iota(ident.begin(), ident.begin() + n, 0u);
+    SH_TRY(upload_vec(c->d_perm, perm, s, B));
+    SH_TRY(upload_vec(c->d_ident, ident, s, B));
     // Bitmap rows; the sentinel row id n maps to the zero row.
-    c->h_bm_row.assign((size_t)A.n + 1, kNoBitmap);
+    c->h_bm_row.assign((size_t)n + 1, kNoBitmap);
     uint32_t nb = 0;
-    for (uint32_t r = 0; r < A.n; ++r)
-        if (A.want_bitmap[r]) c->h_bm_row[r] = nb++;
+    for (uint32_t r = 0; r < n; ++r)
+        if (want_bitmap[r]) c->h_bm_row[r] = nb++;
     c->n_bitmap_rows = nb;
-    c->h_bm_row[A.n] = nb;   // zero row
-    c->bitmaps_complete = (nb == A.n);
-    SH_HIP(hipMalloc(&c->d_bm_row, ((size_t)A.n + 1) * 4));
-    B += ((size_t)A.n + 1) * 4;
+    c->h_bm_row[n] = nb;   // zero row
+    c->bitmaps_complete = (nb == n);
+    SH_HIP(hipMalloc(&c->d_bm_row, ((size_t)n + 1) * 4));
+    B += ((size_t)n + 1) * 4;
     SH_TRY(upload_bitmaps(c, s));
-    SH_HIP(hipMemcpyAsync(c->d_bm_row + A.n, &c->h_bm_row[A.n], 4, hipMemcpyHostToDevice, s));
+    SH_HIP(hipMemcpyAsync(c->d_bm_row + n, &c->h_bm_row[n], 4, hipMemcpyHostToDevice, s));
     B += ((size_t)nb + 1) * c->ld * 8;
     SH_HIP(hipStreamSynchronize(s));
     return STORM_HIP_OK;
 }
-
-}  // namespace
 
 int general_materialize(storm_hip_ctx* ctx, storm_hip_corpus* c) {
     if (c->dense || c->bitmaps_complete) return STORM_HIP_OK;
@@ -356,11 +374,41 @@ int storm_hip_corpus_from_lists(storm_hip_ctx* ctx, const uint64_t* offs, const 
     for (uint32_t r = 0; r < n_rows; ++r)
         if (offs[r + 1] < offs[r]) return STORM_HIP_ERR_INVALID;
     SH_HIP(hipSetDevice(ctx->device));
-    HostArena A;
-    SH_TRY(build_host_arena(offs, pos, n_rows, universe, flags, A));
-    const int s = upload_host_arena(ctx, A, out);
-    if (s != STORM_HIP_OK && *out) { storm_hip_corpus_free(*out); *out = nullptr; }
-    return s;
+    if (flags & STORM_HIP_BUILD_ON_HOST) {
+        HostArena A;
+        SH_TRY(build_host_arena(offs, pos, n_rows, universe, flags, A));
+        const int s = upload_host_arena(ctx, A, flags, out);
+        if (s != STORM_HIP_OK && *out) { storm_hip_corpus_free(*out); *out = nullptr; }
+        return s;
+    }
+    // Lists go up as they are (rebased to offset 0); everything else is built
+    // on the device.
+    storm_hip_corpus* c = corpus_new(ctx, n_rows, universe, flags);
+    if (!c) return STORM_HIP_ERR_NOMEM;
+    hipStream_t s = ctx->stream;
+    std::vector<uint64_t> rebased(offs, offs + (size_t)n_rows + 1);
+    for (auto& x : rebased) x -= offs[0];
+    const uint64_t total = rebased[n_rows];
+    int st = upload_vec(c->d_list_off, rebased, s, c->device_bytes);
+    if (st == STORM_HIP_OK) {
+        if (hipMalloc((void**)&c->d_list, std::max<uint64_t>(total, 1) * 4) != hipSuccess) {
+            st = STORM_HIP_ERR_NOMEM;
+        } else {
+            c->device_bytes += std::max<uint64_t>(total, 1) * 4;
+            if (total && hipMemcpyAsync(c->d_list, pos + offs[0], total * 4,
+                                        hipMemcpyHostToDevice, s) != hipSuccess)
+                st = STORM_HIP_ERR_DEVICE;
+        }
+    }
+    if (st == STORM_HIP_OK) st = device_build(ctx, c, flags);
+    if (st == STORM_HIP_OK)
+        st = corpus_finish(ctx, c, bitmap_rows(c->h_meta, n_rows, universe, flags & 3u));
+    if (st != STORM_HIP_OK) {
+        storm_hip_corpus_free(c);
+        return st;
+    }
+    *out = c;
+    return STORM_HIP_OK;
 }
 
 int storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* a, storm_hip_corpus** out) {
@@ -384,7 +432,7 @@ int storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* a, storm_
             }
         A.want_bitmap[r] = a->bitmap_off && a->bitmap_off[r + 1] > a->bitmap_off[r];
     }
-    const int s = upload_host_arena(ctx, A, out);
+    const int s = upload_host_arena(ctx, A, flags, out);
     if (s != STORM_HIP_OK && *out) { storm_hip_corpus_free(*out); *out = nullptr; }
     return s;
 }
@@ -393,6 +441,55 @@ int storm_hip_corpus_materialize_bitmaps(storm_hip_ctx* ctx, storm_hip_corpus* c
     if (!ctx || !c || c->ctx != ctx) return STORM_HIP_ERR_INVALID;
     SH_HIP(hipSetDevice(ctx->device));
     return general_materialize(ctx, c);
+}
+
+int storm_hip_corpus_arena_sizes(storm_hip_ctx* ctx, const storm_hip_corpus* c,
+                                 uint64_t bytes[STORM_HIP_ARENA_COUNT]) {
+    if (!ctx || !c || !bytes || c->ctx != ctx) return STORM_HIP_ERR_INVALID;
+    if (c->dense) return STORM_HIP_ERR_UNSUPPORTED;
+    SH_HIP(hipSetDevice(ctx->device));
+    SH_HIP(hipStreamSynchronize(ctx->stream));
+    const uint64_t n = c->n_rows;
+    const uint64_t* offs[5] = {c->d_list_off, c->d_run_off, c->d_ewah_off, c->d_mk_off,
+                               c->d_comp_off};
+    uint64_t tot[5];
+    for (int k = 0; k < 5; ++k) SH_HIP(hipMemcpy(&tot[k], offs[k] + n, 8, hipMemcpyDeviceToHost));
+    bytes[STORM_HIP_ARENA_LIST_OFF] = (n + 1) * 8;
+    bytes[STORM_HIP_ARENA_LIST] = tot[0] * 4;
+    bytes[STORM_HIP_ARENA_RUN_OFF] = (n + 1) * 8;
+    bytes[STORM_HIP_ARENA_RUN_START] = tot[1] * 4;
+    bytes[STORM_HIP_ARENA_RUN_END] = tot[1] * 4;
+    bytes[STORM_HIP_ARENA_EWAH_OFF] = (n + 1) * 8;
+    bytes[STORM_HIP_ARENA_EWAH] = tot[2] * 8;
+    bytes[STORM_HIP_ARENA_MK_OFF] = (n + 1) * 8;
+    bytes[STORM_HIP_ARENA_MK_WORD] = tot[3] * 4;
+    bytes[STORM_HIP_ARENA_MK_POS] = tot[3] * 4;
+    bytes[STORM_HIP_ARENA_COMP_OFF] = (n + 1) * 8;
+    bytes[STORM_HIP_ARENA_COMP] = tot[4] * 4;
+    bytes[STORM_HIP_ARENA_META] = n * 32;
+    bytes[STORM_HIP_ARENA_BM_ROW] = n * 4;
+    bytes[STORM_HIP_ARENA_BITMAP] = (uint64_t)c->n_bitmap_rows * c->ld * 8;
+    return STORM_HIP_OK;
+}
+
+int storm_hip_corpus_download_arena(storm_hip_ctx* ctx, const storm_hip_corpus* c, uint32_t which,
+                                    void* dst, uint64_t bytes) {
+    if (which >= STORM_HIP_ARENA_COUNT) return STORM_HIP_ERR_INVALID;
+    uint64_t sizes[STORM_HIP_ARENA_COUNT];
+    SH_TRY(storm_hip_corpus_arena_sizes(ctx, c, sizes));
+    if (bytes != sizes[which]) {
+        set_error("arena %u holds %llu bytes, %llu requested", which,
+                  (unsigned long long)sizes[which], (unsigned long long)bytes);
+        return STORM_HIP_ERR_INVALID;
+    }
+    if (bytes == 0) return STORM_HIP_OK;
+    if (!dst) return STORM_HIP_ERR_NULL_VALUES;
+    const void* src[STORM_HIP_ARENA_COUNT] = {
+        c->d_list_off, c->d_list,  c->d_run_off, c->d_run_start, c->d_run_end,
+        c->d_ewah_off, c->d_ewah,  c->d_mk_off,  c->d_mk_word,   c->d_mk_pos,
+        c->d_comp_off, c->d_comp,  c->d_meta,    c->d_bm_row,    c->d_bitmap};
+    SH_HIP(hipMemcpy(dst, src[which], bytes, hipMemcpyDeviceToHost));
+    return STORM_HIP_OK;
 }
 
 }  // extern "C"

@@ -260,9 +260,13 @@ __device__ inline uint32_t cb_base(const DevCorpus& D, uint32_t i, uint32_t j) {
 }
 
 // Span test (kernels/storm_allpairs.cpp:627-634): true if provably disjoint.
+// Thresholded calls also skip a pair whose length bound min/max is below the
+// threshold (it cannot be reported, whatever its count).
 __device__ inline bool span_disjoint(const DevCorpus& D, uint32_t i, uint32_t j) {
     const uint32_t ci = meta(D, i, 0), cj = meta(D, j, 0);
-    return ci == 0 || cj == 0 || meta(D, i, 5) < meta(D, j, 4) || meta(D, j, 5) < meta(D, i, 4);
+    if (ci == 0 || cj == 0 || meta(D, i, 5) < meta(D, j, 4) || meta(D, j, 5) < meta(D, i, 4))
+        return true;
+    return D.thr > 0.0 && (double)min(ci, cj) / (double)max(ci, cj) < D.thr;
 }
 
 // ---- S x S at tile granularity: a hash join in LDS -------------------------
@@ -303,9 +307,9 @@ __device__ inline uint32_t join_row(const uint32_t* pre, uint32_t nrows, uint32_
 }
 
 __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint32_t ni,
-                             uint32_t ja, uint32_t nj, bool diag, uint32_t* __restrict__ tri,
+                             uint32_t ja, uint32_t nj, bool diag,
                              uint64_t& local, uint64_t& pairs, uint64_t& skipped) {
-    const uint32_t tid = threadIdx.x, n = D.n;
+    const uint32_t tid = threadIdx.x;
     for (uint32_t k = tid; k < 64 * 65; k += blockDim.x) (&S.cnt[0][0])[k] = 0;
     for (uint32_t k = tid; k < kJoinSize; k += blockDim.x) {
         S.key[k] = kJoinEmpty; S.mlo[k] = 0; S.mhi[k] = 0;
@@ -398,7 +402,7 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
         const uint32_t v = S.cnt[a][b];
         if (span_disjoint(D, i, j)) ++skipped; else ++pairs;
         local += v;
-        if (tri) tri[i < j ? tri_index(i, j, n) : tri_index(j, i, n)] = v;
+        emit_pair(D, i, j, v);
     }
     __syncthreads();
 }
@@ -422,10 +426,8 @@ struct StageSmem {
 template <int CELL>
 __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint32_t ni,
                             uint32_t ja, uint32_t nj, bool diag, bool stage_j,
-                            uint32_t* __restrict__ tri, uint64_t& local, uint64_t& pairs,
-                            uint64_t& skipped) {
+                            uint64_t& local, uint64_t& pairs, uint64_t& skipped) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-    const uint32_t n = D.n;
     const uint32_t xs0 = stage_j ? ja : ia, nx = stage_j ? nj : ni;
     const uint32_t ys0 = stage_j ? ia : ja, ny = stage_j ? ni : nj;
     for (uint32_t u = 0; u < nx; ++u) {
@@ -481,7 +483,7 @@ __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint3
             }
             if (lane == 0) {
                 local += cnt;
-                if (tri) tri[x < y ? tri_index(x, y, n) : tri_index(y, x, n)] = cnt;
+                emit_pair(D, x, y, cnt);
             }
         }
     }
@@ -530,7 +532,6 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         const QItem e = queue[t];
         const DevCorpus D = Ds[e.inst];
         const uint32_t n = D.n, tile = D.tile;
-        uint32_t* const tri = D.tri;
         uint64_t local = 0;
         uint32_t i0 = e.bi * tile, j0 = e.bj * tile;
         uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
@@ -557,14 +558,14 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
                 for (uint32_t si = 0; si < ni; si += 64)
                     for (uint32_t sj = diag ? si : 0; sj < nj; sj += 64)
                         ss_tile_join(D, sm.js, i0 + si, min(64u, ni - si), j0 + sj,
-                                     min(64u, nj - sj), diag && si == sj, tri, local, pairs,
+                                     min(64u, nj - sj), diag && si == sj, local, pairs,
                                      skipped);
             } else if (!lane_mode) {
                 const int side = stage_side<CELL>(A, D.tm[e.bj]);
                 if (side >= 0) {
                     joined = true;
-                    tile_staged<CELL>(D, sm.ss, i0, ni, j0, nj, diag, side == 1, tri, local,
-                                      pairs, skipped);
+                    tile_staged<CELL>(D, sm.ss, i0, ni, j0, nj, diag, side == 1, local, pairs,
+                                      skipped);
                 }
             }
         }
@@ -573,8 +574,7 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
             const int side = stage_side<CELL>(A, D.tm[e.bj]);
             if (!lane_mode && side >= 0) {
                 joined = true;
-                tile_staged<CELL>(D, ss, i0, ni, j0, nj, diag, side == 1, tri, local, pairs,
-                                  skipped);
+                tile_staged<CELL>(D, ss, i0, ni, j0, nj, diag, side == 1, local, pairs, skipped);
             }
         }
         if (joined) {
@@ -592,7 +592,7 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
                     if (CELL == STORM_HIP_CELL_CB) cnt = cb_base(D, i, j) - cnt;
                 }
                 local += cnt;
-                if (tri) tri[i < j ? tri_index(i, j, n) : tri_index(j, i, n)] = cnt;
+                emit_pair(D, i, j, cnt);
             }
         } else {
             for (uint32_t p = wave; p < np; p += blockDim.x / 64) {
@@ -609,7 +609,7 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
                 }
                 if (lane == 0) {
                     local += cnt;
-                    if (tri) tri[i < j ? tri_index(i, j, n) : tri_index(j, i, n)] = cnt;
+                    emit_pair(D, i, j, cnt);
                 }
             }
         }

@@ -4,12 +4,16 @@
 
 #include <cstdint>
 
+#include "device_common.h"
 #include "internal.h"
 
 namespace stormhip {
 
 constexpr uint32_t kNoBitmap = 0xFFFFFFFFu;
 constexpr int kNumCells = STORM_HIP_NCELL;   // BB..WW, EMPTY, CB
+
+// One reported pair of a thresholded call (layout of storm_hip_pair).
+struct PairHit { uint32_t i, j, count; };
 
 // Row arenas on the device. Row ids here are ORIGINAL ids; `perm` maps a
 // density-sorted position to its original id.
@@ -33,7 +37,32 @@ struct DevCorpus {
     unsigned long long* sum;          // its sum
     unsigned long long* finish;       // wall clock of its last finished tile (max)
     uint32_t tile;                    // rows per tile
+    // Thresholded calls (hits non-null, tri null): pairs whose Jaccard
+    // |A n B| / |A u B| is at least thr are appended to hits (n_hits counts
+    // every such pair, including those past hit_cap).
+    double thr;
+    PairHit* hits;
+    unsigned long long* n_hits;
+    uint64_t hit_cap;
 };
+
+// Per-pair output of a general call: the packed triangle, or the pair itself
+// when a thresholded call's Jaccard test passes -- the test of the
+// thresholded benchmark (bench/bench_threshold.cpp jac_ok): u = |A| + |B| -
+// |A n B| in double, reported when |A n B| > 0 and |A n B| / u >= thr.
+__device__ inline void emit_pair(const DevCorpus& D, uint32_t i, uint32_t j, uint32_t v) {
+    if (D.tri) {
+        D.tri[i < j ? tri_index(i, j, D.n) : tri_index(j, i, D.n)] = v;
+        return;
+    }
+    if (D.hits && v) {
+        const double ci = D.meta[(uint64_t)i * 8], cj = D.meta[(uint64_t)j * 8];
+        if ((double)v / (ci + cj - (double)v) >= D.thr) {
+            const unsigned long long k = atomicAdd(D.n_hits, 1ull);
+            if (k < D.hit_cap) D.hits[k] = PairHit{i < j ? i : j, i < j ? j : i, v};
+        }
+    }
+}
 
 // One queued tile pair: corpus index, tile row, tile column, and `sub`: 0 for
 // the whole tile, else one block of it: (sub & 0xFFFF) rows per side, the
@@ -50,12 +79,14 @@ __host__ __device__ inline uint32_t make_sub(uint32_t rows, uint32_t bi_blk, uin
 // Rows per block when a tile is split for the per-pair cells.
 constexpr uint32_t kSplitRows = 64;
 
+// thresholded: the corpora carry a Jaccard threshold and pair buffer (their
+// tri is null); the launch uses the B x B instance that reports pairs.
 int launch_bb_valu_queue(hipStream_t s, int bm, const DevCorpus* Ds, const QItem* queue,
                          const unsigned int* qcount, unsigned int* qhead, uint32_t grid,
-                         unsigned long long* tile_ticks);
+                         unsigned long long* tile_ticks, bool thresholded = false);
 int launch_bb_mfma_queue(bool fp4, const MfmaConfig& cfg, hipStream_t s, const DevCorpus* Ds,
                          const QItem* queue, const unsigned int* qcount, unsigned int* qhead,
-                         uint32_t grid, unsigned long long* tile_ticks);
+                         uint32_t grid, unsigned long long* tile_ticks, bool thresholded = false);
 int bb_mfma_check(bool fp4, uint32_t stage_words, uint32_t n_words, uint32_t ld);
 // Workgroups resident per CU: a cell kernel; the B x B kernel of a tile width
 // (256 = matrix cores with this launch shape, else VALU).
@@ -95,6 +126,16 @@ __host__ __device__ inline uint32_t eligible_mask(const TileMeta& a, const TileM
 // Sound tile-level emptiness: every pair of the tile fails the span test.
 __host__ __device__ inline bool tile_empty(const TileMeta& a, const TileMeta& b) {
     return a.max_card == 0 || b.max_card == 0 || a.last < b.first || b.last < a.first;
+}
+
+// Largest min(|A|, |B|) / max(|A|, |B|) over the tile's pairs: an upper bound
+// on every pair's Jaccard (|A n B| <= min, |A u B| >= max). Computed as the
+// same double division as a pair's Jaccard in the equality case, so a tile
+// whose bound is below the threshold holds no reportable pair.
+__host__ __device__ inline double length_bound(const TileMeta& a, const TileMeta& b) {
+    if (a.max_card >= b.min_card && b.max_card >= a.min_card) return 1.0;
+    return a.max_card < b.min_card ? (double)a.max_card / (double)b.min_card
+                                   : (double)b.max_card / (double)a.min_card;
 }
 
 // Device cost-model knobs passed by value.

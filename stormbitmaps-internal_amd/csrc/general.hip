@@ -10,6 +10,7 @@
 // Every count of work lives on the device, so the whole sequence is captured
 // once in a hipGraph and replayed.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -24,7 +25,7 @@ int launch_tile_meta(hipStream_t s, const DevCorpus& D, uint32_t tile, uint32_t 
 int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel& M, uint32_t reps,
                   uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
-                  uint32_t inst, uint32_t tile, DevCounters* ctr);
+                  uint32_t inst, uint32_t tile, double thr, DevCounters* ctr);
 int launch_compose_rowmap(hipStream_t s, const uint32_t* order, const uint32_t* bm_row,
                           uint32_t npad, uint32_t* rowmap);
 int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
@@ -83,11 +84,21 @@ int make_work(const storm_hip_corpus* c, uint32_t tile, Work** out) {
     return STORM_HIP_OK;
 }
 
+// Output of a thresholded single-corpus call: the Jaccard threshold and the
+// device pair buffer with its counter.
+struct Thresh {
+    double thr = 0.0;
+    PairHit* hits = nullptr;
+    unsigned long long* n_hits = nullptr;
+    uint64_t cap = 0;
+};
+
 // Device state of one call over a list of corpora: per-corpus selection
 // state, the device copy of every corpus's view, the queues and the results.
 // Cached and reused (graph included) while the corpora, their device arrays,
 // the tile width and the outputs stay the same.
 struct Plan {
+    Thresh th;                              // thresholded call (th.hits set) or not
     std::vector<storm_hip_corpus*> cs;
     std::vector<uint32_t*> tris;
     std::vector<Work*> works;               // one per corpus, owned
@@ -114,8 +125,9 @@ void free_plan(Plan* p) {
 }
 
 DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
-                     unsigned long long* sum, unsigned long long* finish) {
+                     unsigned long long* sum, unsigned long long* finish, const Thresh& th) {
     DevCorpus D{};
+    D.thr = th.thr; D.hits = th.hits; D.n_hits = th.n_hits; D.hit_cap = th.cap;
     D.n = c->n_rows; D.universe = c->universe; D.nw = c->n_words; D.ld = c->ld;
     D.list_off = c->d_list_off; D.list = c->d_list;
     D.run_off = c->d_run_off; D.run_start = c->d_run_start; D.run_end = c->d_run_end;
@@ -149,19 +161,21 @@ uint32_t reps_of(const storm_hip_corpus* c) {
 // outputs, the tile width, or any device array of a corpus (materialising
 // bitmaps replaces the bitmap store, for instance).
 bool plan_current(const Plan* p, const std::vector<storm_hip_corpus*>& cs,
-                  const std::vector<uint32_t*>& tris, uint32_t tile) {
+                  const std::vector<uint32_t*>& tris, uint32_t tile, const Thresh& th) {
     if (!p || p->cs != cs || p->tris != tris || p->tile != tile) return false;
     for (size_t k = 0; k < cs.size(); ++k) {
-        const DevCorpus D = dev_corpus(cs[k], p->works[k], tris[k], p->sums + k, p->finish + k);
+        const DevCorpus D =
+            dev_corpus(cs[k], p->works[k], tris[k], p->sums + k, p->finish + k, th);
         if (std::memcmp(&D, &p->h_corpora[k], sizeof D) != 0) return false;
     }
     return true;
 }
 
 int build_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
-               const std::vector<uint32_t*>& tris, uint32_t tile) {
+               const std::vector<uint32_t*>& tris, uint32_t tile, const Thresh& th) {
     free_plan(p);
     p = new Plan();
+    p->th = th;
     p->cs = cs;
     p->tris = tris;
     p->tile = tile;
@@ -184,7 +198,8 @@ int build_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
     SH_HIP(hipMalloc(&p->finish, 8 * (n + 1)));
     p->h_corpora.resize(n);
     for (size_t k = 0; k < n; ++k)
-        p->h_corpora[k] = dev_corpus(cs[k], p->works[k], tris[k], p->sums + k, p->finish + k);
+        p->h_corpora[k] =
+            dev_corpus(cs[k], p->works[k], tris[k], p->sums + k, p->finish + k, th);
     if (n)
         SH_HIP(hipMemcpy(p->d_corpora, p->h_corpora.data(), sizeof(DevCorpus) * n,
                          hipMemcpyHostToDevice));
@@ -193,9 +208,9 @@ int build_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
 
 // On failure the plan is freed and left null (never half built).
 int ensure_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
-                const std::vector<uint32_t*>& tris, uint32_t tile) {
-    if (plan_current(p, cs, tris, tile)) return STORM_HIP_OK;
-    const int rc = build_plan(p, cs, tris, tile);
+                const std::vector<uint32_t*>& tris, uint32_t tile, const Thresh& th = Thresh{}) {
+    if (plan_current(p, cs, tris, tile, th)) return STORM_HIP_OK;
+    const int rc = build_plan(p, cs, tris, tile, th);
     if (rc != STORM_HIP_OK) { free_plan(p); p = nullptr; }
     return rc;
 }
@@ -236,6 +251,7 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     SH_TRY(zero_words(s, p->ctr, sizeof(DevCounters)));
     SH_TRY(zero_words(s, p->sums, 8 * n));
     SH_TRY(zero_words(s, p->finish, 8 * n));
+    if (p->th.n_hits) SH_TRY(zero_words(s, p->th.n_hits, 8));
     hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, p->finish + n);
     SH_HIP(hipGetLastError());
     if (timed) SH_HIP(hipEventRecord(ctx->ev[4], s));
@@ -254,7 +270,7 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
         }
         SH_TRY(launch_select(s, w->tm, w->T, M, reps_of(c), o.policy, o.fixed_cell, lc, lok,
                              w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
-                             p->ctr));
+                             p->th.thr, p->ctr));
         SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
         // Only corpora with bitmap rows can queue B x B tiles.
         if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
@@ -280,14 +296,15 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     const QItem* qbb = p->queue + (uint64_t)STORM_HIP_CELL_BB * p->qstride;
     const uint32_t gbb = (uint32_t)std::min<uint64_t>(
         P1, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
+    const bool thresholded = p->th.hits != nullptr;
     if (p->tile == 256)
         rc = launch_bb_mfma_queue(fp4, ctx->mfma, sb, p->d_corpora, qbb,
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
-                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr);
+                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr, thresholded);
     else
         rc = launch_bb_valu_queue(sb, (int)p->tile, p->d_corpora, qbb,
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
-                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr);
+                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr, thresholded);
     if (timed) (void)hipEventRecord(ctx->ev[3], sb);
     // Persistent grids: as many workgroups as fit on the chip at once.
     int k = 0;
@@ -391,6 +408,10 @@ void general_free(storm_hip_corpus* c) {
     c->plan = nullptr;
     free_work(static_cast<Work*>(c->work));
     c->work = nullptr;
+    if (c->d_hits) (void)hipFree(c->d_hits);
+    if (c->d_n_hits) (void)hipFree(c->d_n_hits);
+    c->d_hits = nullptr;
+    c->d_n_hits = nullptr;
     if (!c->dense) general_free_arenas(c);
 }
 
@@ -437,6 +458,37 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
     SH_HIP(hipMemcpyAsync(d_sum, p->sums, 8, hipMemcpyDeviceToDevice, s));
     return STORM_HIP_OK;
 }
+
+namespace {
+
+// One thresholded call into the corpus's pair buffer; returns the number of
+// reportable pairs (which may exceed the buffer's capacity).
+int threshold_pass(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_options& o,
+                   uint32_t tile, double t, storm_hip_stats* st, unsigned long long* n_hits) {
+    Thresh th;
+    th.thr = t;
+    th.hits = static_cast<PairHit*>(c->d_hits);
+    th.n_hits = c->d_n_hits;
+    th.cap = c->hit_cap;
+    Plan* p = static_cast<Plan*>(c->plan);
+    const int rc = ensure_plan(p, {c}, {nullptr}, tile, th);
+    c->plan = p;
+    SH_TRY(rc);
+    uint64_t sum = 0;
+    SH_TRY(run_plan(ctx, p, o, st, &sum, nullptr));
+    SH_HIP(hipMemcpyAsync(n_hits, c->d_n_hits, 8, hipMemcpyDeviceToHost, ctx->stream));
+    SH_HIP(hipStreamSynchronize(ctx->stream));
+    return STORM_HIP_OK;
+}
+
+int grow_hits(storm_hip_corpus* c, uint64_t cap) {
+    if (c->d_hits) { SH_HIP(hipFree(c->d_hits)); c->d_hits = nullptr; c->hit_cap = 0; }
+    SH_HIP(hipMalloc(&c->d_hits, cap * sizeof(PairHit)));
+    c->hit_cap = cap;
+    return STORM_HIP_OK;
+}
+
+}  // namespace
 
 }  // namespace stormhip
 
@@ -780,6 +832,92 @@ int storm_hip_last_decisions(storm_hip_ctx* ctx, const storm_hip_corpus* c, uint
     if (!cells && n_tiles) return STORM_HIP_ERR_NULL_VALUES;
     if (n_tiles > c->last_cells.size()) return STORM_HIP_ERR_RANGE;
     std::memcpy(cells, c->last_cells.data(), n_tiles);
+    return STORM_HIP_OK;
+}
+
+int storm_hip_allpairs_threshold(storm_hip_ctx* ctx, storm_hip_corpus* c,
+                                 const storm_hip_options* opt, double t, storm_hip_stats* st,
+                                 uint64_t* n_pairs) {
+    if (!ctx || !c || c->ctx != ctx || !n_pairs) return STORM_HIP_ERR_INVALID;
+    *n_pairs = 0;
+    if (!(t > 0.0 && t <= 1.0)) {
+        set_error("Jaccard threshold must be in (0, 1] (got %g)", t);
+        return STORM_HIP_ERR_INVALID;
+    }
+    storm_hip_options o{};
+    if (opt) o = *opt;
+    if (o.policy > STORM_HIP_POLICY_EXHAUSTIVE || o.bb_kernel > STORM_HIP_BB_MFMA_FP4 ||
+        o.fixed_cell >= STORM_HIP_NCELL)
+        return STORM_HIP_ERR_INVALID;
+    if (c->dense || (o.policy != STORM_HIP_POLICY_ANALYTIC &&
+                     o.policy != STORM_HIP_POLICY_LEARNED && o.policy != STORM_HIP_POLICY_FIXED)) {
+        set_error("thresholded calls take a general corpus and a selecting policy");
+        return STORM_HIP_ERR_UNSUPPORTED;
+    }
+    const uint32_t tile = o.tile ? o.tile : 64;
+    SH_TRY(check_tile(tile));
+    storm_hip_stats local{};
+    storm_hip_stats& s = st ? *st : local;
+    std::memset(&s, 0, sizeof s);
+    const auto t0 = std::chrono::steady_clock::now();
+    SH_HIP(hipSetDevice(ctx->device));
+    const uint64_t n = c->n_rows;
+    s.pairs = n ? n * (n - 1) / 2 : 0;
+    c->hits_valid = false;
+    c->h_hits.clear();
+    if (n < 2) {
+        c->hits_valid = true;
+        return STORM_HIP_OK;
+    }
+    if (!c->d_n_hits) SH_HIP(hipMalloc((void**)&c->d_n_hits, 8));
+    if (!c->d_hits) SH_TRY(grow_hits(c, 1u << 16));
+    unsigned long long nh = 0;
+    SH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    SH_TRY(threshold_pass(ctx, c, o, tile, t, &s, &nh));
+    if (nh > c->hit_cap) {
+        // The buffer overflowed: grow it to fit and run again.
+        SH_TRY(grow_hits(c, nh + nh / 8));
+        SH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+        SH_TRY(threshold_pass(ctx, c, o, tile, t, &s, &nh));
+        if (nh > c->hit_cap) {
+            set_error("thresholded call: pair count moved between passes");
+            return STORM_HIP_ERR_DEVICE;
+        }
+    }
+    SH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    std::vector<PairHit> h(nh);
+    if (nh) SH_HIP(hipMemcpyAsync(h.data(), c->d_hits, nh * sizeof(PairHit),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    SH_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    s.ms_device = ms;
+    // Reported in (i, j) order whatever order the device appended them in.
+    std::sort(h.begin(), h.end(), [](const PairHit& a, const PairHit& b) {
+        return a.i != b.i ? a.i < b.i : a.j < b.j;
+    });
+    c->h_hits.resize(nh * 3);
+    for (uint64_t k = 0; k < nh; ++k) {
+        c->h_hits[3 * k] = h[k].i;
+        c->h_hits[3 * k + 1] = h[k].j;
+        c->h_hits[3 * k + 2] = h[k].count;
+    }
+    c->hits_valid = true;
+    *n_pairs = nh;
+    s.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return STORM_HIP_OK;
+}
+
+int storm_hip_result_copy_pairs(storm_hip_ctx* ctx, const storm_hip_corpus* c, uint64_t offset,
+                                uint64_t count, storm_hip_pair* out) {
+    if (!ctx || !c || c->ctx != ctx || !c->hits_valid) return STORM_HIP_ERR_INVALID;
+    if (!out && count) return STORM_HIP_ERR_NULL_VALUES;
+    if (offset + count > c->h_hits.size() / 3) return STORM_HIP_ERR_RANGE;
+    for (uint64_t k = 0; k < count; ++k) {
+        out[k].i = c->h_hits[3 * (offset + k)];
+        out[k].j = c->h_hits[3 * (offset + k) + 1];
+        out[k].count = c->h_hits[3 * (offset + k) + 2];
+    }
     return STORM_HIP_OK;
 }
 

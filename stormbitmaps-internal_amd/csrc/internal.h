@@ -113,6 +113,13 @@ struct storm_hip_corpus {
     bool tri_valid = false;
     // Decisions of the last selecting call.
     std::vector<uint8_t> last_cells;
+    // Thresholded calls: device pair buffer (capacity in pairs) and counter,
+    // and the last call's pairs sorted by (i, j), 3 u32 each.
+    void* d_hits = nullptr;
+    uint64_t hit_cap = 0;
+    unsigned long long* d_n_hits = nullptr;
+    std::vector<uint32_t> h_hits;
+    bool hits_valid = false;
 };
 
 namespace stormhip {

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void select_kernel(
     const TileMeta* __restrict__ tm, uint32_t T, uint64_t P, DevModel M, uint32_t reps,
     uint32_t policy, uint32_t fixed_cell, const float* __restrict__ learned_cost,
     const uint8_t* __restrict__ learned_ok, uint8_t* __restrict__ decisions,
-    QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, bool split,
+    QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, bool split, double thr,
     DevCounters* __restrict__ ctr) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P) return;
@@ -201,7 +201,9 @@ __global__ __launch_bounds__(256) void select_kernel(
     const uint32_t elig = eligible_mask(a, b, reps);
     int p;
     bool fallback = false;
-    if (tile_empty(a, b)) {
+    // Thresholded calls: a tile whose length bound is below the threshold
+    // holds no reportable pair.
+    if (tile_empty(a, b) || (thr > 0.0 && length_bound(a, b) < thr)) {
         p = STORM_HIP_CELL_EMPTY;
     } else if (policy == STORM_HIP_POLICY_FIXED) {
         p = (int)fixed_cell;
@@ -276,12 +278,12 @@ int launch_tile_meta(hipStream_t s, const DevCorpus& D, uint32_t tile, uint32_t 
 int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel& M, uint32_t reps,
                   uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
-                  uint32_t inst, uint32_t tile, DevCounters* ctr) {
+                  uint32_t inst, uint32_t tile, double thr, DevCounters* ctr) {
     const uint64_t P = (uint64_t)T * (T + 1) / 2;
     if (P == 0) return STORM_HIP_OK;
     hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, s, tm, T, P,
                        M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions, queue,
-                       qstride, inst, tile > kSplitRows, ctr);
+                       qstride, inst, tile > kSplitRows, thr, ctr);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

@@ -18,6 +18,14 @@ OUT = {"sum": 0, "tri": 1}
 BB = {"auto": 0, "valu": 1, "mfma": 2, "mfma_fp4": 3}
 CELLS = ["BB", "BS", "BR", "BW", "SS", "SR", "SW", "RR", "RW", "WW", "EMPTY", "CB"]
 NCELL = len(CELLS)
+# Representation flags of from_lists (storm_hip.h STORM_HIP_BUILD_*).
+BUILD_RUNS, BUILD_EWAH, BUILD_COMP, BUILD_ON_HOST = 4, 8, 16, 32
+# Arena ids of storm_hip_corpus_download_arena, in order, with element types.
+ARENAS = [("list_off", np.uint64), ("list", np.uint32), ("run_off", np.uint64),
+          ("run_start", np.uint32), ("run_end", np.uint32), ("ewah_off", np.uint64),
+          ("ewah", np.uint64), ("mk_off", np.uint64), ("mk_word", np.uint32),
+          ("mk_pos", np.uint32), ("comp_off", np.uint64), ("comp", np.uint32),
+          ("meta", np.uint32), ("bm_row", np.uint32), ("bitmap", np.uint64)]
 
 
 class StormHipError(RuntimeError):
@@ -150,6 +158,17 @@ def lib():
         "storm_hip_set_bb_config": (i32, [vp, C.POINTER(BBConfig)]),
         "storm_hip_get_bb_config": (i32, [vp, C.POINTER(BBConfig)]),
         "storm_hip_autotune": (i32, [vp, vp, C.POINTER(Options), C.POINTER(TuneResult)]),
+        "storm_hip_corpus_from_device_lists": (i32, [vp, vp, vp, u32, u32, u32, pvp]),
+        "storm_hip_stormbin_read": (i32, [C.c_char_p, u32, u32,
+                                          C.POINTER(C.POINTER(C.c_uint64)),
+                                          C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(u32),
+                                          C.POINTER(u32)]),
+        "storm_hip_corpus_from_stormbin": (i32, [vp, C.c_char_p, u32, u32, u32, pvp]),
+        "storm_hip_corpus_arena_sizes": (i32, [vp, vp, C.POINTER(u64)]),
+        "storm_hip_corpus_download_arena": (i32, [vp, vp, u32, vp, u64]),
+        "storm_hip_allpairs_threshold": (i32, [vp, vp, C.POINTER(Options), C.c_double,
+                                               C.POINTER(Stats), C.POINTER(u64)]),
+        "storm_hip_result_copy_pairs": (i32, [vp, vp, u64, u64, vp]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -158,6 +177,52 @@ def lib():
         f.restype, f.argtypes = res, args
     _lib = L
     return L
+
+
+_hiprt = None
+
+
+def hip_runtime():
+    """The HIP runtime libstorm_hip.so is linked against (loaded with it).
+    Device buffers for the engine come from here: a second HIP runtime in the
+    process (torch bundles its own) does not share devices with this one."""
+    global _hiprt
+    if _hiprt is None:
+        lib()
+        R = C.CDLL("libamdhip64.so.7")
+        R.hipMalloc.argtypes, R.hipMalloc.restype = [C.POINTER(C.c_void_p), C.c_size_t], C.c_int
+        R.hipFree.argtypes, R.hipFree.restype = [C.c_void_p], C.c_int
+        R.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        R.hipMemcpy.restype = C.c_int
+        _hiprt = R
+    return _hiprt
+
+
+class DeviceArray:
+    """A copy of a 1-D numpy array in device memory (hipMalloc + hipMemcpy on
+    the engine's HIP runtime)."""
+
+    def __init__(self, a):
+        a = np.ascontiguousarray(a)
+        self.dtype, self.count = a.dtype, a.size
+        self.ptr = C.c_void_p()
+        R = hip_runtime()
+        if R.hipMalloc(C.byref(self.ptr), max(a.nbytes, 1)) != 0:
+            raise StormHipError("hipMalloc failed")
+        if a.nbytes and R.hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, 1) != 0:
+            self.free()
+            raise StormHipError("hipMemcpy to the device failed")
+
+    def free(self):
+        if self.ptr:
+            hip_runtime().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
 
 
 def check(status, what=""):
@@ -211,18 +276,38 @@ class Context:
               "gen_bernoulli")
         return Corpus(self, h, n_rows)
 
-    def from_lists(self, offs, pos, universe, bitmap_policy=0):
-        """Build every representation on the host (build_row semantics) and
-        upload. offs: n+1 int64 offsets into pos (u32, sorted per row).
-        bitmap_policy: 0 = bitmap for rows of density >= 1/64, 1 = every row,
-        2 = no row."""
+    def from_lists(self, offs, pos, universe, bitmap_policy=0, on_host=False):
+        """Upload CSR lists and build every representation (build_row
+        semantics) on the device, or on host threads with on_host=True.
+        offs: n+1 int64 offsets into pos (u32, sorted per row). bitmap_policy:
+        bits 0-1 are 0 = bitmap for rows of density >= 1/64, 1 = every row,
+        2 = no row; the STORM_HIP_BUILD_* bits (BUILD_*) select arenas."""
         offs = np.ascontiguousarray(offs, dtype=np.uint64)
         pos = np.ascontiguousarray(pos, dtype=np.uint32)
         n = len(offs) - 1
+        flags = bitmap_policy | (BUILD_ON_HOST if on_host else 0)
         h = C.c_void_p()
         check(lib().storm_hip_corpus_from_lists(self.h, _ptr(offs), _ptr(pos), n, universe,
-                                                bitmap_policy, C.byref(h)), "from_lists")
+                                                flags, C.byref(h)), "from_lists")
         return Corpus(self, h, n)
+
+    def from_device_lists(self, offs, pos, universe, bitmap_policy=0):
+        """As from_lists for lists already in device memory: DeviceArray
+        objects holding n+1 u64 offsets and the u32 positions."""
+        n = offs.count - 1
+        h = C.c_void_p()
+        check(lib().storm_hip_corpus_from_device_lists(self.h, offs.ptr, pos.ptr, n, universe,
+                                                       bitmap_policy, C.byref(h)),
+              "from_device_lists")
+        return Corpus(self, h, n)
+
+    def from_stormbin(self, path, max_rows=0, stride=1, bitmap_policy=0):
+        """STORMBIN file -> device corpus (records gathered and built on the
+        device); corpus row k is the k-th kept (non-empty, stride-selected) row."""
+        h = C.c_void_p()
+        check(lib().storm_hip_corpus_from_stormbin(self.h, os.fsencode(path), max_rows, stride,
+                                                   bitmap_policy, C.byref(h)), "from_stormbin")
+        return Corpus(self, h, int(lib().storm_hip_corpus_rows(h)))
 
     def batch(self, corpora):
         return Batch(self, corpora)
@@ -371,6 +456,22 @@ class Corpus:
         check(lib().storm_hip_allpairs(self.ctx.h, self.h, C.byref(o), C.byref(st)), "allpairs")
         return st.as_dict()
 
+    def allpairs_threshold(self, t, policy="analytic", bb="auto", tile=64, fixed_cell=0,
+                           use_graph=False):
+        """Thresholded mode: (stats, pairs) with pairs an (n, 3) uint32 array of
+        (i, j, |Xi n Xj|), i < j, sorted, for every pair whose Jaccard >= t."""
+        o = Options()
+        o.policy, o.out_mode, o.bb_kernel = POLICY[policy], OUT["sum"], BB[bb]
+        o.tile, o.fixed_cell, o.use_graph = tile, fixed_cell, int(use_graph)
+        st = Stats()
+        n = C.c_uint64(0)
+        check(lib().storm_hip_allpairs_threshold(self.ctx.h, self.h, C.byref(o), t, C.byref(st),
+                                                 C.byref(n)), "allpairs_threshold")
+        out = np.zeros((n.value, 3), dtype=np.uint32)
+        check(lib().storm_hip_result_copy_pairs(self.ctx.h, self.h, 0, n.value, _ptr(out)),
+              "copy_pairs")
+        return st.as_dict(), out
+
     def tri(self, offset=0, count=None):
         if count is None:
             count = self.n_pairs - offset
@@ -388,6 +489,18 @@ class Corpus:
         out = np.empty(n_tiles, dtype=np.uint8)
         check(lib().storm_hip_last_decisions(self.ctx.h, self.h, _ptr(out), n_tiles),
               "last_decisions")
+        return out
+
+    def arenas(self):
+        """Every arena of a general corpus as numpy arrays keyed by name."""
+        sizes = (C.c_uint64 * len(ARENAS))()
+        check(lib().storm_hip_corpus_arena_sizes(self.ctx.h, self.h, sizes), "arena_sizes")
+        out = {}
+        for k, (name, dt) in enumerate(ARENAS):
+            a = np.empty(int(sizes[k]) // np.dtype(dt).itemsize, dtype=dt)
+            check(lib().storm_hip_corpus_download_arena(self.ctx.h, self.h, k, _ptr(a),
+                                                        int(sizes[k])), f"download {name}")
+            out[name] = a
         return out
 
 
@@ -457,3 +570,43 @@ def gen_corpus(kind, n_rows, universe, p1, p2=0.0, p3=0.0, seed=1):
         lib().storm_hip_gen_free(C.cast(offs_p, C.c_void_p))
         lib().storm_hip_gen_free(C.cast(pos_p, C.c_void_p))
     return offs, pos
+
+
+def stormbin_read(path, max_rows=0, stride=1):
+    """STORMBIN file -> (offs, pos, universe) host CSR (storm_hip_stormbin_read:
+    the reference loader's row selection)."""
+    offs_p = C.POINTER(C.c_uint64)()
+    pos_p = C.POINTER(C.c_uint32)()
+    n, u = C.c_uint32(0), C.c_uint32(0)
+    check(lib().storm_hip_stormbin_read(os.fsencode(path), max_rows, stride, C.byref(offs_p),
+                                        C.byref(pos_p), C.byref(n), C.byref(u)),
+          "stormbin_read")
+    try:
+        offs = np.ctypeslib.as_array(offs_p, shape=(n.value + 1,)).copy()
+        total = int(offs[-1])
+        pos = (np.ctypeslib.as_array(pos_p, shape=(total,)).copy() if total
+               else np.zeros(0, np.uint32))
+    finally:
+        lib().storm_hip_gen_free(C.cast(offs_p, C.c_void_p))
+        lib().storm_hip_gen_free(C.cast(pos_p, C.c_void_p))
+    return offs, pos, u.value
+
+
+def write_stormbin(path, offs, pos, universe):
+    """Writes CSR rows as STORMBIN (tools/sets2bin.py:23-25 layout: magic,
+    <IIII> version 1, n_rows, n_bits, 0, then per row <I> count + u32 positions)."""
+    offs = np.asarray(offs, dtype=np.uint64)
+    pos = np.asarray(pos, dtype=np.uint32)
+    n = len(offs) - 1
+    counts = np.diff(offs).astype(np.uint32)
+    # Interleave counts and positions in one u32 buffer.
+    buf = np.empty(n + len(pos), dtype="<u4")
+    heads = (np.arange(n, dtype=np.uint64) + offs[:-1]).astype(np.int64)
+    buf[heads] = counts
+    mask = np.ones(len(buf), dtype=bool)
+    mask[heads] = False
+    buf[mask] = pos
+    with open(path, "wb") as f:
+        f.write(b"STORMBIN")
+        f.write(np.array([1, n, universe, 0], dtype="<u4").tobytes())
+        f.write(buf.tobytes())

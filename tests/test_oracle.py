@@ -124,39 +124,7 @@ def ref_row(pos, u):
                 run_start=rs[:nruns], run_end=re_[:nruns], ewah=ew[:newah], comp=comp[:ncomp])
 
 
-class OrcRow(C.Structure):
-    _fields_ = [("bitmap", C.POINTER(C.c_uint64)), ("n_words", C.c_uint32),
-                ("rank", C.POINTER(C.c_uint64)), ("n_rank", C.c_uint32),
-                ("occ", C.POINTER(C.c_uint64)), ("n_occ", C.c_uint32),
-                ("list", C.POINTER(C.c_uint32)), ("n_list", C.c_uint32),
-                ("run_start", C.POINTER(C.c_uint32)), ("run_end", C.POINTER(C.c_uint32)),
-                ("n_runs", C.c_uint32),
-                ("ewah", C.POINTER(C.c_uint64)), ("n_ewah", C.c_uint32),
-                ("comp_list", C.POINTER(C.c_uint32)), ("n_comp", C.c_uint32),
-                ("comp_start", C.POINTER(C.c_uint32)), ("comp_end", C.POINTER(C.c_uint32)),
-                ("n_comp_runs", C.c_uint32),
-                ("meta", C.c_uint32 * 6), ("has_rank", C.c_uint8), ("has_occ", C.c_uint8)]
-
-
-def orc_row(pos, u):
-    L = O.oracle()
-    L.orc_build_row.argtypes = [C.POINTER(OrcRow), C.c_void_p, C.c_uint32, C.c_uint32]
-    L.orc_build_row.restype = C.c_int
-    L.orc_free_row.argtypes = [C.POINTER(OrcRow)]
-    r = OrcRow()
-    assert L.orc_build_row(C.byref(r), P(pos), len(pos), u) == 0
-
-    def arr(p, n, dt):
-        return np.ctypeslib.as_array(p, shape=(n,)).astype(dt).copy() if n else np.zeros(0, dt)
-
-    out = dict(meta=np.array(list(r.meta), np.uint32), bitmap=arr(r.bitmap, r.n_words, np.uint64),
-               rank=arr(r.rank, r.n_rank, np.uint64), occ=arr(r.occ, r.n_occ, np.uint64),
-               run_start=arr(r.run_start, r.n_runs, np.uint32),
-               run_end=arr(r.run_end, r.n_runs, np.uint32),
-               ewah=arr(r.ewah, r.n_ewah, np.uint64), comp=arr(r.comp_list, r.n_comp, np.uint32),
-               has_rank=r.has_rank, has_occ=r.has_occ)
-    L.orc_free_row(C.byref(r))
-    return out
+orc_row = O.build_row
 
 
 def _rowsets():
