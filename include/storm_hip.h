@@ -266,6 +266,18 @@ int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c,
 int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                               uint64_t* row_sum);
 
+/* ---- rectangular form -----------------------------------------------------
+ *
+ * |Xi n Yj| for every row i of x and every row j of y: the two-input form the
+ * reference plans as the STORM_xy family (RESEARCH_PLAN.md:448-449), of which
+ * the upper triangle is the special case x == y. Two dense corpora of the
+ * same universe; matrix-core B x B only (bb_kernel AUTO, MFMA or MFMA_FP4,
+ * else STORM_HIP_ERR_UNSUPPORTED). *sum receives the total over all n_x * n_y
+ * pairs; out (host, n_x * n_y u32, row-major, out[i * n_y + j]) may be NULL. */
+int storm_hip_allpairs_xy(storm_hip_ctx* ctx, const storm_hip_corpus* x,
+                          const storm_hip_corpus* y, uint32_t bb_kernel, uint64_t* sum,
+                          uint32_t* out);
+
 /* ---- thresholded mode -----------------------------------------------------
  *
  * Exact |Xi n Xj| for exactly the pairs i < j whose Jaccard similarity reaches

@@ -315,6 +315,56 @@ int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
     return STORM_HIP_OK;
 }
 
+int storm_hip_allpairs_xy(storm_hip_ctx* ctx, const storm_hip_corpus* x,
+                          const storm_hip_corpus* y, uint32_t bb_kernel, uint64_t* sum,
+                          uint32_t* out) {
+    if (!ctx || !x || !y || x->ctx != ctx || y->ctx != ctx || !sum) return STORM_HIP_ERR_INVALID;
+    *sum = 0;
+    if (!x->dense || !y->dense) {
+        set_error("the rectangular form takes two dense corpora");
+        return STORM_HIP_ERR_UNSUPPORTED;
+    }
+    if (x->universe != y->universe || x->ld != y->ld) {
+        set_error("rectangular corpora differ in universe (%u vs %u bits)", x->universe,
+                  y->universe);
+        return STORM_HIP_ERR_INVALID;
+    }
+    if (bb_kernel > STORM_HIP_BB_MFMA_FP4 || bb_kernel == STORM_HIP_BB_VALU) {
+        set_error("the rectangular form runs on the matrix cores (AUTO, MFMA or MFMA_FP4)");
+        return STORM_HIP_ERR_UNSUPPORTED;
+    }
+    const bool fp4 = bb_kernel == STORM_HIP_BB_MFMA_FP4 ||
+                     (bb_kernel == STORM_HIP_BB_AUTO && x->n_words < (1u << 18));
+    SH_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const uint64_t count = (uint64_t)x->n_rows * y->n_rows;
+    uint32_t* d_out = nullptr;
+    unsigned long long* d_sum = nullptr;
+    if (out && count) SH_HIP(hipMalloc(&d_out, count * 4));
+    hipError_t e = hipMalloc(&d_sum, 8);
+    if (e == hipSuccess) e = hipMemsetAsync(d_sum, 0, 8, s);
+    int rc = e == hipSuccess ? STORM_HIP_OK : STORM_HIP_ERR_DEVICE;
+    if (rc == STORM_HIP_OK)
+        rc = launch_bb_rect(fp4, ctx->mfma, s, x->d_words, x->n_rows, y->d_words, y->n_rows, x->ld,
+                            x->n_words, d_out, d_sum, (uint32_t)ctx->cu_count);
+    unsigned long long h_sum = 0;
+    if (rc == STORM_HIP_OK) {
+        e = hipMemcpyAsync(&h_sum, d_sum, 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && d_out)
+            e = hipMemcpyAsync(out, d_out, count * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            set_error("rectangular call: %s", hipGetErrorString(e));
+            rc = STORM_HIP_ERR_DEVICE;
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    if (d_out) (void)hipFree(d_out);
+    if (d_sum) (void)hipFree(d_sum);
+    if (rc == STORM_HIP_OK) *sum = h_sum;
+    return rc;
+}
+
 int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c, uint64_t offset,
                               uint64_t count, uint32_t* out) {
     if (!ctx || !c || !c->tri_valid) return STORM_HIP_ERR_INVALID;

@@ -88,16 +88,26 @@ struct BBMfmaSmem {
 // One 256 x 256 block over stages [s0, s0 + n_stages) of SW words. With
 // `accumulate`, counts are atomically added to `tri` (a K-split block: other
 // blocks add the other stage ranges of the same pairs into a zeroed triangle).
-// THR (thresholded queue launches): every pair goes through the output of the
-// corpus view `Dp`; a separate instance, so the common epilogue stays small.
-template <bool FP4, int SW, int NW, bool THR = false>
+// THR (thresholded queue launches): the pair counts of the block go to `tri`
+// as a dense 256 x 256 row-major buffer (local row, local column); the caller
+// filters them. Filtering here would hold the test's registers next to the
+// accumulators.
+// RECT (rectangular X . Y^T): the B rows come from `Rx->Y` (same row stride),
+// every (i < n, j < Rx->ny) is a pair, and counts go to Rx->out[i*ny + j].
+struct RectArgs {
+    const uint64_t* Y;
+    uint32_t ny;
+    uint32_t* out;
+};
+
+template <bool FP4, int SW, int NW, bool THR = false, bool RECT = false>
 __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint64_t* X, uint32_t ld,
                                                  uint32_t s0, uint32_t n_stages,
                                                  const uint32_t* __restrict__ rowmap,
                                                  const uint32_t* __restrict__ perm, uint32_t n,
                                                  uint32_t bi, uint32_t bj,
                                                  uint32_t* __restrict__ tri, bool accumulate,
-                                                 const DevCorpus* __restrict__ Dp = nullptr) {
+                                                 const RectArgs* __restrict__ Rx = nullptr) {
     using Acc = typename std::conditional<FP4, f32x16, i32x16>::type;
     constexpr int L = SW / 2;        // 16-byte loads per row per stage
     constexpr int R = SW / 4;        // 16-byte LDS reads per fragment row per stage
@@ -115,8 +125,10 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
     auto src_of = [&](uint32_t r) {
         const uint32_t pos = r < (uint32_t)kTile ? i0 + r : j0 + (r - kTile);
         const uint64_t phys = rowmap ? rowmap[pos] : (uint64_t)pos;
+        const uint64_t* base = X;
+        if constexpr (RECT) base = r < (uint32_t)kTile ? X : Rx->Y;
         // Rows live in global memory: loads through this pointer are global loads.
-        return (global_u32x4*)(X + phys * ld) + (size_t)L * s0;
+        return (global_u32x4*)(base + phys * ld) + (size_t)L * s0;
     };
     global_u32x4* src = src_of(t);
     global_u32x4* src2 = THREADS < 512 ? src_of(t + THREADS) : src;
@@ -238,11 +250,21 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
 #pragma unroll
             for (int q = 0; q < NB; ++q) {
                 const uint32_t j = j0 + wc * 32 * NB + q * 32 + lr;
-                if (j >= n || j <= i) continue;
+                if constexpr (RECT) {
+                    if (j >= Rx->ny) continue;
+                } else {
+                    if (j >= n || j <= i) continue;
+                }
                 const uint32_t val = (uint32_t)acc(m, q)[v];
                 local += val;
-                if constexpr (THR) {
-                    emit_pair(*Dp, oi, perm ? perm[j] : j, val);
+                if constexpr (RECT) {
+                    if (Rx->out) {
+                        uint32_t* dst = &Rx->out[(uint64_t)i * Rx->ny + j];
+                        if (accumulate) { if (val) atomicAdd(dst, val); }
+                        else            *dst = val;
+                    }
+                } else if constexpr (THR) {
+                    tri[(i - i0) * kTile + (j - j0)] = val;
                 } else if (tri) {
                     const uint32_t oj = perm ? perm[j] : j;
                     uint32_t* dst = &tri[oi < oj ? tri_index(oi, oj, n) : tri_index(oj, oi, n)];
@@ -292,9 +314,25 @@ __global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
             const DevCorpus D = Ds[e.inst];
             const uint64_t t0 = wall_clock64();
             const uint32_t ns = max(1u, (D.nw + SW - 1) / SW);
-            uint64_t part = bb_mfma_tile<FP4, SW, NW, THR>(sm, D.bitmap, D.ld, 0, ns, D.rowmap,
-                                                       D.perm, D.n, e.bi, e.bj, D.tri, false,
-                                                       &Ds[e.inst]);
+            uint64_t part;
+            if constexpr (THR) {
+                // The block's counts through its own 256 x 256 buffer, then the
+                // Jaccard test per pair in a plain loop.
+                uint32_t* buf = tri + (uint64_t)blockIdx.x * kTile * kTile;
+                part = bb_mfma_tile<FP4, SW, NW, true>(sm, D.bitmap, D.ld, 0, ns, D.rowmap,
+                                                       D.perm, D.n, e.bi, e.bj, buf, false);
+                __threadfence();
+                __syncthreads();
+                const uint32_t i0 = e.bi * kTile, j0 = e.bj * kTile;
+                for (uint32_t k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
+                    const uint32_t i = i0 + k / kTile, j = j0 + k % kTile;
+                    if (i >= D.n || j >= D.n || j <= i) continue;
+                    emit_pair(Ds[e.inst], D.perm[i], D.perm[j], buf[k]);
+                }
+            } else {
+                part = bb_mfma_tile<FP4, SW, NW>(sm, D.bitmap, D.ld, 0, ns, D.rowmap, D.perm, D.n,
+                                                 e.bi, e.bj, D.tri, false);
+            }
             part = wave_sum_u64(part);
             if ((threadIdx.x & 63) == 0 && part) atomicAdd(D.sum, (unsigned long long)part);
             __syncthreads();
@@ -306,6 +344,31 @@ __global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
         }
         return;
     }
+    local = wave_sum_u64(local);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(sum, (unsigned long long)local);
+}
+
+// Rectangular X . Y^T over Tx x Ty tiles of 256 x 256 pairs, row-major, each
+// tile's stages split over k_split blocks (which then add into a zeroed
+// output); block b takes its XCD-aware position so each XCD sweeps a run of
+// tiles sharing X panels. 8-word stages, 8 waves.
+template <bool FP4>
+__global__ __launch_bounds__(512, 1) void bb_mfma_rect_kernel(
+    const uint64_t* __restrict__ X, uint32_t nx, const uint64_t* __restrict__ Y, uint32_t ny,
+    uint32_t ld, uint32_t n_stages, uint32_t k_split, uint32_t Ty, uint32_t* __restrict__ out,
+    unsigned long long* __restrict__ sum) {
+    __shared__ __attribute__((aligned(16))) BBMfmaSmem<8> sm;
+    const uint32_t p = xcd_position(blockIdx.x, gridDim.x);
+    const uint32_t tile = p / k_split, slice = p % k_split;
+    const uint32_t per = (n_stages + k_split - 1) / k_split;
+    const uint32_t s0 = min(slice * per, n_stages);
+    const uint32_t cnt = min(per, n_stages - s0);
+    const RectArgs R{Y, ny, out};
+    uint64_t local = 0;
+    if (cnt)
+        local = bb_mfma_tile<FP4, 8, 8, false, true>(sm, X, ld, s0, cnt, nullptr, nullptr, nx,
+                                                     tile / Ty, tile % Ty, nullptr, k_split > 1,
+                                                     &R);
     local = wave_sum_u64(local);
     if ((threadIdx.x & 63) == 0 && local) atomicAdd(sum, (unsigned long long)local);
 }
@@ -436,23 +499,64 @@ uint32_t bb_blocks_per_cu(uint32_t tile, const MfmaConfig& cfg) {
 
 // Queue mode over the corpora in Ds. The caller has checked every corpus that
 // can queue a tile with bb_mfma_check (and picks fp4 only when all allow it).
-// Thresholded calls run the 8-word, 8-wave shape's thresholded instance.
+// Thresholded calls (thr_buf: grid x 256 x 256 u32 of block buffers) run the
+// 8-word, 8-wave shape's thresholded instance.
 int launch_bb_mfma_queue(bool fp4, const MfmaConfig& cfg, hipStream_t s, const DevCorpus* Ds,
                          const QItem* queue, const unsigned int* qcount, unsigned int* qhead,
-                         uint32_t grid, unsigned long long* tile_ticks, bool thresholded) {
+                         uint32_t grid, unsigned long long* tile_ticks, uint32_t* thr_buf) {
     if (grid == 0) return STORM_HIP_OK;
-    if (thresholded) {
+    if (thr_buf) {
         if (fp4)
             hipLaunchKernelGGL((bb_mfma_kernel<true, 8, 8, true>), dim3(grid), dim3(512), 0, s,
                                nullptr, 0u, 1u, 1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue,
-                               qcount, qhead, nullptr, nullptr, tile_ticks);
+                               qcount, qhead, thr_buf, nullptr, tile_ticks);
         else
             hipLaunchKernelGGL((bb_mfma_kernel<false, 8, 8, true>), dim3(grid), dim3(512), 0, s,
                                nullptr, 0u, 1u, 1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue,
-                               qcount, qhead, nullptr, nullptr, tile_ticks);
+                               qcount, qhead, thr_buf, nullptr, tile_ticks);
     } else {
         SH_TRY(with_shape(fp4, cfg, QueueLaunch{s, grid, Ds, queue, qcount, qhead, tile_ticks}));
     }
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+// Rectangular launch: rows of X (nx, padded to 256) against rows of Y (ny,
+// padded to 256), same row stride; out (nx * ny, row-major) may be null.
+int launch_bb_rect(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X, uint32_t nx,
+                   const uint64_t* Y, uint32_t ny, uint32_t ld, uint32_t n_words, uint32_t* out,
+                   unsigned long long* sum, uint32_t cu_count) {
+    SH_TRY(bb_mfma_check(fp4, 8, n_words, ld));
+    const uint64_t Tx = (nx + kTile - 1) / kTile, Ty = (ny + kTile - 1) / kTile;
+    const uint64_t tiles = Tx * Ty;
+    if (tiles == 0) return STORM_HIP_OK;
+    const uint32_t n_stages = std::max(1u, (n_words + 7) / 8);
+    const uint64_t want = (uint64_t)std::max(1u, cfg.blocks_per_cu) * cu_count;
+    uint32_t k_split = 1;
+    if (tiles < want) {
+        k_split = (uint32_t)std::min<uint64_t>((want + tiles - 1) / tiles,
+                                               std::max(1u, n_stages / 4));
+        k_split = std::max(1u, k_split);
+        const uint32_t per = (n_stages + k_split - 1) / k_split;
+        k_split = (n_stages + per - 1) / per;
+    }
+    const uint64_t g = tiles * k_split;
+    if (g > 0xFFFFFFFFull) {
+        set_error("bb_mfma: %llu blocks exceed one launch", (unsigned long long)g);
+        return STORM_HIP_ERR_RANGE;
+    }
+    if (k_split > 1 && out) {
+        const uint64_t count = (uint64_t)nx * ny;
+        hipLaunchKernelGGL(zero_u32_kernel, dim3((uint32_t)std::min<uint64_t>((count + 255) / 256, 65536)),
+                           dim3(256), 0, s, out, count);
+        SH_HIP(hipGetLastError());
+    }
+    if (fp4)
+        hipLaunchKernelGGL(bb_mfma_rect_kernel<true>, dim3((uint32_t)g), dim3(512), 0, s, X, nx, Y,
+                           ny, ld, n_stages, k_split, (uint32_t)Ty, out, sum);
+    else
+        hipLaunchKernelGGL(bb_mfma_rect_kernel<false>, dim3((uint32_t)g), dim3(512), 0, s, X, nx, Y,
+                           ny, ld, n_stages, k_split, (uint32_t)Ty, out, sum);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

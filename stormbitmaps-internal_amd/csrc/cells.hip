@@ -418,6 +418,28 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
 // the streamed side is the shorter one; a staged row must fit kStageItems.
 constexpr uint32_t kStageItems = 4096;
 
+// First index p >= q with a[p] > key (a ascending; n if none): doubling steps
+// from q, then bisection, so a search costs the log of the distance moved.
+__device__ inline uint32_t gallop_gt(const uint32_t* a, uint32_t q, uint32_t n, uint32_t key) {
+    if (q >= n || a[q] > key) return q;
+    uint32_t lo = q, step = 1;   // a[lo] <= key
+    while (lo + step < n && a[lo + step] <= key) {
+        lo += step;
+        step <<= 1;
+    }
+    uint32_t hi = min(n, lo + step);   // hi == n or a[hi] > key
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= key) lo = mid; else hi = mid;
+    }
+    return hi;
+}
+
+// First index p >= q with a[p] >= key.
+__device__ inline uint32_t gallop_ge(const uint32_t* a, uint32_t q, uint32_t n, uint32_t key) {
+    return key == 0 ? q : gallop_gt(a, q, n, key - 1);
+}
+
 struct StageSmem {
     uint32_t a[kStageItems];   // run starts / sorted positions
     uint32_t b[kStageItems];   // run ends (R x R)
@@ -453,29 +475,30 @@ __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint3
             if (span_disjoint(D, x, y)) {
                 if (lane == 0) ++skipped;
             } else {
+                // Each lane merges a contiguous chunk of the streamed row
+                // against the staged one: one galloping search per step from
+                // where the previous item of its chunk left off.
                 uint32_t c = 0;
                 if constexpr (CELL == STORM_HIP_CELL_RR) {
                     const RunsV Q = runs_of(D, y);
-                    for (uint32_t k = lane; k < Q.n; k += 64) {
+                    const uint32_t per = (Q.n + 63) / 64;
+                    const uint32_t k1 = min(Q.n, (lane + 1) * per);
+                    uint32_t q = 0;
+                    for (uint32_t k = min(Q.n, lane * per); k < k1; ++k) {
                         const uint32_t rs = Q.s[k], re = Q.e[k];
-                        uint32_t lo = 0, hi = nx_items;   // first staged run ending after rs
-                        while (lo < hi) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (S.b[mid] <= rs) lo = mid + 1; else hi = mid;
-                        }
-                        for (uint32_t q = lo; q < nx_items && S.a[q] < re; ++q)
-                            c += min(re, S.b[q]) - max(rs, S.a[q]);
+                        q = gallop_gt(S.b, q, nx_items, rs);   // first staged run ending after rs
+                        for (uint32_t p = q; p < nx_items && S.a[p] < re; ++p)
+                            c += min(re, S.b[p]) - max(rs, S.a[p]);
                     }
                 } else {
                     const Span32 Q = list_of(D, y);
-                    for (uint32_t k = lane; k < Q.n; k += 64) {
+                    const uint32_t per = (Q.n + 63) / 64;
+                    const uint32_t k1 = min(Q.n, (lane + 1) * per);
+                    uint32_t q = 0;
+                    for (uint32_t k = min(Q.n, lane * per); k < k1; ++k) {
                         const uint32_t e = Q.p[k];
-                        uint32_t lo = 0, hi = nx_items;
-                        while (lo < hi) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (S.a[mid] < e) lo = mid + 1; else hi = mid;
-                        }
-                        c += (lo < nx_items && S.a[lo] == e) ? 1u : 0u;
+                        q = gallop_ge(S.a, q, nx_items, e);
+                        c += (q < nx_items && S.a[q] == e) ? 1u : 0u;
                     }
                 }
                 cnt = wave_sum_u32(c);

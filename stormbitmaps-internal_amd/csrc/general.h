@@ -58,8 +58,18 @@ __device__ inline void emit_pair(const DevCorpus& D, uint32_t i, uint32_t j, uin
     if (D.hits && v) {
         const double ci = D.meta[(uint64_t)i * 8], cj = D.meta[(uint64_t)j * 8];
         if ((double)v / (ci + cj - (double)v) >= D.thr) {
-            const unsigned long long k = atomicAdd(D.n_hits, 1ull);
-            if (k < D.hit_cap) D.hits[k] = PairHit{i < j ? i : j, i < j ? j : i, v};
+            // Global-address-space accesses: a flat store could alias a
+            // kernel's private arrays and pin them in scratch.
+            typedef __attribute__((address_space(1))) unsigned long long g_u64;
+            typedef __attribute__((address_space(1))) uint32_t g_u32;
+            const unsigned long long k = __hip_atomic_fetch_add(
+                (g_u64*)D.n_hits, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (k < D.hit_cap) {
+                g_u32* h = (g_u32*)(D.hits + k);
+                h[0] = i < j ? i : j;
+                h[1] = i < j ? j : i;
+                h[2] = v;
+            }
         }
     }
 }
@@ -79,14 +89,16 @@ __host__ __device__ inline uint32_t make_sub(uint32_t rows, uint32_t bi_blk, uin
 // Rows per block when a tile is split for the per-pair cells.
 constexpr uint32_t kSplitRows = 64;
 
-// thresholded: the corpora carry a Jaccard threshold and pair buffer (their
-// tri is null); the launch uses the B x B instance that reports pairs.
+// Thresholded calls (the corpora carry a Jaccard threshold and pair buffer,
+// their tri is null) use the B x B instances that report pairs; the matrix-core
+// one takes thr_buf, grid blocks x 256 x 256 u32 of per-block count buffers.
 int launch_bb_valu_queue(hipStream_t s, int bm, const DevCorpus* Ds, const QItem* queue,
                          const unsigned int* qcount, unsigned int* qhead, uint32_t grid,
                          unsigned long long* tile_ticks, bool thresholded = false);
 int launch_bb_mfma_queue(bool fp4, const MfmaConfig& cfg, hipStream_t s, const DevCorpus* Ds,
                          const QItem* queue, const unsigned int* qcount, unsigned int* qhead,
-                         uint32_t grid, unsigned long long* tile_ticks, bool thresholded = false);
+                         uint32_t grid, unsigned long long* tile_ticks,
+                         uint32_t* thr_buf = nullptr);
 int bb_mfma_check(bool fp4, uint32_t stage_words, uint32_t n_words, uint32_t ld);
 // Workgroups resident per CU: a cell kernel; the B x B kernel of a tile width
 // (256 = matrix cores with this launch shape, else VALU).
@@ -151,6 +163,9 @@ struct DevModel {
 // Learned selection ranks only cells predicted (analytically) within this
 // factor of the cheapest eligible cell on the tile.
 constexpr double kLearnedPrune = 32.0;
+// ... and replaces the analytic choice only when it predicts its own pick
+// faster by more than this many log2 units (0.15: about 10%).
+constexpr float kLearnedMargin = 0.15f;
 __host__ __device__ double predict(const DevModel& m, int p, const TileMeta& a, const TileMeta& b);
 __host__ __device__ uint32_t learned_candidates(const DevModel& m, const TileMeta& a,
                                                 const TileMeta& b, uint32_t elig);

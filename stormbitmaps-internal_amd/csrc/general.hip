@@ -99,6 +99,8 @@ struct Thresh {
 // the tile width and the outputs stay the same.
 struct Plan {
     Thresh th;                              // thresholded call (th.hits set) or not
+    uint32_t* thr_buf = nullptr;            // thresholded 256-row B x B: block buffers
+    uint32_t thr_grid = 0;                  // blocks (one 256 x 256 buffer each)
     std::vector<storm_hip_corpus*> cs;
     std::vector<uint32_t*> tris;
     std::vector<Work*> works;               // one per corpus, owned
@@ -119,7 +121,7 @@ void free_plan(Plan* p) {
     if (!p) return;
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
     for (Work* w : p->works) free_work(w);
-    void* q[] = {p->d_corpora, p->queue, p->ctr, p->sums, p->finish};
+    void* q[] = {p->d_corpora, p->queue, p->ctr, p->sums, p->finish, p->thr_buf};
     for (void* x : q) if (x) (void)hipFree(x);
     delete p;
 }
@@ -297,10 +299,15 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     const uint32_t gbb = (uint32_t)std::min<uint64_t>(
         P1, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
     const bool thresholded = p->th.hits != nullptr;
-    if (p->tile == 256)
+    if (p->tile == 256 && thresholded)
         rc = launch_bb_mfma_queue(fp4, ctx->mfma, sb, p->d_corpora, qbb,
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
-                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr, thresholded);
+                                  &p->ctr->qhead[STORM_HIP_CELL_BB], p->thr_grid, nullptr,
+                                  p->thr_buf);
+    else if (p->tile == 256)
+        rc = launch_bb_mfma_queue(fp4, ctx->mfma, sb, p->d_corpora, qbb,
+                                  &p->ctr->qcount[STORM_HIP_CELL_BB],
+                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr);
     else
         rc = launch_bb_valu_queue(sb, (int)p->tile, p->d_corpora, qbb,
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
@@ -474,6 +481,12 @@ int threshold_pass(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_opti
     const int rc = ensure_plan(p, {c}, {nullptr}, tile, th);
     c->plan = p;
     SH_TRY(rc);
+    if (tile == 256 && !p->thr_buf) {
+        // One persistent matrix-core block per CU, each with its count buffer.
+        p->thr_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(p->totalP, 1),
+                                                   (uint64_t)ctx->cu_count);
+        SH_HIP(hipMalloc(&p->thr_buf, (size_t)p->thr_grid * 256 * 256 * 4));
+    }
     uint64_t sum = 0;
     SH_TRY(run_plan(ctx, p, o, st, &sum, nullptr));
     SH_HIP(hipMemcpyAsync(n_hits, c->d_n_hits, 8, hipMemcpyDeviceToHost, ctx->stream));

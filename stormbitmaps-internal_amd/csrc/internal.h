@@ -138,4 +138,9 @@ int launch_bb_mfma_fmt(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uin
 int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
                     uint32_t ld, uint32_t n_words, const uint32_t* rowmap, uint32_t n,
                     uint32_t* tri, unsigned long long* sum, uint32_t cu_count);
+// Rectangular X . Y^T on the matrix cores (rows of both padded to 256, same
+// stride); out: nx * ny u32 row-major, or null for the sum only.
+int launch_bb_rect(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X, uint32_t nx,
+                   const uint64_t* Y, uint32_t ny, uint32_t ld, uint32_t n_words, uint32_t* out,
+                   unsigned long long* sum, uint32_t cu_count);
 }  // namespace stormhip

@@ -218,6 +218,15 @@ __global__ __launch_bounds__(256) void select_kernel(
             if (best < 0 || c < bc) { bc = c; best = q; }
         }
         p = best < 0 ? STORM_HIP_CELL_SS : best;
+        // Trust region: the analytic choice stands unless the network predicts
+        // the other cell faster by more than kLearnedMargin (log2 units).
+        double ca;
+        int pa = select_pairing(M, a, b, &ca);
+        if (pa == STORM_HIP_CELL_EMPTY) pa = STORM_HIP_CELL_BB;
+        if (!(elig >> pa & 1)) pa = cheapest_eligible(M, a, b, elig);
+        if (best >= 0 && pa != best && (cand >> pa & 1) &&
+            learned_cost[t * 10 + pa] - bc <= kLearnedMargin)
+            p = pa;
     } else {
         if (policy == STORM_HIP_POLICY_LEARNED) fallback = true;
         double c;

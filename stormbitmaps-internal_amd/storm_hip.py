@@ -169,6 +169,7 @@ def lib():
         "storm_hip_allpairs_threshold": (i32, [vp, vp, C.POINTER(Options), C.c_double,
                                                C.POINTER(Stats), C.POINTER(u64)]),
         "storm_hip_result_copy_pairs": (i32, [vp, vp, u64, u64, vp]),
+        "storm_hip_allpairs_xy": (i32, [vp, vp, vp, u32, C.POINTER(u64), vp]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -311,6 +312,15 @@ class Context:
 
     def batch(self, corpora):
         return Batch(self, corpora)
+
+    def allpairs_xy(self, x, y, bb="auto", matrix=True):
+        """Rectangular form: (sum, counts) with counts[i, j] = |Xi n Yj| for two
+        dense corpora (counts None when matrix is False)."""
+        s = C.c_uint64(0)
+        out = np.zeros((x.n, y.n), dtype=np.uint32) if matrix else None
+        check(lib().storm_hip_allpairs_xy(self.h, x.h, y.h, BB[bb], C.byref(s), _ptr(out)),
+              "allpairs_xy")
+        return s.value, out
 
     # -- learned selector ------------------------------------------------------
     def selector_sample(self, corpus, tile=64, max_tiles=512, seed=1, repeats=2):
