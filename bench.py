@@ -366,10 +366,13 @@ def batch_bench(sh, args):
     res["analytic"] = timed("analytic")
     progress("sampling tiles for the selector")
     Xs, Ys, Ms = [], [], []
-    order = sorted(range(len(corpora)), key=lambda i: -corpora[i].n_pairs)[:8]
+    # Instances spread over the size order (every k-th of the sorted list), so
+    # the training tiles cover the batch's range of universes and densities.
+    by_size = sorted(range(len(corpora)), key=lambda i: -corpora[i].n_pairs)
+    order = by_size[::max(1, len(by_size) // 32)][:32]
     for i in order:
         X, Y, M, _ = ctx.selector_sample(corpora[i], tile=args.tile,
-                                         max_tiles=max(32, args.sample_tiles // 8), seed=11 + i,
+                                         max_tiles=max(8, args.sample_tiles // 32), seed=11 + i,
                                          repeats=2)
         Xs.append(X), Ys.append(Y), Ms.append(M)
         progress(f"sampled instance {i}: {len(X)} tiles")

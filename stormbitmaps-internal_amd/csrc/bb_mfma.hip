@@ -24,11 +24,11 @@
 // same bit positions in the same K slots, so each MFMA sums products over the
 // same set of positions whichever order the matrix core assigns to them. Lane
 // group g (lanes 32g..32g+31) takes dwords g*SW .. g*SW+SW-1 of the stage's
-// 2*SW dwords (natural memory order), one 32-bit chunk per MFMA step; for a
-// chunk w:
-//   FP4: nibble n of dword d (0..3) holds bit 4n+d, encoded 0.5/1/2/2 in A and
-//        2/1/0.5/0.5 in B (a mask, or a shift and a mask, per dword)
-//   I8 : two K=32 steps h = 0, 1 per chunk; dword d (0..3) of step h =
+// 2*SW dwords (natural memory order), 32 bits per lane group per MFMA step:
+//   FP4: step 4r+d takes bit slice d of chunks 4r..4r+3: dword k, nibble n =
+//        bit 4n+d of chunk 4r+k, value 2^(d-1) (d = 3 shifted onto bit 2),
+//        block scales 2^(1-d) on A and B (one mask, or shift and mask, each)
+//   I8 : chunk c, two K=32 steps h = 0, 1; dword d (0..3) of step h =
 //        (w >> (4h+d)) & 0x01010101                   (byte j = bit 8j+4h+d)
 #include "device_common.h"
 #include "general.h"
@@ -38,6 +38,7 @@ namespace stormhip {
 typedef int8_t i8x16 __attribute__((ext_vector_type(16)));
 typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
 typedef int32_t i32x8 __attribute__((ext_vector_type(8)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const u32x4 global_u32x4;
@@ -46,29 +47,23 @@ namespace {
 constexpr int kTile = 256;        // pairs block: kTile x kTile
 constexpr int kUnitScale = 127;   // E8M0 exponent of 1.0
 
-// FP4 operands: A and B give bit 4n+d of a chunk the same K slot (nibble n of
-// dword d) but may encode it differently, as long as a set bit in both
-// multiplies to exactly 1.0. A keeps bits where they are whenever a mask
-// alone yields a usable value (d = 0: 0.5, d = 1: 1.0, d = 2: 2.0; the sign
-// bit d = 3 moves to 2.0); B carries the reciprocal (2.0, 1.0, 0.5, 0.5).
-// A costs 5 VALU operations per chunk, B 7; A fragments are expanded twice as
-// often as B fragments per wave.
-__device__ inline i32x8 expand_fp4_a(uint32_t w) {
-    return i32x8{(int)(w & 0x11111111u), (int)(w & 0x22222222u), (int)(w & 0x44444444u),
-                 (int)((w >> 1) & 0x44444444u), 0, 0, 0, 0};
-}
-__device__ inline i32x8 expand_fp4_b(uint32_t w) {
-    return i32x8{(int)((w << 2) & 0x44444444u), (int)(w & 0x22222222u),
-                 (int)((w >> 2) & 0x11111111u), (int)((w >> 3) & 0x11111111u), 0, 0, 0, 0};
+// FP4 operand of one MFMA: bit slice d of four 32-bit chunks v.x..v.w -- dword
+// k holds chunk k's bits 4n+d at nibble n, left where they are by a mask, so
+// every element is the same e2m1 value 2^(d-1) (0.5, 1, 2; the sign bit d = 3
+// is shifted onto 2). A and B slice alike and the MFMA's E8M0 block scales
+// (2^(1-d) on each side) make every product of two set bits exactly 1.
+// 5 VALU operations per four chunks and operand row.
+__device__ inline i32x8 slice_fp4(const uint4& v, int d) {
+    const uint32_t m = d == 0 ? 0x11111111u : d == 1 ? 0x22222222u : 0x44444444u;
+    const uint32_t s = d == 3 ? 1u : 0u;
+    return i32x8{(int)((v.x >> s) & m), (int)((v.y >> s) & m), (int)((v.z >> s) & m),
+                 (int)((v.w >> s) & m), 0, 0, 0, 0};
 }
 __device__ inline i8x16 expand_i8(uint32_t w, int g) {
-    const uint32_t s = 4u * g;
-    const uint32_t d0 = (w >> s) & 0x01010101u, d1 = (w >> (s + 1)) & 0x01010101u;
-    const uint32_t d2 = (w >> (s + 2)) & 0x01010101u, d3 = (w >> (s + 3)) & 0x01010101u;
-    i8x16 r;
-    uint32_t tmp[4] = {d0, d1, d2, d3};
-    __builtin_memcpy(&r, tmp, 16);
-    return r;
+    const uint32_t s = 4u * g, m = 0x01010101u;
+    const i32x4 d = {(int)((w >> s) & m), (int)((w >> (s + 1)) & m), (int)((w >> (s + 2)) & m),
+                     (int)((w >> (s + 3)) & m)};
+    return __builtin_bit_cast(i8x16, d);
 }
 __device__ inline uint32_t comp(const uint4& v, int c) {
     return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
@@ -205,16 +200,19 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
 #pragma unroll
             for (int q = 0; q < NB; ++q) wb[q] = comp(pb[q][c >> 2], c & 3);
             if constexpr (FP4) {
+                // Step c = 4r + d: bit slice d of the four chunks 4r..4r+3.
+                const int r = c >> 2, d = c & 3;
+                const int sc = kUnitScale + 1 - (d < 2 ? d : 2);
                 i32x8 fb[NB];
 #pragma unroll
-                for (int q = 0; q < NB; ++q) fb[q] = expand_fp4_b(wb[q]);
+                for (int q = 0; q < NB; ++q) fb[q] = slice_fp4(pb[q][r], d);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    const i32x8 fa = expand_fp4_a(wa[m]);
+                    const i32x8 fa = slice_fp4(pa[m][r], d);
 #pragma unroll
                     for (int q = 0; q < NB; ++q)
                         acc(m, q) = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                            fa, fb[q], acc(m, q), 4, 4, 0, kUnitScale, 0, kUnitScale);
+                            fa, fb[q], acc(m, q), 4, 4, 0, sc, 0, sc);
                 }
             } else {
                 // 32 bits per lane group per chunk = two K=32 steps (h = 0, 1

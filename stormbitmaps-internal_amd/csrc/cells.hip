@@ -418,32 +418,31 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
 // the streamed side is the shorter one; a staged row must fit kStageItems.
 constexpr uint32_t kStageItems = 4096;
 
-// First index p >= q with a[p] > key (a ascending; n if none): doubling steps
-// from q, then bisection, so a search costs the log of the distance moved.
-__device__ inline uint32_t gallop_gt(const uint32_t* a, uint32_t q, uint32_t n, uint32_t key) {
-    if (q >= n || a[q] > key) return q;
-    uint32_t lo = q, step = 1;   // a[lo] <= key
-    while (lo + step < n && a[lo + step] <= key) {
-        lo += step;
-        step <<= 1;
-    }
-    uint32_t hi = min(n, lo + step);   // hi == n or a[hi] > key
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] <= key) lo = mid; else hi = mid;
-    }
-    return hi;
-}
-
-// First index p >= q with a[p] >= key.
-__device__ inline uint32_t gallop_ge(const uint32_t* a, uint32_t q, uint32_t n, uint32_t key) {
-    return key == 0 ? q : gallop_gt(a, q, n, key - 1);
-}
+// Bucket index over the staged row: the universe in nb buckets of 2^shift
+// positions (nb the power of two at or above the row's item count, at most
+// kStageBuckets); idx[k] = first staged item whose key (run end for R x R,
+// position for S x S) is above k << shift, so a lookup bisects one bucket's
+// items instead of the whole row, and the index costs O(items) to build.
+constexpr uint32_t kStageBuckets = 4096;
 
 struct StageSmem {
     uint32_t a[kStageItems];   // run starts / sorted positions
     uint32_t b[kStageItems];   // run ends (R x R)
+    uint32_t idx[kStageBuckets];
 };
+
+// Fills S.idx for the staged keys key[0..n) (ascending; run ends, or
+// positions + 1 so that "above" means "at or above the position").
+__device__ inline void stage_index(StageSmem& S, const uint32_t* key, uint32_t n, uint32_t nb,
+                                   uint32_t shift, bool plus_one) {
+    for (uint32_t p = threadIdx.x; p <= n; p += blockDim.x) {
+        // Buckets whose first answer is p: prev_key <= k << shift < key[p].
+        const uint64_t prev = p ? (uint64_t)key[p - 1] + plus_one : 0;
+        const uint64_t lo = p ? (prev + (1ull << shift) - 1) >> shift : 0;
+        const uint64_t hi = p < n ? (((uint64_t)key[p] + plus_one - 1) >> shift) + 1 : (uint64_t)nb;
+        for (uint64_t k = lo; k < hi && k < nb; ++k) S.idx[k] = p;
+    }
+}
 
 template <int CELL>
 __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint32_t ni,
@@ -456,16 +455,35 @@ __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint3
         const uint32_t x = D.perm[xs0 + u];
         __syncthreads();   // the previous staged row is no longer read
         uint32_t nx_items;
+        if constexpr (CELL == STORM_HIP_CELL_RR) nx_items = runs_of(D, x).n;
+        else nx_items = list_of(D, x).n;
+        // nb buckets of 2^shift positions cover the universe.
+        uint32_t nb = 1, shift = 0;
+        while (nb < nx_items && nb < kStageBuckets) nb <<= 1;
+        while (((uint64_t)nb << shift) < D.universe) ++shift;
         if constexpr (CELL == STORM_HIP_CELL_RR) {
             const RunsV R = runs_of(D, x);
-            nx_items = R.n;
             for (uint32_t k = tid; k < R.n; k += blockDim.x) { S.a[k] = R.s[k]; S.b[k] = R.e[k]; }
+            stage_index(S, R.e, R.n, nb, shift, false);
         } else {
             const Span32 L = list_of(D, x);
-            nx_items = L.n;
             for (uint32_t k = tid; k < L.n; k += blockDim.x) S.a[k] = L.p[k];
+            stage_index(S, L.p, L.n, nb, shift, true);
         }
         __syncthreads();
+        // First staged item at or after the bucket of `key` whose value is
+        // above `key` (run ends for R x R, positions + 1 for S x S): a
+        // bisection inside the key's bucket.
+        auto lookup = [&](const uint32_t* vals, uint32_t key, uint32_t plus_one) {
+            const uint32_t k = key >> shift;
+            uint32_t lo = S.idx[k];
+            uint32_t hi = k + 1 < nb ? S.idx[k + 1] : nx_items;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (vals[mid] + plus_one <= key) lo = mid + 1; else hi = mid;
+            }
+            return lo;
+        };
         for (uint32_t v = wave; v < ny; v += nwaves) {
             // Tile coordinates of the pair: (row in I, row in J).
             const uint32_t ti = stage_j ? v : u, tj = stage_j ? u : v;
@@ -475,29 +493,23 @@ __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint3
             if (span_disjoint(D, x, y)) {
                 if (lane == 0) ++skipped;
             } else {
-                // Each lane merges a contiguous chunk of the streamed row
-                // against the staged one: one galloping search per step from
-                // where the previous item of its chunk left off.
+                // Lanes take the streamed row's items 64 apart (coalesced
+                // loads) and look each up in the staged row's bucket index.
                 uint32_t c = 0;
                 if constexpr (CELL == STORM_HIP_CELL_RR) {
                     const RunsV Q = runs_of(D, y);
-                    const uint32_t per = (Q.n + 63) / 64;
-                    const uint32_t k1 = min(Q.n, (lane + 1) * per);
-                    uint32_t q = 0;
-                    for (uint32_t k = min(Q.n, lane * per); k < k1; ++k) {
+                    for (uint32_t k = lane; k < Q.n; k += 64) {
                         const uint32_t rs = Q.s[k], re = Q.e[k];
-                        q = gallop_gt(S.b, q, nx_items, rs);   // first staged run ending after rs
+                        // first staged run ending after rs
+                        const uint32_t q = lookup(S.b, rs, 0u);
                         for (uint32_t p = q; p < nx_items && S.a[p] < re; ++p)
                             c += min(re, S.b[p]) - max(rs, S.a[p]);
                     }
                 } else {
                     const Span32 Q = list_of(D, y);
-                    const uint32_t per = (Q.n + 63) / 64;
-                    const uint32_t k1 = min(Q.n, (lane + 1) * per);
-                    uint32_t q = 0;
-                    for (uint32_t k = min(Q.n, lane * per); k < k1; ++k) {
+                    for (uint32_t k = lane; k < Q.n; k += 64) {
                         const uint32_t e = Q.p[k];
-                        q = gallop_ge(S.a, q, nx_items, e);
+                        const uint32_t q = lookup(S.a, e, 1u);   // first staged position >= e
                         c += (q < nx_items && S.a[q] == e) ? 1u : 0u;
                     }
                 }
