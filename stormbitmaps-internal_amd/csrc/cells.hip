@@ -151,6 +151,7 @@ __device__ uint32_t cell_partial(const DevCorpus& D, const DevModel& M, uint32_t
         uint32_t l = s;
         if (!bm) { bm = bitmap_of(D, s); l = d; }
         const Span32 L = list_of(D, l);
+#pragma unroll 4
         for (uint32_t k = first; k < L.n; k += stride) {
             const uint32_t x = L.p[k];
             c += (uint32_t)(bm[x >> 6] >> (x & 63)) & 1u;
@@ -164,6 +165,7 @@ __device__ uint32_t cell_partial(const DevCorpus& D, const DevModel& M, uint32_t
         const uint64_t* bm = bitmap_of(D, b);
         if constexpr (CELL == STORM_HIP_CELL_BR) {
             const RunsV R = runs_of(D, o);
+#pragma unroll 4
             for (uint32_t k = first; k < R.n; k += stride) c += bitmap_range_count(bm, R.s[k], R.e[k]);
         } else {
             const EwahV E = ewah_of(D, o);
@@ -179,6 +181,7 @@ __device__ uint32_t cell_partial(const DevCorpus& D, const DevModel& M, uint32_t
         }
     } else if constexpr (CELL == STORM_HIP_CELL_SS) {
         const Span32 A = list_of(D, s), B = list_of(D, d);   // |A| <= |B|
+#pragma unroll 4
         for (uint32_t k = first; k < A.n; k += stride) {
             const uint32_t x = A.p[k];
             const uint32_t at = lower_bound(B.p, B.n, x);
@@ -187,6 +190,7 @@ __device__ uint32_t cell_partial(const DevCorpus& D, const DevModel& M, uint32_t
     } else if constexpr (CELL == STORM_HIP_CELL_SR) {
         const Span32 L = list_of(D, s);
         const RunsV R = runs_of(D, d);
+#pragma unroll 4
         for (uint32_t k = first; k < L.n; k += stride) {
             const uint32_t x = L.p[k];
             const uint32_t u = upper_bound(R.s, R.n, x);
@@ -242,6 +246,7 @@ __device__ uint32_t cell_partial(const DevCorpus& D, const DevModel& M, uint32_t
         if (!(2ull * meta(D, d, 0) > D.universe) || !bitmap_of(D, s)) { a = s; b = d; }
         const uint64_t* bm = bitmap_of(D, b);
         const Span32 A = comp_of(D, a);
+#pragma unroll 4
         for (uint32_t k = first; k < A.n; k += stride) {
             const uint32_t x = A.p[k];
             c += (uint32_t)(bm[x >> 6] >> (x & 63)) & 1u;
@@ -428,7 +433,7 @@ constexpr uint32_t kStageBuckets = 4096;
 struct StageSmem {
     uint32_t a[kStageItems];   // run starts / sorted positions
     uint32_t b[kStageItems];   // run ends (R x R)
-    uint32_t idx[kStageBuckets];
+    uint16_t idx[kStageBuckets];   // item indices <= kStageItems fit 16 bits
 };
 
 // Fills S.idx for the staged keys key[0..n) (ascending; run ends, or
@@ -440,7 +445,7 @@ __device__ inline void stage_index(StageSmem& S, const uint32_t* key, uint32_t n
         const uint64_t prev = p ? (uint64_t)key[p - 1] + plus_one : 0;
         const uint64_t lo = p ? (prev + (1ull << shift) - 1) >> shift : 0;
         const uint64_t hi = p < n ? (((uint64_t)key[p] + plus_one - 1) >> shift) + 1 : (uint64_t)nb;
-        for (uint64_t k = lo; k < hi && k < nb; ++k) S.idx[k] = p;
+        for (uint64_t k = lo; k < hi && k < nb; ++k) S.idx[k] = (uint16_t)p;
     }
 }
 
@@ -498,6 +503,7 @@ __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint3
                 uint32_t c = 0;
                 if constexpr (CELL == STORM_HIP_CELL_RR) {
                     const RunsV Q = runs_of(D, y);
+#pragma unroll 4
                     for (uint32_t k = lane; k < Q.n; k += 64) {
                         const uint32_t rs = Q.s[k], re = Q.e[k];
                         // first staged run ending after rs
@@ -507,6 +513,7 @@ __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint3
                     }
                 } else {
                     const Span32 Q = list_of(D, y);
+#pragma unroll 4
                     for (uint32_t k = lane; k < Q.n; k += 64) {
                         const uint32_t e = Q.p[k];
                         const uint32_t q = lookup(S.a, e, 1u);   // first staged position >= e
