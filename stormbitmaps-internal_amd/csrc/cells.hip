@@ -412,6 +412,183 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
     __syncthreads();
 }
 
+// ---- R x R at tile granularity: an interval join in LDS --------------------
+//
+// For a block of up to 64 x 64 pairs of run rows, count[i][j] = sum over run
+// pairs of their overlap. The universe is walked in chunks of kRunBuckets
+// buckets of 2^sh positions. Each "I" run clipped to the chunk sets its row's
+// bit in the 64-bit cover mask of every bucket it covers whole and leaves an
+// entry (row, first, last offset) in the list of a bucket it covers in part.
+// Each "J" run then visits the buckets it touches: every row in a bucket's
+// mask gains the J run's overlap with the bucket, every entry its overlap with
+// the entry. Work is (I runs + J runs) x buckets touched + matches instead of
+// a search per run per pair. The chunk width adapts so that a chunk's I runs
+// fill about half the entry pool; a chunk whose entries overflow the pool is
+// redone at half the width (at width 1 every cover is whole: no entries).
+constexpr uint32_t kRunBuckets = 2048, kRunEntries = 4096, kRunNoEntry = 0xFFFFu;
+constexpr uint32_t kRunMaxShift = 16;   // entry offsets fit 16 bits
+
+struct RunJoinSmem {
+    uint32_t cnt[64][65];
+    unsigned long long cover[kRunBuckets];
+    uint32_t head[kRunBuckets];          // first entry of each bucket's list
+    uint16_t next[kRunEntries];
+    uint32_t span[kRunEntries];          // first | last << 16, offsets in the bucket
+    uint8_t row[kRunEntries];
+    uint32_t start[2][64];               // first run of each row overlapping the chunk
+    uint32_t pre[2][65];               // exclusive prefix of the rows' chunk run counts
+    uint32_t n_entries;
+};
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ inline uint32_t wave_scan_u32(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+// swap: the I side (inserted) is the block's J rows; counts are stored by
+// (I row, J row) and read back transposed.
+__device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, uint32_t ni,
+                             uint32_t ja, uint32_t nj, bool diag, bool swap,
+                             uint64_t& local, uint64_t& pairs, uint64_t& skipped) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t xs0 = swap ? ja : ia, nx = swap ? nj : ni;   // inserted side
+    const uint32_t ys0 = swap ? ia : ja, ny = swap ? ni : nj;   // queried side
+    for (uint32_t k = tid; k < 64 * 65; k += blockDim.x) (&S.cnt[0][0])[k] = 0;
+    for (uint32_t k = tid; k < kRunBuckets; k += blockDim.x) { S.cover[k] = 0; S.head[k] = kRunNoEntry; }
+    if (tid == 0) S.n_entries = 0;
+    // Starting width: about kRunEntries / 2 I runs per chunk at uniform spread.
+    uint64_t total = 0;
+    if (wave == 0) {
+        const uint32_t c = lane < nx ? runs_of(D, D.perm[xs0 + lane]).n : 0u;
+        total = wave_sum_u64(c);
+    }
+    __shared__ uint64_t s_total;
+    if (tid == 0) s_total = total;
+    __syncthreads();
+    const uint64_t U = D.universe;
+    uint32_t sh = 0;
+    {
+        const uint64_t want = s_total ? U * (kRunEntries / 2) / s_total : U;
+        while (sh < kRunMaxShift && ((uint64_t)kRunBuckets << (sh + 1)) <= want) ++sh;
+    }
+    // Threads 0..63 follow the I rows, 64..127 the J rows: st = the row's first
+    // run ending after lo (runs are disjoint and sorted by start, so their ends
+    // are sorted too), en = one past its last run starting before hi.
+    const uint32_t side = (tid >> 6) & 1, r_own = tid & 63;
+    const bool follows = tid < 128 && r_own < (side ? ny : nx);
+    RunsV own{nullptr, nullptr, 0};
+    if (follows) own = runs_of(D, D.perm[(side ? ys0 : xs0) + r_own]);
+    uint32_t st = 0, en = 0;
+    uint64_t lo = 0;
+    while (lo < U) {
+        const uint64_t hi = min(U, lo + ((uint64_t)kRunBuckets << sh));
+        if (tid < 128) {
+            uint32_t c = 0;
+            if (follows) {
+                en = hi >= U ? own.n : st + lower_bound(own.s + st, own.n - st, (uint32_t)hi);
+                S.start[side][r_own] = st;
+                c = en - st;
+            }
+            const uint32_t inc = wave_scan_u32(c);
+            S.pre[side][r_own + 1] = inc;
+            if (r_own == 0) S.pre[side][0] = 0;
+        }
+        __syncthreads();
+        const uint32_t ex = S.pre[0][nx], ey = S.pre[1][ny];
+        const uint32_t w = 1u << sh;
+        // Insert the I runs.
+        for (uint32_t f = tid; f < ex; f += blockDim.x) {
+            const uint32_t r = join_row(S.pre[0], nx, f);
+            const RunsV R = runs_of(D, D.perm[xs0 + r]);
+            const uint32_t k = S.start[0][r] + (f - S.pre[0][r]);
+            const uint32_t rs = (uint32_t)(max<uint64_t>(R.s[k], lo) - lo);
+            const uint32_t re = (uint32_t)(min<uint64_t>(R.e[k], hi) - lo);
+            const uint32_t kb = rs >> sh, ke = (re - 1) >> sh;
+            const unsigned long long bit = 1ull << r;
+            auto part = [&](uint32_t b, uint32_t first, uint32_t last) {
+                const uint32_t x = atomicAdd(&S.n_entries, 1u);
+                if (x >= kRunEntries) return;
+                S.span[x] = first | (last << 16);
+                S.row[x] = (uint8_t)r;
+                S.next[x] = (uint16_t)atomicExch(&S.head[b], x);
+            };
+            if (kb == ke) {
+                if (re - rs == w) atomicOr(&S.cover[kb], bit);
+                else part(kb, rs & (w - 1), (re - 1) & (w - 1));
+            } else {
+                if (rs & (w - 1)) part(kb, rs & (w - 1), w - 1);
+                else atomicOr(&S.cover[kb], bit);
+                for (uint32_t b = kb + 1; b < ke; ++b) atomicOr(&S.cover[b], bit);
+                if (re & (w - 1)) part(ke, 0, (re - 1) & (w - 1));
+                else atomicOr(&S.cover[ke], bit);
+            }
+        }
+        __syncthreads();
+        const bool overflow = S.n_entries > kRunEntries;
+        if (!overflow) {
+            // Query the J runs.
+            for (uint32_t f = tid; f < ey; f += blockDim.x) {
+                const uint32_t r = join_row(S.pre[1], ny, f);
+                const RunsV R = runs_of(D, D.perm[ys0 + r]);
+                const uint32_t k = S.start[1][r] + (f - S.pre[1][r]);
+                const uint32_t rs = (uint32_t)(max<uint64_t>(R.s[k], lo) - lo);
+                const uint32_t re = (uint32_t)(min<uint64_t>(R.e[k], hi) - lo);
+                const uint32_t kb = rs >> sh, ke = (re - 1) >> sh;
+                for (uint32_t b = kb; b <= ke; ++b) {
+                    const uint32_t b0 = b << sh;
+                    const uint32_t qs = max(rs, b0) - b0, qe = min(re, b0 + w) - b0;   // [qs, qe) in the bucket
+                    unsigned long long m = S.cover[b];
+                    while (m) {
+                        const int a = __ffsll(m) - 1;
+                        m &= m - 1;
+                        atomicAdd(&S.cnt[a][r], qe - qs);
+                    }
+                    for (uint32_t x = S.head[b]; x != kRunNoEntry; x = S.next[x]) {
+                        const uint32_t es = S.span[x] & 0xFFFFu, el = S.span[x] >> 16;
+                        const uint32_t o0 = max(qs, es), o1 = min(qe, el + 1);
+                        if (o0 < o1) atomicAdd(&S.cnt[S.row[x]][r], o1 - o0);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < kRunBuckets; k += blockDim.x) { S.cover[k] = 0; S.head[k] = kRunNoEntry; }
+        __syncthreads();
+        if (overflow) {
+            // Redo the chunk at half the width (sh > 0 here: width-1 buckets
+            // take no entries).
+            if (tid == 0) S.n_entries = 0;
+            --sh;
+            __syncthreads();
+            continue;
+        }
+        const uint32_t used = S.n_entries;
+        __syncthreads();
+        if (tid == 0) S.n_entries = 0;
+        // The last run starting before hi may continue past it.
+        if (follows) st = (en > st && (uint64_t)own.e[en - 1] > hi) ? en - 1 : en;
+        lo = hi;
+        if (used < kRunEntries / 8 && sh < kRunMaxShift) ++sh;
+        else if (used > kRunEntries * 3 / 4 && sh > 0) --sh;
+    }
+    for (uint32_t p = tid; p < ni * nj; p += blockDim.x) {
+        const uint32_t a = p / nj, b = p % nj;
+        if (diag && b <= a) continue;
+        const uint32_t i = D.perm[ia + a], j = D.perm[ja + b];
+        const uint32_t v = swap ? S.cnt[b][a] : S.cnt[a][b];
+        if (span_disjoint(D, i, j)) ++skipped; else ++pairs;
+        local += v;
+        emit_pair(D, i, j, v);
+    }
+    __syncthreads();
+}
+
 // ---- R x R and S x S with one row staged in LDS ------------------------------
 //
 // Per-pair search in global memory is a chain of dependent loads (a binary
@@ -554,6 +731,17 @@ __device__ inline bool ss_use_join(const TileMeta& A, const TileMeta& B) {
     return join < per_pair;
 }
 
+// Interval join against staged per-pair search for R x R: the join touches
+// every run of both sides once (plus per-chunk row bookkeeping), the search
+// every run of the shorter row once per pair, at a bisection each.
+__device__ inline bool rr_use_join(const TileMeta& A, const TileMeta& B) {
+    // Per 64 x 64 block, the unit both run in.
+    const double ra = A.runs, rb = B.runs, na = min(A.rows, 64u), nb = min(B.rows, 64u);
+    const double per_pair = na * nb * fmin(ra, rb) * (double)(bit_len(max(A.runs, B.runs) + 2) + 2);
+    const double join = 8.0 * (na * ra + nb * rb) + (fmin(na * ra, nb * rb) / (kRunEntries / 2.0) + 1.0) * 2000.0;
+    return join < per_pair;
+}
+
 }  // namespace
 
 template <int CELL>
@@ -612,11 +800,24 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
             }
         }
         if constexpr (CELL == STORM_HIP_CELL_RR) {
-            __shared__ StageSmem ss;
-            const int side = stage_side<CELL>(A, D.tm[e.bj]);
-            if (!lane_mode && side >= 0) {
+            __shared__ union { RunJoinSmem rj; StageSmem ss; } sm;
+            const TileMeta& B = D.tm[e.bj];
+            if (rr_use_join(A, B)) {
                 joined = true;
-                tile_staged<CELL>(D, ss, i0, ni, j0, nj, diag, side == 1, local, pairs, skipped);
+                // Insert the side with fewer runs: fewer chunks.
+                const bool swap = !diag && (uint64_t)B.runs * B.rows < (uint64_t)A.runs * A.rows;
+                for (uint32_t si = 0; si < ni; si += 64)
+                    for (uint32_t sj = diag ? si : 0; sj < nj; sj += 64)
+                        rr_tile_join(D, sm.rj, i0 + si, min(64u, ni - si), j0 + sj,
+                                     min(64u, nj - sj), diag && si == sj, swap && !(diag && si == sj),
+                                     local, pairs, skipped);
+            } else if (!lane_mode) {
+                const int side = stage_side<CELL>(A, B);
+                if (side >= 0) {
+                    joined = true;
+                    tile_staged<CELL>(D, sm.ss, i0, ni, j0, nj, diag, side == 1, local, pairs,
+                                      skipped);
+                }
             }
         }
         if (joined) {

@@ -135,6 +135,36 @@ def test_graph_replay_matches_eager(ctx, cases):
             ctx.sync()
 
 
+@pytest.mark.parametrize("tile", [64, 256])
+def test_rr_join_and_staged(ctx, tile):
+    """R x R over 320 run rows whose cardinalities span 1..30,000 over a
+    universe of 200,003: tile pairs of similar rows take the interval join,
+    pairs of a tiny-row tile with a large-row tile the staged search; long
+    runs crossing chunk and bucket edges, single-position runs, empty rows,
+    and rows covering the whole universe are all present."""
+    rng = np.random.default_rng(77)
+    u = 200_003
+    rows = []
+    for r in range(320):
+        k = int(np.exp(rng.random() * np.log(30_000)))
+        mean_run = (1, 4, 64, 2_000)[r % 4]
+        rows.append(corpora.runs_row(rng, u, k, mean_run))
+    rows[5] = np.zeros(0, np.uint32)
+    rows[17] = np.arange(u, dtype=np.uint32)
+    rows[29] = np.arange(u - 70_000, u, dtype=np.uint32)
+    offs, pos = corpora.csr(rows)
+    want_sum, want = O.allpairs_lists(offs, pos)
+    print("case rr join tile", tile, flush=True)
+    with ctx.from_lists(offs, pos, u, bitmap_policy=0 | BUILD_ALL) as c:
+        st = c.allpairs(policy="fixed", fixed_cell=CELLS.index("RR"), out="tri", tile=tile)
+        assert st["sum"] == want_sum
+        assert np.array_equal(c.tri(), want)
+        assert set(st["cell_tiles"]) - {"EMPTY"} <= {"RR"}
+        st = c.allpairs(policy="analytic", out="tri", tile=tile)
+        assert st["sum"] == want_sum
+        assert np.array_equal(c.tri(), want)
+
+
 def test_tiny_and_empty_corpora(ctx):
     for rows, u in [([], 64), ([np.array([1], np.uint32)], 64),
                     ([np.zeros(0, np.uint32)] * 5, 100),
