@@ -425,8 +425,10 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
 // a search per run per pair. The chunk width adapts so that a chunk's I runs
 // fill about half the entry pool; a chunk whose entries overflow the pool is
 // redone at half the width (at width 1 every cover is whole: no entries).
-constexpr uint32_t kRunBuckets = 2048, kRunEntries = 4096, kRunNoEntry = 0xFFFFu;
+constexpr uint32_t kRunBuckets = 1024, kRunEntries = 2048, kRunNoEntry = 0xFFFFu;
 constexpr uint32_t kRunMaxShift = 16;   // entry offsets fit 16 bits
+constexpr int kRunFetch = 4;            // runs each thread loads before using them
+constexpr uint32_t kRunLong = 256;      // multi-bucket J runs queued per chunk (more: visited in place)
 
 struct RunJoinSmem {
     uint32_t cnt[64][65];
@@ -435,9 +437,10 @@ struct RunJoinSmem {
     uint16_t next[kRunEntries];
     uint32_t span[kRunEntries];          // first | last << 16, offsets in the bucket
     uint8_t row[kRunEntries];
-    uint32_t start[2][64];               // first run of each row overlapping the chunk
-    uint32_t pre[2][65];               // exclusive prefix of the rows' chunk run counts
-    uint32_t n_entries;
+    uint64_t first[2][64];               // run-array index of each row's first run in the chunk
+    uint32_t pre[2][65];             // exclusive prefix of the rows' chunk run counts
+    uint3 long_run[kRunLong];            // queued J runs over several buckets: row, start, end
+    uint32_t n_entries, n_long;
 };
 
 // Inclusive prefix sum over the 64 lanes of a wave.
@@ -461,7 +464,7 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
     const uint32_t ys0 = swap ? ia : ja, ny = swap ? ni : nj;   // queried side
     for (uint32_t k = tid; k < 64 * 65; k += blockDim.x) (&S.cnt[0][0])[k] = 0;
     for (uint32_t k = tid; k < kRunBuckets; k += blockDim.x) { S.cover[k] = 0; S.head[k] = kRunNoEntry; }
-    if (tid == 0) S.n_entries = 0;
+    if (tid == 0) { S.n_entries = 0; S.n_long = 0; }
     // Starting width: about kRunEntries / 2 I runs per chunk at uniform spread.
     uint64_t total = 0;
     if (wave == 0) {
@@ -483,7 +486,12 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
     const uint32_t side = (tid >> 6) & 1, r_own = tid & 63;
     const bool follows = tid < 128 && r_own < (side ? ny : nx);
     RunsV own{nullptr, nullptr, 0};
-    if (follows) own = runs_of(D, D.perm[(side ? ys0 : xs0) + r_own]);
+    uint64_t own_off = 0;
+    if (follows) {
+        const uint32_t row = D.perm[(side ? ys0 : xs0) + r_own];
+        own = runs_of(D, row);
+        own_off = D.run_off[row];
+    }
     uint32_t st = 0, en = 0;
     uint64_t lo = 0;
     while (lo < U) {
@@ -492,7 +500,7 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
             uint32_t c = 0;
             if (follows) {
                 en = hi >= U ? own.n : st + lower_bound(own.s + st, own.n - st, (uint32_t)hi);
-                S.start[side][r_own] = st;
+                S.first[side][r_own] = own_off + st;
                 c = en - st;
             }
             const uint32_t inc = wave_scan_u32(c);
@@ -502,59 +510,95 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
         __syncthreads();
         const uint32_t ex = S.pre[0][nx], ey = S.pre[1][ny];
         const uint32_t w = 1u << sh;
+        // Each thread fetches kRunFetch runs (clipped to the chunk, relative
+        // to lo) before working on any, so their loads overlap.
+        auto fetch = [&](int sd, uint32_t nr, uint32_t f, uint32_t& r, uint32_t& rs, uint32_t& re) {
+            r = join_row(S.pre[sd], nr, f);
+            const uint64_t g = S.first[sd][r] + (f - S.pre[sd][r]);
+            rs = (uint32_t)(max<uint64_t>(D.run_start[g], lo) - lo);
+            re = (uint32_t)(min<uint64_t>(D.run_end[g], hi) - lo);
+        };
         // Insert the I runs.
-        for (uint32_t f = tid; f < ex; f += blockDim.x) {
-            const uint32_t r = join_row(S.pre[0], nx, f);
-            const RunsV R = runs_of(D, D.perm[xs0 + r]);
-            const uint32_t k = S.start[0][r] + (f - S.pre[0][r]);
-            const uint32_t rs = (uint32_t)(max<uint64_t>(R.s[k], lo) - lo);
-            const uint32_t re = (uint32_t)(min<uint64_t>(R.e[k], hi) - lo);
-            const uint32_t kb = rs >> sh, ke = (re - 1) >> sh;
-            const unsigned long long bit = 1ull << r;
-            auto part = [&](uint32_t b, uint32_t first, uint32_t last) {
-                const uint32_t x = atomicAdd(&S.n_entries, 1u);
-                if (x >= kRunEntries) return;
-                S.span[x] = first | (last << 16);
-                S.row[x] = (uint8_t)r;
-                S.next[x] = (uint16_t)atomicExch(&S.head[b], x);
-            };
-            if (kb == ke) {
-                if (re - rs == w) atomicOr(&S.cover[kb], bit);
-                else part(kb, rs & (w - 1), (re - 1) & (w - 1));
-            } else {
-                if (rs & (w - 1)) part(kb, rs & (w - 1), w - 1);
-                else atomicOr(&S.cover[kb], bit);
-                for (uint32_t b = kb + 1; b < ke; ++b) atomicOr(&S.cover[b], bit);
-                if (re & (w - 1)) part(ke, 0, (re - 1) & (w - 1));
-                else atomicOr(&S.cover[ke], bit);
+        for (uint32_t f0 = tid; f0 < ex; f0 += kRunFetch * blockDim.x) {
+            uint32_t rr[kRunFetch], rss[kRunFetch], rse[kRunFetch];
+#pragma unroll
+            for (int u = 0; u < kRunFetch; ++u)
+                if (f0 + u * blockDim.x < ex) fetch(0, nx, f0 + u * blockDim.x, rr[u], rss[u], rse[u]);
+#pragma unroll
+            for (int u = 0; u < kRunFetch; ++u) {
+                if (f0 + u * blockDim.x >= ex) break;
+                const uint32_t r = rr[u], rs = rss[u], re = rse[u];
+                const uint32_t kb = rs >> sh, ke = (re - 1) >> sh;
+                const unsigned long long bit = 1ull << r;
+                auto part = [&](uint32_t b, uint32_t first, uint32_t last) {
+                    const uint32_t x = atomicAdd(&S.n_entries, 1u);
+                    if (x >= kRunEntries) return;
+                    S.span[x] = first | (last << 16);
+                    S.row[x] = (uint8_t)r;
+                    S.next[x] = (uint16_t)atomicExch(&S.head[b], x);
+                };
+                if (kb == ke) {
+                    if (re - rs == w) atomicOr(&S.cover[kb], bit);
+                    else part(kb, rs & (w - 1), (re - 1) & (w - 1));
+                } else {
+                    if (rs & (w - 1)) part(kb, rs & (w - 1), w - 1);
+                    else atomicOr(&S.cover[kb], bit);
+                    for (uint32_t b = kb + 1; b < ke; ++b) atomicOr(&S.cover[b], bit);
+                    if (re & (w - 1)) part(ke, 0, (re - 1) & (w - 1));
+                    else atomicOr(&S.cover[ke], bit);
+                }
             }
         }
         __syncthreads();
         const bool overflow = S.n_entries > kRunEntries;
         if (!overflow) {
-            // Query the J runs.
-            for (uint32_t f = tid; f < ey; f += blockDim.x) {
-                const uint32_t r = join_row(S.pre[1], ny, f);
-                const RunsV R = runs_of(D, D.perm[ys0 + r]);
-                const uint32_t k = S.start[1][r] + (f - S.pre[1][r]);
-                const uint32_t rs = (uint32_t)(max<uint64_t>(R.s[k], lo) - lo);
-                const uint32_t re = (uint32_t)(min<uint64_t>(R.e[k], hi) - lo);
-                const uint32_t kb = rs >> sh, ke = (re - 1) >> sh;
-                for (uint32_t b = kb; b <= ke; ++b) {
-                    const uint32_t b0 = b << sh;
-                    const uint32_t qs = max(rs, b0) - b0, qe = min(re, b0 + w) - b0;   // [qs, qe) in the bucket
-                    unsigned long long m = S.cover[b];
-                    while (m) {
-                        const int a = __ffsll(m) - 1;
-                        m &= m - 1;
-                        atomicAdd(&S.cnt[a][r], qe - qs);
-                    }
-                    for (uint32_t x = S.head[b]; x != kRunNoEntry; x = S.next[x]) {
-                        const uint32_t es = S.span[x] & 0xFFFFu, el = S.span[x] >> 16;
-                        const uint32_t o0 = max(qs, es), o1 = min(qe, el + 1);
-                        if (o0 < o1) atomicAdd(&S.cnt[S.row[x]][r], o1 - o0);
-                    }
+            // Query the J runs. A run's work in one bucket: every row in the
+            // bucket's cover mask gains the run's overlap with the bucket,
+            // every entry its overlap with the entry.
+            auto visit = [&](uint32_t r, uint32_t rs, uint32_t re, uint32_t b) {
+                const uint32_t b0 = b << sh;
+                const uint32_t qs = max(rs, b0) - b0, qe = min(re, b0 + w) - b0;   // [qs, qe) in the bucket
+                unsigned long long m = S.cover[b];
+                while (m) {
+                    const int a = __ffsll(m) - 1;
+                    m &= m - 1;
+                    atomicAdd(&S.cnt[a][r], qe - qs);
                 }
+                for (uint32_t x = S.head[b]; x != kRunNoEntry; x = S.next[x]) {
+                    const uint32_t es = S.span[x] & 0xFFFFu, el = S.span[x] >> 16;
+                    const uint32_t o0 = max(qs, es), o1 = min(qe, el + 1);
+                    if (o0 < o1) atomicAdd(&S.cnt[S.row[x]][r], o1 - o0);
+                }
+            };
+            // Runs within one bucket are visited here; a run over several
+            // buckets is queued (a lane walking its buckets would hold back
+            // its whole wave) and visited below by a wave, one bucket a lane.
+            for (uint32_t f0 = tid; f0 < ey; f0 += kRunFetch * blockDim.x) {
+                uint32_t rr[kRunFetch], rss[kRunFetch], rse[kRunFetch];
+#pragma unroll
+                for (int u = 0; u < kRunFetch; ++u)
+                    if (f0 + u * blockDim.x < ey) fetch(1, ny, f0 + u * blockDim.x, rr[u], rss[u], rse[u]);
+#pragma unroll
+                for (int u = 0; u < kRunFetch; ++u) {
+                    if (f0 + u * blockDim.x >= ey) break;
+                    const uint32_t r = rr[u], rs = rss[u], re = rse[u];
+                    const uint32_t kb = rs >> sh, ke = (re - 1) >> sh;
+                    if (kb != ke) {
+                        const uint32_t q = atomicAdd(&S.n_long, 1u);
+                        if (q < kRunLong) {
+                            S.long_run[q] = make_uint3(r, rs, re);
+                            continue;
+                        }
+                    }
+                    for (uint32_t b = kb; b <= ke; ++b) visit(r, rs, re, b);
+                }
+            }
+            __syncthreads();
+            const uint32_t n_long = min(S.n_long, kRunLong);
+            for (uint32_t q = wave; q < n_long; q += blockDim.x >> 6) {
+                const uint3 L = S.long_run[q];
+                const uint32_t kb = L.y >> sh, ke = (L.z - 1) >> sh;
+                for (uint32_t b = kb + lane; b <= ke; b += 64) visit(L.x, L.y, L.z, b);
             }
         }
         __syncthreads();
@@ -570,7 +614,7 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
         }
         const uint32_t used = S.n_entries;
         __syncthreads();
-        if (tid == 0) S.n_entries = 0;
+        if (tid == 0) { S.n_entries = 0; S.n_long = 0; }
         // The last run starting before hi may continue past it.
         if (follows) st = (en > st && (uint64_t)own.e[en - 1] > hi) ? en - 1 : en;
         lo = hi;
