@@ -423,7 +423,7 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
 // mask gains the J run's overlap with the bucket, every entry its overlap with
 // the entry. Work is (I runs + J runs) x buckets touched + matches instead of
 // a search per run per pair. The chunk width adapts so that a chunk's I runs
-// fill about half the entry pool; a chunk whose entries overflow the pool is
+// fill about three quarters of the entry pool; a chunk whose entries overflow the pool is
 // redone at half the width (at width 1 every cover is whole: no entries).
 constexpr uint32_t kRunBuckets = 1024, kRunEntries = 2048, kRunNoEntry = 0xFFFFu;
 constexpr uint32_t kRunMaxShift = 16;   // entry offsets fit 16 bits
@@ -465,7 +465,7 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
     for (uint32_t k = tid; k < 64 * 65; k += blockDim.x) (&S.cnt[0][0])[k] = 0;
     for (uint32_t k = tid; k < kRunBuckets; k += blockDim.x) { S.cover[k] = 0; S.head[k] = kRunNoEntry; }
     if (tid == 0) { S.n_entries = 0; S.n_long = 0; }
-    // Starting width: about kRunEntries / 2 I runs per chunk at uniform spread.
+    // Starting width: about 3/4 kRunEntries I runs per chunk at uniform spread.
     uint64_t total = 0;
     if (wave == 0) {
         const uint32_t c = lane < nx ? runs_of(D, D.perm[xs0 + lane]).n : 0u;
@@ -477,7 +477,7 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
     const uint64_t U = D.universe;
     uint32_t sh = 0;
     {
-        const uint64_t want = s_total ? U * (kRunEntries / 2) / s_total : U;
+        const uint64_t want = s_total ? U * (kRunEntries * 3 / 4) / s_total : U;
         while (sh < kRunMaxShift && ((uint64_t)kRunBuckets << (sh + 1)) <= want) ++sh;
     }
     // Threads 0..63 follow the I rows, 64..127 the J rows: st = the row's first
@@ -619,7 +619,7 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
         if (follows) st = (en > st && (uint64_t)own.e[en - 1] > hi) ? en - 1 : en;
         lo = hi;
         if (used < kRunEntries / 8 && sh < kRunMaxShift) ++sh;
-        else if (used > kRunEntries * 3 / 4 && sh > 0) --sh;
+        else if (used > kRunEntries * 15 / 16 && sh > 0) --sh;
     }
     for (uint32_t p = tid; p < ni * nj; p += blockDim.x) {
         const uint32_t a = p / nj, b = p % nj;
