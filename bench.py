@@ -146,6 +146,9 @@ def sparse_bench(sh, args):
     bitmap_bytes = n * ((u + 63) // 64) * 8
     all_bitmaps = bitmap_bytes <= args.bitmap_budget_gib << 30
     flags = (1 if all_bitmaps else 0) | 4 | 8 | 16
+    # Bit-sliced copies of the bitmap groups (as many bytes again) when they fit too.
+    if not all_bitmaps or 2 * bitmap_bytes <= args.bitmap_budget_gib << 30:
+        flags |= 64
     t0 = time.perf_counter()
     corpus = ctx.from_lists(offs, pos, u, bitmap_policy=flags)
     t_up = time.perf_counter() - t0

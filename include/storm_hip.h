@@ -112,12 +112,17 @@ int  storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* arena,
  * STORM_HIP_BUILD_* representation bits. The lists are uploaded and the
  * representations built on the device, one wavefront per row; with
  * STORM_HIP_BUILD_ON_HOST they are built by host threads and uploaded (same
- * arenas byte for byte). */
+ * arenas byte for byte). STORM_HIP_BUILD_SLICES adds a bit-sliced copy of
+ * the bitmaps of every 64 density-ordered rows that all hold one (one 64-bit
+ * row mask per position; as much device memory as those bitmaps), which lets
+ * B x S count a 64-row block against a list with one load per list element;
+ * it has no counterpart in the reference's representations. */
 enum {
     STORM_HIP_BITMAP_DENSE = 0, STORM_HIP_BITMAP_ALL = 1, STORM_HIP_BITMAP_NONE = 2,
     STORM_HIP_BUILD_RUNS = 4, STORM_HIP_BUILD_EWAH = 8, STORM_HIP_BUILD_COMP = 16,
     STORM_HIP_BUILD_ALL = 4 | 8 | 16,
-    STORM_HIP_BUILD_ON_HOST = 32
+    STORM_HIP_BUILD_ON_HOST = 32,
+    STORM_HIP_BUILD_SLICES = 64
 };
 /* Seeded synthetic corpora as host CSR lists (storm::generate's role,
  * kernels/storm_gen.cpp:173-232, with the spectra the benchmark workloads

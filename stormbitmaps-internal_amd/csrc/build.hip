@@ -250,6 +250,69 @@ int upload_bitmaps(storm_hip_corpus* c, hipStream_t s) {
     return STORM_HIP_OK;
 }
 
+// Bit-sliced bitmaps: one wave per (group, 8 words). Lane k holds 8 words of
+// sorted row 64g + k; for each of the 512 positions a ballot gathers the
+// 64 rows' bits, and lane t keeps the mask of position 64w + t, so each word's
+// 64 masks leave as one coalesced 512-byte store.
+__global__ __launch_bounds__(256) void slice_bits_kernel(const uint32_t* __restrict__ perm,
+                                                         const uint32_t* __restrict__ bm_row,
+                                                         const uint64_t* __restrict__ bitmap,
+                                                         const uint32_t* __restrict__ slice_of,
+                                                         uint64_t* __restrict__ slice, uint32_t ld,
+                                                         uint32_t groups) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t w0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 8;
+    if (w0 >= ld) return;
+    for (uint32_t g = blockIdx.y; g < groups; g += gridDim.y) {
+        const uint32_t sl = slice_of[g];
+        if (sl == kNoBitmap) continue;
+        // Padding positions of the density order map to the zero row.
+        const uint64_t* src = bitmap + (uint64_t)bm_row[perm[64 * g + lane]] * ld + w0;
+        uint64_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = src[k];
+        uint64_t* dst = slice + (uint64_t)sl * ld * 64 + (uint64_t)w0 * 64;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            uint64_t mine = 0;
+            for (uint32_t t = 0; t < 64; ++t) {
+                const uint64_t m = __ballot((v[k] >> t) & 1);
+                if (lane == t) mine = m;
+            }
+            dst[k * 64 + lane] = mine;
+        }
+    }
+}
+
+// (Re)builds the slices of every 64-row group of the density order whose rows
+// all hold bitmaps (rows past n are the zero row).
+int build_slices(storm_hip_corpus* c, hipStream_t s) {
+    if (c->d_slice) { SH_HIP(hipFree(c->d_slice)); c->d_slice = nullptr; }
+    if (c->d_slice_of) { SH_HIP(hipFree(c->d_slice_of)); c->d_slice_of = nullptr; }
+    c->device_bytes -= (uint64_t)c->n_slices * c->ld * 512;
+    c->n_slices = 0;
+    if (!c->want_slices || c->n_rows == 0) return STORM_HIP_OK;
+    const uint32_t groups = (c->n_rows + 63) / 64;
+    std::vector<uint32_t> of(groups, kNoBitmap);
+    uint32_t ns = 0;
+    for (uint32_t g = 0; g < groups; ++g) {
+        bool all = true;
+        for (uint32_t k = 64 * g; k < std::min(c->n_rows, 64 * g + 64) && all; ++k)
+            all = c->h_bm_row[c->h_perm[k]] != kNoBitmap;
+        if (all) of[g] = ns++;
+    }
+    SH_TRY(upload_vec(c->d_slice_of, of, s, c->device_bytes));
+    if (!ns) return STORM_HIP_OK;
+    SH_HIP(hipMalloc(&c->d_slice, (size_t)ns * c->ld * 512));
+    c->n_slices = ns;
+    c->device_bytes += (uint64_t)ns * c->ld * 512;
+    const dim3 grid((c->ld / 8 + 3) / 4, std::min<uint32_t>(groups, 65535));
+    hipLaunchKernelGGL(slice_bits_kernel, grid, dim3(256), 0, s, c->d_perm, c->d_bm_row,
+                       c->d_bitmap, c->d_slice_of, c->d_slice, c->ld, groups);
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
 int upload_host_arena(storm_hip_ctx* ctx, const HostArena& A, uint32_t flags,
                       storm_hip_corpus** out) {
     storm_hip_corpus* c = corpus_new(ctx, A.n, A.universe, flags);
@@ -288,6 +351,7 @@ storm_hip_corpus* corpus_new(storm_hip_ctx* ctx, uint32_t n, uint32_t universe, 
     c->has_runs = flags & STORM_HIP_BUILD_RUNS;
     c->has_ewah = flags & STORM_HIP_BUILD_EWAH;
     c->has_comp = flags & STORM_HIP_BUILD_COMP;
+    c->want_slices = flags & STORM_HIP_BUILD_SLICES;
     return c;
 }
 
@@ -331,6 +395,7 @@ int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uin
     SH_TRY(upload_bitmaps(c, s));
     SH_HIP(hipMemcpyAsync(c->d_bm_row + n, &c->h_bm_row[n], 4, hipMemcpyHostToDevice, s));
     B += ((size_t)nb + 1) * c->ld * 8;
+    SH_TRY(build_slices(c, s));
     SH_HIP(hipStreamSynchronize(s));
     return STORM_HIP_OK;
 }
@@ -346,6 +411,7 @@ int general_materialize(storm_hip_ctx* ctx, storm_hip_corpus* c) {
                           hipMemcpyHostToDevice, ctx->stream));
     c->device_bytes += ((size_t)c->n_bitmap_rows + 1) * c->ld * 8;
     c->bitmaps_complete = true;
+    SH_TRY(build_slices(c, ctx->stream));
     SH_HIP(hipStreamSynchronize(ctx->stream));
     return STORM_HIP_OK;
 }
@@ -354,7 +420,8 @@ void general_free_arenas(storm_hip_corpus* c) {
     void* ptrs[] = {c->d_list_off, c->d_list, c->d_run_off, c->d_run_start, c->d_run_end,
                     c->d_ewah_off, c->d_ewah, c->d_mk_off, c->d_mk_word, c->d_mk_pos,
                     c->d_comp_off, c->d_comp, c->d_meta, c->d_bm_row, c->d_bitmap,
-                    c->d_perm, c->d_ident, c->d_bitmap_off, c->d_occ_off, c->d_occ};
+                    c->d_perm, c->d_ident, c->d_bitmap_off, c->d_occ_off, c->d_occ,
+                    c->d_slice, c->d_slice_of};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 

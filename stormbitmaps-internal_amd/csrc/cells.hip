@@ -412,6 +412,67 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
     __syncthreads();
 }
 
+// ---- B x S at tile granularity through bit-sliced bitmaps ------------------
+//
+// A 64-row group of bitmap rows has a slab holding, per position, the mask of
+// the group's rows that contain it. For a block of up to 64 x 64 pairs whose
+// one side ("slab side") is such a group, each row of the other side ("probe
+// side") streams its list once: a wave takes the row, its lanes load 64
+// elements' masks, and the count of each slab row is the popcount of one
+// ballot per row present in any lane's mask. One load per list element serves
+// 64 pairs (the per-pair cell probes the bitmap once per element per pair).
+struct SliceSmem {
+    uint32_t cnt[64][65];   // [slab row][probe row]
+};
+
+__device__ inline uint64_t wave_or_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, off, 64);
+        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
+        v |= ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+// Slab of the 64-row group starting at sorted position p0 (a multiple of 64),
+// or null.
+__device__ inline const uint64_t* slab_of(const DevCorpus& D, uint32_t p0) {
+    if (!D.slice) return nullptr;
+    const uint32_t s = D.slice_of[p0 >> 6];
+    return s == kNoBitmap ? nullptr : D.slice + (uint64_t)s * D.ld * 64;
+}
+
+constexpr int kSliceFetch = 4;   // 64-element batches a wave loads before counting
+
+__device__ void bs_slice_block(const DevCorpus& D, SliceSmem& S, const uint64_t* __restrict__ slab,
+                               uint32_t ps0, uint32_t np) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    for (uint32_t r = wave; r < np; r += nwaves) {
+        const Span32 L = list_of(D, D.perm[ps0 + r]);
+        uint32_t acc = 0;   // lane a: elements of this row held by slab row a
+        for (uint32_t k0 = 0; k0 < L.n; k0 += 64 * kSliceFetch) {
+            uint64_t m[kSliceFetch];
+#pragma unroll
+            for (int u = 0; u < kSliceFetch; ++u) {
+                const uint32_t k = k0 + u * 64 + lane;
+                m[u] = k < L.n ? slab[L.p[k]] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < kSliceFetch; ++u) {
+                unsigned long long any = wave_or_u64(m[u]);
+                while (any) {
+                    const uint32_t a = (uint32_t)__ffsll(any) - 1;
+                    any &= any - 1;
+                    const uint32_t c = (uint32_t)__popcll(__ballot((m[u] >> a) & 1ull));
+                    if (lane == a) acc += c;
+                }
+            }
+        }
+        S.cnt[lane][r] = acc;
+    }
+}
+
 // ---- R x R at tile granularity: an interval join in LDS --------------------
 //
 // For a block of up to 64 x 64 pairs of run rows, count[i][j] = sum over run
@@ -824,6 +885,39 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         const TileMeta& A = D.tm[e.bi];
         const bool lane_mode = CELL == STORM_HIP_CELL_EMPTY || (A.max_card <= 48 && A.max_runs <= 48);
         bool joined = false;
+        if constexpr (CELL == STORM_HIP_CELL_BS) {
+            __shared__ SliceSmem sl;
+            const TileMeta& B = D.tm[e.bj];
+            bool a_ok = D.slice != nullptr, b_ok = D.slice != nullptr;
+            for (uint32_t si = 0; si < ni && a_ok; si += 64) a_ok = slab_of(D, i0 + si) != nullptr;
+            for (uint32_t sj = 0; sj < nj && b_ok; sj += 64) b_ok = slab_of(D, j0 + sj) != nullptr;
+            // Probe the side with fewer elements into the other's slab; with
+            // only the denser side probing, per-pair probes of the sparser
+            // side can be cheaper (64 pairs per slab load against 1).
+            const bool probe_b = a_ok && (!b_ok || B.card <= A.card);
+            const uint32_t probe_card = probe_b ? B.card : A.card;
+            if ((a_ok || b_ok) && (double)probe_card <= 64.0 * min(A.card, B.card) + 64.0) {
+                joined = true;
+                for (uint32_t si = 0; si < ni; si += 64)
+                    for (uint32_t sj = diag ? si : 0; sj < nj; sj += 64) {
+                        const uint32_t bi_n = min(64u, ni - si), bj_n = min(64u, nj - sj);
+                        const bool bdiag = diag && si == sj;
+                        if (probe_b) bs_slice_block(D, sl, slab_of(D, i0 + si), j0 + sj, bj_n);
+                        else bs_slice_block(D, sl, slab_of(D, j0 + sj), i0 + si, bi_n);
+                        __syncthreads();
+                        for (uint32_t p = threadIdx.x; p < bi_n * bj_n; p += blockDim.x) {
+                            const uint32_t a = p / bj_n, b = p % bj_n;
+                            if (bdiag && b <= a) continue;
+                            const uint32_t i = D.perm[i0 + si + a], j = D.perm[j0 + sj + b];
+                            const uint32_t v = probe_b ? sl.cnt[a][b] : sl.cnt[b][a];
+                            if (span_disjoint(D, i, j)) ++skipped; else ++pairs;
+                            local += v;
+                            emit_pair(D, i, j, v);
+                        }
+                        __syncthreads();
+                    }
+            }
+        }
         if constexpr (CELL == STORM_HIP_CELL_SS) {
             // The join and the staged search never run together: one LDS area.
             __shared__ union { JoinSmem js; StageSmem ss; } sm;

@@ -30,6 +30,10 @@ struct DevCorpus {
     const uint32_t* bm_row;   const uint64_t* bitmap;   // bitmap row index, kNoBitmap if none
     const uint32_t* meta;     // 8 per original row (RowMeta + has_rank, has_occ)
     const uint32_t* perm;     // sorted position -> original row id
+    // Bit-sliced bitmaps of 64-row groups of the density order: group g's slab
+    // is slice + slice_of[g] * ld * 64 (kNoBitmap: none); word p has bit k set
+    // when sorted row 64g + k holds position p.
+    const uint64_t* slice;    const uint32_t* slice_of;
     // Per-call state of this corpus inside a (possibly multi-corpus) call.
     const struct TileMeta* tm;        // its row tiles' metadata
     const uint32_t* rowmap;           // sorted position -> bitmap row

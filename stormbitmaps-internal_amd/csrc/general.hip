@@ -139,6 +139,7 @@ DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
     D.bm_row = c->d_bm_row; D.bitmap = c->d_bitmap;
     D.meta = c->d_meta;
     D.perm = c->d_perm;
+    D.slice = c->d_slice; D.slice_of = c->d_slice_of;
     D.tm = w->tm; D.rowmap = w->rowmap; D.tile = w->tile;
     D.tri = tri; D.sum = sum; D.finish = finish;
     return D;

@@ -79,6 +79,14 @@ struct storm_hip_corpus {
     uint32_t universe = 0;     // bits
     bool dense = false;        // contiguous corpus: every row a bitmap
     bool has_runs = false, has_ewah = false, has_comp = false;
+    // Bit-sliced bitmaps (STORM_HIP_BUILD_SLICES): for each group of 64
+    // density-ordered rows that all hold bitmaps, a slab of ld * 64 masks,
+    // slab[p] bit k = row 64g + k holds position p. d_slice_of[g] = slab
+    // index or kNoBitmap.
+    bool want_slices = false;
+    uint64_t* d_slice = nullptr;
+    uint32_t* d_slice_of = nullptr;
+    uint32_t n_slices = 0;
     // Dense layout: n_rows_pad x ld words.
     uint64_t* d_words = nullptr;
     uint32_t ld = 0;

@@ -19,7 +19,7 @@ BB = {"auto": 0, "valu": 1, "mfma": 2, "mfma_fp4": 3}
 CELLS = ["BB", "BS", "BR", "BW", "SS", "SR", "SW", "RR", "RW", "WW", "EMPTY", "CB"]
 NCELL = len(CELLS)
 # Representation flags of from_lists (storm_hip.h STORM_HIP_BUILD_*).
-BUILD_RUNS, BUILD_EWAH, BUILD_COMP, BUILD_ON_HOST = 4, 8, 16, 32
+BUILD_RUNS, BUILD_EWAH, BUILD_COMP, BUILD_ON_HOST, BUILD_SLICES = 4, 8, 16, 32, 64
 # Arena ids of storm_hip_corpus_download_arena, in order, with element types.
 ARENAS = [("list_off", np.uint64), ("list", np.uint32), ("run_off", np.uint64),
           ("run_start", np.uint32), ("run_end", np.uint32), ("ewah_off", np.uint64),

@@ -238,3 +238,27 @@ def test_autotune_general_keeps_counts(ctx, cases):
         assert st["sum"] == want_sum
         assert np.array_equal(c.tri(), want)
     ctx.set_bb_config(8, 4, 2, 8)
+
+
+@pytest.mark.parametrize("tile", [64, 256])
+def test_bs_slices(ctx, tile):
+    """B x S through bit-sliced bitmap groups: 300 rows whose cardinalities span
+    1..40,000 over a universe of 100,003 (five 64-row groups, the last one
+    partial), every row a bitmap or only the dense ones (then only the groups
+    of dense rows have slices and the rest runs per pair); exact against the
+    oracle under the forced cell and under the selecting policies."""
+    rng = np.random.default_rng(91)
+    u = 100_003
+    rows = [corpora.uniform_row(rng, u, int(np.exp(rng.random() * np.log(40_000))))
+            for _ in range(300)]
+    rows[7] = np.zeros(0, np.uint32)
+    rows[100] = np.arange(u, dtype=np.uint32)
+    offs, pos = corpora.csr(rows)
+    want_sum, want = O.allpairs_lists(offs, pos)
+    for bitmaps in (1, 0):
+        print("case bs slices tile", tile, "bitmaps", bitmaps, flush=True)
+        with ctx.from_lists(offs, pos, u, bitmap_policy=bitmaps | BUILD_ALL | 64) as c:
+            for pol in ("fixed", "analytic"):
+                st = c.allpairs(policy=pol, fixed_cell=CELLS.index("BS"), out="tri", tile=tile)
+                assert st["sum"] == want_sum, (bitmaps, pol)
+                assert np.array_equal(c.tri(), want), (bitmaps, pol)
