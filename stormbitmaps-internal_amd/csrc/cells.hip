@@ -888,7 +888,9 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         if constexpr (CELL == STORM_HIP_CELL_BS) {
             __shared__ SliceSmem sl;
             const TileMeta& B = D.tm[e.bj];
-            bool a_ok = D.slice != nullptr, b_ok = D.slice != nullptr;
+            // Slabs cover 64-aligned groups of the density order.
+            bool a_ok = D.slice != nullptr && (i0 & 63) == 0;
+            bool b_ok = D.slice != nullptr && (j0 & 63) == 0;
             for (uint32_t si = 0; si < ni && a_ok; si += 64) a_ok = slab_of(D, i0 + si) != nullptr;
             for (uint32_t sj = 0; sj < nj && b_ok; sj += 64) b_ok = slab_of(D, j0 + sj) != nullptr;
             // Probe the side with fewer elements into the other's slab; with

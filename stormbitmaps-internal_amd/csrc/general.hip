@@ -657,9 +657,11 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
     const uint32_t reps = reps_of(c);
     const DevModel M = dev_model(ctx->model);
     // Cells other than B x B are timed on a kSampleRows x kSampleRows block
-    // of median-density rows of each tile (their cost is per pair) and the
-    // time is scaled by the tile's pair count over the block's.
-    constexpr uint32_t kSampleRows = 32;
+    // of median-density rows of each tile and the time is scaled by the tile's
+    // pair count over the block's. The block is the unit the block joins and
+    // the slice path work in (64 x 64, aligned), so their per-block costs are
+    // sampled as they run.
+    constexpr uint32_t kSampleRows = 64;
     for (uint32_t k = 0; k < S; ++k) {
         uint32_t bi, bj;
         tri_tile(cand[k], w->T, bi, bj);
