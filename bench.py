@@ -231,6 +231,8 @@ def sparse_bench(sh, args):
         "sum": sums.pop() if len(sums) == 1 else None, "column_closed_form_sum": want,
         "policies": res,
         "learned_over_analytic": res["learned"]["pairs_per_s"] / res["analytic"]["pairs_per_s"],
+        "learned_over_analytic_reference_rates":
+            res["learned"]["pairs_per_s"] / res["analytic_reference_rates"]["pairs_per_s"],
         "selector": {"sampled_tiles": int(len(X)), "train_loss": loss, "sample_s": t_sample,
                      "train_s": t_train},
         "calibrated_model": model, "calibrate_s": t_cal,
@@ -405,6 +407,8 @@ def batch_bench(sh, args):
         "exact_all": all(v["exact"] for v in res.values()),
         "policies": res,
         "learned_over_analytic": res["learned"]["pairs_per_s"] / res["analytic"]["pairs_per_s"],
+        "learned_over_analytic_reference_rates":
+            res["learned"]["pairs_per_s"] / res["analytic_reference_rates"]["pairs_per_s"],
         "sequential_learned_ms": seq_ms,
         "batched_over_sequential": seq_ms / res["learned"]["ms"],
         "selector": {"sampled_tiles": int(len(X)), "train_loss": loss},

@@ -412,19 +412,19 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
     __syncthreads();
 }
 
-// ---- B x S at tile granularity through bit-sliced bitmaps ------------------
+// ---- B x S through bit-sliced bitmaps --------------------------------------
 //
-// A 64-row group of bitmap rows has a slab holding, per position, the mask of
-// the group's rows that contain it. For a block of up to 64 x 64 pairs whose
-// one side ("slab side") is such a group, each row of the other side ("probe
-// side") streams its list once: a wave takes the row, its lanes load 64
-// elements' masks, and the count of each slab row is the popcount of one
-// ballot per row present in any lane's mask. One load per list element serves
-// 64 pairs (the per-pair cell probes the bitmap once per element per pair).
-struct SliceSmem {
-    uint32_t cnt[64][65];   // [slab row][probe row]
-};
-
+// A 64-row group of bitmap rows has a slice: per position, the mask of the
+// group's rows that contain it; four consecutive groups keep their masks of a
+// position side by side (32 bytes). A B x S item is a stripe: up to 64 rows of
+// the probed side (lists) against up to 256 rows of the other side, whose
+// groups all have slices. A wave takes one probed row and streams its list
+// once; its lanes load 64 elements' masks (one 32-byte load for four groups),
+// and the count of each slab row is the popcount of one ballot per row present
+// in any lane's mask, kept by the lane of that row. One load per list element
+// serves up to 256 pairs (the per-pair cell probes the bitmap once per element
+// per pair), and each lane ends holding finished pair counts: no LDS, no
+// barriers.
 __device__ inline uint64_t wave_or_u64(uint64_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -435,41 +435,73 @@ __device__ inline uint64_t wave_or_u64(uint64_t v) {
     return v;
 }
 
-// Slab of the 64-row group starting at sorted position p0 (a multiple of 64),
-// or null.
-__device__ inline const uint64_t* slab_of(const DevCorpus& D, uint32_t p0) {
-    if (!D.slice) return nullptr;
-    const uint32_t s = D.slice_of[p0 >> 6];
-    return s == kNoBitmap ? nullptr : D.slice + (uint64_t)s * D.ld * 64;
-}
-
 constexpr int kSliceFetch = 4;   // 64-element batches a wave loads before counting
 
-__device__ void bs_slice_block(const DevCorpus& D, SliceSmem& S, const uint64_t* __restrict__ slab,
-                               uint32_t ps0, uint32_t np) {
+// Probe rows at sorted positions [p0, p0 + np), slab rows [s0, s0 + ns) (s0 a
+// multiple of 64, G = ceil(ns / 64) groups, all with slices). Pairs are (A
+// position, B position) with the probe rows on side A when probe_a; on a
+// diagonal tile (same_tile) only pairs whose B position is the larger count.
+template <int G>
+__device__ void bs_stripe(const DevCorpus& D, uint32_t p0, uint32_t np, uint32_t s0, uint32_t ns,
+                          bool probe_a, bool same_tile, uint64_t& local, uint64_t& pairs,
+                          uint64_t& skipped) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const uint32_t g0 = s0 >> 6;
+    const uint64_t* base[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const uint32_t sl = D.slice_of[g0 + g];
+        base[g] = D.slice + (uint64_t)(sl >> 2) * D.ld * 256 + (sl & 3);
+    }
+    const bool packed = G == 4 && (g0 & 3) == 0;   // the four masks of a position at once
     for (uint32_t r = wave; r < np; r += nwaves) {
-        const Span32 L = list_of(D, D.perm[ps0 + r]);
-        uint32_t acc = 0;   // lane a: elements of this row held by slab row a
+        const Span32 L = list_of(D, D.perm[p0 + r]);
+        uint32_t acc[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g] = 0;
         for (uint32_t k0 = 0; k0 < L.n; k0 += 64 * kSliceFetch) {
-            uint64_t m[kSliceFetch];
+            uint64_t m[kSliceFetch][G];
 #pragma unroll
             for (int u = 0; u < kSliceFetch; ++u) {
                 const uint32_t k = k0 + u * 64 + lane;
-                m[u] = k < L.n ? slab[L.p[k]] : 0ull;
-            }
 #pragma unroll
-            for (int u = 0; u < kSliceFetch; ++u) {
-                unsigned long long any = wave_or_u64(m[u]);
-                while (any) {
-                    const uint32_t a = (uint32_t)__ffsll(any) - 1;
-                    any &= any - 1;
-                    const uint32_t c = (uint32_t)__popcll(__ballot((m[u] >> a) & 1ull));
-                    if (lane == a) acc += c;
+                for (int g = 0; g < G; ++g) m[u][g] = 0;
+                if (k < L.n) {
+                    const uint64_t x = (uint64_t)L.p[k] * 4;
+                    if (packed) {
+                        const ulonglong2* q = reinterpret_cast<const ulonglong2*>(base[0] + x);
+                        const ulonglong2 lo = q[0], hi = q[1];
+                        m[u][0] = lo.x; m[u][1 % G] = lo.y; m[u][2 % G] = hi.x; m[u][3 % G] = hi.y;
+                    } else {
+#pragma unroll
+                        for (int g = 0; g < G; ++g) m[u][g] = base[g][x];
+                    }
                 }
             }
+#pragma unroll
+            for (int u = 0; u < kSliceFetch; ++u)
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    unsigned long long any = wave_or_u64(m[u][g]);
+                    while (any) {
+                        const uint32_t a = (uint32_t)__ffsll(any) - 1;
+                        any &= any - 1;
+                        const uint32_t c = (uint32_t)__popcll(__ballot((m[u][g] >> a) & 1ull));
+                        if (lane == a) acc[g] += c;
+                    }
+                }
         }
-        S.cnt[lane][r] = acc;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const uint32_t sa = 64 * g + lane;
+            if (sa >= ns) continue;
+            const uint32_t pa = probe_a ? p0 + r : s0 + sa, pb = probe_a ? s0 + sa : p0 + r;
+            if (same_tile && pb <= pa) continue;
+            const uint32_t i = D.perm[pa], j = D.perm[pb];
+            if (span_disjoint(D, i, j)) ++skipped; else ++pairs;
+            local += acc[g];
+            emit_pair(D, i, j, acc[g]);
+        }
     }
 }
 
@@ -871,53 +903,54 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         uint32_t i0 = e.bi * tile, j0 = e.bj * tile;
         uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
         if (e.sub) {
-            // One block of the tile (empty when it lies past the last row).
+            // One block of the tile (empty when it lies past the last row), or
+            // a stripe: one side whole.
             const uint32_t sb = e.sub & 0xFFFFu;
-            const uint32_t oi = ((e.sub >> 16) & 0xFFu) * sb, oj = (e.sub >> 24) * sb;
-            ni = oi < ni ? min(ni - oi, sb) : 0u;
-            nj = oj < nj ? min(nj - oj, sb) : 0u;
-            i0 += oi; j0 += oj;
+            const uint32_t xi = (e.sub >> 16) & 0xFFu, xj = e.sub >> 24;
+            if (xi != kWholeSide) {
+                const uint32_t oi = xi * sb;
+                ni = oi < ni ? min(ni - oi, sb) : 0u;
+                i0 += oi;
+            }
+            if (xj != kWholeSide) {
+                const uint32_t oj = xj * sb;
+                nj = oj < nj ? min(nj - oj, sb) : 0u;
+                j0 += oj;
+            }
         }
-        // Pairs i < j only where the block straddles the diagonal.
+        // Pairs i < j only where the block straddles the diagonal (a stripe of
+        // a diagonal tile keeps the pairs whose column position is larger).
         const bool diag = i0 == j0;
+        const bool same_tile = e.bi == e.bj;
         const uint32_t np = ni * nj;
         // Lane mode when the sparser tile's rows are tiny.
         const TileMeta& A = D.tm[e.bi];
         const bool lane_mode = CELL == STORM_HIP_CELL_EMPTY || (A.max_card <= 48 && A.max_runs <= 48);
         bool joined = false;
         if constexpr (CELL == STORM_HIP_CELL_BS) {
-            __shared__ SliceSmem sl;
             const TileMeta& B = D.tm[e.bj];
-            // Slabs cover 64-aligned groups of the density order.
-            bool a_ok = D.slice != nullptr && (i0 & 63) == 0;
-            bool b_ok = D.slice != nullptr && (j0 & 63) == 0;
-            for (uint32_t si = 0; si < ni && a_ok; si += 64) a_ok = slab_of(D, i0 + si) != nullptr;
-            for (uint32_t sj = 0; sj < nj && b_ok; sj += 64) b_ok = slab_of(D, j0 + sj) != nullptr;
-            // Probe the side with fewer elements into the other's slab; with
-            // only the denser side probing, per-pair probes of the sparser
-            // side can be cheaper (64 pairs per slab load against 1).
-            const bool probe_b = a_ok && (!b_ok || B.card <= A.card);
-            const uint32_t probe_card = probe_b ? B.card : A.card;
-            if ((a_ok || b_ok) && (double)probe_card <= 64.0 * min(A.card, B.card) + 64.0) {
+            // A stripe names its probed side; a whole tile or block probes the
+            // side bs_probe_a picks.
+            bool probe_a = bs_probe_a(A, B);
+            if (e.sub && (e.sub >> 24) == kWholeSide) probe_a = true;
+            else if (e.sub && ((e.sub >> 16) & 0xFFu) == kWholeSide) probe_a = false;
+            const uint32_t s0 = probe_a ? j0 : i0, ns = probe_a ? nj : ni;
+            const uint32_t p0 = probe_a ? i0 : j0, np_ = probe_a ? ni : nj;
+            bool ok = D.slice != nullptr && (s0 & 63) == 0 && ns > 0 && ns <= 256 && np_ > 0;
+            for (uint32_t g = 0; ok && g < (ns + 63) / 64; ++g)
+                ok = D.slice_of[(s0 >> 6) + g] != kNoBitmap;
+            // Probing the denser side costs its elements once per stripe; the
+            // per-pair cell probes the sparser side's once per pair.
+            const uint32_t probe_card = probe_a ? A.card : B.card;
+            if (ok && (double)probe_card <= 64.0 * min(A.card, B.card) + 64.0) {
                 joined = true;
-                for (uint32_t si = 0; si < ni; si += 64)
-                    for (uint32_t sj = diag ? si : 0; sj < nj; sj += 64) {
-                        const uint32_t bi_n = min(64u, ni - si), bj_n = min(64u, nj - sj);
-                        const bool bdiag = diag && si == sj;
-                        if (probe_b) bs_slice_block(D, sl, slab_of(D, i0 + si), j0 + sj, bj_n);
-                        else bs_slice_block(D, sl, slab_of(D, j0 + sj), i0 + si, bi_n);
-                        __syncthreads();
-                        for (uint32_t p = threadIdx.x; p < bi_n * bj_n; p += blockDim.x) {
-                            const uint32_t a = p / bj_n, b = p % bj_n;
-                            if (bdiag && b <= a) continue;
-                            const uint32_t i = D.perm[i0 + si + a], j = D.perm[j0 + sj + b];
-                            const uint32_t v = probe_b ? sl.cnt[a][b] : sl.cnt[b][a];
-                            if (span_disjoint(D, i, j)) ++skipped; else ++pairs;
-                            local += v;
-                            emit_pair(D, i, j, v);
-                        }
-                        __syncthreads();
-                    }
+                const bool same = e.bi == e.bj;
+                switch ((ns + 63) / 64) {
+                    case 1: bs_stripe<1>(D, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                    case 2: bs_stripe<2>(D, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                    case 3: bs_stripe<3>(D, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                    default: bs_stripe<4>(D, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                }
             }
         }
         if constexpr (CELL == STORM_HIP_CELL_SS) {
@@ -964,7 +997,7 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         } else if (lane_mode) {
             for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
                 const uint32_t a = i0 + p / nj, b = j0 + p % nj;
-                if (diag && b <= a) continue;
+                if (same_tile && b <= a) continue;
                 const uint32_t i = D.perm[a], j = D.perm[b];
                 uint32_t cnt = 0;
                 if (CELL == STORM_HIP_CELL_EMPTY || span_disjoint(D, i, j)) {
@@ -980,7 +1013,7 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         } else {
             for (uint32_t p = wave; p < np; p += blockDim.x / 64) {
                 const uint32_t a = i0 + p / nj, b = j0 + p % nj;
-                if (diag && b <= a) continue;
+                if (same_tile && b <= a) continue;
                 const uint32_t i = D.perm[a], j = D.perm[b];
                 uint32_t cnt = 0;
                 if (span_disjoint(D, i, j)) {

@@ -30,9 +30,10 @@ struct DevCorpus {
     const uint32_t* bm_row;   const uint64_t* bitmap;   // bitmap row index, kNoBitmap if none
     const uint32_t* meta;     // 8 per original row (RowMeta + has_rank, has_occ)
     const uint32_t* perm;     // sorted position -> original row id
-    // Bit-sliced bitmaps of 64-row groups of the density order: group g's slab
-    // is slice + slice_of[g] * ld * 64 (kNoBitmap: none); word p has bit k set
-    // when sorted row 64g + k holds position p.
+    // Bit-sliced bitmaps of 64-row groups of the density order, four groups
+    // (256 rows) per slab: with s = slice_of[g] (kNoBitmap: none), word
+    // slice[((s >> 2) * ld * 64 + p) * 4 + (s & 3)] has bit k set when sorted
+    // row 64g + k holds position p; groups 4q..4q+3 share slab q, lane g & 3.
     const uint64_t* slice;    const uint32_t* slice_of;
     // Per-call state of this corpus inside a (possibly multi-corpus) call.
     const struct TileMeta* tm;        // its row tiles' metadata
@@ -93,6 +94,12 @@ __host__ __device__ inline uint32_t make_sub(uint32_t rows, uint32_t bi_blk, uin
 // Rows per block when a tile is split for the per-pair cells.
 constexpr uint32_t kSplitRows = 64;
 
+// A block index of kWholeSide in a sub means "the whole side of the tile":
+// B x S items are stripes, kSplitRows rows of the probed (list) side against
+// every row of the other side, whose bitmaps are read through their slices.
+// On a diagonal tile a stripe keeps only the pairs above the diagonal.
+constexpr uint32_t kWholeSide = 0xFF;
+
 // Thresholded calls (the corpora carry a Jaccard threshold and pair buffer,
 // their tri is null) use the B x B instances that report pairs; the matrix-core
 // one takes thr_buf, grid blocks x 256 x 256 u32 of per-block count buffers.
@@ -118,6 +125,12 @@ struct TileMeta {
     uint32_t all_bitmap, all_comp, any_empty;    // AND / AND / OR over rows
     uint32_t max_card, min_card, max_runs, max_ewah, rows;
 };
+
+// B x S probes the lists of side A (else B) against the other side's bitmaps:
+// A when B is all bitmaps and A is the sparser side (or not all bitmaps).
+__host__ __device__ inline bool bs_probe_a(const TileMeta& a, const TileMeta& b) {
+    return b.all_bitmap && (!a.all_bitmap || a.card <= b.card);
+}
 
 // Cells a tile pair can run given the representations resident on the device
 // (bit per STORM_HIP_CELL_*). reps: bit 0 runs, bit 1 EWAH, bit 2 complement.

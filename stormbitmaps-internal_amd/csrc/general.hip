@@ -653,7 +653,8 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
     const uint32_t S = (uint32_t)cand.size();
     std::vector<QItem> items(S);
     std::vector<uint32_t> elig(S);
-    std::vector<double> scale(S, 1.0);
+    std::vector<double> scale(S, 1.0), bs_scale(S, 1.0);
+    std::vector<uint32_t> bs_sub(S, 0);
     const uint32_t reps = reps_of(c);
     const DevModel M = dev_model(ctx->model);
     // Cells other than B x B are timed on a kSampleRows x kSampleRows block
@@ -682,6 +683,16 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             const double full = bi == bj ? (double)ri * (ri - 1) / 2 : (double)ri * rj;
             const double part = bi == bj ? si * (si - 1) / 2 : si * sj;
             scale[k] = part > 0 ? full / part : 1.0;
+            // B x S runs as stripes: the middle stripe of the probed side
+            // against the whole other side.
+            const bool pa = bs_probe_a(tm[bi], tm[bj]);
+            const uint32_t x = pa ? xi : xj, rp = pa ? ri : rj, ro = pa ? rj : ri;
+            const double sp = std::min<double>(kSampleRows, rp - x * kSampleRows);
+            double spart = sp * ro;
+            if (bi == bj)   // pairs above the diagonal: row 64x + q has ro - 1 - (64x + q)
+                spart = sp * (ro - 1.0 - x * (double)kSampleRows) - sp * (sp - 1) / 2;
+            bs_sub[k] = pa ? make_sub(kSampleRows, x, kWholeSide) : make_sub(kSampleRows, kWholeSide, x);
+            bs_scale[k] = spart > 0 ? full / spart : 1.0;
         }
     }
     // Time every eligible cell on its sampled tiles, each class alone, as a
@@ -705,6 +716,7 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             if (elig[k] >> cell & 1) {
                 q.push_back(items[k]);
                 if (cell == STORM_HIP_CELL_BB) q.back().sub = 0;   // whole tiles
+                if (cell == STORM_HIP_CELL_BS && bs_sub[k]) q.back().sub = bs_sub[k];
                 slot_of.push_back(k);
             }
         if (q.empty()) continue;
@@ -736,7 +748,9 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
                 const uint32_t k = slot_of[e];
                 // 100 MHz wall clock -> ns of CU time for the whole tile.
                 const double ns = (double)ticks[e] * 10.0 / sharing *
-                                  (cell == STORM_HIP_CELL_BB ? 1.0 : scale[k]);
+                                  (cell == STORM_HIP_CELL_BB ? 1.0
+                                   : cell == STORM_HIP_CELL_BS && bs_sub[k] ? bs_scale[k]
+                                                                            : scale[k]);
                 if (!has[(size_t)k * 10 + cell] || ns < best[(size_t)k * 10 + cell])
                     best[(size_t)k * 10 + cell] = ns;
                 has[(size_t)k * 10 + cell] = 1;

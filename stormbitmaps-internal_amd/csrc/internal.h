@@ -80,9 +80,10 @@ struct storm_hip_corpus {
     bool dense = false;        // contiguous corpus: every row a bitmap
     bool has_runs = false, has_ewah = false, has_comp = false;
     // Bit-sliced bitmaps (STORM_HIP_BUILD_SLICES): for each group of 64
-    // density-ordered rows that all hold bitmaps, a slab of ld * 64 masks,
-    // slab[p] bit k = row 64g + k holds position p. d_slice_of[g] = slab
-    // index or kNoBitmap.
+    // density-ordered rows that all hold bitmaps, per position p a mask whose
+    // bit k says row 64g + k holds p; the groups of 256 consecutive rows share
+    // a slab of ld * 64 positions x 4 masks. d_slice_of[g] = 4 * slab + (g & 3)
+    // or kNoBitmap.
     bool want_slices = false;
     uint64_t* d_slice = nullptr;
     uint32_t* d_slice_of = nullptr;

@@ -237,7 +237,17 @@ __global__ __launch_bounds__(256) void select_kernel(
         if (!(elig >> p & 1)) p = cheapest_eligible(M, a, b, elig);
     }
     decisions[t] = (uint8_t)p;
-    if (split && p != STORM_HIP_CELL_BB && p != STORM_HIP_CELL_EMPTY) {
+    if (split && p == STORM_HIP_CELL_BS) {
+        // B x S: stripes of kSplitRows probed rows against the whole other
+        // side (read through its slices).
+        const bool pa = bs_probe_a(a, b);
+        const uint32_t ns = ((pa ? a.rows : b.rows) + kSplitRows - 1) / kSplitRows;
+        uint32_t slot = atomicAdd(&ctr->qcount[p], ns);
+        for (uint32_t x = 0; x < ns; ++x)
+            queue[(uint64_t)p * qstride + slot++] =
+                QItem{inst, bi, bj, pa ? make_sub(kSplitRows, x, kWholeSide)
+                                       : make_sub(kSplitRows, kWholeSide, x)};
+    } else if (split && p != STORM_HIP_CELL_BB && p != STORM_HIP_CELL_EMPTY) {
         // Per-pair cells: the tile as kSplitRows x kSplitRows blocks (upper
         // triangle of blocks on a diagonal tile), each its own work item.
         const uint32_t ni = (a.rows + kSplitRows - 1) / kSplitRows;

@@ -240,7 +240,7 @@ def test_autotune_general_keeps_counts(ctx, cases):
     ctx.set_bb_config(8, 4, 2, 8)
 
 
-@pytest.mark.parametrize("tile", [64, 256])
+@pytest.mark.parametrize("tile", [64, 128, 256])
 def test_bs_slices(ctx, tile):
     """B x S through bit-sliced bitmap groups: 300 rows whose cardinalities span
     1..40,000 over a universe of 100,003 (five 64-row groups, the last one
@@ -262,3 +262,14 @@ def test_bs_slices(ctx, tile):
                 st = c.allpairs(policy=pol, fixed_cell=CELLS.index("BS"), out="tri", tile=tile)
                 assert st["sum"] == want_sum, (bitmaps, pol)
                 assert np.array_equal(c.tri(), want), (bitmaps, pol)
+                assert sum(st["cell_pairs"].values()) == st["pairs"], (bitmaps, pol, st)
+            # thresholded calls report through the same stripes
+            st, hits = c.allpairs_threshold(0.2, policy="fixed", fixed_cell=CELLS.index("BS"),
+                                            tile=tile)
+            card = np.diff(offs.astype(np.int64)).astype(np.float64)
+            I, J = np.triu_indices(len(rows), 1)
+            v = want.astype(np.float64)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                ok = (want > 0) & (v / (card[I] + card[J] - v) >= 0.2)
+            exp = np.stack([I[ok], J[ok], want[ok]], 1).astype(np.uint32)
+            assert np.array_equal(hits, exp), bitmaps
