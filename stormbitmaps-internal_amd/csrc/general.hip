@@ -653,15 +653,12 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
     const uint32_t S = (uint32_t)cand.size();
     std::vector<QItem> items(S);
     std::vector<uint32_t> elig(S);
-    std::vector<double> scale(S, 1.0), bs_scale(S, 1.0);
-    std::vector<uint32_t> bs_sub(S, 0);
+    std::vector<double> scale(S, 1.0);
     const uint32_t reps = reps_of(c);
     const DevModel M = dev_model(ctx->model);
-    // Cells other than B x B are timed on a kSampleRows x kSampleRows block
-    // of median-density rows of each tile and the time is scaled by the tile's
-    // pair count over the block's. The block is the unit the block joins and
-    // the slice path work in (64 x 64, aligned), so their per-block costs are
-    // sampled as they run.
+    // The per-pair cells are timed on a kSampleRows x kSampleRows block of
+    // median-density rows of each tile and the time is scaled by the tile's
+    // pair count over the block's (their cost is per pair).
     constexpr uint32_t kSampleRows = 64;
     for (uint32_t k = 0; k < S; ++k) {
         uint32_t bi, bj;
@@ -683,16 +680,6 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             const double full = bi == bj ? (double)ri * (ri - 1) / 2 : (double)ri * rj;
             const double part = bi == bj ? si * (si - 1) / 2 : si * sj;
             scale[k] = part > 0 ? full / part : 1.0;
-            // B x S runs as stripes: the middle stripe of the probed side
-            // against the whole other side.
-            const bool pa = bs_probe_a(tm[bi], tm[bj]);
-            const uint32_t x = pa ? xi : xj, rp = pa ? ri : rj, ro = pa ? rj : ri;
-            const double sp = std::min<double>(kSampleRows, rp - x * kSampleRows);
-            double spart = sp * ro;
-            if (bi == bj)   // pairs above the diagonal: row 64x + q has ro - 1 - (64x + q)
-                spart = sp * (ro - 1.0 - x * (double)kSampleRows) - sp * (sp - 1) / 2;
-            bs_sub[k] = pa ? make_sub(kSampleRows, x, kWholeSide) : make_sub(kSampleRows, kWholeSide, x);
-            bs_scale[k] = spart > 0 ? full / spart : 1.0;
         }
     }
     // Time every eligible cell on its sampled tiles, each class alone, as a
@@ -709,23 +696,68 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
     if (tile == 256 && c->n_bitmap_rows)
         SH_TRY(bb_mfma_check(fp4, ctx->mfma.stage_words, c->n_words, c->ld));
     const uint32_t cu = (uint32_t)std::max(1, ctx->cu_count);
+    // The block-join cells (S x S, R x R) and the B x S stripes are timed on
+    // every block or stripe of the tile, queued as the selector queues them:
+    // their cost per block follows the rows' sizes, which vary across a tile.
+    // The per-pair cells take one block and scale by the pair count.
+    auto whole_tile = [&](int cell) {
+        return tile > kSplitRows && (cell == STORM_HIP_CELL_SS || cell == STORM_HIP_CELL_RR ||
+                                     cell == STORM_HIP_CELL_BS);
+    };
     for (int cell = 0; cell < 10 && rc == STORM_HIP_OK; ++cell) {
         std::vector<QItem> q;
         std::vector<uint32_t> slot_of;
-        for (uint32_t k = 0; k < S; ++k)
-            if (elig[k] >> cell & 1) {
-                q.push_back(items[k]);
+        std::vector<double> factor;
+        for (uint32_t k = 0; k < S; ++k) {
+            if (!(elig[k] >> cell & 1)) continue;
+            const QItem it = items[k];
+            if (whole_tile(cell)) {
+                const TileMeta& a = tm[it.bi];
+                const TileMeta& b = tm[it.bj];
+                const uint32_t ni = (a.rows + kSplitRows - 1) / kSplitRows;
+                const uint32_t nj = (b.rows + kSplitRows - 1) / kSplitRows;
+                if (cell == STORM_HIP_CELL_BS) {
+                    const bool pa = bs_probe_a(a, b);
+                    for (uint32_t x = 0; x < (pa ? ni : nj); ++x) {
+                        q.push_back(QItem{0, it.bi, it.bj,
+                                          pa ? make_sub(kSplitRows, x, kWholeSide)
+                                             : make_sub(kSplitRows, kWholeSide, x)});
+                        slot_of.push_back(k);
+                        factor.push_back(1.0);
+                    }
+                } else {
+                    for (uint32_t x = 0; x < ni; ++x)
+                        for (uint32_t y = it.bi == it.bj ? x : 0; y < nj; ++y) {
+                            q.push_back(QItem{0, it.bi, it.bj, make_sub(kSplitRows, x, y)});
+                            slot_of.push_back(k);
+                            factor.push_back(1.0);
+                        }
+                }
+            } else {
+                q.push_back(it);
                 if (cell == STORM_HIP_CELL_BB) q.back().sub = 0;   // whole tiles
-                if (cell == STORM_HIP_CELL_BS && bs_sub[k]) q.back().sub = bs_sub[k];
                 slot_of.push_back(k);
+                factor.push_back(cell == STORM_HIP_CELL_BB ? 1.0 : scale[k]);
             }
+        }
         if (q.empty()) continue;
+        if (q.size() > ticks.size()) {
+            (void)hipFree(dticks);
+            dticks = nullptr;
+            ticks.resize(q.size());
+            if (hipMalloc(&dticks, sizeof(unsigned long long) * q.size()) != hipSuccess) {
+                rc = STORM_HIP_ERR_NOMEM;
+                break;
+            }
+        }
+        if (q.size() > p->qstride) { rc = STORM_HIP_ERR_INVALID; break; }
         QItem* dq = p->queue + (uint64_t)cell * p->qstride;
         const uint32_t per_cu = cell == STORM_HIP_CELL_BB ? bb_blocks_per_cu(tile, ctx->mfma)
                                                           : cell_blocks_per_cu(cell);
         const uint32_t grid = (uint32_t)std::min<uint64_t>(q.size(), (uint64_t)cu * per_cu);
         const double sharing = std::max(1.0, std::min((double)per_cu, (double)grid / cu));
         (void)hipMemcpyAsync(dq, q.data(), q.size() * sizeof(QItem), hipMemcpyHostToDevice, s);
+        std::vector<double> sum(S);
         for (uint32_t r = 0; r < std::max(1u, repeats) && rc == STORM_HIP_OK; ++r) {
             DevCounters h{};
             h.qcount[cell] = (uint32_t)q.size();
@@ -744,15 +776,14 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             if (rc != STORM_HIP_OK) break;
             (void)hipMemcpyAsync(ticks.data(), dticks, q.size() * 8, hipMemcpyDeviceToHost, s);
             if (hipStreamSynchronize(s) != hipSuccess) { rc = STORM_HIP_ERR_DEVICE; break; }
+            std::fill(sum.begin(), sum.end(), 0.0);
+            // 100 MHz wall clock -> ns of CU time, summed over the tile's items.
+            for (size_t e = 0; e < q.size(); ++e)
+                sum[slot_of[e]] += (double)ticks[e] * 10.0 / sharing * factor[e];
             for (size_t e = 0; e < q.size(); ++e) {
                 const uint32_t k = slot_of[e];
-                // 100 MHz wall clock -> ns of CU time for the whole tile.
-                const double ns = (double)ticks[e] * 10.0 / sharing *
-                                  (cell == STORM_HIP_CELL_BB ? 1.0
-                                   : cell == STORM_HIP_CELL_BS && bs_sub[k] ? bs_scale[k]
-                                                                            : scale[k]);
-                if (!has[(size_t)k * 10 + cell] || ns < best[(size_t)k * 10 + cell])
-                    best[(size_t)k * 10 + cell] = ns;
+                if (!has[(size_t)k * 10 + cell] || sum[k] < best[(size_t)k * 10 + cell])
+                    best[(size_t)k * 10 + cell] = sum[k];
                 has[(size_t)k * 10 + cell] = 1;
             }
         }
