@@ -127,6 +127,30 @@ def column_pair_sum(offs, pos, universe):
     return int((c * (c - np.uint64(1)) // np.uint64(2)).sum(dtype=np.uint64)) if len(c) else 0
 
 
+def selection_regret(ctx, X, Y, M, idx, decisions):
+    """Per-tile regret on held-out sampled tiles: the measured CU-time of each
+    policy's chosen cell, summed, over the sum of the fastest measured cell of
+    every tile (1.0 = always the exhaustively timed best). `learned_argmin` is
+    the network's own ranking before the trust region; the policies' figures
+    are the cells their production calls chose."""
+    if len(X) == 0:
+        return None
+    t = np.where(M > 0, np.exp2(Y.astype(np.float64)), np.inf)
+    best = t.min(axis=1)
+    out = {}
+    pred = np.where(M > 0, ctx.selector_predict(X), np.inf)
+    choice = pred.argmin(axis=1)
+    out["learned_argmin"] = float(t[np.arange(len(t)), choice].sum() / best.sum())
+    for name, dec in decisions.items():
+        c = dec[idx].astype(np.int64)
+        ok = (c < 10)
+        ok[ok] &= M[np.nonzero(ok)[0], c[ok]] > 0
+        if ok.any():
+            out[name] = float(t[np.nonzero(ok)[0], c[ok]].sum() / best[ok].sum())
+            out[name + "_tiles"] = int(ok.sum())
+    return out
+
+
 def sparse_bench(sh, args):
     """Mixed-representation workloads: all-bitmap vs analytic vs learned
     per-tile selection, every policy's sum checked equal."""
@@ -179,7 +203,10 @@ def sparse_bench(sh, args):
                 "cell_tiles": st["cell_tiles"], "skipped": st["skipped"],
                 "fallback_tiles": st["fallback_tiles"], "ms_select": st["ms_select"]}
 
+    T = (n + args.tile - 1) // args.tile
+    decisions = {}
     res["analytic_reference_rates"] = timed("analytic", use_graph=True)
+    decisions["analytic_reference_rates"] = corpus.last_decisions(T * (T + 1) // 2)
     # One sampling pass (every candidate cell timed on sampled tiles) feeds
     # both the calibrated analytic model and the learned selector.
     t0 = time.perf_counter()
@@ -191,16 +218,26 @@ def sparse_bench(sh, args):
                                          seed=11 + ch, repeats=2))
         progress(f"sampled chunk {ch + 1}/{chunks}: {len(parts[-1][0])} tiles")
     X, Y, M, idx = (np.concatenate([p[k] for p in parts]) for k in range(4))
+    # The last chunk is held out of training to measure regret.
+    n_hold = len(parts[-1][0]) if chunks > 1 else 0
     t_sample = time.perf_counter() - t0
     t0 = time.perf_counter()
     model = ctx.calibrate_samples(corpus, args.tile, Y, M, idx)
     t_cal = time.perf_counter() - t0
     res["analytic"] = timed("analytic", use_graph=True)
+    decisions["analytic"] = corpus.last_decisions(T * (T + 1) // 2)
     t0 = time.perf_counter()
-    progress(f"training on {len(X)} tiles")
-    loss = ctx.selector_train(X, Y, M, epochs=300, batch=128, lr=3e-3, seed=5)
+    n_fit = len(X) - n_hold
+    progress(f"training on {n_fit} tiles ({n_hold} held out)")
+    loss = ctx.selector_train(X[:n_fit], Y[:n_fit], M[:n_fit], epochs=300, batch=128, lr=3e-3,
+                              seed=5)
     t_train = time.perf_counter() - t0
     res["learned"] = timed("learned", use_graph=True)
+    decisions["learned"] = corpus.last_decisions(T * (T + 1) // 2)
+    regret = selection_regret(ctx, X[n_fit:], Y[n_fit:], M[n_fit:], idx[n_fit:], decisions)
+    if args.dump_samples:
+        np.savez(args.dump_samples, X=X, Y=Y, M=M, idx=idx, n_fit=n_fit,
+                 **{"dec_" + k: v for k, v in decisions.items()})
     # Thresholded mode: only pairs with Jaccard >= t, against the exact call.
     thresh = {}
     for t in (float(x) for x in args.threshold.split(",") if x):
@@ -234,7 +271,8 @@ def sparse_bench(sh, args):
         "learned_over_analytic_reference_rates":
             res["learned"]["pairs_per_s"] / res["analytic_reference_rates"]["pairs_per_s"],
         "selector": {"sampled_tiles": int(len(X)), "train_loss": loss, "sample_s": t_sample,
-                     "train_s": t_train},
+                     "train_s": t_train, "held_out_tiles": int(n_hold),
+                     "held_out_regret": regret},
         "calibrated_model": model, "calibrate_s": t_cal,
         "setup_s": {"generate": t_gen, "build_upload": t_up},
         "representation_build": build,
@@ -538,7 +576,11 @@ def main():
                     help="sparse configs: a policy slower than this per call runs once")
     ap.add_argument("--bitmap-budget-gib", type=int, default=160,
                     help="sparse configs: keep a bitmap of every row when they fit this")
-    ap.add_argument("--sample-tiles", type=int, default=256)
+    ap.add_argument("--sample-tiles", type=int, default=1024)
+    ap.add_argument("--dump-samples", default="",
+                    help="sparse configs: write the selector's sampled tiles (features, "
+                         "measured log2 CU-time per cell, masks) and each policy's decisions "
+                         "to this .npz")
     ap.add_argument("--sample-chunks", type=int, default=8,
                     help="sampling passes (a progress line after each)")
     ap.add_argument("--no-allbitmap", action="store_true")

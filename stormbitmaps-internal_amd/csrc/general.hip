@@ -741,6 +741,23 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             }
         }
         if (q.empty()) continue;
+        const uint32_t per_cu = cell == STORM_HIP_CELL_BB ? bb_blocks_per_cu(tile, ctx->mfma)
+                                                          : cell_blocks_per_cu(cell);
+        // Production launches keep every CU at full occupancy, and a
+        // latency-bound cell's time per item under that load is not its time
+        // alone on a CU: the items are repeated until the launch fills every
+        // workgroup slot, and each item's cost is the mean over its copies.
+        const size_t base = q.size();
+        const size_t slots = (size_t)cu * per_cu;
+        size_t copies = std::max<size_t>(1, (slots + base - 1) / base);
+        copies = std::min<size_t>(copies, std::max<size_t>(1, p->qstride / base));
+        for (size_t c = 1; c < copies; ++c)
+            for (size_t e = 0; e < base; ++e) {
+                q.push_back(q[e]);
+                slot_of.push_back(slot_of[e]);
+                factor.push_back(factor[e] / (double)copies);
+            }
+        for (size_t e = 0; e < base; ++e) factor[e] /= (double)copies;
         if (q.size() > ticks.size()) {
             (void)hipFree(dticks);
             dticks = nullptr;
@@ -752,8 +769,6 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
         }
         if (q.size() > p->qstride) { rc = STORM_HIP_ERR_INVALID; break; }
         QItem* dq = p->queue + (uint64_t)cell * p->qstride;
-        const uint32_t per_cu = cell == STORM_HIP_CELL_BB ? bb_blocks_per_cu(tile, ctx->mfma)
-                                                          : cell_blocks_per_cu(cell);
         const uint32_t grid = (uint32_t)std::min<uint64_t>(q.size(), (uint64_t)cu * per_cu);
         const double sharing = std::max(1.0, std::min((double)per_cu, (double)grid / cu));
         (void)hipMemcpyAsync(dq, q.data(), q.size() * sizeof(QItem), hipMemcpyHostToDevice, s);
