@@ -405,7 +405,13 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
         if (diag && b <= a) continue;
         const uint32_t i = D.perm[ia + a], j = D.perm[ja + b];
         const uint32_t v = S.cnt[a][b];
-        if (span_disjoint(D, i, j)) ++skipped; else ++pairs;
+        {
+            // Counted as values, not as a choice between two counters (which
+            // the compiler turns into a pointer into scratch memory).
+            const uint64_t dis = span_disjoint(D, i, j) ? 1u : 0u;
+            skipped += dis;
+            pairs += 1u - dis;
+        }
         local += v;
         emit_pair(D, i, j, v);
     }
@@ -498,7 +504,13 @@ __device__ void bs_stripe(const DevCorpus& D, uint32_t p0, uint32_t np, uint32_t
             const uint32_t pa = probe_a ? p0 + r : s0 + sa, pb = probe_a ? s0 + sa : p0 + r;
             if (same_tile && pb <= pa) continue;
             const uint32_t i = D.perm[pa], j = D.perm[pb];
-            if (span_disjoint(D, i, j)) ++skipped; else ++pairs;
+            {
+            // Counted as values, not as a choice between two counters (which
+            // the compiler turns into a pointer into scratch memory).
+            const uint64_t dis = span_disjoint(D, i, j) ? 1u : 0u;
+            skipped += dis;
+            pairs += 1u - dis;
+        }
             local += acc[g];
             emit_pair(D, i, j, acc[g]);
         }
@@ -719,7 +731,13 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
         if (diag && b <= a) continue;
         const uint32_t i = D.perm[ia + a], j = D.perm[ja + b];
         const uint32_t v = swap ? S.cnt[b][a] : S.cnt[a][b];
-        if (span_disjoint(D, i, j)) ++skipped; else ++pairs;
+        {
+            // Counted as values, not as a choice between two counters (which
+            // the compiler turns into a pointer into scratch memory).
+            const uint64_t dis = span_disjoint(D, i, j) ? 1u : 0u;
+            skipped += dis;
+            pairs += 1u - dis;
+        }
         local += v;
         emit_pair(D, i, j, v);
     }
@@ -809,9 +827,9 @@ __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint3
             if (diag && tj <= ti) continue;
             const uint32_t y = D.perm[ys0 + v];
             uint32_t cnt = 0;
-            if (span_disjoint(D, x, y)) {
-                if (lane == 0) ++skipped;
-            } else {
+            const bool dis = span_disjoint(D, x, y);
+            if (lane == 0) { skipped += dis ? 1u : 0u; pairs += dis ? 0u : 1u; }
+            if (!dis) {
                 // Lanes take the streamed row's items 64 apart (coalesced
                 // loads) and look each up in the staged row's bucket index.
                 uint32_t c = 0;
@@ -835,7 +853,6 @@ __device__ void tile_staged(const DevCorpus& D, StageSmem& S, uint32_t ia, uint3
                     }
                 }
                 cnt = wave_sum_u32(c);
-                if (lane == 0) ++pairs;
             }
             if (lane == 0) {
                 local += cnt;
@@ -1000,10 +1017,10 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
                 if (same_tile && b <= a) continue;
                 const uint32_t i = D.perm[a], j = D.perm[b];
                 uint32_t cnt = 0;
-                if (CELL == STORM_HIP_CELL_EMPTY || span_disjoint(D, i, j)) {
-                    ++skipped;
-                } else {
-                    ++pairs;
+                const bool dis = CELL == STORM_HIP_CELL_EMPTY || span_disjoint(D, i, j);
+                skipped += dis ? 1u : 0u;
+                pairs += dis ? 0u : 1u;
+                if (!dis) {
                     cnt = cell_partial<CELL>(D, M, i, j, 0, 1);
                     if (CELL == STORM_HIP_CELL_CB) cnt = cb_base(D, i, j) - cnt;
                 }
@@ -1016,12 +1033,11 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
                 if (same_tile && b <= a) continue;
                 const uint32_t i = D.perm[a], j = D.perm[b];
                 uint32_t cnt = 0;
-                if (span_disjoint(D, i, j)) {
-                    if (lane == 0) ++skipped;
-                } else {
+                const bool dis = span_disjoint(D, i, j);
+                if (lane == 0) { skipped += dis ? 1u : 0u; pairs += dis ? 0u : 1u; }
+                if (!dis) {
                     cnt = wave_sum_u32(cell_partial<CELL>(D, M, i, j, lane, 64));
                     if (CELL == STORM_HIP_CELL_CB) cnt = cb_base(D, i, j) - cnt;
-                    if (lane == 0) ++pairs;
                 }
                 if (lane == 0) {
                     local += cnt;
