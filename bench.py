@@ -205,6 +205,22 @@ def sparse_bench(sh, args):
 
     T = (n + args.tile - 1) // args.tile
     decisions = {}
+    if args.only_policy:
+        # One policy alone (for profiling its kernels without the others').
+        name = args.only_policy
+        r = timed("analytic" if name == "analytic_reference_rates" else name,
+                  use_graph=name != "all_bitmap")
+        out = {"metric": METRIC, "value": r["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": 1, "ms_per_step": r["ms"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+               "data": "synthetic (seeded host generator)",
+               "config": {"workload": desc, "n_rows": n, "universe": u, "pairs": pairs,
+                          "tile": args.tile},
+               "policy": name, "exact": r["sum"] == want, "result": r}
+        corpus.free()
+        ctx.close()
+        print(json.dumps(out), flush=True)
+        return
     res["analytic_reference_rates"] = timed("analytic", use_graph=True)
     decisions["analytic_reference_rates"] = corpus.last_decisions(T * (T + 1) // 2)
     # One sampling pass (every candidate cell timed on sampled tiles) feeds
@@ -577,6 +593,9 @@ def main():
     ap.add_argument("--bitmap-budget-gib", type=int, default=160,
                     help="sparse configs: keep a bitmap of every row when they fit this")
     ap.add_argument("--sample-tiles", type=int, default=1024)
+    ap.add_argument("--only-policy", default="",
+                    choices=["", "analytic_reference_rates", "all_bitmap"],
+                    help="sparse configs: time this one policy only (no sampling or training)")
     ap.add_argument("--dump-samples", default="",
                     help="sparse configs: write the selector's sampled tiles (features, "
                          "measured log2 CU-time per cell, masks) and each policy's decisions "
