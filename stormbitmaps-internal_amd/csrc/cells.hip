@@ -22,6 +22,10 @@
 // Work split: a pair's work items (list elements, runs, markers) are spread
 // over the 64 lanes of a wave and wave-reduced ("wave mode"), or, for tiles
 // whose rows are tiny, each lane evaluates whole pairs ("lane mode").
+// Three cells also work on whole blocks of pairs at once, touching each row's
+// items once per block instead of once per pair: S x S as an LDS hash join and
+// R x R as an LDS interval join over 64 x 64 blocks, and B x S as stripes of
+// 64 list rows against a 256-row bitmap tile read through its bit slices.
 #include <atomic>
 
 #include "device_common.h"
