@@ -429,22 +429,10 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
 // position side by side (32 bytes). A B x S item is a stripe: up to 64 rows of
 // the probed side (lists) against up to 256 rows of the other side, whose
 // groups all have slices. A wave takes one probed row and streams its list
-// once; its lanes load 64 elements' masks (one 32-byte load for four groups),
-// and the count of each slab row is the popcount of one ballot per row present
-// in any lane's mask, kept by the lane of that row. One load per list element
-// serves up to 256 pairs (the per-pair cell probes the bitmap once per element
-// per pair), and each lane ends holding finished pair counts: no LDS, no
-// barriers.
-__device__ inline uint64_t wave_or_u64(uint64_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const uint32_t lo = __shfl_xor((uint32_t)v, off, 64);
-        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
-        v |= ((uint64_t)hi << 32) | lo;
-    }
-    return v;
-}
-
+// once; its lanes load 64 elements' masks (32 bytes for four groups) and add 1
+// to the wave's LDS counter of every slab row set in them. One load per list
+// element serves up to 256 pairs (the per-pair cell probes the bitmap once per
+// element per pair); the counters are private to the wave, so no barrier.
 constexpr int kSliceFetch = 4;   // 64-element batches a wave loads before counting
 
 // Probe rows at sorted positions [p0, p0 + np), slab rows [s0, s0 + ns) (s0 a
@@ -452,9 +440,9 @@ constexpr int kSliceFetch = 4;   // 64-element batches a wave loads before count
 // position, B position) with the probe rows on side A when probe_a; on a
 // diagonal tile (same_tile) only pairs whose B position is the larger count.
 template <int G>
-__device__ void bs_stripe(const DevCorpus& D, uint32_t p0, uint32_t np, uint32_t s0, uint32_t ns,
-                          bool probe_a, bool same_tile, uint64_t& local, uint64_t& pairs,
-                          uint64_t& skipped) {
+__device__ void bs_stripe(const DevCorpus& D, uint32_t* __restrict__ wave_cnt, uint32_t p0,
+                          uint32_t np, uint32_t s0, uint32_t ns, bool probe_a, bool same_tile,
+                          uint64_t& local, uint64_t& pairs, uint64_t& skipped) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     const uint32_t g0 = s0 >> 6;
     const uint64_t* base[G];
@@ -464,11 +452,13 @@ __device__ void bs_stripe(const DevCorpus& D, uint32_t p0, uint32_t np, uint32_t
         base[g] = D.slice + (uint64_t)(sl >> 2) * D.ld * 256 + (sl & 3);
     }
     const bool packed = G == 4 && (g0 & 3) == 0;   // the four masks of a position at once
+    // This wave's counters, one per slab row: each lane adds 1 for every set
+    // bit of its elements' masks (LDS atomics, a few per element at most
+    // densities), then the row's counts are read back and cleared.
+    uint32_t* cnt = wave_cnt + wave * 256;
+    for (uint32_t q = lane; q < 256; q += 64) cnt[q] = 0;
     for (uint32_t r = wave; r < np; r += nwaves) {
         const Span32 L = list_of(D, D.perm[p0 + r]);
-        uint32_t acc[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) acc[g] = 0;
         for (uint32_t k0 = 0; k0 < L.n; k0 += 64 * kSliceFetch) {
             uint64_t m[kSliceFetch][G];
 #pragma unroll
@@ -492,31 +482,30 @@ __device__ void bs_stripe(const DevCorpus& D, uint32_t p0, uint32_t np, uint32_t
             for (int u = 0; u < kSliceFetch; ++u)
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
-                    unsigned long long any = wave_or_u64(m[u][g]);
-                    while (any) {
-                        const uint32_t a = (uint32_t)__ffsll(any) - 1;
-                        any &= any - 1;
-                        const uint32_t c = (uint32_t)__popcll(__ballot((m[u][g] >> a) & 1ull));
-                        if (lane == a) acc[g] += c;
+                    unsigned long long x = m[u][g];
+                    while (x) {
+                        const uint32_t a = (uint32_t)__ffsll(x) - 1;
+                        x &= x - 1;
+                        atomicAdd(&cnt[64 * g + a], 1u);
                     }
                 }
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const uint32_t sa = 64 * g + lane;
+            const uint32_t v = cnt[sa];
+            cnt[sa] = 0;
             if (sa >= ns) continue;
             const uint32_t pa = probe_a ? p0 + r : s0 + sa, pb = probe_a ? s0 + sa : p0 + r;
             if (same_tile && pb <= pa) continue;
             const uint32_t i = D.perm[pa], j = D.perm[pb];
             {
-            // Counted as values, not as a choice between two counters (which
-            // the compiler turns into a pointer into scratch memory).
-            const uint64_t dis = span_disjoint(D, i, j) ? 1u : 0u;
-            skipped += dis;
-            pairs += 1u - dis;
-        }
-            local += acc[g];
-            emit_pair(D, i, j, acc[g]);
+                const uint64_t dis = span_disjoint(D, i, j) ? 1u : 0u;
+                skipped += dis;
+                pairs += 1u - dis;
+            }
+            local += v;
+            emit_pair(D, i, j, v);
         }
     }
 }
@@ -966,11 +955,12 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
             if (ok && (double)probe_card <= 64.0 * min(A.card, B.card) + 64.0) {
                 joined = true;
                 const bool same = e.bi == e.bj;
+                __shared__ uint32_t wave_cnt[4 * 256];   // 256 counters per wave (4 waves)
                 switch ((ns + 63) / 64) {
-                    case 1: bs_stripe<1>(D, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
-                    case 2: bs_stripe<2>(D, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
-                    case 3: bs_stripe<3>(D, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
-                    default: bs_stripe<4>(D, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                    case 1: bs_stripe<1>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                    case 2: bs_stripe<2>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                    case 3: bs_stripe<3>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                    default: bs_stripe<4>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
                 }
             }
         }
