@@ -314,14 +314,31 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
                                   &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr, thresholded);
     if (timed) (void)hipEventRecord(ctx->ev[3], sb);
-    // Persistent grids: as many workgroups as fit on the chip at once.
-    int k = 0;
-    for (int cell = 0; cell < kNumCells && rc == STORM_HIP_OK; ++cell) {
-        if (cell == STORM_HIP_CELL_BB) continue;
+    // Persistent grids: as many workgroups as fit on the chip at once. The
+    // classes that usually carry the most work (B x S stripes, the R x R and
+    // S x S joins) start at once on different streams: B x S first on the
+    // main stream, the joins first on side stream 1; the per-pair classes
+    // follow them.
+    static const int kOrder[2][5] = {
+        {STORM_HIP_CELL_BS, STORM_HIP_CELL_SR, STORM_HIP_CELL_SW, STORM_HIP_CELL_RW,
+         STORM_HIP_CELL_CB},
+        {STORM_HIP_CELL_RR, STORM_HIP_CELL_SS, STORM_HIP_CELL_BR, STORM_HIP_CELL_BW,
+         STORM_HIP_CELL_WW}};
+    static_assert(STORM_HIP_NCELL == 12, "every cell class but B x B and EMPTY in kOrder");
+    for (int x = 0; x < 5 && rc == STORM_HIP_OK; ++x)
+        for (int side = 0; side < 2 && rc == STORM_HIP_OK; ++side) {
+            const int cell = kOrder[side][x];
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(
+                p->qstride, (uint64_t)ctx->cu_count * cell_blocks_per_cu(cell));
+            rc = launch_cell(cell, side ? ctx->aux[1] : s, grid, p->d_corpora, M,
+                             p->queue + (uint64_t)cell * p->qstride, p->ctr, nullptr);
+        }
+    // Tiles settled empty (zeros only) go last.
+    if (rc == STORM_HIP_OK) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(
-            p->qstride, (uint64_t)ctx->cu_count * cell_blocks_per_cu(cell));
-        rc = launch_cell(cell, (k++ & 1) ? ctx->aux[1] : s, grid, p->d_corpora, M,
-                         p->queue + (uint64_t)cell * p->qstride, p->ctr, nullptr);
+            p->qstride, (uint64_t)ctx->cu_count * cell_blocks_per_cu(STORM_HIP_CELL_EMPTY));
+        rc = launch_cell(STORM_HIP_CELL_EMPTY, ctx->aux[1], grid, p->d_corpora, M,
+                         p->queue + (uint64_t)STORM_HIP_CELL_EMPTY * p->qstride, p->ctr, nullptr);
     }
     // Join the side streams back, whatever happened (a capture must close).
     for (int a = 0; a < 2; ++a) {
