@@ -665,7 +665,13 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
         rs += kGolden;
         std::swap(cand[i - 1], cand[splitmix_mix(rs) % i]);
     }
-    if (cand.size() > max_tiles) cand.resize(max_tiles);
+    // The rest of the shuffled order: tiles whose items fill the workgroup
+    // slots while the sampled ones are timed.
+    std::vector<uint32_t> rest;
+    if (cand.size() > max_tiles) {
+        rest.assign(cand.begin() + max_tiles, cand.end());
+        cand.resize(max_tiles);
+    }
     std::sort(cand.begin(), cand.end());
     const uint32_t S = (uint32_t)cand.size();
     std::vector<QItem> items(S);
@@ -721,22 +727,25 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
         return tile > kSplitRows && (cell == STORM_HIP_CELL_SS || cell == STORM_HIP_CELL_RR ||
                                      cell == STORM_HIP_CELL_BS);
     };
+    constexpr uint32_t kFiller = 0xFFFFFFFFu;   // an item that loads the GPU, not timed
+    size_t next_rest = 0;
     for (int cell = 0; cell < 10 && rc == STORM_HIP_OK; ++cell) {
         std::vector<QItem> q;
         std::vector<uint32_t> slot_of;
         std::vector<double> factor;
-        for (uint32_t k = 0; k < S; ++k) {
-            if (!(elig[k] >> cell & 1)) continue;
-            const QItem it = items[k];
+        // The items of tile (bi, bj) as this cell runs them, for sampled tile
+        // k (or a filler).
+        auto add_items = [&](uint32_t bi, uint32_t bj, uint32_t k, const QItem& single,
+                             double single_factor) {
             if (whole_tile(cell)) {
-                const TileMeta& a = tm[it.bi];
-                const TileMeta& b = tm[it.bj];
+                const TileMeta& a = tm[bi];
+                const TileMeta& b = tm[bj];
                 const uint32_t ni = (a.rows + kSplitRows - 1) / kSplitRows;
                 const uint32_t nj = (b.rows + kSplitRows - 1) / kSplitRows;
                 if (cell == STORM_HIP_CELL_BS) {
                     const bool pa = bs_probe_a(a, b);
                     for (uint32_t x = 0; x < (pa ? ni : nj); ++x) {
-                        q.push_back(QItem{0, it.bi, it.bj,
+                        q.push_back(QItem{0, bi, bj,
                                           pa ? make_sub(kSplitRows, x, kWholeSide)
                                              : make_sub(kSplitRows, kWholeSide, x)});
                         slot_of.push_back(k);
@@ -744,30 +753,52 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
                     }
                 } else {
                     for (uint32_t x = 0; x < ni; ++x)
-                        for (uint32_t y = it.bi == it.bj ? x : 0; y < nj; ++y) {
-                            q.push_back(QItem{0, it.bi, it.bj, make_sub(kSplitRows, x, y)});
+                        for (uint32_t y = bi == bj ? x : 0; y < nj; ++y) {
+                            q.push_back(QItem{0, bi, bj, make_sub(kSplitRows, x, y)});
                             slot_of.push_back(k);
                             factor.push_back(1.0);
                         }
                 }
             } else {
-                q.push_back(it);
+                q.push_back(single);
                 if (cell == STORM_HIP_CELL_BB) q.back().sub = 0;   // whole tiles
                 slot_of.push_back(k);
-                factor.push_back(cell == STORM_HIP_CELL_BB ? 1.0 : scale[k]);
+                factor.push_back(cell == STORM_HIP_CELL_BB ? 1.0 : single_factor);
             }
-        }
+        };
+        for (uint32_t k = 0; k < S; ++k)
+            if (elig[k] >> cell & 1) add_items(items[k].bi, items[k].bj, k, items[k], scale[k]);
         if (q.empty()) continue;
         const uint32_t per_cu = cell == STORM_HIP_CELL_BB ? bb_blocks_per_cu(tile, ctx->mfma)
                                                           : cell_blocks_per_cu(cell);
         // Production launches keep every CU at full occupancy, and a
         // latency-bound cell's time per item under that load is not its time
-        // alone on a CU: the items are repeated until the launch fills every
-        // workgroup slot, and each item's cost is the mean over its copies.
+        // alone on a CU. Free workgroup slots are filled with the items of
+        // unsampled tiles (other rows, so the caches are as cold as in
+        // production; not timed); where there are too few, the sampled items
+        // are repeated and each costs the mean over its copies.
         const size_t base = q.size();
         const size_t slots = (size_t)cu * per_cu;
-        size_t copies = std::max<size_t>(1, (slots + base - 1) / base);
-        copies = std::min<size_t>(copies, std::max<size_t>(1, p->qstride / base));
+        const size_t cap = std::min<size_t>(p->qstride, std::max(slots, base));
+        // Fillers are drawn from the rest of the shuffled tile order: any rows
+        // of the corpus, so the mix of sizes and the cache misses are those of
+        // a production launch.
+        for (size_t tried = 0; q.size() < slots && tried < rest.size(); ++tried) {
+            const uint32_t t = rest[next_rest++ % rest.size()];
+            uint32_t bi, bj;
+            tri_tile(t, w->T, bi, bj);
+            if (!(eligible_mask(tm[bi], tm[bj], reps) >> cell & 1)) continue;
+            const size_t before = q.size();
+            add_items(bi, bj, kFiller, QItem{0, bi, bj, items[0].sub}, 0.0);
+            if (q.size() > cap) {   // keep within the queue
+                q.resize(before);
+                slot_of.resize(before);
+                factor.resize(before);
+                break;
+            }
+        }
+        size_t copies = q.size() >= slots ? 1 : std::max<size_t>(1, (slots + base - 1) / base);
+        copies = std::min<size_t>(copies, std::max<size_t>(1, p->qstride / std::max<size_t>(q.size(), 1)));
         for (size_t c = 1; c < copies; ++c)
             for (size_t e = 0; e < base; ++e) {
                 q.push_back(q[e]);
@@ -811,9 +842,11 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
             std::fill(sum.begin(), sum.end(), 0.0);
             // 100 MHz wall clock -> ns of CU time, summed over the tile's items.
             for (size_t e = 0; e < q.size(); ++e)
-                sum[slot_of[e]] += (double)ticks[e] * 10.0 / sharing * factor[e];
+                if (slot_of[e] != kFiller)
+                    sum[slot_of[e]] += (double)ticks[e] * 10.0 / sharing * factor[e];
             for (size_t e = 0; e < q.size(); ++e) {
                 const uint32_t k = slot_of[e];
+                if (k == kFiller) continue;
                 if (!has[(size_t)k * 10 + cell] || sum[k] < best[(size_t)k * 10 + cell])
                     best[(size_t)k * 10 + cell] = sum[k];
                 has[(size_t)k * 10 + cell] = 1;
