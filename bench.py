@@ -615,7 +615,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the all-pairs triangle would shard by tile rows, see DESIGN.md)", file=sys.stderr)
+              "(the all-pairs triangle would shard by tile rows across GPUs, each holding "
+              "every row panel)", file=sys.stderr)
         sys.exit(2)
 
     if args.config == "copy":
