@@ -408,9 +408,13 @@ int storm_hip_last_decisions(storm_hip_ctx* ctx, const storm_hip_corpus* c,
 #define STORM_HIP_MLP_FEATURES 16
 int storm_hip_mlp_shape(int* features, int* h1, int* h2, int* outputs, int* n_params);
 /* Time every eligible cell on up to max_tiles sampled non-empty tile pairs
- * (tile rows each), `repeats` times (min kept). Outputs per sampled tile:
- * features[F], log2 of the tile's wall time in ns per cell (mask=1 where the
- * cell was eligible and timed), the tile-pair index and its pair count. */
+ * (tile rows each), `repeats` times (min kept). Each timing launch runs the
+ * tiles' work items as production queues them, filled to every workgroup slot
+ * with items of other tiles (run, not timed), and a tile's cost is its
+ * items' CU-time (latency over the workgroups sharing a CU); per-pair cells
+ * are timed on one 64 x 64 block and scaled by the pair count. Outputs per
+ * sampled tile: features[F], log2 of that cost in ns per cell (mask=1 where
+ * the cell was eligible and timed), the tile-pair index and its pair count. */
 int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
                               uint32_t max_tiles, uint64_t seed, uint32_t repeats,
                               float* features, float* log2_ns, uint8_t* mask,
