@@ -31,6 +31,8 @@ def oracle():
     vp, u32, u64, i32, dbl = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_double
     sig = {
         "orc_rng_next": (u64, [vp]),
+        "orc_rng_below": (u32, [vp, u32]),
+        "orc_rng_seed": (u64, [u64]),
         "orc_gen_bernoulli_rows": (None, [u64, dbl, u32, u32, u32, vp]),
         "orc_popcount_bitloop": (u32, [u64]),
         "orc_intersect_lists": (u64, [vp, u32, u32, vp, u32, u32]),
@@ -96,6 +98,7 @@ def reference():
     vp, u32, u64, i32, dbl = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_double
     sig = {
         "ref_rng_next": (None, [u64, u32, vp]),
+        "ref_rng_below": (None, [u64, u32, u32, vp]),
         "ref_build_row": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "ref_build_rank": (None, [vp, u32, vp]),
         "ref_select_pairing": (i32, [vp, vp, vp]),

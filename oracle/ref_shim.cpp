@@ -26,6 +26,12 @@ void ref_rng_next(uint64_t seed, uint32_t n, uint64_t* out) {
     for (uint32_t i = 0; i < n; ++i) out[i] = r.next();
 }
 
+/* n draws of storm::Rng(seed).below(bound) (kernels/storm_gen.h:93-99). */
+void ref_rng_below(uint64_t seed, uint32_t bound, uint32_t n, uint32_t* out) {
+    Rng r(seed);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.below(bound);
+}
+
 /* build_row (kernels/storm_repr.cpp:102-207): sizes first, then contents.
  * sizes[0..8] = n_words, n_rank, n_occ, n_runs, n_ewah, n_comp, n_comp_runs,
  * has_rank, has_occ. Arrays may be NULL to query sizes only. meta6 receives

@@ -369,7 +369,8 @@ int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c, uin
                               uint64_t count, uint32_t* out) {
     if (!ctx || !c || !c->tri_valid) return STORM_HIP_ERR_INVALID;
     if (!out && count) return STORM_HIP_ERR_NULL_VALUES;
-    if (offset + count > c->tri_count) return STORM_HIP_ERR_RANGE;
+    // Written so that no sum can wrap: offset and count may each be near 2^64.
+    if (offset > c->tri_count || count > c->tri_count - offset) return STORM_HIP_ERR_RANGE;
     if (!count) return STORM_HIP_OK;
     SH_HIP(hipSetDevice(ctx->device));
     SH_HIP(hipMemcpyAsync(out, c->d_tri + offset, count * 4, hipMemcpyDeviceToHost, ctx->stream));

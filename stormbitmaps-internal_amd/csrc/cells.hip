@@ -433,7 +433,13 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
 // to the wave's LDS counter of every slab row set in them. One load per list
 // element serves up to 256 pairs (the per-pair cell probes the bitmap once per
 // element per pair); the counters are private to the wave, so no barrier.
-constexpr int kSliceFetch = 4;   // 64-element batches a wave loads before counting
+constexpr int kSliceFetch = 4;
+
+// Orders one wave's LDS accesses across its lanes (no cross-wave effect).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}   // 64-element batches a wave loads before counting
 
 // Probe rows at sorted positions [p0, p0 + np), slab rows [s0, s0 + ns) (s0 a
 // multiple of 64, G = ceil(ns / 64) groups, all with slices). Pairs are (A
@@ -457,6 +463,10 @@ __device__ void bs_stripe(const DevCorpus& D, uint32_t* __restrict__ wave_cnt, u
     // densities), then the row's counts are read back and cleared.
     uint32_t* cnt = wave_cnt + wave * 256;
     for (uint32_t q = lane; q < 256; q += 64) cnt[q] = 0;
+    // Lanes hand counters to each other through LDS (every lane adds to any
+    // counter, lane k reads back counters k, k+64, ...): a wave-scope fence
+    // and barrier order the clears, the adds and the read-back.
+    wave_lds_sync();
     for (uint32_t r = wave; r < np; r += nwaves) {
         const Span32 L = list_of(D, D.perm[p0 + r]);
         for (uint32_t k0 = 0; k0 < L.n; k0 += 64 * kSliceFetch) {
@@ -490,6 +500,7 @@ __device__ void bs_stripe(const DevCorpus& D, uint32_t* __restrict__ wave_cnt, u
                     }
                 }
         }
+        wave_lds_sync();
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const uint32_t sa = 64 * g + lane;
@@ -507,6 +518,7 @@ __device__ void bs_stripe(const DevCorpus& D, uint32_t* __restrict__ wave_cnt, u
             local += v;
             emit_pair(D, i, j, v);
         }
+        wave_lds_sync();
     }
 }
 

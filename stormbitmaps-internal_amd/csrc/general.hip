@@ -1038,7 +1038,8 @@ int storm_hip_result_copy_pairs(storm_hip_ctx* ctx, const storm_hip_corpus* c, u
                                 uint64_t count, storm_hip_pair* out) {
     if (!ctx || !c || c->ctx != ctx || !c->hits_valid) return STORM_HIP_ERR_INVALID;
     if (!out && count) return STORM_HIP_ERR_NULL_VALUES;
-    if (offset + count > c->h_hits.size() / 3) return STORM_HIP_ERR_RANGE;
+    const uint64_t total = c->h_hits.size() / 3;
+    if (offset > total || count > total - offset) return STORM_HIP_ERR_RANGE;
     for (uint64_t k = 0; k < count; ++k) {
         out[k].i = c->h_hits[3 * (offset + k)];
         out[k].j = c->h_hits[3 * (offset + k) + 1];

@@ -339,3 +339,175 @@ def test_adam_matches_published_update():
         L.orc_adam_step(3, P(p), P(g), P(m), P(v), step, 0.01, 0.9, 0.999, 1e-8)
     want = np.array([1.0, -2.0, 0.5]) - 5 * 0.01 * g / (np.abs(g) + 1e-8)
     assert np.allclose(p, want, rtol=0, atol=1e-12)
+
+
+# ---------------------------------------------------------------- driver helpers
+class OrcMeta(C.Structure):
+    """orc_meta (storm_oracle.h): the six RowMeta integers, has_rank, has_occ."""
+    _fields_ = [("a", C.c_uint32 * 6), ("hr", C.c_uint8), ("ho", C.c_uint8)]
+
+
+def _orc_metas(offs, pos, u):
+    n = len(offs) - 1
+    arr = (OrcMeta * n)()
+    for i in range(n):
+        r = O.build_row(pos[int(offs[i]):int(offs[i + 1])], u)
+        for k in range(6):
+            arr[i].a[k] = int(r["meta"][k])
+        arr[i].hr, arr[i].ho = int(r["has_rank"]), int(r["has_occ"])
+    return arr
+
+
+@needs_ref
+def test_rng_below_matches_reference():
+    """Rng::below (kernels/storm_gen.h:93-99), including bound 0 and bounds
+    near 2^32 where the rejection limit is furthest from 2^64."""
+    L, R = O.oracle(), O.reference()
+    for seed in (0, 3, 0x9911, 0xFFFFFFFFFFFF):
+        for bound in (0, 1, 2, 7, 1000, 3866847, (1 << 31) + 1, 3_000_000_000, (1 << 32) - 1):
+            want = np.zeros(500, np.uint32)
+            R.ref_rng_below(seed, bound, 500, P(want))
+            s = C.c_uint64(L.orc_rng_seed(seed))
+            got = [L.orc_rng_below(C.byref(s), bound) for _ in range(500)]
+            assert np.array_equal(np.array(got, np.uint32), want), (seed, bound)
+            if bound:
+                assert int(want.max()) < bound
+
+
+def test_density_order_is_a_stable_sort():
+    """density_order (kernels/storm_allpairs.cpp:316-323) is std::stable_sort by
+    cardinality: equal to Python's (stable) sorted() on every tie pattern."""
+    L = O.oracle()
+    rng = np.random.default_rng(17)
+    for n in (0, 1, 2, 3, 17, 64, 65, 1000):
+        for levels in (1, 3, 50, 10 ** 6):
+            cards = rng.integers(0, levels, size=n)
+            arr = (OrcMeta * max(n, 1))()
+            for i in range(n):
+                arr[i].a[0] = int(cards[i])
+            order = np.zeros(max(n, 1), np.uint32)
+            L.orc_density_order(arr, n, P(order))
+            want = sorted(range(n), key=lambda i: int(cards[i]))
+            assert list(order[:n]) == want, (n, levels)
+
+
+def test_tile_meta_brute_force():
+    """tile_meta (kernels/storm_allpairs.cpp:281-312) on random metadata:
+    integer-truncated means, min first / max last over non-empty rows only
+    (first 0 when all are empty), AND of the index flags, max n_words; an
+    empty range keeps the flags false and divides by 1."""
+    L = O.oracle()
+    rng = np.random.default_rng(23)
+    n = 300
+    arr = (OrcMeta * n)()
+    rows = []
+    for i in range(n):
+        card = int(rng.integers(0, 4)) * int(rng.integers(0, 1 << 20))
+        first = int(rng.integers(0, 1 << 30))
+        m = [card, int(rng.integers(0, 1 << 20)), int(rng.integers(1, 1 << 16)),
+             int(rng.integers(0, 1 << 16)), first, first + int(rng.integers(0, 1 << 30))]
+        hr, ho = int(rng.random() < 0.8), int(rng.random() < 0.8)
+        for k in range(6):
+            arr[i].a[k] = m[k]
+        arr[i].hr, arr[i].ho = hr, ho
+        rows.append((m, hr, ho))
+    order = rng.permutation(n).astype(np.uint32)
+    for lo, hi in [(0, 0), (5, 5), (0, 1), (0, 64), (10, 77), (250, 300), (0, 300)]:
+        out = OrcMeta()
+        L.orc_tile_meta(arr, P(order), lo, hi, C.byref(out))
+        sel = [rows[int(order[k])] for k in range(lo, hi)]
+        cnt = max(1, hi - lo)
+        live = [m for m, _, _ in sel if m[0]]
+        want = [sum(m[0] for m, _, _ in sel) // cnt, sum(m[1] for m, _, _ in sel) // cnt,
+                max([m[2] for m, _, _ in sel], default=0), sum(m[3] for m, _, _ in sel) // cnt,
+                min([m[4] for m in live], default=0), max([m[5] for m in live], default=0)]
+        assert list(out.a) == want, (lo, hi)
+        assert out.hr == int(bool(sel) and all(h for _, h, _ in sel))
+        assert out.ho == int(bool(sel) and all(o for _, _, o in sel))
+
+
+def _per_tile_histogram(metas, n, tile):
+    """The PerTile loop of allpairs_sum (kernels/storm_allpairs.cpp:547-667)
+    driven by the oracle: density order, tile_meta of both tiles,
+    select_pairing (Empty -> BB), then per pair the span test (Empty) or the
+    tile's cell. Returns the 11 cell_pairs counters."""
+    L = O.oracle()
+    model = (C.c_double * 16)()
+    L.orc_default_model(model)
+    order = np.zeros(max(n, 1), np.uint32)
+    L.orc_density_order(metas, n, P(order))
+    hist = np.zeros(11, np.uint64)
+    a, b = OrcMeta(), OrcMeta()
+    for i0 in range(0, n, tile):
+        i1 = min(i0 + tile, n)
+        L.orc_tile_meta(metas, P(order), i0, i1, C.byref(a))
+        for j0 in range(i0, n, tile):
+            j1 = min(j0 + tile, n)
+            L.orc_tile_meta(metas, P(order), j0, j1, C.byref(b))
+            tp = L.orc_select_pairing(model, C.byref(a), C.byref(b), None)
+            if tp == 10:
+                tp = 0
+            for i in range(i0, i1):
+                mi = metas[int(order[i])].a
+                for j in range(i + 1 if j0 == i0 else j0, j1):
+                    mj = metas[int(order[j])].a
+                    if mi[0] == 0 or mj[0] == 0 or mi[5] < mj[4] or mj[5] < mi[4]:
+                        hist[10] += 1
+                    else:
+                        hist[tp] += 1
+    return hist
+
+
+def _tie_heavy_corpus(u, seed):
+    """Rows with many equal cardinalities but different shapes (runs, spans,
+    index flags), so which tile a tied row lands in changes the tile means."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for k in range(120):
+        card = int(rng.choice([0, 1, 5, 64, 512, u // 3, u // 2 + 1]))
+        if card == 0:
+            rows.append(np.zeros(0, np.uint32))
+        elif rng.random() < 0.5:            # one run
+            s = int(rng.integers(0, u - card + 1))
+            rows.append(np.arange(s, s + card, dtype=np.uint32))
+        else:                               # scattered
+            rows.append(np.sort(rng.choice(u, card, replace=False)).astype(np.uint32))
+    offs = np.zeros(len(rows) + 1, np.uint64)
+    offs[1:] = np.cumsum([len(r) for r in rows])
+    return offs, np.concatenate(rows).astype(np.uint32)
+
+
+@needs_ref
+@pytest.mark.parametrize("shape", ["sweep", "ties"])
+def test_tile_meta_and_density_order_reproduce_reference_histogram(shape):
+    """The reference's PerTile cell_pairs histogram (allpairs_sum, policy
+    PerTile, kernels/storm_allpairs.cpp:547-667) equals the one the oracle's
+    density_order + tile_meta + select_pairing + span test produce, over the
+    tests/test_cells.cpp:237-287 corpus sweep and a tie-heavy corpus, at tile
+    widths 8, 13, 16 and 64. A wrong mean, a lost min/max, an unstable sort or
+    a flag OR instead of AND moves pairs between cells."""
+    R = O.reference()
+    cases = []
+    if shape == "sweep":
+        for u in (1024, 4033, 16384):
+            for structure in (0, 1, 2):
+                for spectrum in (0, 1, 2):
+                    cases.append((u, *O.ref_generate(40, u, 0.02, structure, spectrum,
+                                                      0x9911 + u)))
+        cases.append((65536, *O.ref_generate(200, 65536, 0.01, 2, 1, 0x1234)))
+    else:
+        for u, seed in ((4096, 1), (100000, 2), (1 << 20, 3)):
+            cases.append((u, *_tie_heavy_corpus(u, seed)))
+    moved = 0
+    for u, offs, pos in cases:
+        n = len(offs) - 1
+        metas = _orc_metas(offs, pos, u)
+        o32 = offs.astype(np.uint32)
+        for tile in (8, 13, 16, 64):
+            want = np.zeros(11, np.uint64)
+            R.ref_allpairs_sum(P(o32), P(pos), n, u, 2, tile, P(want), None, None)
+            got = _per_tile_histogram(metas, n, tile)
+            assert np.array_equal(got, want), (u, n, tile, got, want)
+            moved += int((want[:10] > 0).sum() > 1)
+    # The sweep must exercise more than one cell per histogram somewhere.
+    assert moved > 0
