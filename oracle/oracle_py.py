@@ -46,6 +46,8 @@ def oracle():
         "orc_allpairs_words": (u64, [vp, u32, u32, vp, i32]),
         "orc_column_counts_words": (None, [vp, u32, u32, vp]),
         "orc_total_from_columns": (u64, [vp, u64]),
+        "orc_column_closed_forms_words": (None, [vp, u32, u32, vp, vp, i32]),
+        "orc_band_words": (u64, [vp, u32, u32, u32, u32, vp, i32]),
         "orc_row_sums_words": (None, [vp, u32, u32, vp, vp]),
         "orc_row_sums_from_tri": (None, [vp, u32, vp]),
         "orc_select_pairing": (i32, [vp, vp, vp, vp]),
@@ -155,17 +157,31 @@ def allpairs_lists(offs, pos, tri=True, threads=8):
     return s, out
 
 
-def column_closed_forms(words):
+def column_closed_forms(words, threads=1):
     """(total, row_sums) from column counts -- exact, O(N*m)."""
     words = np.ascontiguousarray(words, dtype=np.uint64)
     n, nw = words.shape
     col = np.zeros(nw * 64, dtype=np.uint32)
     L = oracle()
+    rs = np.zeros(n, dtype=np.uint64)
+    if threads > 1:
+        L.orc_column_closed_forms_words(_p(words), n, nw, _p(col), _p(rs), threads)
+        return L.orc_total_from_columns(_p(col), nw * 64), rs
     L.orc_column_counts_words(_p(words), n, nw, _p(col))
     total = L.orc_total_from_columns(_p(col), nw * 64)
-    rs = np.zeros(n, dtype=np.uint64)
     L.orc_row_sums_words(_p(words), n, nw, _p(col), _p(rs))
     return total, rs
+
+
+def band_words(words, r0, r1, threads=16):
+    """Triangle entries of rows [r0, r1) (every j > i), in triangle order."""
+    words = np.ascontiguousarray(words, dtype=np.uint64)
+    n, nw = words.shape
+    r1 = min(r1, n)
+    cnt = sum(n - i - 1 for i in range(r0, r1))
+    out = np.zeros(max(cnt, 1), dtype=np.uint32)
+    s = oracle().orc_band_words(_p(words), n, nw, r0, r1, _p(out), threads)
+    return s, out[:cnt]
 
 
 def lists_closed_forms(offs, pos, universe):

@@ -467,6 +467,100 @@ uint64_t orc_allpairs_words(const uint64_t* words, uint32_t n, uint32_t nw,
     return orc_run_jobs(orc_words_worker, p, threads);
 }
 
+/* Rows [r0, r1) of the packed triangle: for each such i every j in (i, n), in
+ * triangle order (one contiguous range of it). Threads split each row's j. */
+typedef struct {
+    const uint64_t* words; uint32_t n, nw, r0, r1; uint32_t* out; int tid, nt; uint64_t sum;
+} orc_band_job;
+
+static void* orc_band_worker(void* vp) {
+    orc_band_job* j = (orc_band_job*)vp;
+    uint64_t s = 0;
+    const uint64_t base = tri_index(j->r0, j->r0 + 1, j->n);
+    for (uint32_t i = j->r0; i < j->r1; ++i) {
+        const uint64_t* a = j->words + (size_t)i * j->nw;
+        for (uint32_t k = i + 1 + (uint32_t)j->tid; k < j->n; k += (uint32_t)j->nt) {
+            const uint64_t* b = j->words + (size_t)k * j->nw;
+            uint64_t c = 0;
+            for (uint32_t w = 0; w < j->nw; ++w) c += (uint64_t)__builtin_popcountll(a[w] & b[w]);
+            if (j->out) j->out[tri_index(i, k, j->n) - base] = (uint32_t)c;
+            s += c;
+        }
+    }
+    j->sum = s;
+    return NULL;
+}
+
+uint64_t orc_band_words(const uint64_t* words, uint32_t n, uint32_t nw, uint32_t r0, uint32_t r1,
+                        uint32_t* out, int threads) {
+    if (r1 > n) r1 = n;
+    if (r0 >= r1 || r0 + 1 >= n) return 0;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    orc_band_job jobs[256];
+    pthread_t th[256];
+    for (int t = 0; t < threads; ++t) {
+        orc_band_job b = {words, n, nw, r0, r1, out, t, threads, 0};
+        jobs[t] = b;
+        if (threads > 1) pthread_create(&th[t], NULL, orc_band_worker, &jobs[t]);
+        else orc_band_worker(&jobs[t]);
+    }
+    uint64_t s = 0;
+    for (int t = 0; t < threads; ++t) {
+        if (threads > 1) pthread_join(th[t], NULL);
+        s += jobs[t].sum;
+    }
+    return s;
+}
+
+/* Column counts and row sums of a dense matrix on `threads` threads: column
+ * words split between threads for the counts, rows for the sums. */
+typedef struct {
+    const uint64_t* words; uint32_t n, nw; uint32_t* col; uint64_t* row_sum; int tid, nt, phase;
+} orc_cols_job;
+
+static void* orc_cols_worker(void* vp) {
+    orc_cols_job* j = (orc_cols_job*)vp;
+    if (j->phase == 0) {
+        const uint32_t w0 = (uint32_t)((uint64_t)j->nw * j->tid / j->nt);
+        const uint32_t w1 = (uint32_t)((uint64_t)j->nw * (j->tid + 1) / j->nt);
+        memset(j->col + (size_t)w0 * 64, 0, (size_t)(w1 - w0) * 64 * 4);
+        for (uint32_t i = 0; i < j->n; ++i)
+            for (uint32_t w = w0; w < w1; ++w) {
+                uint64_t x = j->words[(size_t)i * j->nw + w];
+                while (x) { j->col[(size_t)w * 64 + __builtin_ctzll(x)]++; x &= x - 1; }
+            }
+    } else {
+        for (uint32_t i = (uint32_t)j->tid; i < j->n; i += (uint32_t)j->nt) {
+            uint64_t s = 0;
+            for (uint32_t w = 0; w < j->nw; ++w) {
+                uint64_t x = j->words[(size_t)i * j->nw + w];
+                while (x) { s += j->col[(size_t)w * 64 + __builtin_ctzll(x)] - 1; x &= x - 1; }
+            }
+            j->row_sum[i] = s;
+        }
+    }
+    return NULL;
+}
+
+void orc_column_closed_forms_words(const uint64_t* words, uint32_t n, uint32_t nw, uint32_t* col,
+                                   uint64_t* row_sum, int threads) {
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    orc_cols_job jobs[256];
+    pthread_t th[256];
+    for (int phase = 0; phase < 2; ++phase) {
+        if (phase == 1 && !row_sum) break;
+        for (int t = 0; t < threads; ++t) {
+            orc_cols_job b = {words, n, nw, col, row_sum, t, threads, phase};
+            jobs[t] = b;
+            if (threads > 1) pthread_create(&th[t], NULL, orc_cols_worker, &jobs[t]);
+            else orc_cols_worker(&jobs[t]);
+        }
+        for (int t = 0; t < threads && threads > 1; ++t) pthread_join(th[t], NULL);
+    }
+}
+
 void orc_column_counts_words(const uint64_t* words, uint32_t n, uint32_t nw, uint32_t* col) {
     memset(col, 0, (size_t)nw * 64 * 4);
     for (uint32_t i = 0; i < n; ++i)

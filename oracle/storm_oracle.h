@@ -114,6 +114,13 @@ uint64_t orc_allpairs_words(const uint64_t* words, uint32_t n, uint32_t nw,
 void     orc_column_counts_words(const uint64_t* words, uint32_t n, uint32_t nw,
                                  uint32_t* col /* nw*64 */);
 uint64_t orc_total_from_columns(const uint32_t* col, uint64_t n_cols);
+/* Column counts and (optional) row sums as above, on `threads` threads. */
+void     orc_column_closed_forms_words(const uint64_t* words, uint32_t n, uint32_t nw,
+                                       uint32_t* col, uint64_t* row_sum, int threads);
+/* The triangle's rows [r0, r1): every pair (i, j), r0 <= i < r1, i < j < n,
+ * written in triangle order from out[0] (out may be NULL); returns their sum. */
+uint64_t orc_band_words(const uint64_t* words, uint32_t n, uint32_t nw, uint32_t r0,
+                        uint32_t r1, uint32_t* out, int threads);
 void     orc_row_sums_words(const uint64_t* words, uint32_t n, uint32_t nw,
                             const uint32_t* col, uint64_t* row_sum);
 /* Row sums implied by a packed upper-triangle output: row_sum[i] =

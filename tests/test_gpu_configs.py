@@ -53,3 +53,32 @@ def test_workload_prefix_exact(sh, ctx, name):
                     assert tri[a * n - a * (a + 1) // 2 + (b - a - 1)] == want
             else:
                 assert np.array_equal(tri, ref_tri), (policy, kw)
+
+
+def test_dense_config_full_size(ctx):
+    """The dense contiguous workload at its stated size: N = 32,768 rows x
+    M = 1,048,576 bits, Bernoulli(0.05), seed 2, generated on the device.
+    Under the fp4 and the int8 matrix-core kernels: the total equals
+    sum_b C(c_b, 2) and every row sum equals sum_{b in Xi} (c_b - 1), from the
+    column counts of the downloaded rows (the generator itself is checked
+    against the oracle stream on a band of rows); three bands of the triangle
+    (first, middle and last rows, every j > i) equal the oracle pair by pair."""
+    n, bits, p, seed = 32768, 1 << 20, 0.05, 2
+    nw = bits // 64
+    print("case dense config full size", flush=True)
+    with ctx.gen_bernoulli(n, bits, p, seed) as c:
+        words = c.download_words(0, n, nw)
+        assert np.array_equal(words[1000:1004], O.bernoulli_rows(seed, p, bits, 1000, 4))
+        total, row_sums = O.column_closed_forms(words, threads=16)
+        bands = [(0, 32), (n // 2 - 16, n // 2 + 16), (n - 32, n)]
+        want_bands = [O.band_words(words, r0, r1, threads=16)[1] for r0, r1 in bands]
+        del words
+        for bb in ("auto", "mfma"):
+            print("case dense config full size", bb, flush=True)
+            st = c.allpairs(out="tri", bb=bb)
+            assert st["pairs"] == n * (n - 1) // 2
+            assert st["sum"] == total, bb
+            assert np.array_equal(c.row_sums(), row_sums), bb
+            for (r0, r1), want in zip(bands, want_bands):
+                lo = r0 * n - r0 * (r0 + 1) // 2
+                assert np.array_equal(c.tri(lo, len(want)), want), (bb, r0, r1)

@@ -511,3 +511,25 @@ def test_tile_meta_and_density_order_reproduce_reference_histogram(shape):
             moved += int((want[:10] > 0).sum() > 1)
     # The sweep must exercise more than one cell per histogram somewhere.
     assert moved > 0
+
+
+def test_threaded_closed_forms_and_band_agree_with_brute_force():
+    """The threaded column closed forms and the triangle band (used on the
+    full-size dense configuration) equal the single-thread forms and the
+    brute-force triangle on ragged shapes."""
+    rng = np.random.default_rng(31)
+    for n, nw, p in ((1, 3, 0.5), (2, 1, 0.5), (37, 5, 0.3), (130, 17, 0.05), (65, 64, 0.9)):
+        words = np.zeros((n, nw), np.uint64)
+        bits = rng.random((n, nw * 64)) < p
+        for k in range(64):
+            words |= bits[:, k::64].astype(np.uint64) << np.uint64(k)
+        t1, rs1 = O.column_closed_forms(words)
+        t8, rs8 = O.column_closed_forms(words, threads=7)
+        want_sum, tri = O.allpairs_words(words)
+        assert t1 == t8 == want_sum
+        assert np.array_equal(rs1, rs8)
+        for r0, r1 in ((0, 1), (0, n), (n // 3, n // 2 + 1), (n - 1, n)):
+            s, band = O.band_words(words, r0, r1, threads=5)
+            lo = r0 * n - r0 * (r0 + 1) // 2
+            assert np.array_equal(band, tri[lo:lo + len(band)]), (n, r0, r1)
+            assert s == int(band.sum(dtype=np.uint64))
