@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-#define STORM_HIP_ABI_VERSION 1
+#define STORM_HIP_ABI_VERSION 2
 
 /* Status codes. Negative = failure; the reference's C ABI uses the same
  * convention (storm.cpp:1160-1219: NULL handle -1, NULL values -2, bad_alloc -4). */
@@ -70,19 +70,22 @@ int storm_hip_corpus_download_words(storm_hip_ctx* ctx, const storm_hip_corpus* 
                                     uint32_t row0, uint32_t n_rows, uint64_t* out);
 
 /* General corpus: per-row arenas in the storm_repr byte layouts
- * (kernels/storm_repr.h:112-276). All offsets are element offsets into the
- * matching arena, n_rows+1 entries each (row r spans [off[r], off[r+1])).
- * A row whose bitmap span is empty has no bitmap on the device; cells that
- * need one are not eligible for pairs that touch it.
+ * (kernels/storm_repr.h:112-276), the layouts build_row writes
+ * (kernels/storm_repr.cpp:102-207). All offsets are element offsets into the
+ * matching arena, n_rows+1 entries each (row r spans [off[r], off[r+1]); the
+ * first offset may be non-zero). An arena whose offsets are NULL is absent.
  *   lists        sorted distinct u32 positions          (ListView)
  *   runs         [start,end) u32 SoA, maximal           (RunView)
  *   ewah         EWAH-64 markers + literals             (EwahView)
- *   bitmap       n_words u64 per bitmap row             (BitmapView::w)
+ *   bitmap       n_words u64 per row, or an empty span  (BitmapView::w)
+ *   rank         rank9, 2*(ceil(n_words/8)+1) u64       (BitmapView::rank)
  *   occ          zone map, 1 bit per 8 words            (BitmapView::occ)
  *   comp lists   complement positions, rows with 2n > universe (ComplementView)
  *   meta         8 u32 per row: cardinality, n_runs, n_words, n_nonzero_w,
  *                first_set, last_set, has_rank, has_occ  (RowMeta)
- * Pointers for absent arenas may be NULL (their offsets then all zero). */
+ * Every row needs a list, runs or a bitmap; lists are optional, so rows held
+ * only as runs or only as bitmaps can be uploaded (cells that read lists are
+ * then not eligible). */
 typedef struct {
     uint32_t        n_rows;
     uint32_t        universe;
@@ -90,16 +93,22 @@ typedef struct {
     const uint64_t* run_off;    const uint32_t* run_start;  const uint32_t* run_end;
     const uint64_t* ewah_off;   const uint64_t* ewah;
     const uint64_t* bitmap_off; const uint64_t* bitmap;
+    const uint64_t* rank_off;   const uint64_t* rank;
     const uint64_t* occ_off;    const uint64_t* occ;
     const uint64_t* comp_off;   const uint32_t* comp;
     const uint32_t* meta;       /* 8 per row */
 } storm_hip_arena;
 
-/* Upload from arenas. The lists are authoritative: runs, EWAH and complement
- * (present when their offset arrays are non-NULL) are rebuilt on the host from
- * the lists in the same layouts, and `meta` must agree with them (else
- * STORM_HIP_ERR_INVALID). A bitmap is placed on the device for exactly the rows
- * whose bitmap span is non-empty. */
+/* Upload from arenas, each taken as given after a host check against the row
+ * it describes (sorted distinct lists inside the universe, maximal runs, EWAH
+ * decoding to the row, complements, bitmaps of n_words words with no bit past
+ * the universe, rank9 / zone map equal to build_rank / build_occ of the
+ * bitmap, RowMeta equal to the row's; a row without a bitmap may carry
+ * build_row's sparse-only meta, n_nonzero_w = n_runs and no index flags).
+ * A bitmap is placed on the device for exactly the rows whose bitmap span is
+ * non-empty; rank9 and zone maps for those of them whose meta has_rank /
+ * has_occ is set (built on the device, the same bytes). Violations are
+ * STORM_HIP_ERR_INVALID with the row and the reason in storm_hip_last_error. */
 int  storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* arena,
                              storm_hip_corpus** out);
 
@@ -151,6 +160,31 @@ int  storm_hip_corpus_from_device_lists(storm_hip_ctx* ctx, const uint64_t* d_of
                                         const uint32_t* d_pos, uint32_t n_rows,
                                         uint32_t universe, uint32_t flags,
                                         storm_hip_corpus** out);
+/* The seeded workloads generated on the device, each row in the
+ * representation it is stored in (no host lists, no upload):
+ *   BERNOULLI, p1 >= 1/32  every row a bitmap only: bit b of row r set iff
+ *                          draw r*universe+b of Rng(seed) is below p1 (as
+ *                          storm_hip_corpus_gen_bernoulli)
+ *   BERNOULLI, p1 < 1/32   lists, the rows of storm_hip_gen_corpus
+ *                          (geometric skips on each row's stream), then built
+ *                          like storm_hip_corpus_from_lists with `flags`
+ *   RUNS                   maximal runs only, the rows of storm_hip_gen_corpus
+ *                          (Poisson(p1) runs of mean length p2 + p3 singletons);
+ *                          bitmaps placed by flags bits 0-1
+ * The same position sets as the host generator, bit for bit (both draw from
+ * the same row streams through the same deterministic log and exp).
+ * ZIPF is STORM_HIP_ERR_UNSUPPORTED here. */
+int  storm_hip_corpus_gen(storm_hip_ctx* ctx, uint32_t kind, uint32_t n_rows, uint32_t universe,
+                          double p1, double p2, double p3, uint64_t seed, uint32_t flags,
+                          storm_hip_corpus** out);
+/* Column closed forms of a corpus (no counterpart in the reference): with c_b
+ * the number of rows holding position b, *total = sum_b C(c_b, 2) -- the sum
+ * over pairs i < j of |Xi n Xj| -- and row_sums[i] (optional, n_rows) =
+ * sum_{b in Xi} (c_b - 1) = sum_{j != i} |Xi n Xj|. Computed from the rows'
+ * own representation (lists, else runs, else bitmaps) without any cell: an
+ * exact check of an all-pairs result at any size. */
+int  storm_hip_corpus_closed_form(storm_hip_ctx* ctx, const storm_hip_corpus* c, uint64_t* total,
+                                  uint64_t* row_sums);
 
 /* STORMBIN ingest. Format (tools/sets2bin.py:23-25): b"STORMBIN", then u32
  * version, n_rows, n_bits, pad, then per row a u32 count followed by count
@@ -173,14 +207,18 @@ int  storm_hip_corpus_from_stormbin(storm_hip_ctx* ctx, const char* path, uint32
 /* Arenas of a general corpus back to the host (layouts as storm_hip_arena;
  * EWAH marker tables: per marker its first decoded word and its offset in the
  * row's EWAH buffer; BM_ROW: bitmap slot of each row or 0xFFFFFFFF; BITMAP:
- * one row of `ld` words per slot, ld a multiple of 64). Sizes are in bytes;
- * a download must ask for exactly the arena's size. */
+ * one row of `ld` words per slot, ld a multiple of 64; RANK_ROW / OCC_ROW:
+ * index slot of each row or 0xFFFFFFFF; RANK: 2*(ceil(n_words/8)+1) words per
+ * slot; OCC: ceil(ceil(n_words/8)/64) words per slot). Sizes are in bytes; a
+ * download must ask for exactly the arena's size. */
 enum {
     STORM_HIP_ARENA_LIST_OFF = 0, STORM_HIP_ARENA_LIST, STORM_HIP_ARENA_RUN_OFF,
     STORM_HIP_ARENA_RUN_START, STORM_HIP_ARENA_RUN_END, STORM_HIP_ARENA_EWAH_OFF,
     STORM_HIP_ARENA_EWAH, STORM_HIP_ARENA_MK_OFF, STORM_HIP_ARENA_MK_WORD,
     STORM_HIP_ARENA_MK_POS, STORM_HIP_ARENA_COMP_OFF, STORM_HIP_ARENA_COMP,
     STORM_HIP_ARENA_META, STORM_HIP_ARENA_BM_ROW, STORM_HIP_ARENA_BITMAP,
+    STORM_HIP_ARENA_RANK_ROW, STORM_HIP_ARENA_RANK, STORM_HIP_ARENA_OCC_ROW,
+    STORM_HIP_ARENA_OCC,
     STORM_HIP_ARENA_COUNT
 };
 int  storm_hip_corpus_arena_sizes(storm_hip_ctx* ctx, const storm_hip_corpus* c,

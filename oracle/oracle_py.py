@@ -243,3 +243,27 @@ def ref_generate(n_rows, universe, density, structure, spectrum, seed):
     pos = np.zeros(max(tot, 1), dtype=np.uint32)
     L.ref_generate(n_rows, universe, density, structure, spectrum, seed, _p(offs), _p(pos))
     return offs.astype(np.uint64), pos[:tot]
+
+
+def ref_build_row(pos, u):
+    """The reference's build_row (kernels/storm_repr.cpp:102-207) arenas of one
+    row: bitmap, rank, occ, runs, ewah, complement and the six meta integers,
+    plus has_rank / has_occ in sizes[7:9]."""
+    L = reference()
+    pos = np.ascontiguousarray(pos, dtype=np.uint32)
+    sizes = np.zeros(9, np.uint32)
+    meta = np.zeros(6, np.uint32)
+    L.ref_build_row(_p(pos), len(pos), u, _p(sizes), _p(meta), None, None, None, None, None,
+                    None, None)
+    nw, nrank, nocc, nruns, newah, ncomp = (int(x) for x in sizes[:6])
+    bm = np.zeros(max(nw, 1), np.uint64)
+    rank = np.zeros(max(nrank, 1), np.uint64)
+    occ = np.zeros(max(nocc, 1), np.uint64)
+    rs = np.zeros(max(nruns, 1), np.uint32)
+    re_ = np.zeros(max(nruns, 1), np.uint32)
+    ew = np.zeros(max(newah, 1), np.uint64)
+    comp = np.zeros(max(ncomp, 1), np.uint32)
+    L.ref_build_row(_p(pos), len(pos), u, _p(sizes), _p(meta), _p(bm), _p(rank), _p(occ),
+                    _p(rs), _p(re_), _p(ew), _p(comp))
+    return dict(sizes=sizes, meta=meta, bitmap=bm[:nw], rank=rank[:nrank], occ=occ[:nocc],
+                run_start=rs[:nruns], run_end=re_[:nruns], ewah=ew[:newah], comp=comp[:ncomp])

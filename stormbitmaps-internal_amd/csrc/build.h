@@ -31,6 +31,9 @@ int upload_vec(T*& d, const std::vector<T>& h, hipStream_t s, uint64_t& bytes) {
 // Density order, identity order, bitmap row map and bitmap store of a corpus
 // whose arenas and host meta are in place; synchronises the stream.
 int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uint8_t>& want_bitmap);
+// The same for a corpus whose bitmap store is already complete (one slot per
+// row, c->h_bm_row set): orders, indexes and slices only.
+int corpus_finish_store(storm_hip_ctx* ctx, storm_hip_corpus* c);
 
 // Builds runs, EWAH (+ marker tables), complements and meta on the device
 // from the lists already at c->d_list_off / c->d_list (flags: the

@@ -242,10 +242,15 @@ int upload_bitmaps(storm_hip_corpus* c, hipStream_t s) {
     SH_HIP(hipMemsetAsync(c->d_bitmap, 0, bytes, s));
     SH_HIP(hipMemcpyAsync(c->d_bm_row, c->h_bm_row.data(), (size_t)c->n_rows * 4,
                           hipMemcpyHostToDevice, s));
-    if (c->n_rows) {
+    if (c->n_rows && c->has_lists) {
         hipLaunchKernelGGL(scatter_bits_kernel, dim3(c->n_rows), dim3(256), 0, s, c->d_list_off,
                            c->d_list, c->d_bm_row, (unsigned long long*)c->d_bitmap, c->ld);
         SH_HIP(hipGetLastError());
+    } else if (c->n_rows && c->has_runs) {
+        SH_TRY(fill_bitmaps_from_runs(c, s));
+    } else if (c->n_rows && c->n_bitmap_rows) {
+        set_error("bitmap rows requested for a corpus held without lists or runs");
+        return STORM_HIP_ERR_INVALID;
     }
     return STORM_HIP_OK;
 }
@@ -373,12 +378,12 @@ std::vector<uint8_t> bitmap_rows(const std::vector<uint32_t>& meta, uint32_t n, 
     return want;
 }
 
-int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uint8_t>& want_bitmap) {
+namespace {
+
+// Density order (stable by cardinality) and the identity order.
+int upload_orders(storm_hip_corpus* c, hipStream_t s) {
     const uint32_t n = c->n_rows;
-    hipStream_t s = ctx->stream;
     uint64_t& B = c->device_bytes;
-    // Density order: stable by cardinality (kernels/storm_allpairs.cpp:316-323),
-    // then padding positions map to the sentinel row id n (never read as data).
     const uint32_t npad = (n + kRowPad - 1) / kRowPad * kRowPad + kRowPad;
     c->h_perm.resize(n);
     std::iota(c->h_perm.begin(), c->h_perm.end(), 0u);
@@ -390,6 +395,27 @@ int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uin
     std::iota(ident.begin(), ident.begin() + n, 0u);
     SH_TRY(upload_vec(c->d_perm, perm, s, B));
     SH_TRY(upload_vec(c->d_ident, ident, s, B));
+    return STORM_HIP_OK;
+}
+
+}  // namespace
+
+int corpus_finish_store(storm_hip_ctx* ctx, storm_hip_corpus* c) {
+    hipStream_t s = ctx->stream;
+    SH_TRY(upload_orders(c, s));
+    SH_TRY(build_indexes(c, s));
+    SH_TRY(build_slices(c, s));
+    SH_HIP(hipStreamSynchronize(s));
+    return STORM_HIP_OK;
+}
+
+int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uint8_t>& want_bitmap) {
+    const uint32_t n = c->n_rows;
+    hipStream_t s = ctx->stream;
+    uint64_t& B = c->device_bytes;
+    // Density order: stable by cardinality (kernels/storm_allpairs.cpp:316-323),
+    // padding positions map to the sentinel row id n (never read as data).
+    SH_TRY(upload_orders(c, s));
     // Bitmap rows; the sentinel row id n maps to the zero row.
     c->h_bm_row.assign((size_t)n + 1, kNoBitmap);
     uint32_t nb = 0;
@@ -403,6 +429,7 @@ int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uin
     SH_TRY(upload_bitmaps(c, s));
     SH_HIP(hipMemcpyAsync(c->d_bm_row + n, &c->h_bm_row[n], 4, hipMemcpyHostToDevice, s));
     B += ((size_t)nb + 1) * c->ld * 8;
+    SH_TRY(build_indexes(c, s));
     SH_TRY(build_slices(c, s));
     SH_HIP(hipStreamSynchronize(s));
     return STORM_HIP_OK;
@@ -410,6 +437,10 @@ int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uin
 
 int general_materialize(storm_hip_ctx* ctx, storm_hip_corpus* c) {
     if (c->dense || c->bitmaps_complete) return STORM_HIP_OK;
+    if (!c->has_lists && !c->has_runs) {
+        set_error("materialise: rows without a bitmap hold no list or runs");
+        return STORM_HIP_ERR_INVALID;
+    }
     c->device_bytes -= ((size_t)c->n_bitmap_rows + 1) * c->ld * 8;
     for (uint32_t r = 0; r < c->n_rows; ++r) c->h_bm_row[r] = r;
     c->n_bitmap_rows = c->n_rows;
@@ -419,6 +450,7 @@ int general_materialize(storm_hip_ctx* ctx, storm_hip_corpus* c) {
                           hipMemcpyHostToDevice, ctx->stream));
     c->device_bytes += ((size_t)c->n_bitmap_rows + 1) * c->ld * 8;
     c->bitmaps_complete = true;
+    SH_TRY(build_indexes(c, ctx->stream));
     SH_TRY(build_slices(c, ctx->stream));
     SH_HIP(hipStreamSynchronize(ctx->stream));
     return STORM_HIP_OK;
@@ -428,8 +460,8 @@ void general_free_arenas(storm_hip_corpus* c) {
     void* ptrs[] = {c->d_list_off, c->d_list, c->d_run_off, c->d_run_start, c->d_run_end,
                     c->d_ewah_off, c->d_ewah, c->d_mk_off, c->d_mk_word, c->d_mk_pos,
                     c->d_comp_off, c->d_comp, c->d_meta, c->d_bm_row, c->d_bitmap,
-                    c->d_perm, c->d_ident, c->d_bitmap_off, c->d_occ_off, c->d_occ,
-                    c->d_slice, c->d_slice_of};
+                    c->d_perm, c->d_ident, c->d_occ,
+                    c->d_slice, c->d_slice_of, c->d_rank, c->d_rank_row, c->d_occ_row};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 
@@ -486,32 +518,6 @@ int storm_hip_corpus_from_lists(storm_hip_ctx* ctx, const uint64_t* offs, const 
     return STORM_HIP_OK;
 }
 
-int storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* a, storm_hip_corpus** out) {
-    if (!ctx || !a || !out) return STORM_HIP_ERR_INVALID;
-    *out = nullptr;
-    if (!a->list_off || !a->meta) return STORM_HIP_ERR_NULL_VALUES;
-    // The lists carry the row; every other arena is rebuilt from them with the
-    // same layouts, and the given meta must agree with the lists.
-    uint32_t flags = 2;   // bitmaps only where the arena supplies one
-    if (a->run_off) flags |= STORM_HIP_BUILD_RUNS;
-    if (a->ewah_off) flags |= STORM_HIP_BUILD_EWAH;
-    if (a->comp_off) flags |= STORM_HIP_BUILD_COMP;
-    SH_HIP(hipSetDevice(ctx->device));
-    HostArena A;
-    SH_TRY(build_host_arena(a->list_off, a->list, a->n_rows, a->universe, flags, A));
-    for (uint32_t r = 0; r < a->n_rows; ++r) {
-        for (int k = 0; k < 6; ++k)
-            if (A.meta[(size_t)r * 8 + k] != a->meta[(size_t)r * 8 + k]) {
-                set_error("row %u: meta field %d disagrees with its list", r, k);
-                return STORM_HIP_ERR_INVALID;
-            }
-        A.want_bitmap[r] = a->bitmap_off && a->bitmap_off[r + 1] > a->bitmap_off[r];
-    }
-    const int s = upload_host_arena(ctx, A, flags, out);
-    if (s != STORM_HIP_OK && *out) { storm_hip_corpus_free(*out); *out = nullptr; }
-    return s;
-}
-
 int storm_hip_corpus_materialize_bitmaps(storm_hip_ctx* ctx, storm_hip_corpus* c) {
     if (!ctx || !c || c->ctx != ctx) return STORM_HIP_ERR_INVALID;
     SH_HIP(hipSetDevice(ctx->device));
@@ -544,6 +550,10 @@ int storm_hip_corpus_arena_sizes(storm_hip_ctx* ctx, const storm_hip_corpus* c,
     bytes[STORM_HIP_ARENA_META] = n * 32;
     bytes[STORM_HIP_ARENA_BM_ROW] = n * 4;
     bytes[STORM_HIP_ARENA_BITMAP] = (uint64_t)c->n_bitmap_rows * c->ld * 8;
+    bytes[STORM_HIP_ARENA_RANK_ROW] = n * 4;
+    bytes[STORM_HIP_ARENA_RANK] = (uint64_t)c->n_rank_rows * c->rank_ld * 8;
+    bytes[STORM_HIP_ARENA_OCC_ROW] = n * 4;
+    bytes[STORM_HIP_ARENA_OCC] = (uint64_t)c->n_occ_rows * c->occ_ld * 8;
     return STORM_HIP_OK;
 }
 
@@ -562,7 +572,8 @@ int storm_hip_corpus_download_arena(storm_hip_ctx* ctx, const storm_hip_corpus* 
     const void* src[STORM_HIP_ARENA_COUNT] = {
         c->d_list_off, c->d_list,  c->d_run_off, c->d_run_start, c->d_run_end,
         c->d_ewah_off, c->d_ewah,  c->d_mk_off,  c->d_mk_word,   c->d_mk_pos,
-        c->d_comp_off, c->d_comp,  c->d_meta,    c->d_bm_row,    c->d_bitmap};
+        c->d_comp_off, c->d_comp,  c->d_meta,    c->d_bm_row,    c->d_bitmap,
+        c->d_rank_row, c->d_rank,  c->d_occ_row, c->d_occ};
     SH_HIP(hipMemcpy(dst, src[which], bytes, hipMemcpyDeviceToHost));
     return STORM_HIP_OK;
 }

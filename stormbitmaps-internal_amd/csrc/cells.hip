@@ -60,6 +60,29 @@ __device__ inline const uint64_t* bitmap_of(const DevCorpus& D, uint32_t r) {
     return b == kNoBitmap ? nullptr : D.bitmap + (uint64_t)b * D.ld;
 }
 __device__ inline uint32_t meta(const DevCorpus& D, uint32_t r, int k) { return D.meta[(uint64_t)r * 8 + k]; }
+__device__ inline const uint64_t* rank_of(const DevCorpus& D, uint32_t r) {
+    const uint32_t k = D.rank_row[r];
+    return k == kNoBitmap ? nullptr : D.rank + (uint64_t)k * D.rank_ld;
+}
+__device__ inline const uint64_t* occ_of(const DevCorpus& D, uint32_t r) {
+    const uint32_t k = D.occ_row[r];
+    return k == kNoBitmap ? nullptr : D.occ + (uint64_t)k * D.occ_ld;
+}
+
+// Set bits strictly below position x through the rank9 index (rank_at,
+// kernels/storm_repr.h:129-137): block count, 9-bit in-block count, one word.
+__device__ inline uint64_t rank_at(const DevCorpus& D, const uint64_t* bm, const uint64_t* rk,
+                                   uint32_t x) {
+    const uint32_t wi = x >> 6;
+    if (wi >= D.nw) return rk[2 * ((D.nw + 7) / 8)];
+    const uint32_t blk = wi >> 3, k = wi & 7;
+    uint64_t r = rk[2 * blk];
+    if (k) r += (rk[2 * blk + 1] >> (9 * (k - 1))) & 0x1FF;
+    return r + __popcll(bm[wi] & ((1ull << (x & 63)) - 1));
+}
+// Runs at least this many bits long are counted by two rank lookups, shorter
+// ones by a masked popcount (br_hybrid<4>, kernels/cell_br.cpp:187-199).
+constexpr uint32_t kRankMinBits = 4 * 64;
 
 // First index with a[idx] >= x.
 __device__ inline uint32_t lower_bound(const uint32_t* a, uint32_t n, uint32_t x) {
@@ -169,8 +192,18 @@ __device__ uint32_t cell_partial(const DevCorpus& D, const DevModel& M, uint32_t
         const uint64_t* bm = bitmap_of(D, b);
         if constexpr (CELL == STORM_HIP_CELL_BR) {
             const RunsV R = runs_of(D, o);
+            const uint64_t* rk = rank_of(D, b);
+            if (rk) {
+                for (uint32_t k = first; k < R.n; k += stride) {
+                    const uint32_t rs = R.s[k], re = R.e[k];
+                    c += re - rs >= kRankMinBits
+                             ? (uint32_t)(rank_at(D, bm, rk, re) - rank_at(D, bm, rk, rs))
+                             : bitmap_range_count(bm, rs, re);
+                }
+            } else {
 #pragma unroll 4
-            for (uint32_t k = first; k < R.n; k += stride) c += bitmap_range_count(bm, R.s[k], R.e[k]);
+                for (uint32_t k = first; k < R.n; k += stride) c += bitmap_range_count(bm, R.s[k], R.e[k]);
+            }
         } else {
             const EwahV E = ewah_of(D, o);
             for (uint32_t k = first; k < E.nm; k += stride) {
@@ -250,10 +283,28 @@ __device__ uint32_t cell_partial(const DevCorpus& D, const DevModel& M, uint32_t
         if (!(2ull * meta(D, d, 0) > D.universe) || !bitmap_of(D, s)) { a = s; b = d; }
         const uint64_t* bm = bitmap_of(D, b);
         const Span32 A = comp_of(D, a);
+        const RunsV R = runs_of(D, a);
+        // cb_adaptive (kernels/cell_comp.cpp:70-74): the complement as runs
+        // (the gaps between the row's runs) when it averages at least 4
+        // positions per gap or no complement list is held, else as positions.
+        const uint32_t comp_n = D.universe - meta(D, a, 0);
+        if (R.n && (A.n != comp_n || comp_n >= 4u * (R.n + 1))) {
+            const uint64_t* rk = rank_of(D, b);
+            // Gap k: [end of run k-1, start of run k), with run -1 ending at 0
+            // and run R.n starting at the universe.
+            for (uint32_t k = first; k <= R.n; k += stride) {
+                const uint32_t gs = k ? R.e[k - 1] : 0, ge = k < R.n ? R.s[k] : D.universe;
+                if (gs >= ge) continue;
+                c += rk && ge - gs >= kRankMinBits
+                         ? (uint32_t)(rank_at(D, bm, rk, ge) - rank_at(D, bm, rk, gs))
+                         : bitmap_range_count(bm, gs, ge);
+            }
+        } else {
 #pragma unroll 4
-        for (uint32_t k = first; k < A.n; k += stride) {
-            const uint32_t x = A.p[k];
-            c += (uint32_t)(bm[x >> 6] >> (x & 63)) & 1u;
+            for (uint32_t k = first; k < A.n; k += stride) {
+                const uint32_t x = A.p[k];
+                c += (uint32_t)(bm[x >> 6] >> (x & 63)) & 1u;
+            }
         }
     }
     return c;

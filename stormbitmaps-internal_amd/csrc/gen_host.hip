@@ -6,8 +6,9 @@
 // from its OWN stream, seeded by draw r of Rng(seed), so rows generate in
 // parallel and any single row can be regenerated alone. Integer sampling only
 // where the result must not depend on libm (positions); the Zipf inverse CDF
-// and the geometric/Poisson transforms use IEEE double arithmetic on the same
-// draws.
+// and the geometric/Poisson transforms use the deterministic log/exp of
+// detmath.h on the same draws, so the device generators (gen_dev.hip) write
+// the same rows.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -16,29 +17,16 @@
 #include <thread>
 #include <vector>
 
+#include "detmath.h"
 #include "device_common.h"
 #include "internal.h"
 
 namespace stormhip {
 namespace {
 
-struct Rng {
-    uint64_t s;
-    explicit Rng(uint64_t seed) : s(seed ? seed : kGolden) {}
-    uint64_t next() { s += kGolden; return splitmix_mix(s); }
-    uint32_t below(uint32_t n) {
-        if (n == 0) return 0;
-        const uint64_t lim = UINT64_MAX - (UINT64_MAX % n);
-        uint64_t r;
-        do { r = next(); } while (r >= lim);
-        return (uint32_t)(r % n);
-    }
-    double unit() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
-};
+using Rng = DetRng;
 
-inline uint64_t row_seed(uint64_t seed, uint64_t r) {
-    return splitmix_mix((seed ? seed : kGolden) + (r + 1) * kGolden);
-}
+inline uint64_t row_seed(uint64_t seed, uint64_t r) { return det_row_seed(seed, r); }
 
 // k distinct uniform positions in [0, u), sorted: draw, sort, unique, top up.
 void distinct_uniform(Rng& g, uint32_t u, uint32_t k, std::vector<uint32_t>& out) {
@@ -63,38 +51,13 @@ void distinct_uniform(Rng& g, uint32_t u, uint32_t k, std::vector<uint32_t>& out
     }
 }
 
-uint32_t geometric(Rng& g, double mean) {   // support >= 1
-    if (mean <= 1.0) return 1;
-    double u = g.unit();
-    if (u <= 0.0) u = 1e-300;
-    const double x = std::floor(std::log(u) / std::log1p(-1.0 / mean)) + 1.0;
-    return (uint32_t)std::min(x, 4.0e9);
-}
-
-uint32_t poisson(Rng& g, double lambda) {   // inversion by sequential search
-    const double u = g.unit();
-    double p = std::exp(-lambda), c = p;
-    uint32_t k = 0;
-    while (u >= c && k < 100000) { ++k; p *= lambda / k; c += p; }
-    return k;
-}
-
-// Runs + singletons row: `nruns` runs of geometric length (mean `mean_run`)
-// at uniform starts, plus round(frac * run bits) uniform singletons.
-void run_row(Rng& g, uint32_t u, uint32_t nruns, double mean_run, double frac,
+// Runs + singletons row (det_runs_row), as sorted distinct positions.
+void run_row(Rng& g, uint32_t u, double lambda, double mean_run, double frac,
              std::vector<uint32_t>& out) {
-    std::vector<std::pair<uint32_t, uint32_t>> iv;
-    uint64_t bits = 0;
-    for (uint32_t q = 0; q < nruns; ++q) {
-        const uint32_t s = g.below(u);
-        const uint32_t len = std::min<uint64_t>(geometric(g, mean_run), (uint64_t)u - s);
-        iv.push_back({s, s + len});
-        bits += len;
-    }
-    const uint32_t extra = (uint32_t)std::llround(frac * (double)bits);
     out.clear();
-    for (auto& x : iv) for (uint32_t p = x.first; p < x.second; ++p) out.push_back(p);
-    for (uint32_t q = 0; q < extra; ++q) out.push_back(g.below(u));
+    det_runs_row(g, u, lambda, mean_run, frac, [&](uint32_t s, uint32_t e) {
+        for (uint32_t p = s; p < e; ++p) out.push_back(p);
+    });
     std::sort(out.begin(), out.end());
     out.erase(std::unique(out.begin(), out.end()), out.end());
 }
@@ -104,15 +67,10 @@ void bernoulli_row(Rng& g, uint32_t u, double d, std::vector<uint32_t>& out) {
     out.clear();
     if (d <= 0.0) return;
     if (d >= 1.0) { out.resize(u); for (uint32_t i = 0; i < u; ++i) out[i] = i; return; }
-    const double l1 = std::log1p(-d);
-    double pos = -1.0;
-    for (;;) {
-        double v = g.unit();
-        if (v <= 0.0) v = 1e-300;
-        pos += std::floor(std::log(v) / l1) + 1.0;
-        if (pos >= (double)u) break;
+    const double l1 = det_log1p(-d);
+    for (double pos = det_bernoulli_next(g, l1, -1.0); pos < (double)u;
+         pos = det_bernoulli_next(g, l1, pos))
         out.push_back((uint32_t)pos);
-    }
 }
 
 template <class F>
@@ -174,7 +132,7 @@ int storm_hip_gen_corpus(uint32_t kind, uint32_t n_rows, uint32_t universe, doub
             // Poisson(p1) runs of geometric length (mean p2) + p3 singletons.
             return gen_rows(n_rows, [&](uint32_t r, std::vector<uint32_t>& out) {
                 Rng g(row_seed(seed, r));
-                run_row(g, universe, poisson(g, p1), p2, p3, out);
+                run_row(g, universe, p1, p2, p3, out);
             }, offs, pos, total);
         case STORM_HIP_GEN_BERNOULLI:
             return gen_rows(n_rows, [&](uint32_t r, std::vector<uint32_t>& out) {

@@ -28,6 +28,10 @@ struct DevCorpus {
     const uint64_t* mk_off;   const uint32_t* mk_word; const uint32_t* mk_pos;
     const uint64_t* comp_off; const uint32_t* comp;
     const uint32_t* bm_row;   const uint64_t* bitmap;   // bitmap row index, kNoBitmap if none
+    // rank9 and zone map of a bitmap row (kNoBitmap in the row map: none):
+    // rank + rank_row[r] * rank_ld, occ + occ_row[r] * occ_ld.
+    const uint32_t* rank_row; const uint64_t* rank; uint32_t rank_ld;
+    const uint32_t* occ_row;  const uint64_t* occ;  uint32_t occ_ld;
     const uint32_t* meta;     // 8 per original row (RowMeta + has_rank, has_occ)
     const uint32_t* perm;     // sorted position -> original row id
     // Bit-sliced bitmaps of 64-row groups of the density order, four groups
@@ -132,22 +136,30 @@ __host__ __device__ inline bool bs_probe_a(const TileMeta& a, const TileMeta& b)
     return b.all_bitmap && (!a.all_bitmap || a.card <= b.card);
 }
 
+// Representations held for every row of a corpus (reps bits).
+enum : uint32_t { kRepRuns = 1, kRepEwah = 2, kRepComp = 4, kRepLists = 8 };
+
 // Cells a tile pair can run given the representations resident on the device
-// (bit per STORM_HIP_CELL_*). reps: bit 0 runs, bit 1 EWAH, bit 2 complement.
+// (bit per STORM_HIP_CELL_*): bitmaps per row (the tiles' all_bitmap), the
+// other representations per corpus (reps). C x B takes the complement as a
+// list or, from the runs, as the gaps between them.
 __host__ __device__ inline uint32_t eligible_mask(const TileMeta& a, const TileMeta& b,
                                                   uint32_t reps) {
-    const bool runs = reps & 1, ewah = reps & 2, comp = reps & 4;
+    const bool runs = reps & kRepRuns, ewah = reps & kRepEwah, comp = reps & kRepComp;
+    const bool lists = reps & kRepLists;
     const bool bb = a.all_bitmap && b.all_bitmap, b1 = a.all_bitmap || b.all_bitmap;
     uint32_t e = 0;
     if (bb) e |= 1u << STORM_HIP_CELL_BB;
-    if (b1) e |= 1u << STORM_HIP_CELL_BS;
+    if (b1 && lists) e |= 1u << STORM_HIP_CELL_BS;
     if (b1 && runs) e |= 1u << STORM_HIP_CELL_BR;
     if (b1 && ewah) e |= 1u << STORM_HIP_CELL_BW;
-    e |= 1u << STORM_HIP_CELL_SS;
-    if (runs) e |= (1u << STORM_HIP_CELL_SR) | (1u << STORM_HIP_CELL_RR);
-    if (ewah) e |= (1u << STORM_HIP_CELL_SW) | (1u << STORM_HIP_CELL_WW);
+    if (lists) e |= 1u << STORM_HIP_CELL_SS;
+    if (runs && lists) e |= 1u << STORM_HIP_CELL_SR;
+    if (runs) e |= 1u << STORM_HIP_CELL_RR;
+    if (ewah && lists) e |= 1u << STORM_HIP_CELL_SW;
+    if (ewah) e |= 1u << STORM_HIP_CELL_WW;
     if (runs && ewah) e |= 1u << STORM_HIP_CELL_RW;
-    if (comp && ((a.all_comp && b.all_bitmap) || (b.all_comp && a.all_bitmap)))
+    if ((comp || runs) && ((a.all_comp && b.all_bitmap) || (b.all_comp && a.all_bitmap)))
         e |= 1u << STORM_HIP_CELL_CB;
     return e;
 }

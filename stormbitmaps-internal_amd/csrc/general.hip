@@ -137,6 +137,8 @@ DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
     D.mk_off = c->d_mk_off; D.mk_word = c->d_mk_word; D.mk_pos = c->d_mk_pos;
     D.comp_off = c->d_comp_off; D.comp = c->d_comp;
     D.bm_row = c->d_bm_row; D.bitmap = c->d_bitmap;
+    D.rank_row = c->d_rank_row; D.rank = c->d_rank; D.rank_ld = c->rank_ld;
+    D.occ_row = c->d_occ_row; D.occ = c->d_occ; D.occ_ld = c->occ_ld;
     D.meta = c->d_meta;
     D.perm = c->d_perm;
     D.slice = c->d_slice; D.slice_of = c->d_slice_of;
@@ -156,7 +158,8 @@ DevModel dev_model(const DeviceCostModel& m) {
 }
 
 uint32_t reps_of(const storm_hip_corpus* c) {
-    return (c->has_runs ? 1u : 0u) | (c->has_ewah ? 2u : 0u) | (c->has_comp ? 4u : 0u);
+    return (c->has_runs ? kRepRuns : 0u) | (c->has_ewah ? kRepEwah : 0u) |
+           (c->has_comp ? kRepComp : 0u) | (c->has_lists ? kRepLists : 0u);
 }
 
 // (Re)build the plan for this list of corpora, tile width and outputs.

@@ -78,6 +78,10 @@ struct storm_hip_corpus {
     uint32_t n_words = 0;      // logical words per row (ceil(universe/64))
     uint32_t universe = 0;     // bits
     bool dense = false;        // contiguous corpus: every row a bitmap
+    // Representations held for every row (bitmaps are per row, below). A
+    // corpus without lists holds its rows as runs or as bitmaps only; the
+    // cells that read a list are then not eligible.
+    bool has_lists = true;
     bool has_runs = false, has_ewah = false, has_comp = false;
     // Bit-sliced bitmaps (STORM_HIP_BUILD_SLICES): for each group of 64
     // density-ordered rows that all hold bitmaps, per position p a mask whose
@@ -94,7 +98,7 @@ struct storm_hip_corpus {
     uint32_t n_rows_pad = 0;
     // General layout (arenas). Offsets are u64 element offsets, n_rows+1.
     uint64_t *d_list_off = nullptr, *d_run_off = nullptr, *d_ewah_off = nullptr;
-    uint64_t *d_bitmap_off = nullptr, *d_occ_off = nullptr, *d_comp_off = nullptr;
+    uint64_t* d_comp_off = nullptr;
     uint32_t *d_list = nullptr, *d_run_start = nullptr, *d_run_end = nullptr;
     uint64_t *d_ewah = nullptr, *d_bitmap = nullptr, *d_occ = nullptr;
     uint32_t* d_comp = nullptr;
@@ -107,6 +111,17 @@ struct storm_hip_corpus {
     uint32_t* d_bm_row = nullptr;
     uint32_t n_bitmap_rows = 0, bm_capacity = 0;
     std::vector<uint32_t> h_bm_row;
+    // The store already holds every bitmap row (generated there): finishing
+    // the corpus keeps it instead of filling it from lists or runs.
+    bool store_ready = false;
+    // rank9 (kernels/storm_repr.cpp:25-45) and zone maps (:47-59) of the
+    // bitmap rows whose build_row gates pass (meta fields 6 and 7): row r's
+    // index is d_rank + d_rank_row[r] * rank_ld (kNoBitmap: none), likewise
+    // d_occ / d_occ_row / occ_ld.
+    uint64_t* d_rank = nullptr;
+    uint32_t *d_rank_row = nullptr, *d_occ_row = nullptr;
+    uint32_t rank_ld = 0, occ_ld = 0, n_rank_rows = 0, n_occ_rows = 0;
+    uint64_t index_bytes = 0;
     // Density order (stable by cardinality) and the identity order, both
     // padded with the zero-row sentinel mapping.
     uint32_t *d_perm = nullptr, *d_ident = nullptr;
@@ -132,6 +147,10 @@ struct storm_hip_corpus {
 };
 
 namespace stormhip {
+// rank9 and zone maps of the bitmap rows whose gates pass (gen_dev.hip).
+int build_indexes(storm_hip_corpus* c, hipStream_t s);
+// Fill the bitmap rows of a corpus held as runs (gen_dev.hip).
+int fill_bitmaps_from_runs(storm_hip_corpus* c, hipStream_t s);
 // Grid launches over all upper-triangle tiles of one bitmap matrix.
 int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32_t n_words,
                    const uint32_t* rowmap, const uint32_t* perm, uint32_t n, uint32_t* tri,

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void select_kernel(
             const float c = learned_cost[t * 10 + q];
             if (best < 0 || c < bc) { bc = c; best = q; }
         }
-        p = best < 0 ? STORM_HIP_CELL_SS : best;
+        p = best < 0 ? cheapest_eligible(M, a, b, elig) : best;
         // Trust region: the analytic choice stands unless the network predicts
         // the other cell faster by more than kLearnedMargin (log2 units).
         double ca;

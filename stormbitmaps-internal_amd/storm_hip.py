@@ -25,7 +25,10 @@ ARENAS = [("list_off", np.uint64), ("list", np.uint32), ("run_off", np.uint64),
           ("run_start", np.uint32), ("run_end", np.uint32), ("ewah_off", np.uint64),
           ("ewah", np.uint64), ("mk_off", np.uint64), ("mk_word", np.uint32),
           ("mk_pos", np.uint32), ("comp_off", np.uint64), ("comp", np.uint32),
-          ("meta", np.uint32), ("bm_row", np.uint32), ("bitmap", np.uint64)]
+          ("meta", np.uint32), ("bm_row", np.uint32), ("bitmap", np.uint64),
+          ("rank_row", np.uint32), ("rank", np.uint64), ("occ_row", np.uint32),
+          ("occ", np.uint64)]
+GEN = {"zipf": 0, "runs": 1, "bernoulli": 2}
 
 
 class StormHipError(RuntimeError):
@@ -63,6 +66,7 @@ class Arena(C.Structure):
                 ("run_off", P64), ("run_start", P32), ("run_end", P32),
                 ("ewah_off", P64), ("ewah", P64),
                 ("bitmap_off", P64), ("bitmap", P64),
+                ("rank_off", P64), ("rank", P64),
                 ("occ_off", P64), ("occ", P64),
                 ("comp_off", P64), ("comp", P32),
                 ("meta", P32)]
@@ -170,6 +174,9 @@ def lib():
                                                C.POINTER(Stats), C.POINTER(u64)]),
         "storm_hip_result_copy_pairs": (i32, [vp, vp, u64, u64, vp]),
         "storm_hip_allpairs_xy": (i32, [vp, vp, vp, u32, C.POINTER(u64), vp]),
+        "storm_hip_corpus_gen": (i32, [vp, u32, u32, u32, C.c_double, C.c_double, C.c_double,
+                                       u64, u32, pvp]),
+        "storm_hip_corpus_closed_form": (i32, [vp, vp, C.POINTER(u64), vp]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -309,6 +316,36 @@ class Context:
         check(lib().storm_hip_corpus_from_stormbin(self.h, os.fsencode(path), max_rows, stride,
                                                    bitmap_policy, C.byref(h)), "from_stormbin")
         return Corpus(self, h, int(lib().storm_hip_corpus_rows(h)))
+
+    def upload_arenas(self, universe, n_rows, arenas):
+        """storm_hip_corpus_upload: arenas is a dict of numpy arrays keyed
+        list_off/list, run_off/run_start/run_end, ewah_off/ewah,
+        bitmap_off/bitmap, rank_off/rank, occ_off/occ, comp_off/comp and meta
+        (8 u32 per row); a family whose offsets are missing is absent."""
+        a = Arena()
+        a.n_rows, a.universe = n_rows, universe
+        keep = []
+        types = {"list": C.c_uint32, "run_start": C.c_uint32, "run_end": C.c_uint32,
+                 "comp": C.c_uint32, "meta": C.c_uint32}
+        for name, _ in Arena._fields_[2:]:
+            if name not in arenas or arenas[name] is None:
+                continue
+            ct = types.get(name, C.c_uint64)
+            arr = np.ascontiguousarray(arenas[name],
+                                       dtype=np.uint32 if ct is C.c_uint32 else np.uint64)
+            keep.append(arr)
+            setattr(a, name, arr.ctypes.data_as(C.POINTER(ct)))
+        h = C.c_void_p()
+        check(lib().storm_hip_corpus_upload(self.h, C.byref(a), C.byref(h)), "corpus_upload")
+        return Corpus(self, h, n_rows)
+
+    def gen_device(self, kind, n_rows, universe, p1, p2=0.0, p3=0.0, seed=1, bitmap_policy=0):
+        """storm_hip_corpus_gen: a seeded workload generated on the device in
+        its stored representation (Bernoulli bitmaps / lists, runs)."""
+        h = C.c_void_p()
+        check(lib().storm_hip_corpus_gen(self.h, GEN[kind], n_rows, universe, p1, p2, p3, seed,
+                                         bitmap_policy, C.byref(h)), "corpus_gen")
+        return Corpus(self, h, n_rows)
 
     def batch(self, corpora):
         return Batch(self, corpora)
@@ -494,6 +531,15 @@ class Corpus:
         out = np.empty(self.n, dtype=np.uint64)
         check(lib().storm_hip_result_row_sums(self.ctx.h, self.h, _ptr(out)), "row_sums")
         return out
+
+    def closed_form(self, row_sums=True):
+        """(total, row_sums): sum_b C(c_b, 2) and sum_{b in Xi} (c_b - 1) from
+        the column counts, computed on the device without any cell."""
+        t = C.c_uint64(0)
+        rs = np.zeros(self.n, dtype=np.uint64) if row_sums else None
+        check(lib().storm_hip_corpus_closed_form(self.ctx.h, self.h, C.byref(t), _ptr(rs)),
+              "closed_form")
+        return t.value, rs
 
     def last_decisions(self, n_tiles):
         out = np.empty(n_tiles, dtype=np.uint8)

@@ -104,24 +104,7 @@ def edge_rows(u):
     return [np.array(sorted({p for p in r if p < u}), dtype=np.uint32) for r in rows]
 
 
-def ref_row(pos, u):
-    R = O.reference()
-    sizes = np.zeros(9, np.uint32)
-    meta = np.zeros(6, np.uint32)
-    R.ref_build_row(P(pos), len(pos), u, P(sizes), P(meta), None, None, None, None, None, None,
-                    None)
-    nw, nrank, nocc, nruns, newah, ncomp = (int(x) for x in sizes[:6])
-    bm = np.zeros(max(nw, 1), np.uint64)
-    rank = np.zeros(max(nrank, 1), np.uint64)
-    occ = np.zeros(max(nocc, 1), np.uint64)
-    rs = np.zeros(max(nruns, 1), np.uint32)
-    re_ = np.zeros(max(nruns, 1), np.uint32)
-    ew = np.zeros(max(newah, 1), np.uint64)
-    comp = np.zeros(max(ncomp, 1), np.uint32)
-    R.ref_build_row(P(pos), len(pos), u, P(sizes), P(meta), P(bm), P(rank), P(occ), P(rs),
-                    P(re_), P(ew), P(comp))
-    return dict(sizes=sizes, meta=meta, bitmap=bm[:nw], rank=rank[:nrank], occ=occ[:nocc],
-                run_start=rs[:nruns], run_end=re_[:nruns], ewah=ew[:newah], comp=comp[:ncomp])
+ref_row = O.ref_build_row
 
 
 orc_row = O.build_row
