@@ -328,149 +328,148 @@ def build_timing(sh, ctx, offs, pos, u):
     return out
 
 
-class _Rng:
-    """splitmix64 counter stream (the project's generator convention)."""
-    G = 0x9E3779B97F4A7C15
-    M = (1 << 64) - 1
-
-    def __init__(self, seed):
-        self.s = seed & self.M
-
-    def next(self):
-        self.s = (self.s + self.G) & self.M
-        z = self.s
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.M
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.M
-        return z ^ (z >> 31)
-
-    def unit(self):
-        return (self.next() >> 11) * 2.0 ** -53
-
-
-def batch_instances(count, seed):
-    """Instance k: (n_rows, universe, density, kind, seed_k)."""
-    g = _Rng(seed)
-    out = []
-    for k in range(count):
-        n = 64 + int(g.unit() * 4033)
-        u = int(math.exp(math.log(2e5) + g.unit() * (math.log(1.3e8) - math.log(2e5))))
-        d = math.exp(math.log(1e-6) + g.unit() * (math.log(0.17) - math.log(1e-6)))
-        out.append((n, u, d, "bernoulli" if k % 2 == 0 else "runs", g.next() & 0xFFFFFFFF))
-    return out
-
-
 def batch_bench(sh, args):
-    """Many independent corpora through one set of launches per policy.
-    The full workload does not fit one GPU (about 1.2e12 list elements), so a
-    bounded sample runs: the instances, in seeded order, whose estimated
-    device size is at most --instance-mib, up to --budget-gib in total."""
+    """The batched workload in full: all 512 independent instances, each
+    generated on the device in its stored representation (bitmaps, lists or
+    runs), run in HBM-bounded batches of instances; each batch is one set of
+    launches per cell class (one captured graph) per policy. Every instance's
+    sum is checked against its column closed form (computed on the device
+    from the rows themselves, no cell involved); aggregate pairs/s is the
+    total pairs over the sum of the batches' call times; per-instance latency
+    is the time from its batch's start to its last tile."""
     count, seed, desc = BATCH_CONFIGS[args.config]
-    inst = batch_instances(count, seed)
-    cap, budget = args.instance_mib << 20, args.budget_gib << 30
-    chosen, used, skipped = [], 0, 0
-    for k, (n, u, d, kind, sk) in enumerate(inst):
-        elems = n * u * d
-        est = 4 * elems * 2 + (n * u / 8 if d >= 1 / 64 else 0)    # lists+runs, bitmaps
-        if est > cap or used + est > budget:
-            skipped += 1
-            continue
-        chosen.append(k)
-        used += est
-    progress(f"{len(chosen)} of {count} instances chosen (~{used / 2**30:.1f} GiB)")
+    inst = sh.batched_instances(count, seed)
+    if args.instances:
+        inst = inst[:args.instances]
+    budget = args.budget_gib << 30
+    batches, cur, used = [], [], 0
+    for x in inst:
+        if cur and used + x["est_bytes"] > budget:
+            batches.append(cur)
+            cur, used = [], 0
+        cur.append(x)
+        used += x["est_bytes"]
+    if cur:
+        batches.append(cur)
+    progress(f"{len(inst)} instances in {len(batches)} batches "
+             f"(~{sum(x['est_bytes'] for x in inst) / 2**30:.0f} GiB stored)")
     ctx = sh.Context(0)
-    corpora, wants, t_gen = [], [], 0.0
-    for k in chosen:
-        n, u, d, kind, sk = inst[k]
-        t0 = time.perf_counter()
-        if kind == "bernoulli":
-            offs, pos = sh.gen_corpus("bernoulli", n, u, d, 0.0, 0.0, sk)
-        else:
-            # clustered runs: Poisson(r) runs of geometric length L with r*L = d*U
-            L = max(1.0, min(2000.0, d * u / 20.0))
-            offs, pos = sh.gen_corpus("runs", n, u, max(d * u / L, 1e-3), L, 0.05, sk)
-        t_gen += time.perf_counter() - t0
-        wants.append(column_pair_sum(offs, pos, u))
-        corpora.append(ctx.from_lists(offs, pos, u, bitmap_policy=0 | 4 | 8))
-        del offs, pos
-        if len(corpora) % 32 == 0:
-            progress(f"built {len(corpora)} of {len(chosen)} corpora")
-    pairs = sum(c.n_pairs for c in corpora)
-    progress(f"built {len(corpora)} corpora, {pairs} pairs")
-    res = {}
-    batch = ctx.batch(corpora)
+    ref_model = ctx.get_cost_model()
+    cal_model = None
+    pol_names = ["analytic_reference_rates", "analytic", "learned"]
+    res = {k: {"ms": 0.0, "lat": {}, "exact": True, "cell_tiles": {}} for k in pol_names}
+    t_gen = t_check = 0.0
+    selector = None
+    n_done = 0
 
-    def timed(policy):
-        batch.allpairs(policy=policy, tile=args.tile, use_graph=True)
-        ctx.sync()
-        ts, lats, st, sums = [], [], None, None
-        for k in range(args.steps):
+    def timed(batch, name, wants):
+        policy = "learned" if name == "learned" else "analytic"
+        t1 = time.perf_counter()
+        st, sums, lat = batch.allpairs(policy=policy, out="tri", tile=args.tile, use_graph=True)
+        warm = time.perf_counter() - t1
+        progress(f"  {name}: warm-up {1000 * warm:.1f} ms, cells {st['cell_tiles']}")
+        ts, lats = [], []
+        # A policy slower than --slow-s per call is timed on its warm-up call
+        # alone (graph capture included), and says so.
+        for k in range(args.steps if warm < args.slow_s else 0):
             t1 = time.perf_counter()
-            st, sums, lat = batch.allpairs(policy=policy, tile=args.tile, use_graph=True)
+            st, sums, lat = batch.allpairs(policy=policy, out="tri", tile=args.tile,
+                                           use_graph=True)
             ts.append(time.perf_counter() - t1)
             lats.append(lat)
-            progress(f"batch {policy}: step {k} {1000 * ts[-1]:.1f} ms")
+            progress(f"  {name}: step {k} {1000 * ts[-1]:.1f} ms")
+        if not ts:
+            ts, lats = [warm], [lat]
         ms = 1000 * float(np.median(ts))
-        lat = np.median(np.stack(lats), axis=0)
-        return {"pairs_per_s": pairs / (ms / 1000), "ms": ms, "ms_device": st["ms_device"],
-                "cell_tiles": st["cell_tiles"],
-                "latency_ms_p50": float(np.percentile(lat, 50)),
-                "latency_ms_p99": float(np.percentile(lat, 99)),
-                "exact": [int(s) for s in sums] == wants}
+        return ms, np.median(np.stack(lats), axis=0), [int(x) for x in sums] == wants, st, \
+            len(ts) if warm < args.slow_s else 0
 
-    res["analytic_reference_rates"] = timed("analytic")
-    big = max(range(len(corpora)), key=lambda i: corpora[i].n_pairs)
-    progress("calibrating on the largest instance")
-    model = ctx.calibrate(corpora[big], tile=args.tile, max_tiles=args.sample_tiles, seed=7,
-                          repeats=2)
-    res["analytic"] = timed("analytic")
-    progress("sampling tiles for the selector")
-    Xs, Ys, Ms = [], [], []
-    # Instances spread over the size order (every k-th of the sorted list), so
-    # the training tiles cover the batch's range of universes and densities.
-    by_size = sorted(range(len(corpora)), key=lambda i: -corpora[i].n_pairs)
-    order = by_size[::max(1, len(by_size) // 32)][:32]
-    for i in order:
-        X, Y, M, _ = ctx.selector_sample(corpora[i], tile=args.tile,
-                                         max_tiles=max(8, args.sample_tiles // 32), seed=11 + i,
-                                         repeats=2)
-        Xs.append(X), Ys.append(Y), Ms.append(M)
-        progress(f"sampled instance {i}: {len(X)} tiles")
-    X, Y, M = np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
-    progress(f"training on {len(X)} tiles")
-    loss = ctx.selector_train(X, Y, M, epochs=300, batch=128, lr=3e-3, seed=5)
-    res["learned"] = timed("learned")
-    # The same corpora one call at a time (no batching) for comparison.
-    ts = []
-    for k in range(max(1, args.steps // 2)):
-        t1 = time.perf_counter()
+    for bi, group in enumerate(batches):
+        corpora, wants = [], []
+        t0 = time.perf_counter()
+        for x in group:
+            corpora.append(sh.gen_instance(ctx, x))
+        ctx.sync()
+        t_gen += time.perf_counter() - t0
+        t0 = time.perf_counter()
         for c in corpora:
-            c.allpairs(policy="learned", tile=args.tile, use_graph=True)
-        ts.append(time.perf_counter() - t1)
-    seq_ms = 1000 * float(np.median(ts))
+            wants.append(int(c.closed_form(row_sums=False)[0]))
+        t_check += time.perf_counter() - t0
+        dev = sum(c.device_bytes() for c in corpora)
+        progress(f"batch {bi + 1}/{len(batches)}: {len(group)} instances generated "
+                 f"({dev / 2**30:.1f} GiB on the device)")
+        batch = ctx.batch(corpora)
+        ctx.put_cost_model(ref_model)
+        out = {}
+        out["analytic_reference_rates"] = timed(batch, "analytic_reference_rates", wants)
+        if selector is None:
+            # One sampling pass over instances spread across the size order
+            # feeds both the calibrated analytic model (pooled over the
+            # sampled instances) and the learned selector.
+            t0 = time.perf_counter()
+            Xs, Ys, Ms, Is, Cs = [], [], [], [], []
+            by_size = sorted(range(len(corpora)), key=lambda i: -corpora[i].n_pairs)
+            picked = by_size[::max(1, len(by_size) // 32)][:32]
+            for q, i in enumerate(picked):
+                X, Y, M, idx = ctx.selector_sample(corpora[i], tile=args.tile,
+                                                   max_tiles=max(8, args.sample_tiles // 32),
+                                                   seed=11 + i, repeats=2)
+                Xs.append(X), Ys.append(Y), Ms.append(M), Is.append(idx)
+                Cs.append(np.full(len(X), q, np.uint32))
+            X, Y, M = np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
+            t_sample = time.perf_counter() - t0
+            progress(f"sampled {len(X)} tiles of {len(picked)} instances in {t_sample:.1f} s")
+            ctx.calibrate_corpora([corpora[i] for i in picked], args.tile, Y, M,
+                                  np.concatenate(Is), np.concatenate(Cs))
+            cal_model = ctx.get_cost_model()
+            loss = ctx.selector_train(X, Y, M, epochs=300, batch=128, lr=3e-3, seed=5)
+            selector = {"sampled_tiles": int(len(X)), "sampled_instances": len(picked),
+                        "train_loss": loss, "sample_s": t_sample}
+        ctx.put_cost_model(cal_model)
+        out["analytic"] = timed(batch, "analytic", wants)
+        out["learned"] = timed(batch, "learned", wants)
+        for name, (ms, lat, ok, st, n_timed) in out.items():
+            r = res[name]
+            r.setdefault("timed_calls", []).append(n_timed)
+            r["ms"] += ms
+            r["exact"] = r["exact"] and ok
+            for x, l in zip(group, lat):
+                r["lat"][x["k"]] = float(l)
+            for cell, t in st["cell_tiles"].items():
+                r["cell_tiles"][cell] = r["cell_tiles"].get(cell, 0) + t
+        n_done += len(group)
+        batch.free()
+        for c in corpora:
+            c.free()
+    pairs = sum(x["pairs"] for x in inst)
+    policies = {}
+    for name, r in res.items():
+        lat = np.array(list(r["lat"].values()))
+        policies[name] = {"pairs_per_s": pairs / (r["ms"] / 1000), "ms": r["ms"],
+                          "exact_all_instances": r["exact"],
+                          "timed_calls_per_batch": r["timed_calls"],
+                          "latency_ms_p50": float(np.percentile(lat, 50)),
+                          "latency_ms_p99": float(np.percentile(lat, 99)),
+                          "cell_tiles": r["cell_tiles"]}
     out = {
         "metric": METRIC,
-        "value": res["learned"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": 1, "ms_per_step": res["learned"]["ms"],
+        "value": policies["learned"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": policies["learned"]["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
-        "data": "synthetic (seeded host generator)",
-        "config": {"workload": desc + f" (bounded sample: {len(chosen)} instances of at most "
-                                      f"{args.instance_mib} MiB estimated, {args.budget_gib} GiB "
-                                      f"total; {skipped} larger instances not run)",
-                   "instances_run": len(chosen), "instances_skipped": skipped,
-                   "pairs": pairs, "tile": args.tile},
-        "exact_all": all(v["exact"] for v in res.values()),
-        "policies": res,
-        "learned_over_analytic": res["learned"]["pairs_per_s"] / res["analytic"]["pairs_per_s"],
+        "data": "synthetic (seeded device generators)",
+        "config": {"workload": desc, "instances_run": n_done, "instances_total": count,
+                   "batches": len(batches), "pairs": pairs, "tile": args.tile,
+                   "batch_budget_gib": args.budget_gib,
+                   "instances_with_bitmaps_added": sum(1 for x in inst if x["bitmaps_added"]),
+                   "stored_gib_estimate": sum(x["est_bytes"] for x in inst) / 2**30},
+        "exact_all": all(p["exact_all_instances"] for p in policies.values()),
+        "policies": policies,
+        "learned_over_analytic": policies["learned"]["pairs_per_s"] / policies["analytic"]["pairs_per_s"],
         "learned_over_analytic_reference_rates":
-            res["learned"]["pairs_per_s"] / res["analytic_reference_rates"]["pairs_per_s"],
-        "sequential_learned_ms": seq_ms,
-        "batched_over_sequential": seq_ms / res["learned"]["ms"],
-        "selector": {"sampled_tiles": int(len(X)), "train_loss": loss},
-        "calibrated_model": model, "setup_s": {"generate": t_gen},
+            policies["learned"]["pairs_per_s"] / policies["analytic_reference_rates"]["pairs_per_s"],
+        "selector": selector, "setup_s": {"generate_on_device": t_gen, "closed_forms": t_check},
+        "all_bitmap": "not run: every instance as bitmaps is about 2.7 TB",
     }
-    batch.free()
-    for c in corpora:
-        c.free()
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -608,10 +607,10 @@ def main():
     ap.add_argument("--threshold", default="0.01,0.1,0.5",
                     help="sparse configs: Jaccard thresholds to time in thresholded mode "
                          "(comma list, empty for none)")
-    ap.add_argument("--instance-mib", type=int, default=256,
-                    help="batch configs: largest estimated instance size run")
-    ap.add_argument("--budget-gib", type=int, default=16,
-                    help="batch configs: total estimated size of the instances run")
+    ap.add_argument("--budget-gib", type=int, default=150,
+                    help="batch configs: estimated stored size of one batch of instances")
+    ap.add_argument("--instances", type=int, default=0,
+                    help="batch configs: the first N instances only (0: all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "

@@ -429,6 +429,15 @@ int storm_hip_calibrate(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
 int storm_hip_calibrate_samples(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
                                 uint32_t n, const float* log2_ns, const uint8_t* mask,
                                 const uint32_t* tile_index, storm_hip_cost_model* out);
+/* The same fit pooled over samples of several corpora (e.g. the instances of
+ * a batch): sample k was drawn by storm_hip_selector_sample on
+ * corpora[corpus_index[k]] at this tile width; each cell's rate is the median
+ * over all its samples. */
+int storm_hip_calibrate_corpora(storm_hip_ctx* ctx, storm_hip_corpus* const* corpora,
+                                uint32_t n_corpora, uint32_t tile, uint32_t n,
+                                const float* log2_ns, const uint8_t* mask,
+                                const uint32_t* tile_index, const uint32_t* corpus_index,
+                                storm_hip_cost_model* out);
 /* Per-tile decisions of the last selecting call: `cells[t]` for tile pair t
  * in row-major upper-triangle tile order. */
 int storm_hip_last_decisions(storm_hip_ctx* ctx, const storm_hip_corpus* c,

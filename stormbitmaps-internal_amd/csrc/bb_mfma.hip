@@ -328,8 +328,15 @@ __global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
                     emit_pair(Ds[e.inst], D.perm[i], D.perm[j], buf[k]);
                 }
             } else {
-                part = bb_mfma_tile<FP4, SW, NW>(sm, D.bitmap, D.ld, 0, ns, D.rowmap, D.perm, D.n,
-                                                 e.bi, e.bj, D.tri, false);
+                // K slice `slice` of ks (sub 0: the whole tile); slices add
+                // into the triangle the caller zeroed.
+                const uint32_t ks = max(1u, e.sub >> 16), slice = e.sub & 0xFFFFu;
+                const uint32_t per = (ns + ks - 1) / ks;
+                const uint32_t s0 = min(slice * per, ns);
+                const uint32_t cnt = min(per, ns - s0);
+                part = cnt ? bb_mfma_tile<FP4, SW, NW>(sm, D.bitmap, D.ld, s0, cnt, D.rowmap,
+                                                       D.perm, D.n, e.bi, e.bj, D.tri, ks > 1)
+                           : 0;
             }
             part = wave_sum_u64(part);
             if ((threadIdx.x & 63) == 0 && part) atomicAdd(D.sum, (unsigned long long)part);

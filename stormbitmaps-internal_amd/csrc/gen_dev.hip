@@ -364,28 +364,58 @@ __global__ void col_from_runs_kernel(const uint32_t* __restrict__ rs,
     }
 }
 
-// Prefix sum of the difference array, in place, one workgroup walking
-// chunks: col[b] = sum_{x <= b} diff[x].
-__global__ __launch_bounds__(256) void diff_prefix_kernel(int* __restrict__ diff, uint32_t n) {
+// Prefix sums over a u32 / i32 array in three phases: per-chunk sums, a scan
+// of the chunk sums (one workgroup), then each chunk rescanned from its
+// offset. kChunk elements per workgroup (16 per thread).
+constexpr uint32_t kScanChunk = 256 * 16;
+
+template <class T>
+__global__ __launch_bounds__(256) void chunk_sums_kernel(const T* __restrict__ x, uint32_t n,
+                                                         int64_t* __restrict__ sums) {
+    const uint64_t c0 = (uint64_t)blockIdx.x * kScanChunk;
+    int64_t v = 0;
+    for (uint32_t k = 0; k < 16; ++k) {
+        const uint64_t b = c0 + threadIdx.x * 16 + k;
+        if (b < n) v += (int64_t)x[b];
+    }
+    const uint64_t t = block_sum_u64((uint64_t)v);
+    if (threadIdx.x == 0) sums[blockIdx.x] = (int64_t)t;
+}
+
+__global__ __launch_bounds__(256) void scan_sums_kernel(int64_t* __restrict__ sums, uint32_t m) {
     uint64_t carry = 0;
-    for (uint32_t b0 = 0; b0 < n; b0 += blockDim.x * 16) {
-        int64_t v[16];
-        int64_t mine = 0;
-        for (int k = 0; k < 16; ++k) {
-            const uint32_t b = b0 + threadIdx.x * 16 + k;
-            v[k] = b < n ? diff[b] : 0;
-            mine += v[k];
-        }
+    for (uint32_t b0 = 0; b0 < m; b0 += blockDim.x) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint64_t v = b < m ? (uint64_t)sums[b] : 0;
         uint64_t total;
-        const uint64_t before = block_scan_u64((uint64_t)mine, &total);
-        int64_t acc = (int64_t)(carry + before);
-        for (int k = 0; k < 16; ++k) {
-            const uint32_t b = b0 + threadIdx.x * 16 + k;
-            acc += v[k];
-            if (b < n) diff[b] = (int)acc;
-        }
+        const uint64_t before = block_scan_u64(v, &total);
+        if (b < m) sums[b] = (int64_t)(carry + before);
         carry += total;
-        __syncthreads();
+    }
+}
+
+// Inclusive prefix in place (out == x as int), or exclusive into out64 (with
+// out64[n] = total) when out64 is set.
+template <class T>
+__global__ __launch_bounds__(256) void chunk_scan_kernel(T* __restrict__ x, uint32_t n,
+                                                         const int64_t* __restrict__ sums,
+                                                         unsigned long long* __restrict__ out64) {
+    const uint64_t c0 = (uint64_t)blockIdx.x * kScanChunk;
+    int64_t v[16], mine = 0;
+    for (uint32_t k = 0; k < 16; ++k) {
+        const uint64_t b = c0 + threadIdx.x * 16 + k;
+        v[k] = b < n ? (int64_t)x[b] : 0;
+        mine += v[k];
+    }
+    uint64_t total;
+    int64_t acc = sums[blockIdx.x] + (int64_t)block_scan_u64((uint64_t)mine, &total);
+    for (uint32_t k = 0; k < 16; ++k) {
+        const uint64_t b = c0 + threadIdx.x * 16 + k;
+        if (b >= n) break;
+        if (out64) out64[b] = (unsigned long long)acc;
+        acc += v[k];
+        if (!out64) x[b] = (T)acc;
+        if (out64 && b + 1 == n) out64[n] = (unsigned long long)acc;
     }
 }
 
@@ -460,29 +490,6 @@ __global__ __launch_bounds__(256) void row_sums_kernel(DevCorpus D, int src,
     if (threadIdx.x == 0) out[r] = s;
 }
 
-// Exclusive prefix over the column counts (u64), one workgroup.
-__global__ __launch_bounds__(256) void col_prefix_kernel(const uint32_t* __restrict__ col,
-                                                         uint32_t n,
-                                                         unsigned long long* __restrict__ pre) {
-    uint64_t carry = 0;
-    for (uint32_t b0 = 0; b0 < n; b0 += blockDim.x * 16) {
-        uint64_t mine = 0;
-        for (int k = 0; k < 16; ++k) {
-            const uint32_t b = b0 + threadIdx.x * 16 + k;
-            mine += b < n ? col[b] : 0;
-        }
-        uint64_t total;
-        uint64_t acc = carry + block_scan_u64(mine, &total);
-        for (int k = 0; k < 16; ++k) {
-            const uint32_t b = b0 + threadIdx.x * 16 + k;
-            if (b < n) { pre[b] = acc; acc += col[b]; }
-        }
-        carry += total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) pre[n] = carry;
-}
-
 struct Dev {
     void* p = nullptr;
     ~Dev() { if (p) (void)hipFree(p); }
@@ -490,6 +497,23 @@ struct Dev {
 
 uint32_t grid_for(uint64_t n, uint32_t block = 256, uint32_t cap = 65535u * 4) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + block - 1) / block, cap));
+}
+
+// In-place inclusive prefix of x[0, n) (out64 null), or the exclusive prefix
+// of x into out64[0, n] (out64[n] the total).
+template <class T>
+int prefix_scan(hipStream_t s, T* x, uint32_t n, unsigned long long* out64) {
+    const uint32_t m = std::max<uint32_t>(1, (uint32_t)(((uint64_t)n + kScanChunk - 1) / kScanChunk));
+    Dev sums;
+    SH_HIP(hipMalloc(&sums.p, (size_t)m * 8));
+    int64_t* d = (int64_t*)sums.p;
+    hipLaunchKernelGGL(chunk_sums_kernel<T>, dim3(m), dim3(256), 0, s, x, n, d);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, s, d, m);
+    hipLaunchKernelGGL(chunk_scan_kernel<T>, dim3(m), dim3(256), 0, s, x, n, (const int64_t*)d,
+                       out64);
+    SH_HIP(hipGetLastError());
+    SH_HIP(hipStreamSynchronize(s));   // the chunk sums are freed on return
+    return STORM_HIP_OK;
 }
 
 // Scatter runs into bitmap slots (rows whose bm_row is set).
@@ -858,7 +882,7 @@ int storm_hip_corpus_closed_form(storm_hip_ctx* ctx, const storm_hip_corpus* c, 
             hipLaunchKernelGGL(col_from_runs_kernel, dim3(grid_for(run_total, 256, 8192)),
                                dim3(256), 0, s, c->d_run_start, c->d_run_end, run_total,
                                (int*)col.p);
-        hipLaunchKernelGGL(diff_prefix_kernel, dim3(1), dim3(256), 0, s, (int*)col.p, U);
+        SH_TRY(prefix_scan<int>(s, (int*)col.p, U, nullptr));
     } else {
         if (!c->bitmaps_complete) {
             set_error("closed form: rows without a list, runs or bitmap");
@@ -875,8 +899,7 @@ int storm_hip_corpus_closed_form(storm_hip_ctx* ctx, const storm_hip_corpus* c, 
         SH_HIP(hipMalloc(&rs.p, (size_t)n * 8));
         if (src == 1) {
             SH_HIP(hipMalloc(&pre.p, ((uint64_t)U + 1) * 8));
-            hipLaunchKernelGGL(col_prefix_kernel, dim3(1), dim3(256), 0, s, (const uint32_t*)col.p,
-                               U, (unsigned long long*)pre.p);
+            SH_TRY(prefix_scan<uint32_t>(s, (uint32_t*)col.p, U, (unsigned long long*)pre.p));
         }
         hipLaunchKernelGGL(row_sums_kernel, dim3(n), dim3(256), 0, s, D, src,
                            (const uint32_t*)col.p, (const unsigned long long*)pre.p,

@@ -46,6 +46,9 @@ struct DevCorpus {
     unsigned long long* sum;          // its sum
     unsigned long long* finish;       // wall clock of its last finished tile (max)
     uint32_t tile;                    // rows per tile
+    // 256-row B x B tiles are queued as this many K slices (1: whole tiles);
+    // the slices add into the corpus's triangle, zeroed before the call.
+    uint32_t bb_ksplit;
     // Thresholded calls (hits non-null, tri null): pairs whose Jaccard
     // |A n B| / |A u B| is at least thr are appended to hits (n_hits counts
     // every such pair, including those past hit_cap).
@@ -97,6 +100,17 @@ __host__ __device__ inline uint32_t make_sub(uint32_t rows, uint32_t bi_blk, uin
 
 // Rows per block when a tile is split for the per-pair cells.
 constexpr uint32_t kSplitRows = 64;
+
+// B x B items: sub = slice | (k_split << 16) names K slice `slice` of k_split
+// (sub 0: the whole tile). A 256-row tile of a corpus with n_words words is
+// cut into about n_words / kBBSliceWords slices (at most kBBMaxSlices), so
+// that a wide universe does not leave one workgroup per tile running for
+// seconds while the rest of the chip idles.
+constexpr uint32_t kBBSliceWords = 32768, kBBMaxSlices = 64;
+__host__ __device__ inline uint32_t bb_slices_for(uint32_t n_words) {
+    const uint32_t k = (n_words + kBBSliceWords - 1) / kBBSliceWords;
+    return k < 1 ? 1 : (k > kBBMaxSlices ? kBBMaxSlices : k);
+}
 
 // A block index of kWholeSide in a sub means "the whole side of the tile":
 // B x S items are stripes, kSplitRows rows of the probed (list) side against
@@ -202,14 +216,20 @@ __host__ __device__ double predict(const DevModel& m, int p, const TileMeta& a, 
 __host__ __device__ uint32_t learned_candidates(const DevModel& m, const TileMeta& a,
                                                 const TileMeta& b, uint32_t elig);
 
+// B x B tiles of corpora whose counts can exceed fp4's exact range (2^18
+// words or more) go to a queue of their own, run by the int8 matrix-core
+// kernel, so the other corpora of a batch keep the fp4 one.
+constexpr int kQueueBBWide = kNumCells;
+constexpr int kNumQueues = kNumCells + 1;
+
 // Counters written by the selection and cell kernels.
 struct DevCounters {
     unsigned long long cell_pairs[kNumCells];
     unsigned long long cell_tiles[kNumCells];
     unsigned long long skipped;
     unsigned long long fallback_tiles;
-    unsigned int qcount[kNumCells];   // tiles queued per cell
-    unsigned int qhead[kNumCells];    // next tile to take, per cell
+    unsigned int qcount[kNumCells + 1];   // items queued per cell (+ wide B x B)
+    unsigned int qhead[kNumCells + 1];    // next item to take, per queue
 };
 
 }  // namespace stormhip

@@ -25,7 +25,8 @@ int launch_tile_meta(hipStream_t s, const DevCorpus& D, uint32_t tile, uint32_t 
 int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel& M, uint32_t reps,
                   uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
-                  uint32_t inst, uint32_t tile, double thr, DevCounters* ctr);
+                  uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
+                  bool bb_wide, DevCounters* ctr);
 int launch_compose_rowmap(hipStream_t s, const uint32_t* order, const uint32_t* bm_row,
                           uint32_t npad, uint32_t* rowmap);
 int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
@@ -109,7 +110,7 @@ struct Plan {
     uint64_t totalP = 0;
     uint64_t qstride = 0;                   // queue slots per cell class
     DevCorpus* d_corpora = nullptr;
-    QItem* queue = nullptr;                 // kNumCells x qstride
+    QItem* queue = nullptr;                 // kNumQueues x qstride
     DevCounters* ctr = nullptr;
     unsigned long long* sums = nullptr;     // one per corpus
     unsigned long long* finish = nullptr;   // one per corpus, then the start stamp
@@ -144,6 +145,9 @@ DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
     D.slice = c->d_slice; D.slice_of = c->d_slice_of;
     D.tm = w->tm; D.rowmap = w->rowmap; D.tile = w->tile;
     D.tri = tri; D.sum = sum; D.finish = finish;
+    // K slices for wide 256-row B x B tiles (not in thresholded calls, whose
+    // B x B tiles test each pair from the whole count).
+    D.bb_ksplit = (w->tile == 256 && !th.hits) ? bb_slices_for(c->n_words) : 1;
     return D;
 }
 
@@ -194,11 +198,14 @@ int build_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
         p->totalP += w->P;
     }
     const uint64_t P1 = p->totalP ? p->totalP : 1;
-    // A tile of a per-pair cell may be queued as (tile / kSplitRows)^2 blocks.
-    const uint64_t split = tile > kSplitRows ? (uint64_t)(tile / kSplitRows) * (tile / kSplitRows) : 1;
+    // A tile of a per-pair cell may be queued as (tile / kSplitRows)^2 blocks,
+    // a B x B tile as up to kBBMaxSlices K slices.
+    uint64_t split = tile > kSplitRows ? (uint64_t)(tile / kSplitRows) * (tile / kSplitRows) : 1;
+    for (storm_hip_corpus* c : cs)
+        if (tile == 256 && !th.hits) split = std::max<uint64_t>(split, bb_slices_for(c->n_words));
     p->qstride = P1 * split;
     SH_HIP(hipMalloc(&p->d_corpora, sizeof(DevCorpus) * std::max<size_t>(n, 1)));
-    SH_HIP(hipMalloc(&p->queue, sizeof(QItem) * p->qstride * kNumCells));
+    SH_HIP(hipMalloc(&p->queue, sizeof(QItem) * p->qstride * kNumQueues));
     SH_HIP(hipMalloc(&p->ctr, sizeof(DevCounters)));
     SH_HIP(hipMalloc(&p->sums, 8 * std::max<size_t>(n, 1)));
     SH_HIP(hipMalloc(&p->finish, 8 * (n + 1)));
@@ -234,19 +241,29 @@ int ensure_bitmap_rowmap(storm_hip_corpus* c, Work** out) {
 }
 
 // Zero fill as a kernel (graph replays must not depend on memset nodes).
-__global__ void zero_kernel(uint32_t* p, uint32_t n_words) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n_words) p[k] = 0;
+__global__ void zero_kernel(uint32_t* p, uint64_t n_words) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_words;
+         k += (uint64_t)gridDim.x * blockDim.x)
+        p[k] = 0;
 }
 __global__ void stamp_kernel(unsigned long long* out) { *out = wall_clock64(); }
 
 int zero_words(hipStream_t s, void* p, uint64_t bytes) {
-    const uint32_t n = (uint32_t)(bytes / 4);
+    const uint64_t n = bytes / 4;
     if (!n) return STORM_HIP_OK;
-    hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
-                       reinterpret_cast<uint32_t*>(p), n);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 1u << 16);
+    hipLaunchKernelGGL(zero_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<uint32_t*>(p), n);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
+}
+
+bool thresholded_plan(const Plan* p) { return p->th.hits != nullptr; }
+
+// A corpus whose 256-row B x B tiles go to the int8 queue: counts can reach
+// 2^24 (2^18 words or more). Thresholded calls and explicit int8 use one
+// queue for every corpus.
+bool bb_wide(const storm_hip_corpus* c, const storm_hip_options& o, uint32_t tile, bool thr) {
+    return tile == 256 && !thr && o.bb_kernel != STORM_HIP_BB_MFMA && c->n_words >= (1u << 18);
 }
 
 // The kernel sequence of one selecting call on the context's stream.
@@ -274,21 +291,29 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
                                   w->learned_ok, &have));
             if (have) { lc = w->learned_cost; lok = w->learned_ok; }
         }
+        // Corpora whose B x B tiles run as K slices add into a zeroed triangle.
+        const uint32_t ks = p->h_corpora[k].bb_ksplit;
+        if (ks > 1 && p->h_corpora[k].tri)
+            SH_TRY(zero_words(s, p->h_corpora[k].tri, (uint64_t)c->n_rows * (c->n_rows - 1) / 2 * 4));
         SH_TRY(launch_select(s, w->tm, w->T, M, reps_of(c), o.policy, o.fixed_cell, lc, lok,
                              w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
-                             p->th.thr, p->ctr));
+                             p->th.thr, ks, bb_wide(c, o, p->tile, thresholded_plan(p)), p->ctr));
         SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
         // Only corpora with bitmap rows can queue B x B tiles.
         if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
     }
     // fp4 with f32 accumulation is exact below 2^18 words (counts < 2^24);
-    // 256-row B x B tiles run on the matrix cores, fp4 unless int8 is asked for.
-    const bool fp4 = max_words < (1u << 18) && o.bb_kernel != STORM_HIP_BB_MFMA;
+    // 256-row B x B tiles run on the matrix cores, fp4 unless int8 is asked
+    // for, and the tiles of wider corpora in the int8 queue (in thresholded
+    // calls every corpus takes the one format the widest allows).
+    const bool thr_call = thresholded_plan(p);
+    const bool fp4 = o.bb_kernel != STORM_HIP_BB_MFMA && (!thr_call || max_words < (1u << 18));
     if (p->tile == 256)
         for (storm_hip_corpus* c : p->cs)
-            if (c->n_bitmap_rows) SH_TRY(bb_mfma_check(fp4, ctx->mfma.stage_words, c->n_words, c->ld));
+            if (c->n_bitmap_rows)
+                SH_TRY(bb_mfma_check(fp4 && !bb_wide(c, o, p->tile, thr_call),
+                                     ctx->mfma.stage_words, c->n_words, c->ld));
     if (timed) SH_HIP(hipEventRecord(ctx->ev[5], s));
-    const uint64_t P1 = p->totalP ? p->totalP : 1;
     // Dispatch on three streams so the classes overlap: B x B first on side
     // stream 0 (its tiles are the longest, and one launch holds at most one
     // workgroup per CU), the cell classes alternating between the main stream
@@ -300,8 +325,9 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     hipStream_t sb = ctx->aux[0];
     if (timed) (void)hipEventRecord(ctx->ev[2], sb);
     const QItem* qbb = p->queue + (uint64_t)STORM_HIP_CELL_BB * p->qstride;
+    // B x B items: whole tiles or K slices (at most qstride of them).
     const uint32_t gbb = (uint32_t)std::min<uint64_t>(
-        P1, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
+        p->qstride, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
     const bool thresholded = p->th.hits != nullptr;
     if (p->tile == 256 && thresholded)
         rc = launch_bb_mfma_queue(fp4, ctx->mfma, sb, p->d_corpora, qbb,
@@ -317,6 +343,12 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
                                   &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr, thresholded);
     if (timed) (void)hipEventRecord(ctx->ev[3], sb);
+    // Wide corpora's B x B tiles (int8) on side stream 1, ahead of the joins.
+    if (rc == STORM_HIP_OK && p->tile == 256 && !thresholded)
+        rc = launch_bb_mfma_queue(false, ctx->mfma, ctx->aux[1], p->d_corpora,
+                                  p->queue + (uint64_t)kQueueBBWide * p->qstride,
+                                  &p->ctr->qcount[kQueueBBWide], &p->ctr->qhead[kQueueBBWide],
+                                  gbb, nullptr);
     // Persistent grids: as many workgroups as fit on the chip at once. The
     // classes that usually carry the most work (B x S stripes, the R x R and
     // S x S joins) start at once on different streams: B x S first on the
@@ -410,7 +442,7 @@ int run_plan(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, storm_hip_
     st->skipped = h.skipped;
     st->fallback_tiles = h.fallback_tiles;
     st->decisions = decisions;
-    st->bb_launches = h.qcount[STORM_HIP_CELL_BB] ? 1 : 0;
+    st->bb_launches = (h.qcount[STORM_HIP_CELL_BB] ? 1 : 0) + (h.qcount[kQueueBBWide] ? 1 : 0);
     if (!o.use_graph) {
         float ms = 0;
         hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
@@ -885,32 +917,63 @@ int storm_hip_calibrate(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
 int storm_hip_calibrate_samples(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
                                 uint32_t n, const float* log2_ns, const uint8_t* mask,
                                 const uint32_t* tile_index, storm_hip_cost_model* out) {
-    if (!ctx || !c || c->ctx != ctx || c->dense) return STORM_HIP_ERR_INVALID;
-    if (n && (!log2_ns || !mask || !tile_index)) return STORM_HIP_ERR_NULL_VALUES;
+    if (!c) return STORM_HIP_ERR_INVALID;
+    std::vector<uint32_t> which(n, 0);
+    return storm_hip_calibrate_corpora(ctx, &c, 1, tile, n, log2_ns, mask, tile_index,
+                                       which.data(), out);
+}
+
+int storm_hip_calibrate_corpora(storm_hip_ctx* ctx, storm_hip_corpus* const* corpora,
+                                uint32_t n_corpora, uint32_t tile, uint32_t n,
+                                const float* log2_ns, const uint8_t* mask,
+                                const uint32_t* tile_index, const uint32_t* corpus_index,
+                                storm_hip_cost_model* out) {
+    if (!ctx || !corpora || n_corpora == 0) return STORM_HIP_ERR_INVALID;
+    for (uint32_t q = 0; q < n_corpora; ++q)
+        if (!corpora[q] || corpora[q]->ctx != ctx || corpora[q]->dense) return STORM_HIP_ERR_INVALID;
+    if (n && (!log2_ns || !mask || !tile_index || !corpus_index)) return STORM_HIP_ERR_NULL_VALUES;
     SH_TRY(check_tile(tile));
     if (n == 0) {
         set_error("calibrate: no timed tile pair");
         return STORM_HIP_ERR_INVALID;
     }
     SH_HIP(hipSetDevice(ctx->device));
-    // This corpus's tile metadata at this tile width.
-    Plan* plan = static_cast<Plan*>(c->plan);
-    const int prc = ensure_plan(plan, {c}, {nullptr}, tile);
-    c->plan = plan;
-    SH_TRY(prc);
-    const Work* w = plan->works[0];
-    SH_TRY(launch_tile_meta(ctx->stream, plan->h_corpora[0], w->tile, w->T, w->tm));
-    std::vector<TileMeta> tm(w->T);
-    SH_HIP(hipMemcpyAsync(tm.data(), w->tm, tm.size() * sizeof(TileMeta), hipMemcpyDeviceToHost,
-                          ctx->stream));
+    // Each corpus's tile metadata at this tile width.
+    std::vector<std::vector<TileMeta>> tms(n_corpora);
+    std::vector<uint64_t> Ps(n_corpora, 0);
+    std::vector<uint32_t> Ts(n_corpora, 0);
+    std::vector<bool> used(n_corpora, false);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (corpus_index[k] >= n_corpora) {
+            set_error("calibrate: corpus index %u out of range (%u corpora)", corpus_index[k],
+                      n_corpora);
+            return STORM_HIP_ERR_RANGE;
+        }
+        used[corpus_index[k]] = true;
+    }
+    for (uint32_t q = 0; q < n_corpora; ++q) {
+        if (!used[q]) continue;
+        storm_hip_corpus* c = corpora[q];
+        Plan* plan = static_cast<Plan*>(c->plan);
+        const int prc = ensure_plan(plan, {c}, {nullptr}, tile);
+        c->plan = plan;
+        SH_TRY(prc);
+        const Work* w = plan->works[0];
+        SH_TRY(launch_tile_meta(ctx->stream, plan->h_corpora[0], w->tile, w->T, w->tm));
+        tms[q].resize(w->T);
+        SH_HIP(hipMemcpyAsync(tms[q].data(), w->tm, tms[q].size() * sizeof(TileMeta),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        Ps[q] = w->P;
+        Ts[q] = w->T;
+    }
     SH_HIP(hipStreamSynchronize(ctx->stream));
     const float* Y = log2_ns;
     const uint8_t* Mk = mask;
     const uint32_t* idx = tile_index;
     for (uint32_t k = 0; k < n; ++k)
-        if (idx[k] >= w->P) {
+        if (idx[k] >= Ps[corpus_index[k]]) {
             set_error("calibrate: tile index %u out of range (%llu tile pairs)", idx[k],
-                      (unsigned long long)w->P);
+                      (unsigned long long)Ps[corpus_index[k]]);
             return STORM_HIP_ERR_RANGE;
         }
     DeviceCostModel m = ctx->model;
@@ -919,8 +982,9 @@ int storm_hip_calibrate_samples(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_
         std::vector<double> r;
         for (uint32_t k = 0; k < n; ++k) {
             if (!Mk[(size_t)k * 10 + p]) continue;
+            const std::vector<TileMeta>& tm = tms[corpus_index[k]];
             uint32_t bi, bj;
-            tri_tile(idx[k], w->T, bi, bj);
+            tri_tile(idx[k], Ts[corpus_index[k]], bi, bj);
             const double na = tm[bi].rows, nb = tm[bj].rows;
             const double pairs = bi == bj ? na * (na - 1) / 2 : na * nb;
             const double units = std::max(1.0, host_work_units(p, tm[bi], tm[bj], m.br_margin));

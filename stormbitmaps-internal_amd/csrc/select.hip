@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void select_kernel(
     uint32_t policy, uint32_t fixed_cell, const float* __restrict__ learned_cost,
     const uint8_t* __restrict__ learned_ok, uint8_t* __restrict__ decisions,
     QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, bool split, double thr,
-    DevCounters* __restrict__ ctr) {
+    uint32_t bb_ksplit, bool bb_wide, DevCounters* __restrict__ ctr) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P) return;
     uint32_t bi, bj;
@@ -258,6 +258,13 @@ __global__ __launch_bounds__(256) void select_kernel(
             for (uint32_t y = bi == bj ? x : 0; y < nj; ++y)
                 queue[(uint64_t)p * qstride + slot++] =
                     QItem{inst, bi, bj, make_sub(kSplitRows, x, y)};
+    } else if (p == STORM_HIP_CELL_BB) {
+        const uint32_t q = bb_wide ? kQueueBBWide : STORM_HIP_CELL_BB;
+        const uint32_t ks = bb_ksplit > 1 ? bb_ksplit : 1;
+        uint32_t slot = atomicAdd(&ctr->qcount[q], ks);
+        for (uint32_t x = 0; x < ks; ++x)
+            queue[(uint64_t)q * qstride + slot++] =
+                QItem{inst, bi, bj, ks > 1 ? (x | (ks << 16)) : 0u};
     } else {
         const uint32_t slot = atomicAdd(&ctr->qcount[p], 1u);
         queue[(uint64_t)p * qstride + slot] = QItem{inst, bi, bj, 0};
@@ -297,12 +304,13 @@ int launch_tile_meta(hipStream_t s, const DevCorpus& D, uint32_t tile, uint32_t 
 int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel& M, uint32_t reps,
                   uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
-                  uint32_t inst, uint32_t tile, double thr, DevCounters* ctr) {
+                  uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
+                  bool bb_wide, DevCounters* ctr) {
     const uint64_t P = (uint64_t)T * (T + 1) / 2;
     if (P == 0) return STORM_HIP_OK;
     hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, s, tm, T, P,
                        M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions, queue,
-                       qstride, inst, tile > kSplitRows, thr, ctr);
+                       qstride, inst, tile > kSplitRows, thr, bb_ksplit, bb_wide, ctr);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

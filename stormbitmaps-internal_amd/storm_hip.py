@@ -177,6 +177,8 @@ def lib():
         "storm_hip_corpus_gen": (i32, [vp, u32, u32, u32, C.c_double, C.c_double, C.c_double,
                                        u64, u32, pvp]),
         "storm_hip_corpus_closed_form": (i32, [vp, vp, C.POINTER(u64), vp]),
+        "storm_hip_calibrate_corpora": (i32, [vp, vp, u32, u32, u32, vp, vp, vp, vp,
+                                              C.POINTER(CostModel)]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -430,6 +432,20 @@ class Context:
               "calibrate_samples")
         return m.as_dict()
 
+    def calibrate_corpora(self, corpora, tile, Y, M, idx, cidx):
+        """Fit the analytic rates from samples of several corpora (sample k
+        from corpora[cidx[k]])."""
+        Y = np.ascontiguousarray(Y, np.float32)
+        M = np.ascontiguousarray(M, np.uint8)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        cidx = np.ascontiguousarray(cidx, np.uint32)
+        hs = (C.c_void_p * len(corpora))(*[c.h for c in corpora])
+        m = CostModel()
+        check(lib().storm_hip_calibrate_corpora(self.h, hs, len(corpora), tile, len(idx), _ptr(Y),
+                                                _ptr(M), _ptr(idx), _ptr(cidx), C.byref(m)),
+              "calibrate_corpora")
+        return m.as_dict()
+
     def set_cost_model(self, rates, ns_fixed=2.0, ns_per_occ_word=0.30, hot=0.0, cold=0.0,
                        bb_zonemap=1.0):
         m = CostModel()
@@ -443,6 +459,10 @@ class Context:
         m = CostModel()
         check(lib().storm_hip_get_cost_model(self.h, C.byref(m)), "get_cost_model")
         return m
+
+    def put_cost_model(self, m):
+        """Install a CostModel returned by get_cost_model."""
+        check(lib().storm_hip_set_cost_model(self.h, C.byref(m)), "set_cost_model")
 
     # -- launch shapes -----------------------------------------------------------
     def set_bb_config(self, stage_words=8, band=4, blocks_per_cu=2, waves=8):
@@ -666,3 +686,92 @@ def write_stormbin(path, offs, pos, universe):
         f.write(b"STORMBIN")
         f.write(np.array([1, n, universe, 0], dtype="<u4").tobytes())
         f.write(buf.tobytes())
+
+
+# ---- the batched-instances workload ----------------------------------------
+_GOLDEN = 0x9E3779B97F4A7C15
+_MASK64 = (1 << 64) - 1
+
+
+class _Stream:
+    """splitmix64 stream (kernels/storm_gen.h:84-102): next() and unit()."""
+
+    def __init__(self, seed):
+        self.s = (seed if seed else _GOLDEN) & _MASK64
+
+    def next(self):
+        self.s = (self.s + _GOLDEN) & _MASK64
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK64
+        return z ^ (z >> 31)
+
+    def unit(self):
+        return (self.next() >> 11) * 2.0 ** -53
+
+
+# Representation planning for the batched workload: a bitmap of every row is
+# added to an instance held as lists or runs when the matrix-core B x B is
+# predicted faster than the instance's list or run cell. Rates measured on
+# MI355X: B x B at about 3e15 (int8) / 6e15 (fp4) bit-pair operations per
+# second, the S x S hash join at 6e-13 s and the R x R interval join at
+# 1e-12 s per pair and list element (run).
+BB_RATE_I8, BB_RATE_FP4 = 3.0e15, 6.0e15
+SS_S_PER_PAIR_ITEM, RR_S_PER_PAIR_ITEM = 6e-13, 1e-12
+
+
+def batched_instances(count=512, seed=5, plan_bitmaps=True, bitmap_cap_bytes=64 << 30):
+    """The independent all-pairs problems of the batched workload, drawn in
+    order from Rng(seed): instance k has N uniform in [64, 4096], universe
+    log-uniform in [2e5, 1.3e8], density log-uniform in [1e-6, 0.17];
+    even instances are i.i.d. Bernoulli rows, odd ones clustered-run rows
+    (Poisson runs of geometric length L = min(2000, d*U/20), at least 1, so
+    that lambda * L = d * U, plus 5% extra singletons). Each entry is a dict
+    with the generator call that builds it on the device (gen_device), its
+    stored size estimate in bytes and whether every row also gets a bitmap
+    (plan_bitmaps: when the B x B is predicted faster than the list or run
+    cell and the bitmaps take at most bitmap_cap_bytes)."""
+    import math
+    g = _Stream(seed)
+    out = []
+    for k in range(count):
+        n = 64 + int(g.unit() * 4033)
+        u = int(math.exp(math.log(2e5) + g.unit() * (math.log(1.3e8) - math.log(2e5))))
+        d = math.exp(math.log(1e-6) + g.unit() * (math.log(0.17) - math.log(1e-6)))
+        sk = g.next() & 0xFFFFFFFF
+        nw = (u + 63) // 64
+        pairs = n * (n - 1) // 2
+        bm_bytes = n * ((nw + 63) // 64 * 64) * 8
+        t_bb = pairs * 2.0 * u / (BB_RATE_FP4 if nw < (1 << 18) else BB_RATE_I8)
+        if k % 2 == 0:
+            kind = "bernoulli"
+            if d * 32 >= 1:      # bitmaps only
+                est, flags, t_other = bm_bytes, 0, t_bb
+            else:                # lists; bitmaps where a row's density is >= 1/64
+                est = 4 * n * u * d + (bm_bytes if d * 64 >= 1 else 0)
+                flags, t_other = 0, pairs * 2.0 * d * u * SS_S_PER_PAIR_ITEM
+            args = ["bernoulli", n, u, d, 0.0, 0.0, sk, flags]
+        else:
+            kind = "runs"
+            L = max(1.0, min(2000.0, d * u / 20.0))
+            lam = max(d * u / L, 1e-3)
+            runs = lam * (1 + 0.05 * L)
+            est = 8 * n * (runs + 1)
+            t_other = pairs * 2.0 * runs * RR_S_PER_PAIR_ITEM
+            args = ["runs", n, u, lam, L, 0.05, sk, 2]   # runs only
+        bitmaps = False
+        if plan_bitmaps and args[7] != 1 and t_bb < t_other and bm_bytes <= bitmap_cap_bytes \
+                and not (kind == "bernoulli" and d * 32 >= 1):
+            args[7] = 1                                  # a bitmap for every row
+            est += bm_bytes if not (kind == "bernoulli" and d * 64 >= 1) else 0
+            bitmaps = True
+        out.append({"k": k, "n": n, "universe": u, "density": d, "kind": kind, "seed": sk,
+                    "gen": tuple(args), "est_bytes": int(est) + 64 * n + (1 << 20),
+                    "pairs": pairs, "bitmaps_added": bitmaps,
+                    "predicted_s": min(t_bb, t_other) if bitmaps else t_other})
+    return out
+
+
+def gen_instance(ctx, inst):
+    kind, n, u, p1, p2, p3, seed, flags = inst["gen"]
+    return ctx.gen_device(kind, n, u, p1, p2, p3, seed, bitmap_policy=flags)

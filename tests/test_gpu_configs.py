@@ -82,3 +82,82 @@ def test_dense_config_full_size(ctx):
             for (r0, r1), want in zip(bands, want_bands):
                 lo = r0 * n - r0 * (r0 + 1) // 2
                 assert np.array_equal(c.tri(lo, len(want)), want), (bb, r0, r1)
+
+
+def _instance_rows(c, inst):
+    """Rows of a generated instance as CSR from its stored representation."""
+    a = c.arenas()
+    n, u = inst["n"], inst["universe"]
+    if len(a["list"]) or inst["kind"] == "bernoulli" and inst["density"] * 32 < 1:
+        return a["list_off"].astype(np.uint64), a["list"]
+    if inst["kind"] == "runs":
+        rows = []
+        for r in range(n):
+            lo, hi = int(a["run_off"][r]), int(a["run_off"][r + 1])
+            rows.append(np.concatenate([np.arange(s, e, dtype=np.uint32) for s, e in
+                                        zip(a["run_start"][lo:hi], a["run_end"][lo:hi])])
+                        if hi > lo else np.zeros(0, np.uint32))
+    else:
+        nw = (u + 63) // 64
+        ld = len(a["bitmap"]) // n
+        bm = a["bitmap"].reshape(n, ld)[:, :nw]
+        rows = [np.flatnonzero(np.unpackbits(w.view(np.uint8), bitorder="little")[:u])
+                .astype(np.uint32) for w in bm]
+    offs = np.zeros(n + 1, np.uint64)
+    offs[1:] = np.cumsum([len(r) for r in rows])
+    return offs, (np.concatenate(rows).astype(np.uint32) if offs[-1] else np.zeros(0, np.uint32))
+
+
+def test_batched_instances_subset(sh, ctx):
+    """The batched workload's own instances (seeded table of 512): the twelve
+    smallest and every 24th of the rest up to 2 GiB each, generated on the
+    device and run as one batch. Every instance's sum equals its column closed
+    form (computed on the device without any cell, and on the CPU for the
+    small ones), every row sum equals sum_{b in Xi} (c_b - 1), and the small
+    instances equal the oracle pair by pair and the host generator row by
+    row."""
+    inst = sh.batched_instances(512, 5)
+    by_size = sorted(inst, key=lambda x: x["est_bytes"])
+    chosen = {x["k"]: x for x in by_size[:12]}
+    for x in inst[::24]:
+        if x["est_bytes"] <= 2 << 30:
+            chosen[x["k"]] = x
+    chosen = [chosen[k] for k in sorted(chosen)]
+    corpora, wants = [], []
+    for x in chosen:
+        print("case batched instance", x["k"], x["kind"], x["n"], x["universe"],
+              f"{x['density']:.3g}", flush=True)
+        c = sh.gen_instance(ctx, x)
+        total, rs = c.closed_form()
+        corpora.append(c)
+        wants.append((total, rs))
+        small = x["n"] * x["universe"] * x["density"] <= 3e6 and x["n"] <= 1500
+        if small:
+            offs, pos = _instance_rows(c, x)
+            t2, rs2 = O.lists_closed_forms(offs, pos, x["universe"])
+            assert t2 == total and np.array_equal(rs2, rs), x["k"]
+            kind, n, u, p1, p2, p3, seed, _ = x["gen"]
+            if kind == "runs" or p1 * 32 < 1:
+                ho, hp = sh.gen_corpus(kind, n, u, p1, p2, p3, seed)
+                assert np.array_equal(ho, offs) and np.array_equal(hp, pos), x["k"]
+            else:
+                nw = (u + 63) // 64
+                words = O.bernoulli_rows(seed, p1, u, 0, n)
+                ld_rows = [np.flatnonzero(np.unpackbits(w.view(np.uint8),
+                                                        bitorder="little")[:u]) for w in words]
+                assert all(np.array_equal(a, pos[int(offs[r]):int(offs[r + 1])])
+                           for r, a in enumerate(ld_rows)), x["k"]
+            x["oracle"] = O.allpairs_lists(offs, pos)
+    print("case batched run", len(corpora), flush=True)
+    with ctx.batch(corpora) as b:
+        for policy in ("analytic", "fixed"):
+            st, sums, lat = b.allpairs(policy=policy, out="tri", tile=256, fixed_cell=7,
+                                       use_graph=True)
+            for c, x, (total, rs), s in zip(corpora, chosen, wants, sums):
+                assert int(s) == total, (policy, x["k"])
+                assert np.array_equal(c.row_sums(), rs), (policy, x["k"])
+                if "oracle" in x:
+                    assert np.array_equal(c.tri(), x["oracle"][1]), (policy, x["k"])
+            assert (lat > 0).sum() == sum(1 for x in chosen if x["n"] >= 2)
+    for c in corpora:
+        c.free()

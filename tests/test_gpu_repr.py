@@ -358,3 +358,52 @@ def test_upload_rejects_bad_arenas(sh, ctx):
         else:
             a["meta"][8 * 0 + 2] += 1
     bad(past_universe, -1)
+
+
+def test_wide_bb_tiles_in_k_slices(sh, ctx):
+    """256-row B x B tiles of wide rows (78,125 words, 3 K slices per tile)
+    queued as slices that add into the zeroed triangle: exact pair by pair,
+    alone and in a batch with a second corpus whose rows are narrow."""
+    n, u, p, seed = 300, 5_000_000, 0.04, 51
+    words = O.bernoulli_rows(seed, p, u, 0, n)
+    want_sum, want = O.allpairs_words(words)
+    total, rs = O.column_closed_forms(words, threads=8)
+    with ctx.gen_device("bernoulli", n, u, p, seed=seed) as c, \
+            ctx.gen_device("bernoulli", 200, 4096, 0.5, seed=52) as c2:
+        for policy in ("analytic", "fixed"):
+            st = c.allpairs(policy=policy, out="tri", tile=256, fixed_cell=0)
+            assert st["sum"] == want_sum == total
+            assert np.array_equal(c.tri(), want)
+            assert np.array_equal(c.row_sums(), rs)
+        w2 = O.bernoulli_rows(52, 0.5, 4096, 0, 200)
+        s2, t2 = O.allpairs_words(w2)
+        with ctx.batch([c, c2]) as b:
+            for _ in range(2):   # graph replays re-zero the sliced triangle
+                st, sums, lat = b.allpairs(policy="analytic", out="tri", tile=256, use_graph=True)
+                assert [int(x) for x in sums] == [want_sum, s2]
+                assert np.array_equal(c.tri(), want)
+                assert np.array_equal(c2.tri(), t2)
+
+
+def test_wide_corpus_bb_int8_queue_beside_fp4(sh, ctx):
+    """A corpus of 2^18 words or more per row (counts may pass fp4's exact
+    range) runs its B x B tiles in the int8 queue, in K slices, while a
+    narrow corpus of the same batch keeps the fp4 queue: both exact."""
+    n, u, p, seed = 100, (1 << 24) + 4097, 0.04, 53   # p >= 1/32: rows held as bitmaps
+    words = O.bernoulli_rows(seed, p, u, 0, n)
+    want_sum, want = O.allpairs_words(words)
+    w2 = O.bernoulli_rows(54, 0.5, 4096, 0, 300)
+    s2, t2 = O.allpairs_words(w2)
+    with ctx.gen_device("bernoulli", n, u, p, seed=seed) as c, \
+            ctx.gen_device("bernoulli", 300, 4096, 0.5, seed=54) as c2:
+        st = c.allpairs(policy="analytic", out="tri", tile=256)
+        assert st["sum"] == want_sum
+        assert np.array_equal(c.tri(), want)
+        with ctx.batch([c, c2]) as b:
+            for graph in (False, True, True):
+                st, sums, lat = b.allpairs(policy="analytic", out="tri", tile=256,
+                                           use_graph=graph)
+                assert [int(x) for x in sums] == [want_sum, s2]
+                assert np.array_equal(c.tri(), want)
+                assert np.array_equal(c2.tri(), t2)
+                assert st["bb_launches"] == 2
