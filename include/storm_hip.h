@@ -309,6 +309,28 @@ int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c,
 int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                               uint64_t* row_sum);
 
+/* ---- allocation-free dense call ------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc: the dense all-pairs call (as storm_hip_allpairs on
+ * a STORM_contiguous_t corpus, matrix-core B x B) over rows the caller holds
+ * in device memory, without any allocation by the library. d_words: n_rows
+ * rows of n_words u64, row stride ld words (ld >= n_words, a multiple of 8,
+ * and at least 16; the words of a row from n_words up to the next multiple of
+ * 8 are read and must be zero); no row padding is needed. Outputs in device
+ * memory:
+ * *d_sum = sum over pairs of |Xi n Xj|; d_tri (optional) = the packed upper
+ * triangle, idx(i,j) = i*N - i(i+1)/2 + (j-i-1). bb_kernel: AUTO, MFMA or
+ * MFMA_FP4. The call is enqueued on the context's stream and returns
+ * without waiting (storm_hip_sync waits). Limit: at most 2^20 rows (the
+ * library's fixed scratch array, 4 MiB per GPU, holds the B x B row map;
+ * larger calls are STORM_HIP_ERR_RANGE). Every call on one GPU shares that
+ * array: calls that overlap on one GPU (two contexts or threads) race. */
+int storm_hip_allpairs_contig_noalloc(storm_hip_ctx* ctx, const uint64_t* d_words,
+                                      uint32_t n_rows, uint32_t n_words, uint32_t ld,
+                                      uint32_t bb_kernel, uint32_t* d_tri,
+                                      unsigned long long* d_sum);
+
 /* ---- rectangular form -----------------------------------------------------
  *
  * |Xi n Yj| for every row i of x and every row j of y: the two-input form the

@@ -145,6 +145,7 @@ int storm_hip_ctx_create(int device, uint32_t flags, storm_hip_ctx** out) {
         set_error("device %d is %s; this build targets gfx950 only", device, prop.gcnArchName);
         return STORM_HIP_ERR_UNSUPPORTED;
     }
+    SH_TRY(noalloc_register_device(device));
     storm_hip_ctx* c = new (std::nothrow) storm_hip_ctx();
     if (!c) return STORM_HIP_ERR_NOMEM;
     c->device = device;

@@ -147,6 +147,9 @@ struct storm_hip_corpus {
 };
 
 namespace stormhip {
+// Looks up the allocation-free call's scratch array on this (current)
+// device, once per device (noalloc.hip).
+int noalloc_register_device(int device);
 // rank9 and zone maps of the bitmap rows whose gates pass (gen_dev.hip).
 int build_indexes(storm_hip_corpus* c, hipStream_t s);
 // Fill the bitmap rows of a corpus held as runs (gen_dev.hip).

@@ -179,6 +179,7 @@ def lib():
         "storm_hip_corpus_closed_form": (i32, [vp, vp, C.POINTER(u64), vp]),
         "storm_hip_calibrate_corpora": (i32, [vp, vp, u32, u32, u32, vp, vp, vp, vp,
                                               C.POINTER(CostModel)]),
+        "storm_hip_allpairs_contig_noalloc": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
