@@ -223,19 +223,32 @@ def sparse_bench(sh, args):
         return
     res["analytic_reference_rates"] = timed("analytic", use_graph=True)
     decisions["analytic_reference_rates"] = corpus.last_decisions(T * (T + 1) // 2)
-    # One sampling pass (every candidate cell timed on sampled tiles) feeds
-    # both the calibrated analytic model and the learned selector.
+    # One pass of training data feeds both the calibrated analytic model and
+    # the learned selector: exploration calls (production calls in which
+    # every tile runs a seeded random candidate cell, each item timed in the
+    # production mix), or with --sampler sample the per-class timing
+    # launches of selector_sample.
     t0 = time.perf_counter()
     parts = []
     chunks = max(1, args.sample_chunks)
-    for ch in range(chunks):
-        parts.append(ctx.selector_sample(corpus, tile=args.tile,
-                                         max_tiles=max(1, args.sample_tiles // chunks),
-                                         seed=11 + ch, repeats=2))
-        progress(f"sampled chunk {ch + 1}/{chunks}: {len(parts[-1][0])} tiles")
+    n_hold = 0
+    if args.sampler == "explore":
+        for ch in range(args.explore_calls):
+            X_, Y_, M_, idx_, s_ = ctx.selector_explore(corpus, tile=args.tile, seed=101 + ch,
+                                                        fraction=args.explore_fraction)
+            if s_ != want:
+                raise RuntimeError(f"exploration call {ch}: sum {s_} != closed form {want}")
+            parts.append((X_, Y_, M_, idx_))
+            progress(f"exploration call {ch + 1}/{args.explore_calls}: {len(X_)} tiles labelled")
+    else:
+        for ch in range(chunks):
+            parts.append(ctx.selector_sample(corpus, tile=args.tile,
+                                             max_tiles=max(1, args.sample_tiles // chunks),
+                                             seed=11 + ch, repeats=2))
+            progress(f"sampled chunk {ch + 1}/{chunks}: {len(parts[-1][0])} tiles")
+        # The last chunk is held out of training to measure regret.
+        n_hold = len(parts[-1][0]) if chunks > 1 else 0
     X, Y, M, idx = (np.concatenate([p[k] for p in parts]) for k in range(4))
-    # The last chunk is held out of training to measure regret.
-    n_hold = len(parts[-1][0]) if chunks > 1 else 0
     t_sample = time.perf_counter() - t0
     t0 = time.perf_counter()
     model = ctx.calibrate_samples(corpus, args.tile, Y, M, idx)
@@ -245,7 +258,9 @@ def sparse_bench(sh, args):
     t0 = time.perf_counter()
     n_fit = len(X) - n_hold
     progress(f"training on {n_fit} tiles ({n_hold} held out)")
-    loss = ctx.selector_train(X[:n_fit], Y[:n_fit], M[:n_fit], epochs=300, batch=128, lr=3e-3,
+    # About 40,000 minibatch steps whatever the number of labelled tiles.
+    epochs = max(3, min(300, int(40000 * 128 / max(1, n_fit))))
+    loss = ctx.selector_train(X[:n_fit], Y[:n_fit], M[:n_fit], epochs=epochs, batch=128, lr=3e-3,
                               seed=5)
     t_train = time.perf_counter() - t0
     res["learned"] = timed("learned", use_graph=True)
@@ -286,7 +301,8 @@ def sparse_bench(sh, args):
         "learned_over_analytic": res["learned"]["pairs_per_s"] / res["analytic"]["pairs_per_s"],
         "learned_over_analytic_reference_rates":
             res["learned"]["pairs_per_s"] / res["analytic_reference_rates"]["pairs_per_s"],
-        "selector": {"sampled_tiles": int(len(X)), "train_loss": loss, "sample_s": t_sample,
+        "selector": {"sampler": args.sampler, "labelled_tiles": int(len(X)),
+                     "train_loss": loss, "sample_s": t_sample,
                      "train_s": t_train, "held_out_tiles": int(n_hold),
                      "held_out_regret": regret},
         "calibrated_model": model, "calibrate_s": t_cal,
@@ -601,6 +617,14 @@ def main():
                          "to this .npz")
     ap.add_argument("--sample-chunks", type=int, default=8,
                     help="sampling passes (a progress line after each)")
+    ap.add_argument("--sampler", default="explore", choices=["explore", "sample"],
+                    help="sparse configs: training data from exploration calls in the "
+                         "production mix, or from per-class sampling launches")
+    ap.add_argument("--explore-calls", type=int, default=8,
+                    help="sparse configs: exploration calls (one label per tile each)")
+    ap.add_argument("--explore-fraction", type=float, default=0.125,
+                    help="sparse configs: share of tiles that run a random candidate cell in "
+                         "each exploration call (the others run the analytic choice)")
     ap.add_argument("--no-allbitmap", action="store_true")
     ap.add_argument("--no-build-timing", action="store_true",
                     help="sparse configs: skip timing the representation builders")

@@ -488,6 +488,29 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
                               uint32_t max_tiles, uint64_t seed, uint32_t repeats,
                               float* features, float* log2_ns, uint8_t* mask,
                               uint32_t* tile_index, uint32_t* n_sampled);
+/* Exploration (the production-mix alternative to storm_hip_selector_sample):
+ * one all-pairs call, exact as any other, in which a seeded share `fraction`
+ * of the non-empty tile pairs runs a cell drawn from the cells the learned
+ * selector may rank on it and the others run the analytic choice, and every
+ * work item's time is recorded as the launches run, alongside the other cell
+ * classes on their streams. Outputs, per tile pair that ran a
+ * cell (at most `capacity`): its features, log2 of its cost in ns -- the
+ * CU-time of its items, wall time over its class's workgroups per CU -- for
+ * the cell it ran (mask 1 there only), and its tile-pair index; the format
+ * storm_hip_selector_train and storm_hip_calibrate_samples take; *sum
+ * (optional) the call's exact total. Repeated calls with other seeds label
+ * other cells of the same tiles. */
+int storm_hip_selector_explore(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
+                               uint64_t seed, double fraction, uint32_t capacity, float* features,
+                               float* log2_ns, uint8_t* mask, uint32_t* tile_index,
+                               uint32_t* n_sampled, uint64_t* sum);
+/* The same over a batch (corpus_index[k]: the batch member of sample k;
+ * sums, optional: each member's total). */
+int storm_hip_batch_explore(storm_hip_ctx* ctx, storm_hip_batch* b, uint32_t tile, uint64_t seed,
+                            double fraction, uint32_t capacity, float* features, float* log2_ns,
+                            uint8_t* mask,
+                            uint32_t* tile_index, uint32_t* corpus_index, uint32_t* n_sampled,
+                            uint64_t* sums);
 /* Train on the device (He init from `seed`, Adam, minibatches of `batch`);
  * the trained model then serves POLICY_LEARNED on this context. */
 int storm_hip_selector_train(storm_hip_ctx* ctx, const float* features, const float* log2_ns,

@@ -203,6 +203,10 @@ struct DevModel {
     double bb_zonemap;
 };
 
+// Internal policy of exploration calls: each non-empty tile pair runs a cell
+// drawn (seeded) from the cells the learned selector may rank on it.
+constexpr uint32_t kPolicyExplore = 5;
+
 // Learned selection ranks only cells predicted (analytically) within this
 // factor of the cheapest eligible cell on the tile.
 constexpr double kLearnedPrune = 32.0;

@@ -26,7 +26,8 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
-                  bool bb_wide, DevCounters* ctr);
+                  bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
+                  DevCounters* ctr);
 int launch_compose_rowmap(hipStream_t s, const uint32_t* order, const uint32_t* bm_row,
                           uint32_t npad, uint32_t* rowmap);
 int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
@@ -116,13 +117,18 @@ struct Plan {
     unsigned long long* finish = nullptr;   // one per corpus, then the start stamp
     hipGraphExec_t graph = nullptr;
     uint32_t graph_key[4] = {0, 0, 0, 0};
+    // Exploration calls (eager only): the seed of the cell draws and the
+    // per-item times, kNumQueues x qstride, indexed like the queues.
+    uint64_t explore_seed = 0;
+    uint32_t explore_share = 0;            // tiles exploring, out of 2^24
+    unsigned long long* ticks = nullptr;
 };
 
 void free_plan(Plan* p) {
     if (!p) return;
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
     for (Work* w : p->works) free_work(w);
-    void* q[] = {p->d_corpora, p->queue, p->ctr, p->sums, p->finish, p->thr_buf};
+    void* q[] = {p->d_corpora, p->queue, p->ctr, p->sums, p->finish, p->thr_buf, p->ticks};
     for (void* x : q) if (x) (void)hipFree(x);
     delete p;
 }
@@ -297,7 +303,9 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
             SH_TRY(zero_words(s, p->h_corpora[k].tri, (uint64_t)c->n_rows * (c->n_rows - 1) / 2 * 4));
         SH_TRY(launch_select(s, w->tm, w->T, M, reps_of(c), o.policy, o.fixed_cell, lc, lok,
                              w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
-                             p->th.thr, ks, bb_wide(c, o, p->tile, thresholded_plan(p)), p->ctr));
+                             p->th.thr, ks, bb_wide(c, o, p->tile, thresholded_plan(p)),
+                             o.policy == kPolicyExplore ? p->explore_seed : 0, p->explore_share,
+                             p->ctr));
         SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
         // Only corpora with bitmap rows can queue B x B tiles.
         if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
@@ -329,6 +337,11 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     const uint32_t gbb = (uint32_t)std::min<uint64_t>(
         p->qstride, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
     const bool thresholded = p->th.hits != nullptr;
+    // Per-item times of an exploration call, per queue.
+    auto ticks_of = [&](int q) -> unsigned long long* {
+        return (o.policy == kPolicyExplore && p->ticks) ? p->ticks + (uint64_t)q * p->qstride
+                                                        : nullptr;
+    };
     if (p->tile == 256 && thresholded)
         rc = launch_bb_mfma_queue(fp4, ctx->mfma, sb, p->d_corpora, qbb,
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
@@ -337,18 +350,20 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     else if (p->tile == 256)
         rc = launch_bb_mfma_queue(fp4, ctx->mfma, sb, p->d_corpora, qbb,
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
-                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr);
+                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb,
+                                  ticks_of(STORM_HIP_CELL_BB));
     else
         rc = launch_bb_valu_queue(sb, (int)p->tile, p->d_corpora, qbb,
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
-                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb, nullptr, thresholded);
+                                  &p->ctr->qhead[STORM_HIP_CELL_BB], gbb,
+                                  ticks_of(STORM_HIP_CELL_BB), thresholded);
     if (timed) (void)hipEventRecord(ctx->ev[3], sb);
     // Wide corpora's B x B tiles (int8) on side stream 1, ahead of the joins.
     if (rc == STORM_HIP_OK && p->tile == 256 && !thresholded)
         rc = launch_bb_mfma_queue(false, ctx->mfma, ctx->aux[1], p->d_corpora,
                                   p->queue + (uint64_t)kQueueBBWide * p->qstride,
                                   &p->ctr->qcount[kQueueBBWide], &p->ctr->qhead[kQueueBBWide],
-                                  gbb, nullptr);
+                                  gbb, ticks_of(kQueueBBWide));
     // Persistent grids: as many workgroups as fit on the chip at once. The
     // classes that usually carry the most work (B x S stripes, the R x R and
     // S x S joins) start at once on different streams: B x S first on the
@@ -366,7 +381,7 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
             const uint32_t grid = (uint32_t)std::min<uint64_t>(
                 p->qstride, (uint64_t)ctx->cu_count * cell_blocks_per_cu(cell));
             rc = launch_cell(cell, side ? ctx->aux[1] : s, grid, p->d_corpora, M,
-                             p->queue + (uint64_t)cell * p->qstride, p->ctr, nullptr);
+                             p->queue + (uint64_t)cell * p->qstride, p->ctr, ticks_of(cell));
         }
     // Tiles settled empty (zeros only) go last.
     if (rc == STORM_HIP_OK) {
@@ -450,6 +465,89 @@ int run_plan(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, storm_hip_
         hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
         st->ms_bb = ms;
     }
+    return STORM_HIP_OK;
+}
+
+// One exploration call over the plan's corpora, then per non-empty tile pair
+// its features, the cell it ran and that cell's cost: the summed CU-time of
+// its work items (wall time over the class's workgroups per CU), in ns.
+int explore_plan(storm_hip_ctx* ctx, Plan* p, uint32_t tile, uint64_t seed, double fraction,
+                 uint32_t capacity, float* features, float* log2_ns, uint8_t* mask,
+                 uint32_t* tile_index, uint32_t* corpus_index, uint32_t* n_out, uint64_t* sums) {
+    hipStream_t s = ctx->stream;
+    if (!p->ticks)
+        SH_HIP(hipMalloc(&p->ticks, sizeof(unsigned long long) * p->qstride * kNumQueues));
+    p->explore_seed = seed ? seed : kGolden;
+    p->explore_share = (uint32_t)std::min(16777216.0, std::max(0.0, fraction) * 16777216.0);
+    storm_hip_options o{};
+    o.policy = kPolicyExplore;
+    o.tile = tile;
+    storm_hip_stats st{};
+    const int rc = run_plan(ctx, p, o, &st, sums, nullptr);
+    p->explore_seed = 0;
+    SH_TRY(rc);
+    DevCounters h;
+    SH_HIP(hipMemcpy(&h, p->ctr, sizeof h, hipMemcpyDeviceToHost));
+    const int F = STORM_HIP_MLP_FEATURES;
+    const size_t n = p->cs.size();
+    // Tile-pair index base of each corpus in one flat array of costs.
+    std::vector<uint64_t> base(n + 1, 0);
+    for (size_t k = 0; k < n; ++k) base[k + 1] = base[k] + p->works[k]->P;
+    std::vector<double> cost(base[n], 0.0);
+    std::vector<int8_t> cell_of(base[n], -1);
+    const uint32_t cu = (uint32_t)std::max(1, ctx->cu_count);
+    for (int q = 0; q < kNumQueues; ++q) {
+        const uint32_t cnt = h.qcount[q];
+        if (!cnt || q == STORM_HIP_CELL_EMPTY) continue;
+        const int cell = q == kQueueBBWide ? STORM_HIP_CELL_BB : q;
+        std::vector<QItem> items(cnt);
+        std::vector<unsigned long long> ticks(cnt);
+        SH_HIP(hipMemcpy(items.data(), p->queue + (uint64_t)q * p->qstride, cnt * sizeof(QItem),
+                         hipMemcpyDeviceToHost));
+        SH_HIP(hipMemcpy(ticks.data(), p->ticks + (uint64_t)q * p->qstride, cnt * 8,
+                         hipMemcpyDeviceToHost));
+        const uint32_t per_cu = cell == STORM_HIP_CELL_BB ? bb_blocks_per_cu(tile, ctx->mfma)
+                                                          : cell_blocks_per_cu(cell);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(p->qstride, (uint64_t)cu * per_cu);
+        const double sharing = std::max(1.0, std::min((double)per_cu, (double)grid / cu));
+        for (uint32_t e = 0; e < cnt; ++e) {
+            const QItem& it = items[e];
+            const uint32_t T = p->works[it.inst]->T;
+            const uint64_t t = base[it.inst] + (uint64_t)it.bi * T -
+                               (uint64_t)it.bi * (it.bi - 1) / 2 + (it.bj - it.bi);
+            cost[t] += (double)ticks[e] * 10.0 / sharing;   // 100 MHz clock -> ns
+            cell_of[t] = (int8_t)cell;
+        }
+    }
+    uint32_t m = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const Work* w = p->works[k];
+        if (!w->P) continue;
+        float* dX = nullptr;
+        SH_HIP(hipMalloc(&dX, sizeof(float) * F * w->P));
+        std::vector<float> X((size_t)F * w->P);
+        int frc = launch_features(s, w->tm, w->T, w->P, p->cs[k]->universe, dX);
+        if (frc == STORM_HIP_OK &&
+            (hipMemcpyAsync(X.data(), dX, X.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+             hipStreamSynchronize(s) != hipSuccess))
+            frc = STORM_HIP_ERR_DEVICE;
+        (void)hipFree(dX);
+        SH_TRY(frc);
+        for (uint64_t t = 0; t < w->P && m < capacity; ++t) {
+            const uint64_t g = base[k] + t;
+            if (cell_of[g] < 0) continue;
+            std::memcpy(features + (size_t)m * F, &X[(size_t)t * F], F * 4);
+            for (int c = 0; c < 10; ++c) {
+                mask[(size_t)m * 10 + c] = c == cell_of[g];
+                log2_ns[(size_t)m * 10 + c] =
+                    c == cell_of[g] ? (float)std::log2(std::max(cost[g], 10.0)) : 0.f;
+            }
+            tile_index[m] = (uint32_t)t;
+            if (corpus_index) corpus_index[m] = (uint32_t)k;
+            ++m;
+        }
+    }
+    *n_out = m;
     return STORM_HIP_OK;
 }
 
@@ -912,6 +1010,41 @@ int storm_hip_calibrate(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
     SH_TRY(storm_hip_selector_sample(ctx, c, tile, max_tiles, seed, repeats, X.data(), Y.data(),
                                      Mk.data(), idx.data(), &n));
     return storm_hip_calibrate_samples(ctx, c, tile, n, Y.data(), Mk.data(), idx.data(), out);
+}
+
+int storm_hip_selector_explore(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,
+                               uint64_t seed, double fraction, uint32_t capacity, float* features,
+                               float* log2_ns, uint8_t* mask, uint32_t* tile_index,
+                               uint32_t* n_sampled, uint64_t* sum) {
+    if (!ctx || !c || c->ctx != ctx || c->dense || !n_sampled) return STORM_HIP_ERR_INVALID;
+    if (capacity && (!features || !log2_ns || !mask || !tile_index)) return STORM_HIP_ERR_NULL_VALUES;
+    SH_TRY(check_tile(tile));
+    *n_sampled = 0;
+    if (c->n_rows < 2) return STORM_HIP_OK;
+    SH_HIP(hipSetDevice(ctx->device));
+    Plan* p = static_cast<Plan*>(c->plan);
+    const int rc = ensure_plan(p, {c}, {nullptr}, tile);
+    c->plan = p;
+    SH_TRY(rc);
+    return explore_plan(ctx, p, tile, seed, fraction, capacity, features, log2_ns, mask,
+                        tile_index, nullptr, n_sampled, sum);
+}
+
+int storm_hip_batch_explore(storm_hip_ctx* ctx, storm_hip_batch* b, uint32_t tile, uint64_t seed,
+                            double fraction, uint32_t capacity, float* features, float* log2_ns,
+                            uint8_t* mask,
+                            uint32_t* tile_index, uint32_t* corpus_index, uint32_t* n_sampled,
+                            uint64_t* sums) {
+    if (!ctx || !b || b->ctx != ctx || !n_sampled) return STORM_HIP_ERR_INVALID;
+    if (capacity && (!features || !log2_ns || !mask || !tile_index || !corpus_index))
+        return STORM_HIP_ERR_NULL_VALUES;
+    SH_TRY(check_tile(tile));
+    *n_sampled = 0;
+    SH_HIP(hipSetDevice(ctx->device));
+    std::vector<uint32_t*> tris(b->cs.size(), nullptr);
+    SH_TRY(ensure_plan(b->plan, b->cs, tris, tile));
+    return explore_plan(ctx, b->plan, tile, seed, fraction, capacity, features, log2_ns, mask,
+                        tile_index, corpus_index, n_sampled, sums);
 }
 
 int storm_hip_calibrate_samples(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t tile,

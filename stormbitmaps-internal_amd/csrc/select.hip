@@ -192,7 +192,8 @@ __global__ __launch_bounds__(256) void select_kernel(
     uint32_t policy, uint32_t fixed_cell, const float* __restrict__ learned_cost,
     const uint8_t* __restrict__ learned_ok, uint8_t* __restrict__ decisions,
     QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, bool split, double thr,
-    uint32_t bb_ksplit, bool bb_wide, DevCounters* __restrict__ ctr) {
+    uint32_t bb_ksplit, bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
+    DevCounters* __restrict__ ctr) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P) return;
     uint32_t bi, bj;
@@ -205,6 +206,23 @@ __global__ __launch_bounds__(256) void select_kernel(
     // holds no reportable pair.
     if (tile_empty(a, b) || (thr > 0.0 && length_bound(a, b) < thr)) {
         p = STORM_HIP_CELL_EMPTY;
+    } else if (policy == kPolicyExplore &&
+               (splitmix_mix(explore_seed ^ (t * 0xD1B54A32D192ED03ull)) >> 40) <
+                   (uint64_t)explore_share) {
+        // This tile explores: a cell drawn among the learned selector's
+        // candidates; the other tiles of an exploration call run the analytic
+        // choice (below) and are labelled with its cost.
+        const uint32_t cand = learned_candidates(M, a, b, elig);
+        const uint32_t k = __popc(cand);
+        if (k == 0) {
+            p = cheapest_eligible(M, a, b, elig);
+        } else {
+            uint32_t r = (uint32_t)(splitmix_mix(explore_seed + (t + 1) * kGolden) % k);
+            p = 0;
+            for (uint32_t m = cand;; m &= m - 1) {
+                if (r-- == 0) { p = __ffs(m) - 1; break; }
+            }
+        }
     } else if (policy == STORM_HIP_POLICY_FIXED) {
         p = (int)fixed_cell;
         if (p == STORM_HIP_CELL_EMPTY || !(elig >> p & 1)) { p = cheapest_eligible(M, a, b, elig); fallback = true; }
@@ -305,12 +323,14 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   uint32_t policy, uint32_t fixed_cell, const float* learned_cost,
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
-                  bool bb_wide, DevCounters* ctr) {
+                  bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
+                  DevCounters* ctr) {
     const uint64_t P = (uint64_t)T * (T + 1) / 2;
     if (P == 0) return STORM_HIP_OK;
     hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, s, tm, T, P,
                        M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions, queue,
-                       qstride, inst, tile > kSplitRows, thr, bb_ksplit, bb_wide, ctr);
+                       qstride, inst, tile > kSplitRows, thr, bb_ksplit, bb_wide, explore_seed,
+                       explore_share, ctr);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

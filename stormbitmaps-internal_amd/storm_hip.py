@@ -180,6 +180,10 @@ def lib():
         "storm_hip_calibrate_corpora": (i32, [vp, vp, u32, u32, u32, vp, vp, vp, vp,
                                               C.POINTER(CostModel)]),
         "storm_hip_allpairs_contig_noalloc": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
+        "storm_hip_selector_explore": (i32, [vp, vp, u32, u64, C.c_double, u32, vp, vp, vp,
+                                             vp, C.POINTER(u32), C.POINTER(u64)]),
+        "storm_hip_batch_explore": (i32, [vp, vp, u32, u64, C.c_double, u32, vp, vp, vp, vp,
+                                          vp, C.POINTER(u32), vp]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -375,6 +379,24 @@ class Context:
               "selector_sample")
         k = n.value
         return X[:k], Y[:k], M[:k], idx[:k]
+
+    def selector_explore(self, corpus, tile=256, seed=1, fraction=1.0):
+        """One exploration call: (X, Y, M, idx, sum) with one labelled cell per
+        non-empty tile pair, its cost measured in the production mix."""
+        F, O = mlp_shape()[0], 10
+        T = (corpus.n + tile - 1) // tile
+        cap = max(1, T * (T + 1) // 2)
+        X = np.zeros((cap, F), np.float32)
+        Y = np.zeros((cap, O), np.float32)
+        M = np.zeros((cap, O), np.uint8)
+        idx = np.zeros(cap, np.uint32)
+        n, s = C.c_uint32(0), C.c_uint64(0)
+        check(lib().storm_hip_selector_explore(self.h, corpus.h, tile, seed, fraction, cap,
+                                               _ptr(X), _ptr(Y),
+                                               _ptr(M), _ptr(idx), C.byref(n), C.byref(s)),
+              "selector_explore")
+        k = n.value
+        return X[:k], Y[:k], M[:k], idx[:k], s.value
 
     def selector_train(self, X, Y, M, epochs=200, batch=256, lr=3e-3, seed=1):
         X = np.ascontiguousarray(X, np.float32)
@@ -602,6 +624,28 @@ class Batch:
 
     def __exit__(self, *a):
         self.free()
+
+    def explore(self, tile=256, seed=1, fraction=1.0):
+        """Exploration over the batch: (X, Y, M, idx, corpus index, sums)."""
+        F, O = mlp_shape()[0], 10
+        cap = 0
+        for c in self.corpora:
+            T = (c.n + tile - 1) // tile
+            cap += T * (T + 1) // 2
+        cap = max(cap, 1)
+        X = np.zeros((cap, F), np.float32)
+        Y = np.zeros((cap, O), np.float32)
+        M = np.zeros((cap, O), np.uint8)
+        idx = np.zeros(cap, np.uint32)
+        cidx = np.zeros(cap, np.uint32)
+        sums = np.zeros(max(1, len(self.corpora)), np.uint64)
+        n = C.c_uint32(0)
+        check(lib().storm_hip_batch_explore(self.ctx.h, self.h, tile, seed, fraction, cap,
+                                            _ptr(X), _ptr(Y),
+                                            _ptr(M), _ptr(idx), _ptr(cidx), C.byref(n),
+                                            _ptr(sums)), "batch_explore")
+        k = n.value
+        return X[:k], Y[:k], M[:k], idx[:k], cidx[:k], sums[:len(self.corpora)]
 
     def allpairs(self, policy="analytic", out="sum", bb="auto", tile=64, fixed_cell=0,
                  use_graph=False):

@@ -107,47 +107,86 @@ def _mixed(seed, n, u):
     return rows
 
 
-def test_learned_policy_is_exact_and_beats_untrained_analytic(sh, ctx):
-    u = 200_000
-    n = 1600
+REF_RATES = [0.116, 0.30, 0.60, 0.55, 0.22, 0.35, 0.40, 0.30, 0.45, 0.50]
+
+
+def _median_ms(c, policy, tile, n=5):
+    import time
+    c.allpairs(policy=policy, tile=tile, use_graph=True)
+    ts = []
+    for _ in range(n):
+        t0 = time.perf_counter()
+        c.allpairs(policy=policy, tile=tile, use_graph=True)
+        ts.append(time.perf_counter() - t0)
+    return 1000 * float(np.median(ts))
+
+
+def test_explore_labels_every_tile_and_is_exact(sh, ctx):
+    """An exploration call is an exact all-pairs call that labels every
+    non-empty tile pair with the cell it ran and that cell's measured cost;
+    other seeds draw other cells; batches label every member."""
+    u, n = 200_000, 1600
     rows = _mixed(21, n, u)
     offs, pos = corpora.csr(rows)
-    want_sum, want = O.allpairs_lists(offs, pos)
-    # The untrained method: the analytic model with the reference's rates (it
-    # also bounds the cells the learned selector may rank, at sampling and at
-    # selection alike).
-    ctx.set_cost_model([0.116, 0.30, 0.60, 0.55, 0.22, 0.35, 0.40, 0.30, 0.45, 0.50])
-    with ctx.from_lists(offs, pos, u, bitmap_policy=0 | 4 | 8 | 16) as c:
-        X, Y, M, idx = ctx.selector_sample(c, tile=64, max_tiles=400, seed=3, repeats=2)
-        assert len(X) > 200
-        assert M.any(axis=1).all()
-        c.allpairs(policy="analytic", tile=64)
-        T = (n + 63) // 64
-        analytic = c.last_decisions(T * (T + 1) // 2)[idx]
-        # Training and held-out tiles drawn from the same distribution (the
-        # sample comes back in tile order, which runs from sparse to dense).
-        order = np.random.default_rng(11).permutation(len(X))
-        X, Y, M, idx, analytic = X[order], Y[order], M[order], idx[order], analytic[order]
-        half = len(X) // 2
-        loss = ctx.selector_train(X[:half], Y[:half], M[:half], epochs=400, batch=64, lr=3e-3,
-                                  seed=7)
-        print("train loss", loss, "labels per cell", M[:half].sum(axis=0), flush=True)
-        # Held-out tiles: time of each method's choice against the
-        # exhaustively timed best cell.
-        pred = ctx.selector_predict(X[half:])
-        Yh, Mh = Y[half:], M[half:].astype(bool)
-        pred[~Mh] = np.inf
-        rows = np.arange(len(Yh))
-        t_best = np.exp2(np.where(Mh, Yh, np.inf).min(axis=1)).sum()
-        t_learned = np.exp2(Yh[rows, pred.argmin(axis=1)]).sum()
-        a = analytic[half:].astype(np.int64)
-        assert Mh[rows, a].all()                  # analytic choices were timed
-        t_analytic = np.exp2(Yh[rows, a]).sum()
-        print("held-out regret: learned", t_learned / t_best - 1, "analytic",
-              t_analytic / t_best - 1, flush=True)
-        assert t_learned <= 1.05 * t_analytic
-        st = c.allpairs(policy="learned", out="tri")
-        assert st["sum"] == want_sum
-        assert np.array_equal(c.tri(), want)
-        st2 = c.allpairs(policy="analytic", out="tri")
-        assert st2["sum"] == want_sum
+    want_sum, _ = O.allpairs_lists(offs, pos)
+    ctx.set_cost_model(REF_RATES)
+    with ctx.from_lists(offs, pos, u, bitmap_policy=1 | 4 | 8 | 16 | 64) as c:
+        for tile in (64, 256):
+            T = (n + tile - 1) // tile
+            c.allpairs(policy="analytic", tile=tile)
+            dec = c.last_decisions(T * (T + 1) // 2)
+            live = set(np.flatnonzero(dec != 10).tolist())
+            seen = {}
+            for seed in (1, 2, 3):
+                X, Y, M, idx, s = ctx.selector_explore(c, tile=tile, seed=seed)
+                assert s == want_sum, (tile, seed)
+                assert (M.sum(axis=1) == 1).all()
+                assert (Y[M > 0] > 0).all()
+                assert len(set(idx.tolist())) == len(idx)
+                assert set(idx.tolist()) == live, tile
+                for t, cell in zip(idx.tolist(), M.argmax(axis=1).tolist()):
+                    seen.setdefault(t, set()).add(cell)
+            assert sum(len(v) > 1 for v in seen.values()) > len(seen) // 4
+    # A batch labels the tiles of every member.
+    rows2 = _mixed(22, 300, 50_000)
+    o2, p2 = corpora.csr(rows2)
+    s2, _ = O.allpairs_lists(o2, p2)
+    with ctx.from_lists(offs, pos, u, bitmap_policy=0 | 4 | 8 | 16) as c1, \
+            ctx.from_lists(o2, p2, 50_000, bitmap_policy=0 | 4 | 8 | 16) as c2, \
+            ctx.batch([c1, c2]) as b:
+        X, Y, M, idx, cidx, sums = b.explore(tile=256, seed=5)
+        assert [int(x) for x in sums] == [want_sum, s2]
+        assert set(cidx.tolist()) == {0, 1}
+        assert (M.sum(axis=1) == 1).all()
+
+
+def test_learned_policy_is_exact_and_beats_untrained_analytic(sh, ctx):
+    """Run-heavy rows with every row's bitmap resident (cells B x B, B x R,
+    R x R). The untrained method -- the analytic model with the reference's
+    CPU rates -- against the learned selector trained on four exploration
+    calls (costs measured in the production mix): learned must be strictly
+    faster in production, by at least 20%, and both exact (total and every
+    row sum against the column closed forms of the host generator's rows)."""
+    n, u = 2048, 4_000_000
+    offs, pos = sh.gen_corpus("runs", n, u, 20.0, 2000.0, 0.05, 71)
+    total, rs = O.lists_closed_forms(offs, pos, u)
+    ctx.set_cost_model(REF_RATES)
+    with ctx.gen_device("runs", n, u, 20.0, 2000.0, 0.05, 71, bitmap_policy=1) as c:
+        tile = 256
+        parts = [ctx.selector_explore(c, tile=tile, seed=sd) for sd in (1, 2, 3, 4)]
+        for p in parts:
+            assert p[4] == total
+        X, Y, M = (np.concatenate([p[k] for p in parts]) for k in range(3))
+        loss = ctx.selector_train(X, Y, M, epochs=300, batch=128, lr=3e-3, seed=7)
+        print("explored samples", len(X), "labels per cell", M.sum(axis=0), "loss", loss,
+              flush=True)
+        for policy in ("analytic", "learned"):
+            st = c.allpairs(policy=policy, out="tri", tile=tile)
+            assert st["sum"] == total, policy
+            assert np.array_equal(c.row_sums(), rs), policy
+            print(policy, st["cell_tiles"], flush=True)
+        t_analytic = _median_ms(c, "analytic", tile)
+        t_learned = _median_ms(c, "learned", tile)
+        print(f"production: analytic (reference rates) {t_analytic:.2f} ms, learned "
+              f"{t_learned:.2f} ms", flush=True)
+        assert t_learned < 0.8 * t_analytic
