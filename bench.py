@@ -82,11 +82,26 @@ def traffic_from_profiles(kernel, workload):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(sh, corpus, n_bits, target_s=12.0):
-    """Reference all-bitmap B x B (kernels/cell_bb.cpp 'dense') with project
-    threading over all pairs of the first R rows of the same corpus."""
+def _oracle_mod():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O  # CPU checker: the reference build lives under oracle/_ref
+    return O
+
+
+def host_info(O):
+    return {"cpu_model": O.cpu_model(), "reference_isa_build": O.ref_isa(),
+            "host_threads": os.cpu_count()}
+
+
+def cpu_baseline(sh, corpus, n_bits, target_s=12.0, single_target_s=6.0):
+    """The reference on this host's cores, beside the GPU (a reported
+    baseline): leg (ii) the reference's all-bitmap B x B kernel
+    (kernels/cell_bb.cpp 'dense') with project threading over all pairs of the
+    first R rows of the same corpus, its sum checked against the GPU's counts
+    of the same pairs; leg (i) the reference as shipped on one thread,
+    STORM_contig_pairw_intersect_cardinality (storm.cpp:1232-1240) on the
+    first rows of the corpus."""
+    O = _oracle_mod()
     if not O.ref_available():
         return None
     import ctypes as C
@@ -104,8 +119,8 @@ def cpu_baseline(sh, corpus, n_bits, target_s=12.0):
     r = 256
     _, t = run(r)
     rate = (r * (r - 1) / 2) / max(t, 1e-6)
-    r = int(min(8192, max(256, (2 * rate * target_s) ** 0.5)))
-    progress(f"cpu baseline: {r} rows")
+    r = int(min(8192, max(256, (2 * rate * target_s) ** 0.5), corpus.n))
+    progress(f"cpu baseline (threaded reference kernel): {r} rows")
     s, t = run(r)
     pairs = r * (r - 1) // 2
     # The GPU's counts for the same pairs must agree.
@@ -114,10 +129,83 @@ def cpu_baseline(sh, corpus, n_bits, target_s=12.0):
     for i in range(r - 1):
         off = i * n - i * (i + 1) // 2
         gpu += int(corpus.tri(off, r - i - 1).sum(dtype=np.uint64))
-    return {"value": pairs / t, "unit": "pairs/s", "cores": threads, "kind": "reference",
-            "sample": f"all {pairs} pairs of the first {r} rows of the same corpus, "
-                      f"{t:.1f} s, reference bb 'dense' kernel, {threads} threads",
-            "sample_sum": int(s), "sample_sum_matches_gpu": bool(gpu == s)}
+    out = {"value": pairs / t, "unit": "pairs/s", "cores": threads, "kind": "reference",
+           "sample": f"all {pairs} pairs of the first {r} rows of the same corpus, "
+                     f"{t:.1f} s, reference bb 'dense' kernel, project threading over "
+                     f"{threads} threads",
+           "sample_sum": int(s), "sample_sum_matches_gpu": bool(gpu == s)}
+    out.update(host_info(O))
+    # Leg (i): the reference as shipped, one thread.
+    r1 = 128
+    words = corpus.download_words(0, r1, nw)
+    secs = C.c_double(0)
+    R_.ref_time_contig(words.ctypes.data_as(C.c_void_p), r1, nw, n_bits, C.byref(secs))
+    rate1 = (r1 * (r1 - 1) / 2) / max(secs.value, 1e-6)
+    r1 = int(min(4096, max(128, (2 * rate1 * single_target_s) ** 0.5), corpus.n))
+    words = corpus.download_words(0, r1, nw)
+    s1 = R_.ref_time_contig(words.ctypes.data_as(C.c_void_p), r1, nw, n_bits, C.byref(secs))
+    p1 = r1 * (r1 - 1) // 2
+    gpu1 = 0
+    for i in range(r1 - 1):
+        gpu1 += int(corpus.tri(i * n - i * (i + 1) // 2, r1 - i - 1).sum(dtype=np.uint64))
+    out["single_thread_reference_as_shipped"] = {
+        "value": p1 / secs.value, "unit": "pairs/s", "cores": 1,
+        "sample": f"all {p1} pairs of the first {r1} rows, "
+                  f"STORM_contig_pairw_intersect_cardinality, {secs.value:.1f} s",
+        "sample_sum_matches_gpu": bool(int(s1) == gpu1)}
+    return out
+
+
+def cpu_baseline_sparse(offs, pos, universe, max_rows=2048, tile=64, target_s=8.0):
+    """The reference as shipped on one thread (leg (i)) over a stride sample
+    of the corpus's rows (every k-th row, so the sample spans the density
+    spectrum): allpairs_sum under PerTile with the model calibrate() fits on
+    this host, and under AllBitmap (kernels/storm_allpairs.cpp:486-675),
+    rows built untimed; each sample sized from a 128-row probe for about
+    target_s seconds; every sum checked against the sample's column closed
+    form."""
+    O = _oracle_mod()
+    if not O.ref_available():
+        return None
+    import ctypes as C
+    n = len(offs) - 1
+    R_ = O.reference()
+
+    def sample(rows_wanted):
+        k = max(1, n // rows_wanted)
+        rows = [pos[int(offs[i]):int(offs[i + 1])] for i in range(0, n, k)][:rows_wanted]
+        so = np.zeros(len(rows) + 1, np.uint32)
+        so[1:] = np.cumsum([len(r) for r in rows])
+        sp = np.concatenate(rows).astype(np.uint32) if so[-1] else np.zeros(1, np.uint32)
+        return k, rows, so, sp
+
+    def run(policy, cal, rows_wanted):
+        k, rows, so, sp = sample(rows_wanted)
+        secs = C.c_double(0)
+        got = R_.ref_time_allpairs(so.ctypes.data_as(C.c_void_p), sp.ctypes.data_as(C.c_void_p),
+                                   len(rows), universe, policy, tile, cal, C.byref(secs))
+        want = column_pair_sum(so.astype(np.int64), sp[:int(so[-1])], universe)
+        return k, len(rows), secs.value, int(got) == want
+
+    res = {}
+    for name, policy, cal in (("per_tile_calibrated", 2, 1), ("all_bitmap", 0, 0)):
+        _, r0, t0, _ = run(policy, cal, 128)
+        rate = (r0 * (r0 - 1) / 2) / max(t0, 1e-6)
+        want_rows = int(min(max_rows, max(128, (2 * rate * target_s) ** 0.5)))
+        k, r, t, ok = run(policy, cal, want_rows)
+        pairs = r * (r - 1) // 2
+        res[name] = {"value": pairs / t, "unit": "pairs/s", "seconds": t, "rows": r,
+                     "stride": k, "exact": ok}
+        progress(f"cpu reference {name}: {t:.1f} s on {r} rows")
+    r = res["per_tile_calibrated"]
+    out = {"value": r["value"], "unit": "pairs/s", "cores": 1, "kind": "reference",
+           "sample": f"all pairs of every {r['stride']}-th row ({r['rows']} rows), reference "
+                     f"allpairs_sum PerTile + calibrate(), one thread, tile {tile}",
+           "legs": res,
+           "threaded_leg": "not run for this workload (the threaded leg runs the reference's "
+                           "B x B kernel on the dense workload)"}
+    out.update(host_info(O))
+    return out
 
 
 def column_pair_sum(offs, pos, universe):
@@ -178,6 +266,7 @@ def sparse_bench(sh, args):
     t_up = time.perf_counter() - t0
     progress(f"built and uploaded in {t_up:.1f} s (bitmaps for every row: {all_bitmaps})")
     build = None if args.no_build_timing else build_timing(sh, ctx, offs, pos, u)
+    pos_keep = pos if not args.no_cpu else None
     del pos
     pairs = n * (n - 1) // 2
     res = {}
@@ -219,8 +308,7 @@ def sparse_bench(sh, args):
                "policy": name, "exact": r["sum"] == want, "result": r}
         corpus.free()
         ctx.close()
-        print(json.dumps(out), flush=True)
-        return
+        return out
     res["analytic_reference_rates"] = timed("analytic", use_graph=True)
     decisions["analytic_reference_rates"] = corpus.last_decisions(T * (T + 1) // 2)
     # One pass of training data feeds both the calibrated analytic model and
@@ -314,7 +402,9 @@ def sparse_bench(sh, args):
         out["learned_over_all_bitmap"] = res["learned"]["pairs_per_s"] / res["all_bitmap"]["pairs_per_s"]
     corpus.free()
     ctx.close()
-    print(json.dumps(out), flush=True)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_sparse(offs, pos_keep, u)
+    return out
 
 
 def build_timing(sh, ctx, offs, pos, u):
@@ -487,7 +577,7 @@ def batch_bench(sh, args):
         "all_bitmap": "not run: every instance as bitmaps is about 2.7 TB",
     }
     ctx.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def dense_bench(sh, args):
@@ -567,7 +657,7 @@ def dense_bench(sh, args):
     }
     corpus.free()
     ctx.close()
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def copy_probe(args):
@@ -595,6 +685,8 @@ def main():
                     + ["copy"])
     ap.add_argument("--bb", default="auto", choices=["auto", "mfma", "mfma_fp4", "valu"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default run: the headline workload only (no other workloads)")
     ap.add_argument("--no-tune", action="store_true",
                     help="dense configs: keep the default B x B launch shape")
     ap.add_argument("--shape", default="",
@@ -647,11 +739,55 @@ def main():
         return
     sh = load_storm_hip()
     if args.config in SPARSE_CONFIGS:
-        sparse_bench(sh, args)
+        out = sparse_bench(sh, args)
     elif args.config in BATCH_CONFIGS:
-        batch_bench(sh, args)
+        out = batch_bench(sh, args)
     else:
-        dense_bench(sh, args)
+        out = dense_bench(sh, args)
+    if args.config == "dense2" and not args.no_extra:
+        out["other_configs"] = other_configs(sh, args)
+    print(json.dumps(out), flush=True)
+
+
+def other_configs(sh, args):
+    """The default run's other workloads, each at its stated size, after the
+    headline line's own measurement: the small dense workload (fp4 B x B,
+    full), the skewed and the run-heavy mixed workloads (analytic with the
+    reference's rates, learned trained on two exploration calls, all-bitmap)
+    with the reference's single-thread CPU leg on a row sample. The batched
+    workload (about five minutes) runs with --config batch5."""
+    import copy
+    out = {}
+    for name in ("dense1", "zipf3", "runs4"):
+        a = copy.copy(args)
+        a.config = name
+        a.steps = 3
+        a.warmup = 1
+        a.no_build_timing = True
+        a.threshold = ""
+        a.explore_calls = 2
+        a.explore_fraction = 1.0 / 32
+        a.slow_s = 5.0
+        a.no_tune = False
+        a.shape = ""
+        t0 = time.perf_counter()
+        progress(f"other workload {name}")
+        try:
+            r = sparse_bench(sh, a) if name in SPARSE_CONFIGS else dense_bench(sh, a)
+            keep = ("value", "unit", "ms_per_step", "dtype", "config", "roofline", "exact_sum",
+                    "exact_sums_agree", "column_closed_form_sum", "cpu_baseline",
+                    "learned_over_analytic_reference_rates", "learned_over_analytic",
+                    "learned_over_all_bitmap", "selector", "launch_shape")
+            r2 = {k: r[k] for k in keep if k in r}
+            if "policies" in r:
+                r2["policies"] = {k: {q: v[q] for q in ("pairs_per_s", "ms", "cell_tiles",
+                                                         "timed_calls") if q in v}
+                                  for k, v in r["policies"].items()}
+            r2["wall_s"] = time.perf_counter() - t0
+            out[name] = r2
+        except Exception as e:   # one workload's failure does not hide the others
+            out[name] = {"error": repr(e)}
+    return out
 
 
 if __name__ == "__main__":

@@ -67,15 +67,26 @@ def oracle():
 
 
 def _cpu_has_avx512():
+    """CPUID through liboracle (built without ISA flags): AVX-512 F, BW, VL
+    and VPOPCNTDQ all present."""
+    L = oracle()
+    L.orc_cpu_has_avx512_vpopcntdq.restype = C.c_int
+    return bool(L.orc_cpu_has_avx512_vpopcntdq())
+
+
+def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("flags"):
-                    fl = set(line.split(":", 1)[1].split())
-                    return {"avx512f", "avx512bw", "avx512vl", "avx512vpopcntdq"} <= fl
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return False
+    return "unknown"
+
+
+def ref_isa():
+    return "avx512 (F/BW/VL/VPOPCNTDQ)" if _cpu_has_avx512() else "x86-64-v3 (AVX2)"
 
 
 def ref_path():
@@ -109,6 +120,8 @@ def reference():
         "ref_allpairs_tiles_tri": (u64, [vp, vp, u32, u32, u32, vp]),
         "ref_generate": (u64, [u32, u32, dbl, i32, i32, u64, vp, vp]),
         "ref_bb_dense_allpairs": (u64, [vp, u32, u32, i32, vp]),
+        "ref_time_allpairs": (u64, [vp, vp, u32, u32, i32, u32, i32, vp]),
+        "ref_time_contig": (u64, [vp, u32, u32, u32, vp]),
         "STORM_contig_new": (vp, [C.c_size_t]),
         "STORM_contig_free": (None, [vp]),
         "STORM_contig_add": (i32, [vp, vp, u32]),

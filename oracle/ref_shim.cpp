@@ -5,6 +5,7 @@
  *
  * Each function names the reference routine it drives.
  */
+#include "storm.h"
 #include "kernels/storm_allpairs.h"
 #include "kernels/storm_cells.h"
 #include "kernels/storm_cost.h"
@@ -147,6 +148,51 @@ uint64_t ref_generate(uint32_t n_rows, uint32_t universe, double density, int st
     }
     if (offs) offs[n_rows] = (uint32_t)tot;
     return tot;
+}
+
+/* CPU baseline leg (i), the reference as shipped on one thread: rows built
+ * with build_row (kernels/storm_repr.cpp:102-207, untimed), then
+ * allpairs_sum (kernels/storm_allpairs.cpp:486-675) under `policy`
+ * (0 AllBitmap, 2 PerTile) with the default model or, when calibrate_first,
+ * the model calibrate() (kernels/storm_cost.cpp:556-769) fits on this host
+ * (untimed). Returns the sum; *seconds is allpairs_sum's wall time. */
+uint64_t ref_time_allpairs(const uint32_t* offs, const uint32_t* pos, uint32_t n,
+                           uint32_t universe, int policy, uint32_t tile, int calibrate_first,
+                           double* seconds) {
+    std::vector<Row> rows;
+    rows_from_csr(offs, pos, n, universe, rows);
+    CostModel m; default_model(m);
+    if (calibrate_first) calibrate(m);
+    auto t0 = std::chrono::steady_clock::now();
+    AllPairsStats s = allpairs_sum(rows, m, (Policy)policy, tile);
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return s.sum;
+}
+
+/* CPU baseline leg (i) for the contiguous container: n rows of nw words (row
+ * major) added to a STORM_contiguous_t of n_bits (storm.cpp:1159-1220,
+ * untimed), then STORM_contig_pairw_intersect_cardinality (storm.cpp:1232-1240)
+ * timed on one thread. */
+uint64_t ref_time_contig(const uint64_t* words, uint32_t n, uint32_t nw, uint32_t n_bits,
+                         double* seconds) {
+    STORM_contiguous_t* c = STORM_contig_new(n_bits);
+    if (!c) return 0;
+    std::vector<uint32_t> p;
+    for (uint32_t i = 0; i < n; ++i) {
+        p.clear();
+        for (uint32_t w = 0; w < nw; ++w) {
+            uint64_t x = words[(size_t)i * nw + w];
+            while (x) { p.push_back(w * 64 + (uint32_t)__builtin_ctzll(x)); x &= x - 1; }
+        }
+        STORM_contig_add(c, p.data(), (uint32_t)p.size());
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    const uint64_t s = STORM_contig_pairw_intersect_cardinality(c);
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    STORM_contig_free(c);
+    return s;
 }
 
 /* CPU baseline: the reference's shipped all-bitmap B x B kernel ("dense",

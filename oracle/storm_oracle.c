@@ -686,3 +686,12 @@ void orc_adam_step(int n, double* p, const double* g, double* m, double* v, int 
         p[i] -= lr * mh / (sqrt(vh) + eps);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Host ISA check (which reference build may run here).                      */
+/* ------------------------------------------------------------------------ */
+int orc_cpu_has_avx512_vpopcntdq(void) {
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") &&
+           __builtin_cpu_supports("avx512vl") && __builtin_cpu_supports("avx512vpopcntdq");
+}

@@ -143,6 +143,10 @@ double orc_mlp_forward_backward(int f, int h1, int h2, int o, int batch,
 void orc_adam_step(int n, double* p, const double* g, double* m, double* v, int t,
                    double lr, double beta1, double beta2, double eps);
 
+/* 1 when this CPU runs AVX-512 F/BW/VL/VPOPCNTDQ (the reference build that
+ * uses them may be loaded), from the CPUID bits. */
+int orc_cpu_has_avx512_vpopcntdq(void);
+
 #ifdef __cplusplus
 }
 #endif

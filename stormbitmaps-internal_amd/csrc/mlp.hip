@@ -113,56 +113,85 @@ __device__ inline void mlp_forward(const float* __restrict__ P, const float* x, 
 // One thread per sample; gradients reduced in LDS, then once per block into
 // `grad` (which the caller zeroes). Loss = sum over masked outputs of
 // (y - t)^2 / 2, divided by `count` (the number of masked outputs in the batch).
+// Forward and backward over kSB samples per workgroup: lanes 0..kSB-1 run
+// one sample each and leave its activations and deltas in LDS; then every
+// thread sums its parameters' gradients over the block's samples (one global
+// add per parameter and block, no contended atomics).
+constexpr int kSB = 64;
+
 __global__ __launch_bounds__(256) void mlp_fwd_bwd_kernel(
     const float* __restrict__ P, const float* __restrict__ X, const float* __restrict__ Tg,
     const uint8_t* __restrict__ M, uint32_t n, float inv_count, float* __restrict__ grad,
     float* __restrict__ loss) {
-    __shared__ float g[kParams];
-    __shared__ float lsum;
-    for (int k = threadIdx.x; k < kParams; k += blockDim.x) g[k] = 0.f;
-    if (threadIdx.x == 0) lsum = 0.f;
-    __syncthreads();
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) {
+    __shared__ float sx[kSB][kF], sa1[kSB][kH1], sa2[kSB][kH2];
+    __shared__ float sd1[kSB][kH1], sd2[kSB][kH2], sdy[kSB][kO];
+    __shared__ float lpart[kSB];
+    const uint32_t s0 = blockIdx.x * kSB;
+    const int t = threadIdx.x;
+    if (t < kSB) {
+        const uint32_t s = s0 + t;
         float x[kF], a1[kH1], a2[kH2], y[kO], dy[kO], d2[kH2];
-        for (int k = 0; k < kF; ++k) x[k] = X[(uint64_t)s * kF + k];
-        mlp_forward(P, x, a1, a2, y);
         float l = 0.f;
-        for (int i = 0; i < kO; ++i) {
-            const bool on = M[(uint64_t)s * kO + i] != 0;
-            const float e = on ? (y[i] - Tg[(uint64_t)s * kO + i]) : 0.f;
-            l += 0.5f * e * e * inv_count;
-            dy[i] = e * inv_count;
+        if (s < n) {
+            for (int k = 0; k < kF; ++k) x[k] = X[(uint64_t)s * kF + k];
+            mlp_forward(P, x, a1, a2, y);
+            for (int i = 0; i < kO; ++i) {
+                const bool on = M[(uint64_t)s * kO + i] != 0;
+                const float e = on ? (y[i] - Tg[(uint64_t)s * kO + i]) : 0.f;
+                l += 0.5f * e * e * inv_count;
+                dy[i] = e * inv_count;
+            }
+            for (int k = 0; k < kH2; ++k) {
+                float acc = 0.f;
+                for (int i = 0; i < kO; ++i) acc = fmaf(P[oW3 + i * kH2 + k], dy[i], acc);
+                d2[k] = a2[k] > 0.f ? acc : 0.f;
+            }
+            for (int k = 0; k < kH1; ++k) {
+                float acc = 0.f;
+                for (int i = 0; i < kH2; ++i) acc = fmaf(P[oW2 + i * kH1 + k], d2[i], acc);
+                sd1[t][k] = a1[k] > 0.f ? acc : 0.f;
+            }
+        } else {
+            for (int k = 0; k < kF; ++k) x[k] = 0.f;
+            for (int k = 0; k < kH1; ++k) { a1[k] = 0.f; sd1[t][k] = 0.f; }
+            for (int k = 0; k < kH2; ++k) { a2[k] = 0.f; d2[k] = 0.f; }
+            for (int i = 0; i < kO; ++i) dy[i] = 0.f;
         }
-        atomicAdd(&lsum, l);
-        for (int i = 0; i < kO; ++i) {
-            if (dy[i] == 0.f) continue;
-            atomicAdd(&g[oB3 + i], dy[i]);
-            for (int k = 0; k < kH2; ++k) atomicAdd(&g[oW3 + i * kH2 + k], dy[i] * a2[k]);
-        }
-        for (int k = 0; k < kH2; ++k) {
-            float acc = 0.f;
-            for (int i = 0; i < kO; ++i) acc = fmaf(P[oW3 + i * kH2 + k], dy[i], acc);
-            d2[k] = a2[k] > 0.f ? acc : 0.f;
-        }
-        for (int i = 0; i < kH2; ++i) {
-            if (d2[i] == 0.f) continue;
-            atomicAdd(&g[oB2 + i], d2[i]);
-            for (int k = 0; k < kH1; ++k) atomicAdd(&g[oW2 + i * kH1 + k], d2[i] * a1[k]);
-        }
-        for (int k = 0; k < kH1; ++k) {
-            float acc = 0.f;
-            for (int i = 0; i < kH2; ++i) acc = fmaf(P[oW2 + i * kH1 + k], d2[i], acc);
-            const float d1 = a1[k] > 0.f ? acc : 0.f;
-            if (d1 == 0.f) continue;
-            atomicAdd(&g[oB1 + k], d1);
-            for (int q = 0; q < kF; ++q) atomicAdd(&g[oW1 + k * kF + q], d1 * x[q]);
-        }
+        for (int k = 0; k < kF; ++k) sx[t][k] = x[k];
+        for (int k = 0; k < kH1; ++k) sa1[t][k] = a1[k];
+        for (int k = 0; k < kH2; ++k) { sa2[t][k] = a2[k]; sd2[t][k] = d2[k]; }
+        for (int i = 0; i < kO; ++i) sdy[t][i] = dy[i];
+        lpart[t] = l;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < kParams; k += blockDim.x)
-        if (g[k] != 0.f) atomicAdd(&grad[k], g[k]);
-    if (threadIdx.x == 0 && loss) atomicAdd(loss, lsum);
+    if (t == 0 && loss) {
+        float sum = 0.f;
+        for (int q = 0; q < kSB; ++q) sum += lpart[q];
+        atomicAdd(loss, sum);
+    }
+    for (int k = t; k < kParams; k += blockDim.x) {
+        float acc = 0.f;
+        if (k < oB1) {
+            const int i = k / kF, j = k % kF;
+            for (int q = 0; q < kSB; ++q) acc = fmaf(sd1[q][i], sx[q][j], acc);
+        } else if (k < oW2) {
+            const int i = k - oB1;
+            for (int q = 0; q < kSB; ++q) acc += sd1[q][i];
+        } else if (k < oB2) {
+            const int i = (k - oW2) / kH1, j = (k - oW2) % kH1;
+            for (int q = 0; q < kSB; ++q) acc = fmaf(sd2[q][i], sa1[q][j], acc);
+        } else if (k < oW3) {
+            const int i = k - oB2;
+            for (int q = 0; q < kSB; ++q) acc += sd2[q][i];
+        } else if (k < oB3) {
+            const int i = (k - oW3) / kH2, j = (k - oW3) % kH2;
+            for (int q = 0; q < kSB; ++q) acc = fmaf(sdy[q][i], sa2[q][j], acc);
+        } else {
+            const int i = k - oB3;
+            for (int q = 0; q < kSB; ++q) acc += sdy[q][i];
+        }
+        if (acc != 0.f) atomicAdd(&grad[k], acc);
+    }
 }
 
 // Adam (Kingma & Ba 2015, Algorithm 1), one thread per parameter.
@@ -256,8 +285,8 @@ int fwd_bwd(hipStream_t st, Selector* s, const float* x, const float* t, const u
     SH_HIP(hipMemsetAsync(s->grad, 0, kParams * 4, st));
     if (d_loss) SH_HIP(hipMemsetAsync(d_loss, 0, 4, st));
     if (n == 0) return STORM_HIP_OK;
-    hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((n + 255) / 256), dim3(256), 0, st, s->params, x, t,
-                       mask, n, count ? 1.f / (float)count : 0.f, s->grad, d_loss);
+    hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((n + kSB - 1) / kSB), dim3(256), 0, st, s->params,
+                       x, t, mask, n, count ? 1.f / (float)count : 0.f, s->grad, d_loss);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }
