@@ -309,6 +309,39 @@ int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c,
 int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                               uint64_t* row_sum);
 
+/* ---- streamed tile rows and cross blocks (general corpora) ---------------
+ *
+ * storm::allpairs_tiles (kernels/storm_allpairs.cpp:677-732) visits the upper
+ * triangle one tile row at a time, in density order under PerTile and in row
+ * order otherwise, so a consumer never holds N^2 counts. A band call computes
+ * one tile row of that walk and nothing else: only the tile pairs (band, bj),
+ * bj >= band, are selected and dispatched; the device holds one tile x N
+ * count buffer, not the triangle.
+ *   order   STORM_HIP_ORDER_DENSITY: positions of the density order (stable by
+ *           cardinality, kernels/storm_allpairs.cpp:314-323; the permutation
+ *           is storm_hip_corpus_order); STORM_HIP_ORDER_IDENTITY: row order.
+ *   out     (rows in the band) x N u32, out[(i - band*tile) * N + j] = the
+ *           count of positions i < j; entries with j <= i are 0.
+ * opt->tile (64, 128, 256) is the band height. Policies: ANALYTIC, LEARNED,
+ * FIXED, ALL_BITMAP (every tile B x B over materialised bitmaps). */
+enum { STORM_HIP_ORDER_DENSITY = 0, STORM_HIP_ORDER_IDENTITY = 1 };
+int storm_hip_allpairs_band(storm_hip_ctx* ctx, storm_hip_corpus* c,
+                            const storm_hip_options* opt, uint32_t order, uint32_t band,
+                            uint32_t* out, storm_hip_stats* stats);
+/* Density order of a general corpus: perm[k] = the row at position k
+ * (identity for a dense corpus). */
+int storm_hip_corpus_order(storm_hip_ctx* ctx, const storm_hip_corpus* c, uint32_t* perm);
+/* The n1 x n2 block of a corpus whose rows [0, split) are one set and
+ * [split, N) another (STORM_intersect_cardinality_square, storm.cpp:1027-1046:
+ * every pair across the two, none within either): only tile pairs across
+ * the split are selected and dispatched. split is a multiple of opt->tile
+ * (pad the first set with empty rows). stats->sum is the sum over the block;
+ * out (optional, split x (N - split) u32) receives
+ * out[i * (N - split) + (j - split)] = |X_i n X_j|. */
+int storm_hip_allpairs_cross(storm_hip_ctx* ctx, storm_hip_corpus* c,
+                             const storm_hip_options* opt, uint32_t split, uint32_t* out,
+                             storm_hip_stats* stats);
+
 /* ---- allocation-free dense call ------------------------------------------
  * No counterpart in the reference. For callers that cannot allocate device
  * memory -- code under HIP graph capture, tight serving loops, harnesses

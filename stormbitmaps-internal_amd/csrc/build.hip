@@ -393,8 +393,12 @@ int upload_orders(storm_hip_corpus* c, hipStream_t s) {
     std::vector<uint32_t> perm(npad, n), ident(npad, n);
     std::copy(c->h_perm.begin(), c->h_perm.end(), perm.begin());
     std::iota(ident.begin(), ident.begin() + n, 0u);
+    // Padded like the orders: the sentinel row n maps to position n.
+    std::vector<uint32_t> inv(npad, n);
+    for (uint32_t k = 0; k < n; ++k) inv[c->h_perm[k]] = k;
     SH_TRY(upload_vec(c->d_perm, perm, s, B));
     SH_TRY(upload_vec(c->d_ident, ident, s, B));
+    SH_TRY(upload_vec(c->d_inv, inv, s, B));
     return STORM_HIP_OK;
 }
 
@@ -460,7 +464,7 @@ void general_free_arenas(storm_hip_corpus* c) {
     void* ptrs[] = {c->d_list_off, c->d_list, c->d_run_off, c->d_run_start, c->d_run_end,
                     c->d_ewah_off, c->d_ewah, c->d_mk_off, c->d_mk_word, c->d_mk_pos,
                     c->d_comp_off, c->d_comp, c->d_meta, c->d_bm_row, c->d_bitmap,
-                    c->d_perm, c->d_ident, c->d_occ,
+                    c->d_perm, c->d_ident, c->d_inv, c->d_occ,
                     c->d_slice, c->d_slice_of, c->d_rank, c->d_rank_row, c->d_occ_row};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }

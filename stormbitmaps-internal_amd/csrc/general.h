@@ -56,6 +56,13 @@ struct DevCorpus {
     PairHit* hits;
     unsigned long long* n_hits;
     uint64_t hit_cap;
+    // Band and cross calls (emit_kind 1 / 2, tri null): counts go to emit_buf
+    // by position in the call's order (inv: original row -> position):
+    //   1  band of one tile row:  emit_buf[(lo % tile) * n + hi]
+    //   2  cross rows [0, split) x [split, n): emit_buf[lo * emit_cols + hi - split]
+    uint32_t emit_kind, emit_cols, emit_split;
+    uint32_t* emit_buf;
+    const uint32_t* inv;
 };
 
 // Per-pair output of a general call: the packed triangle, or the pair itself
@@ -65,6 +72,16 @@ struct DevCorpus {
 __device__ inline void emit_pair(const DevCorpus& D, uint32_t i, uint32_t j, uint32_t v) {
     if (D.tri) {
         D.tri[i < j ? tri_index(i, j, D.n) : tri_index(j, i, D.n)] = v;
+        return;
+    }
+    if (D.emit_kind) {
+        const uint32_t a = D.inv[i], b = D.inv[j];
+        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        if (hi >= D.n) return;   // padding positions carry no pair
+        if (D.emit_kind == 1)
+            D.emit_buf[(uint64_t)(lo % D.tile) * D.n + hi] = v;
+        else if (lo < D.emit_split && hi >= D.emit_split)
+            D.emit_buf[(uint64_t)lo * D.emit_cols + (hi - D.emit_split)] = v;
         return;
     }
     if (D.hits && v) {

@@ -27,7 +27,7 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
                   bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
-                  DevCounters* ctr);
+                  const uint32_t* sel, DevCounters* ctr);
 int launch_compose_rowmap(hipStream_t s, const uint32_t* order, const uint32_t* bm_row,
                           uint32_t npad, uint32_t* rowmap);
 int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
@@ -93,6 +93,13 @@ struct Thresh {
     PairHit* hits = nullptr;
     unsigned long long* n_hits = nullptr;
     uint64_t cap = 0;
+    // Band / cross output (kind 1 / 2, see DevCorpus) and the call's order.
+    uint32_t kind = 0, cols = 0, split = 0;
+    uint32_t* buf = nullptr;
+    uint64_t buf_bytes = 0;
+    const uint32_t* inv = nullptr;
+    bool identity = false;    // identity order instead of the density order
+    bool emits() const { return hits != nullptr || kind != 0; }
 };
 
 // Device state of one call over a list of corpora: per-corpus selection
@@ -121,6 +128,12 @@ struct Plan {
     // per-item times, kNumQueues x qstride, indexed like the queues.
     uint64_t explore_seed = 0;
     uint32_t explore_share = 0;            // tiles exploring, out of 2^24
+    // Tile pairs (bi, bj) queued: bi in [sel[0], sel[1]), bj in [sel[2], sel[3]).
+    // The select kernel reads d_sel, so one captured graph serves every range;
+    // sel_dev mirrors what d_sel holds.
+    uint32_t sel[4] = {0, 0xFFFFFFFFu, 0, 0xFFFFFFFFu};
+    uint32_t sel_dev[4] = {0, 0, 0, 0};
+    uint32_t* d_sel = nullptr;
     unsigned long long* ticks = nullptr;
 };
 
@@ -128,7 +141,8 @@ void free_plan(Plan* p) {
     if (!p) return;
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
     for (Work* w : p->works) free_work(w);
-    void* q[] = {p->d_corpora, p->queue, p->ctr, p->sums, p->finish, p->thr_buf, p->ticks};
+    void* q[] = {p->d_corpora, p->queue, p->ctr,   p->sums,
+                 p->finish,    p->thr_buf, p->ticks, p->d_sel};
     for (void* x : q) if (x) (void)hipFree(x);
     delete p;
 }
@@ -147,13 +161,15 @@ DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
     D.rank_row = c->d_rank_row; D.rank = c->d_rank; D.rank_ld = c->rank_ld;
     D.occ_row = c->d_occ_row; D.occ = c->d_occ; D.occ_ld = c->occ_ld;
     D.meta = c->d_meta;
-    D.perm = c->d_perm;
+    D.perm = th.identity ? c->d_ident : c->d_perm;
+    D.emit_kind = th.kind; D.emit_cols = th.cols; D.emit_split = th.split;
+    D.emit_buf = th.buf; D.inv = th.inv;
     D.slice = c->d_slice; D.slice_of = c->d_slice_of;
     D.tm = w->tm; D.rowmap = w->rowmap; D.tile = w->tile;
     D.tri = tri; D.sum = sum; D.finish = finish;
     // K slices for wide 256-row B x B tiles (not in thresholded calls, whose
     // B x B tiles test each pair from the whole count).
-    D.bb_ksplit = (w->tile == 256 && !th.hits) ? bb_slices_for(c->n_words) : 1;
+    D.bb_ksplit = (w->tile == 256 && !th.emits()) ? bb_slices_for(c->n_words) : 1;
     return D;
 }
 
@@ -208,11 +224,12 @@ int build_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
     // a B x B tile as up to kBBMaxSlices K slices.
     uint64_t split = tile > kSplitRows ? (uint64_t)(tile / kSplitRows) * (tile / kSplitRows) : 1;
     for (storm_hip_corpus* c : cs)
-        if (tile == 256 && !th.hits) split = std::max<uint64_t>(split, bb_slices_for(c->n_words));
+        if (tile == 256 && !th.emits()) split = std::max<uint64_t>(split, bb_slices_for(c->n_words));
     p->qstride = P1 * split;
     SH_HIP(hipMalloc(&p->d_corpora, sizeof(DevCorpus) * std::max<size_t>(n, 1)));
     SH_HIP(hipMalloc(&p->queue, sizeof(QItem) * p->qstride * kNumQueues));
     SH_HIP(hipMalloc(&p->ctr, sizeof(DevCounters)));
+    SH_HIP(hipMalloc(&p->d_sel, sizeof p->sel));
     SH_HIP(hipMalloc(&p->sums, 8 * std::max<size_t>(n, 1)));
     SH_HIP(hipMalloc(&p->finish, 8 * (n + 1)));
     p->h_corpora.resize(n);
@@ -263,7 +280,8 @@ int zero_words(hipStream_t s, void* p, uint64_t bytes) {
     return STORM_HIP_OK;
 }
 
-bool thresholded_plan(const Plan* p) { return p->th.hits != nullptr; }
+// Calls whose pairs all go through emit_pair (thresholded, band, cross).
+bool thresholded_plan(const Plan* p) { return p->th.emits(); }
 
 // A corpus whose 256-row B x B tiles go to the int8 queue: counts can reach
 // 2^24 (2^18 words or more). Thresholded calls and explicit int8 use one
@@ -281,6 +299,7 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     SH_TRY(zero_words(s, p->sums, 8 * n));
     SH_TRY(zero_words(s, p->finish, 8 * n));
     if (p->th.n_hits) SH_TRY(zero_words(s, p->th.n_hits, 8));
+    if (p->th.kind) SH_TRY(zero_words(s, p->th.buf, p->th.buf_bytes));
     hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, p->finish + n);
     SH_HIP(hipGetLastError());
     if (timed) SH_HIP(hipEventRecord(ctx->ev[4], s));
@@ -305,8 +324,8 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
                              w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
                              p->th.thr, ks, bb_wide(c, o, p->tile, thresholded_plan(p)),
                              o.policy == kPolicyExplore ? p->explore_seed : 0, p->explore_share,
-                             p->ctr));
-        SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
+                             p->d_sel, p->ctr));
+        SH_TRY(launch_compose_rowmap(s, p->h_corpora[k].perm, c->d_bm_row, w->npad, w->rowmap));
         // Only corpora with bitmap rows can queue B x B tiles.
         if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
     }
@@ -336,7 +355,7 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     // B x B items: whole tiles or K slices (at most qstride of them).
     const uint32_t gbb = (uint32_t)std::min<uint64_t>(
         p->qstride, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
-    const bool thresholded = p->th.hits != nullptr;
+    const bool thresholded = p->th.emits();
     // Per-item times of an exploration call, per queue.
     auto ticks_of = [&](int q) -> unsigned long long* {
         return (o.policy == kPolicyExplore && p->ticks) ? p->ticks + (uint64_t)q * p->qstride
@@ -406,6 +425,10 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
 int run_plan(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, storm_hip_stats* st,
              uint64_t* sums, double* latency_ms) {
     hipStream_t s = ctx->stream;
+    if (std::memcmp(p->sel, p->sel_dev, sizeof p->sel) != 0) {
+        SH_HIP(hipMemcpyAsync(p->d_sel, p->sel, sizeof p->sel, hipMemcpyHostToDevice, s));
+        std::memcpy(p->sel_dev, p->sel, sizeof p->sel);
+    }
     if (o.use_graph) {
         const uint32_t key[4] = {o.policy, o.fixed_cell, o.bb_kernel, ctx->model_version};
         if (!p->graph || std::memcmp(key, p->graph_key, sizeof key) != 0) {
@@ -568,6 +591,9 @@ void general_free(storm_hip_corpus* c) {
     c->work = nullptr;
     if (c->d_hits) (void)hipFree(c->d_hits);
     if (c->d_n_hits) (void)hipFree(c->d_n_hits);
+    if (c->d_emit) (void)hipFree(c->d_emit);
+    c->d_emit = nullptr;
+    c->emit_bytes = 0;
     c->d_hits = nullptr;
     c->d_n_hits = nullptr;
     if (!c->dense) general_free_arenas(c);
@@ -619,6 +645,16 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
 
 namespace {
 
+// 256-row B x B tiles of calls that emit per pair run on one persistent
+// matrix-core block per CU, each with its own 256 x 256 count buffer.
+int ensure_thr_buf(storm_hip_ctx* ctx, Plan* p) {
+    if (p->tile != 256 || p->thr_buf) return STORM_HIP_OK;
+    p->thr_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(p->totalP, 1),
+                                               (uint64_t)ctx->cu_count);
+    SH_HIP(hipMalloc(&p->thr_buf, (size_t)p->thr_grid * 256 * 256 * 4));
+    return STORM_HIP_OK;
+}
+
 // One thresholded call into the corpus's pair buffer; returns the number of
 // reportable pairs (which may exceed the buffer's capacity).
 int threshold_pass(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_options& o,
@@ -632,16 +668,70 @@ int threshold_pass(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_opti
     const int rc = ensure_plan(p, {c}, {nullptr}, tile, th);
     c->plan = p;
     SH_TRY(rc);
-    if (tile == 256 && !p->thr_buf) {
-        // One persistent matrix-core block per CU, each with its count buffer.
-        p->thr_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(p->totalP, 1),
-                                                   (uint64_t)ctx->cu_count);
-        SH_HIP(hipMalloc(&p->thr_buf, (size_t)p->thr_grid * 256 * 256 * 4));
-    }
+    SH_TRY(ensure_thr_buf(ctx, p));
     uint64_t sum = 0;
     SH_TRY(run_plan(ctx, p, o, st, &sum, nullptr));
     SH_HIP(hipMemcpyAsync(n_hits, c->d_n_hits, 8, hipMemcpyDeviceToHost, ctx->stream));
     SH_HIP(hipStreamSynchronize(ctx->stream));
+    return STORM_HIP_OK;
+}
+
+int ensure_emit(storm_hip_corpus* c, uint64_t bytes) {
+    if (c->d_emit && c->emit_bytes == bytes) return STORM_HIP_OK;
+    if (c->d_emit) { SH_HIP(hipFree(c->d_emit)); c->d_emit = nullptr; c->emit_bytes = 0; }
+    SH_HIP(hipMalloc((void**)&c->d_emit, std::max<uint64_t>(bytes, 4)));
+    c->emit_bytes = bytes;
+    return STORM_HIP_OK;
+}
+
+// Options of a band or cross call: a general corpus and a selecting policy;
+// the all-bitmap policy runs every tile B x B over materialised bitmaps.
+int range_call_options(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_options* opt,
+                       storm_hip_options* o, uint32_t* tile) {
+    *o = storm_hip_options{};
+    if (opt) *o = *opt;
+    if (o->policy > STORM_HIP_POLICY_EXHAUSTIVE || o->bb_kernel > STORM_HIP_BB_MFMA_FP4 ||
+        o->fixed_cell >= STORM_HIP_NCELL)
+        return STORM_HIP_ERR_INVALID;
+    if (c->dense) {
+        set_error("band and cross calls take a general corpus");
+        return STORM_HIP_ERR_UNSUPPORTED;
+    }
+    if (o->policy == STORM_HIP_POLICY_EXHAUSTIVE) {
+        set_error("band and cross calls take a selecting policy or all-bitmap");
+        return STORM_HIP_ERR_UNSUPPORTED;
+    }
+    *tile = o->tile ? o->tile : 64;
+    SH_TRY(check_tile(*tile));
+    o->tile = *tile;
+    if (o->policy == STORM_HIP_POLICY_ALL_BITMAP) {
+        SH_TRY(general_materialize(ctx, c));
+        o->policy = STORM_HIP_POLICY_FIXED;
+        o->fixed_cell = STORM_HIP_CELL_BB;
+    }
+    return STORM_HIP_OK;
+}
+
+// Run the corpus's plan for `th` over tile pairs sel, timing the call.
+int run_range(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_options& o, uint32_t tile,
+              const Thresh& th, const uint32_t sel[4], storm_hip_stats* st) {
+    Plan* p = static_cast<Plan*>(c->plan);
+    const int rc = ensure_plan(p, {c}, {nullptr}, tile, th);
+    c->plan = p;
+    SH_TRY(rc);
+    if (th.emits()) SH_TRY(ensure_thr_buf(ctx, p));
+    std::memcpy(p->sel, sel, sizeof p->sel);
+    SH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    uint64_t sum = 0;
+    const int rr = run_plan(ctx, p, o, st, &sum, nullptr);
+    static const uint32_t kAll[4] = {0, 0xFFFFFFFFu, 0, 0xFFFFFFFFu};
+    std::memcpy(p->sel, kAll, sizeof p->sel);
+    SH_TRY(rr);
+    SH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    SH_HIP(hipEventSynchronize(ctx->ev[1]));
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    st->ms_device = ms;
     return STORM_HIP_OK;
 }
 
@@ -1231,6 +1321,99 @@ int storm_hip_allpairs_threshold(storm_hip_ctx* ctx, storm_hip_corpus* c,
     c->hits_valid = true;
     *n_pairs = nh;
     s.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return STORM_HIP_OK;
+}
+
+int storm_hip_allpairs_band(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_options* opt,
+                            uint32_t order, uint32_t band, uint32_t* out, storm_hip_stats* st) {
+    if (!ctx || !c || c->ctx != ctx || order > STORM_HIP_ORDER_IDENTITY) return STORM_HIP_ERR_INVALID;
+    storm_hip_stats local{};
+    storm_hip_stats& s = st ? *st : local;
+    std::memset(&s, 0, sizeof s);
+    const auto t0 = std::chrono::steady_clock::now();
+    SH_HIP(hipSetDevice(ctx->device));
+    storm_hip_options o;
+    uint32_t tile = 0;
+    SH_TRY(range_call_options(ctx, c, opt, &o, &tile));
+    const uint64_t n = c->n_rows;
+    const uint64_t T = (n + tile - 1) / tile;
+    if (band >= T) {
+        set_error("band %u of a corpus with %llu tile rows", band, (unsigned long long)T);
+        return STORM_HIP_ERR_RANGE;
+    }
+    const uint64_t i0 = (uint64_t)band * tile, i1 = std::min<uint64_t>(n, i0 + tile);
+    if (!out) return STORM_HIP_ERR_NULL_VALUES;
+    // Pairs (i, j), i in the band, j > i.
+    s.pairs = (i1 - i0) * (n - i0) - (i1 - i0) * (i1 - i0 + 1) / 2;
+    if (n < 2) {
+        std::memset(out, 0, (size_t)((i1 - i0) * n * 4));
+        return STORM_HIP_OK;
+    }
+    SH_TRY(ensure_emit(c, (uint64_t)tile * n * 4));
+    Thresh th;
+    th.kind = 1;
+    th.buf = c->d_emit;
+    th.buf_bytes = (uint64_t)tile * n * 4;
+    th.identity = order == STORM_HIP_ORDER_IDENTITY;
+    th.inv = th.identity ? c->d_ident : c->d_inv;
+    const uint32_t sel[4] = {band, band + 1, 0, 0xFFFFFFFFu};
+    SH_TRY(run_range(ctx, c, o, tile, th, sel, &s));
+    SH_HIP(hipMemcpyAsync(out, c->d_emit, (size_t)((i1 - i0) * n * 4), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    SH_HIP(hipStreamSynchronize(ctx->stream));
+    s.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return STORM_HIP_OK;
+}
+
+int storm_hip_allpairs_cross(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_options* opt,
+                             uint32_t split, uint32_t* out, storm_hip_stats* st) {
+    if (!ctx || !c || c->ctx != ctx) return STORM_HIP_ERR_INVALID;
+    storm_hip_stats local{};
+    storm_hip_stats& s = st ? *st : local;
+    std::memset(&s, 0, sizeof s);
+    const auto t0 = std::chrono::steady_clock::now();
+    SH_HIP(hipSetDevice(ctx->device));
+    storm_hip_options o;
+    uint32_t tile = 0;
+    SH_TRY(range_call_options(ctx, c, opt, &o, &tile));
+    const uint64_t n = c->n_rows;
+    if (split == 0 || split >= n || split % tile != 0) {
+        set_error("cross split %u must be a positive multiple of the tile (%u) below the %llu rows",
+                  split, tile, (unsigned long long)n);
+        return STORM_HIP_ERR_INVALID;
+    }
+    const uint64_t cols = n - split;
+    s.pairs = (uint64_t)split * cols;
+    Thresh th;
+    th.identity = true;
+    if (out) {
+        SH_TRY(ensure_emit(c, (uint64_t)split * cols * 4));
+        th.kind = 2;
+        th.buf = c->d_emit;
+        th.buf_bytes = (uint64_t)split * cols * 4;
+        th.cols = (uint32_t)cols;
+        th.split = split;
+        th.inv = c->d_ident;
+    }
+    const uint32_t sel[4] = {0, split / tile, split / tile, 0xFFFFFFFFu};
+    SH_TRY(run_range(ctx, c, o, tile, th, sel, &s));
+    if (out) {
+        SH_HIP(hipMemcpyAsync(out, c->d_emit, (size_t)split * cols * 4, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        SH_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    s.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return STORM_HIP_OK;
+}
+
+int storm_hip_corpus_order(storm_hip_ctx* ctx, const storm_hip_corpus* c, uint32_t* perm) {
+    if (!ctx || !c || c->ctx != ctx) return STORM_HIP_ERR_INVALID;
+    if (!perm && c->n_rows) return STORM_HIP_ERR_NULL_VALUES;
+    if (c->dense || c->h_perm.size() != c->n_rows) {
+        for (uint32_t k = 0; k < c->n_rows; ++k) perm[k] = k;
+    } else {
+        std::memcpy(perm, c->h_perm.data(), (size_t)c->n_rows * 4);
+    }
     return STORM_HIP_OK;
 }
 

@@ -123,8 +123,9 @@ struct storm_hip_corpus {
     uint32_t rank_ld = 0, occ_ld = 0, n_rank_rows = 0, n_occ_rows = 0;
     uint64_t index_bytes = 0;
     // Density order (stable by cardinality) and the identity order, both
-    // padded with the zero-row sentinel mapping.
-    uint32_t *d_perm = nullptr, *d_ident = nullptr;
+    // padded with the zero-row sentinel mapping; d_inv: row -> its position
+    // in the density order.
+    uint32_t *d_perm = nullptr, *d_ident = nullptr, *d_inv = nullptr;
     std::vector<uint32_t> h_perm;
     uint64_t device_bytes = 0;
     std::vector<uint32_t> h_meta;    // host copy of meta (8 per row)
@@ -141,6 +142,9 @@ struct storm_hip_corpus {
     // and the last call's pairs sorted by (i, j), 3 u32 each.
     void* d_hits = nullptr;
     uint64_t hit_cap = 0;
+    // Band and cross calls: device count buffer, sized per call shape.
+    uint32_t* d_emit = nullptr;
+    uint64_t emit_bytes = 0;
     unsigned long long* d_n_hits = nullptr;
     std::vector<uint32_t> h_hits;
     bool hits_valid = false;

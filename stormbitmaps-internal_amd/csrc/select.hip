@@ -193,11 +193,16 @@ __global__ __launch_bounds__(256) void select_kernel(
     const uint8_t* __restrict__ learned_ok, uint8_t* __restrict__ decisions,
     QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, bool split, double thr,
     uint32_t bb_ksplit, bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
-    DevCounters* __restrict__ ctr) {
+    const uint32_t* __restrict__ sel, DevCounters* __restrict__ ctr) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P) return;
     uint32_t bi, bj;
     tri_tile((uint32_t)t, T, bi, bj);
+    // Band and cross calls queue only the tile pairs of their range.
+    if (bi < sel[0] || bi >= sel[1] || bj < sel[2] || bj >= sel[3]) {
+        decisions[t] = 0xFF;
+        return;
+    }
     const TileMeta a = tm[bi], b = tm[bj];
     const uint32_t elig = eligible_mask(a, b, reps);
     int p;
@@ -324,13 +329,13 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
                   bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
-                  DevCounters* ctr) {
+                  const uint32_t* sel, DevCounters* ctr) {
     const uint64_t P = (uint64_t)T * (T + 1) / 2;
     if (P == 0) return STORM_HIP_OK;
     hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, s, tm, T, P,
                        M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions, queue,
                        qstride, inst, tile > kSplitRows, thr, bb_ksplit, bb_wide, explore_seed,
-                       explore_share, ctr);
+                       explore_share, sel, ctr);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

@@ -184,6 +184,11 @@ def lib():
                                              vp, C.POINTER(u32), C.POINTER(u64)]),
         "storm_hip_batch_explore": (i32, [vp, vp, u32, u64, C.c_double, u32, vp, vp, vp, vp,
                                           vp, C.POINTER(u32), vp]),
+        "storm_hip_allpairs_band": (i32, [vp, vp, C.POINTER(Options), u32, u32, vp,
+                                          C.POINTER(Stats)]),
+        "storm_hip_allpairs_cross": (i32, [vp, vp, C.POINTER(Options), u32, vp,
+                                           C.POINTER(Stats)]),
+        "storm_hip_corpus_order": (i32, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -561,6 +566,40 @@ class Corpus:
         check(lib().storm_hip_result_copy_pairs(self.ctx.h, self.h, 0, n.value, _ptr(out)),
               "copy_pairs")
         return st.as_dict(), out
+
+    def allpairs_band(self, band, policy="analytic", order="density", bb="auto", tile=64,
+                      fixed_cell=0, use_graph=False):
+        """(stats, counts): tile row `band` of the upper triangle in the density
+        or identity order, counts[(i - band*tile), j] for positions i < j."""
+        o = Options()
+        o.policy, o.out_mode, o.bb_kernel = POLICY[policy], OUT["sum"], BB[bb]
+        o.tile, o.fixed_cell, o.use_graph = tile, fixed_cell, int(use_graph)
+        rows = max(0, min(tile, self.n - band * tile))
+        out = np.zeros((rows, self.n), dtype=np.uint32)
+        st = Stats()
+        check(lib().storm_hip_allpairs_band(self.ctx.h, self.h, C.byref(o),
+                                            {"density": 0, "identity": 1}[order], band,
+                                            _ptr(out), C.byref(st)), "allpairs_band")
+        return st.as_dict(), out
+
+    def allpairs_cross(self, split, policy="analytic", bb="auto", tile=64, fixed_cell=0,
+                       matrix=True, use_graph=False):
+        """(stats, counts): rows [0, split) against rows [split, n); counts is
+        split x (n - split), or None with matrix=False (sum only)."""
+        o = Options()
+        o.policy, o.out_mode, o.bb_kernel = POLICY[policy], OUT["sum"], BB[bb]
+        o.tile, o.fixed_cell, o.use_graph = tile, fixed_cell, int(use_graph)
+        out = np.zeros((split, self.n - split), dtype=np.uint32) if matrix else None
+        st = Stats()
+        check(lib().storm_hip_allpairs_cross(self.ctx.h, self.h, C.byref(o), split, _ptr(out),
+                                             C.byref(st)), "allpairs_cross")
+        return st.as_dict(), out
+
+    def order(self):
+        """Density order: perm[k] = the row at position k."""
+        out = np.empty(self.n, dtype=np.uint32)
+        check(lib().storm_hip_corpus_order(self.ctx.h, self.h, _ptr(out)), "corpus_order")
+        return out
 
     def tri(self, offset=0, count=None):
         if count is None:
