@@ -3,6 +3,7 @@
 // the engine's per-tile selection on the device. Results equal those of
 // storm.cpp:872-1046 on sorted input.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -33,14 +34,28 @@ uint64_t serialized_bytes(const uint32_t* v, uint32_t n) {
     return total;
 }
 
+// Generations are unique over the process (a freed handle's address may be
+// reused by a new one, its generation never).
+uint64_t next_gen() {
+    static std::atomic<uint64_t> g{0};
+    return ++g;
+}
+
 }  // namespace
 
 struct STORM_s {
     std::vector<std::vector<uint32_t>> rows;   // sorted, distinct
     uint64_t ser_bytes = 8;                    // STORM_serialized_size
+    uint64_t gen = next_gen();                 // renewed by every add / clear
     mutable storm_hip_ctx* ctx = nullptr;
     mutable storm_hip_corpus* corpus = nullptr;
     mutable bool dirty = true;
+    // The last square call with this handle first: the two-set corpus, the
+    // other handle and both generations it was built from.
+    mutable storm_hip_corpus* sq_corpus = nullptr;
+    mutable const STORM_s* sq_other = nullptr;
+    mutable uint64_t sq_gen_self = 0, sq_gen_other = 0;
+    mutable uint32_t sq_split = 0;
 };
 
 namespace {
@@ -91,6 +106,42 @@ int pairwise_sum(storm_hip_ctx* ctx, const std::vector<const std::vector<uint32_
     return STORM_HIP_OK;
 }
 
+// Corpus of the rows of a then b, a padded with empty rows to a multiple
+// of kCrossTile; each side in density order (stable by cardinality), which
+// keeps tiles homogeneous and does not change the sum over the block.
+constexpr uint32_t kCrossTile = 64;
+
+int cross_corpus(storm_hip_ctx* ctx, const STORM_s* a, const STORM_s* b, storm_hip_corpus** out,
+                 uint32_t* split) {
+    std::vector<uint64_t> offs{0};
+    std::vector<uint32_t> pos;
+    uint64_t universe = 1;
+    auto add_side = [&](const STORM_s* h) {
+        std::vector<uint32_t> ord(h->rows.size());
+        for (uint32_t k = 0; k < ord.size(); ++k) ord[k] = k;
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+            return h->rows[x].size() < h->rows[y].size();
+        });
+        for (uint32_t k : ord) {
+            const auto& r = h->rows[k];
+            pos.insert(pos.end(), r.begin(), r.end());
+            offs.push_back(pos.size());
+            if (!r.empty()) universe = std::max<uint64_t>(universe, (uint64_t)r.back() + 1);
+        }
+    };
+    add_side(a);
+    const uint64_t n1 = a->rows.size();
+    *split = (uint32_t)((n1 + kCrossTile - 1) / kCrossTile * kCrossTile);
+    for (uint64_t k = n1; k < *split; ++k) offs.push_back(pos.size());
+    add_side(b);
+    if (offs.size() - 1 > 0xFFFFFFFFull) return STORM_HIP_ERR_RANGE;
+    return storm_hip_corpus_from_lists(ctx, offs.data(), pos.data(), (uint32_t)(offs.size() - 1),
+                                       (uint32_t)universe,
+                                       STORM_HIP_BITMAP_DENSE | STORM_HIP_BUILD_RUNS |
+                                           STORM_HIP_BUILD_EWAH | STORM_HIP_BUILD_COMP,
+                                       out);
+}
+
 uint64_t pairw(STORM_t* h, const char* what) {
     if (h == nullptr) return UINT64_MAX;
     if (h->rows.size() < 2) return 0;
@@ -122,6 +173,7 @@ STORM_t* STORM_new(void) { return new (std::nothrow) STORM_s(); }
 void STORM_free(STORM_t* bitmap) {
     if (bitmap == nullptr) return;
     if (bitmap->corpus) storm_hip_corpus_free(bitmap->corpus);
+    if (bitmap->sq_corpus) storm_hip_corpus_free(bitmap->sq_corpus);
     if (bitmap->ctx) storm_hip_ctx_free(bitmap->ctx);
     delete bitmap;
 }
@@ -148,6 +200,7 @@ int STORM_add(STORM_t* bitmap, const uint32_t* values, const uint32_t n_values) 
         return -4;
     }
     bitmap->dirty = true;
+    bitmap->gen = next_gen();
     return 1;
 }
 
@@ -156,6 +209,7 @@ int STORM_clear(STORM_t* bitmap) {
     bitmap->rows.clear();
     bitmap->ser_bytes = 8;
     bitmap->dirty = true;
+    bitmap->gen = next_gen();
     return 1;
 }
 
@@ -168,24 +222,42 @@ uint64_t STORM_pairw_intersect_cardinality_blocked(STORM_t* bitmap, uint32_t bsi
     return pairw(bitmap, "STORM_pairw_intersect_cardinality_blocked");
 }
 
-// Every (i in the first, j in the second) pair: the pairs of the union minus
-// the pairs inside each side, all exact integers.
+// Every (i in the first, j in the second) pair and nothing else: one corpus
+// holding both sets, and only the tile pairs across the split dispatched
+// (n1 x n2 pair evaluations, as storm.cpp:1027-1046 visits). The corpus is
+// kept on the first handle until either handle changes.
 uint64_t STORM_intersect_cardinality_square(const STORM_t* bitmap1, const STORM_t* bitmap2) {
     if (bitmap1 == nullptr || bitmap2 == nullptr) return 0;
     if (bitmap1->rows.empty() || bitmap2->rows.empty()) return 0;
     int s = ensure_ctx(bitmap1);
-    std::vector<const std::vector<uint32_t>*> a, b, u;
-    for (const auto& r : bitmap1->rows) { a.push_back(&r); u.push_back(&r); }
-    for (const auto& r : bitmap2->rows) { b.push_back(&r); u.push_back(&r); }
-    uint64_t su = 0, sa = 0, sb = 0;
-    if (s == STORM_HIP_OK) s = pairwise_sum(bitmap1->ctx, u, nullptr, &su);
-    if (s == STORM_HIP_OK) s = pairwise_sum(bitmap1->ctx, a, nullptr, &sa);
-    if (s == STORM_HIP_OK) s = pairwise_sum(bitmap1->ctx, b, nullptr, &sb);
+    const bool cached = bitmap1->sq_corpus && bitmap1->sq_other == bitmap2 &&
+                        bitmap1->sq_gen_self == bitmap1->gen &&
+                        bitmap1->sq_gen_other == bitmap2->gen;
+    if (s == STORM_HIP_OK && !cached) {
+        if (bitmap1->sq_corpus) storm_hip_corpus_free(bitmap1->sq_corpus);
+        bitmap1->sq_corpus = nullptr;
+        s = cross_corpus(bitmap1->ctx, bitmap1, bitmap2, &bitmap1->sq_corpus, &bitmap1->sq_split);
+        if (s == STORM_HIP_OK) {
+            bitmap1->sq_other = bitmap2;
+            bitmap1->sq_gen_self = bitmap1->gen;
+            bitmap1->sq_gen_other = bitmap2->gen;
+        }
+    }
+    storm_hip_stats st{};
+    if (s == STORM_HIP_OK) {
+        storm_hip_options o;
+        std::memset(&o, 0, sizeof o);
+        o.policy = STORM_HIP_POLICY_ANALYTIC;
+        o.out_mode = STORM_HIP_OUT_SUM;
+        o.tile = kCrossTile;
+        s = storm_hip_allpairs_cross(bitmap1->ctx, bitmap1->sq_corpus, &o, bitmap1->sq_split,
+                                     nullptr, &st);
+    }
     if (s != STORM_HIP_OK) {
         report(s, "STORM_intersect_cardinality_square");
         return 0;
     }
-    return su - sa - sb;
+    return st.sum;
 }
 
 uint64_t STORM_serialized_size(const STORM_t* bitmap) {

@@ -1,6 +1,7 @@
 // storm::gpu::allpairs_sum / allpairs_tiles against brute force, in the shape
 // of the reference's check_allpairs (tests/test_cells.cpp:237-288): every
-// policy, both API forms, tiles 8 / 16 / 64, several corpus structures.
+// policy, both API forms, tiles 8 / 16 / 64 / 100 / 128 (several tiles per
+// engine band, bands per callback tile), several corpus structures.
 // Needs a GPU; prints one line per failure and exits non-zero on any.
 #include <algorithm>
 #include <cinttypes>
@@ -73,8 +74,22 @@ uint64_t inter(const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) {
     return c;
 }
 
+// The reference's coordinates: density-order positions under PerTile (stable
+// by cardinality, kernels/storm_allpairs.cpp:316-323), row order otherwise.
+std::vector<uint32_t> order_for(Policy pol, const std::vector<std::vector<uint32_t>>& rows) {
+    std::vector<uint32_t> o(rows.size());
+    for (uint32_t k = 0; k < o.size(); ++k) o[k] = k;
+    if (pol == Policy::PerTile)
+        std::stable_sort(o.begin(), o.end(), [&](uint32_t a, uint32_t b) {
+            return rows[a].size() < rows[b].size();
+        });
+    return o;
+}
+
 struct CbState {
-    const std::vector<std::vector<uint32_t>>* rows;
+    const std::vector<uint32_t>* full;   // n x n brute-force counts
+    uint32_t n;
+    std::vector<uint32_t> ord;
     uint32_t tile;
     uint64_t sum = 0, bad = 0, calls = 0;
 };
@@ -88,7 +103,7 @@ void on_tile(uint32_t i0, uint32_t j0, uint32_t nr, uint32_t nc, const uint32_t*
             if (j <= i) continue;
             const uint32_t v = cnt[(size_t)a * s->tile + b];
             s->sum += v;
-            if (v != inter((*s->rows)[i], (*s->rows)[j])) ++s->bad;
+            if (v != (*s->full)[(size_t)s->ord[i] * s->n + s->ord[j]]) ++s->bad;
         }
 }
 
@@ -103,7 +118,7 @@ int main() {
     for (uint32_t u : {1024u, 4033u, 16384u})
         for (int st = 0; st < 3; ++st)
             for (int sp = 0; sp < 2; ++sp) {
-                const auto rows = corpus(40, u, 0.02, st, sp, 0x9911u + u * 7 + st * 3 + sp);
+                const auto rows = corpus(300, u, 0.02, st, sp, 0x9911u + u * 7 + st * 3 + sp);
                 std::vector<uint64_t> offs{0};
                 std::vector<uint32_t> pos;
                 for (const auto& r : rows) {
@@ -114,27 +129,34 @@ int main() {
                 R.offs = offs.data(); R.pos = pos.data(); R.n_rows = (uint32_t)rows.size();
                 R.universe = u;
                 uint64_t want = 0;
-                for (size_t i = 0; i < rows.size(); ++i)
-                    for (size_t j = i + 1; j < rows.size(); ++j) want += inter(rows[i], rows[j]);
+                const size_t n = rows.size();
+                std::vector<uint32_t> full(n * n, 0);
+                for (size_t i = 0; i < n; ++i)
+                    for (size_t j = i + 1; j < n; ++j) {
+                        full[i * n + j] = full[j * n + i] = (uint32_t)inter(rows[i], rows[j]);
+                        want += full[i * n + j];
+                    }
                 char ctx[96];
                 for (Policy pol : {Policy::AllBitmap, Policy::PerTile, Policy::Probe, Policy::Fixed}) {
-                    for (uint32_t tile : {8u, 16u, 64u}) {
+                    for (uint32_t tile : {8u, 16u, 64u, 100u, 128u}) {
                         std::snprintf(ctx, sizeof ctx, "%s u=%u st=%d sp=%d tile=%u", name_of(pol),
                                       u, st, sp, tile);
                         const AllPairsStats s1 = allpairs_sum(eng, R, nullptr, pol, tile, false,
                                                               Pairing::SS);
                         check(s1.status == STORM_HIP_OK, "status", ctx, (uint64_t)-s1.status, 0);
                         check(s1.sum == want, "allpairs_sum", ctx, s1.sum, want);
-                        check(s1.pairs == 40ull * 39 / 2, "pairs", ctx, s1.pairs, 780);
+                        check(s1.pairs == 300ull * 299 / 2, "pairs", ctx, s1.pairs, 300ull * 299 / 2);
                         CbState cs;
-                        cs.rows = &rows;
+                        cs.full = &full;
+                        cs.n = (uint32_t)n;
+                        cs.ord = order_for(pol, rows);
                         cs.tile = tile;
                         const AllPairsStats s2 = allpairs_tiles(eng, R, nullptr, pol, on_tile, &cs, tile);
                         check(s2.status == STORM_HIP_OK, "tiles status", ctx, (uint64_t)-s2.status, 0);
                         check(s2.sum == want, "allpairs_tiles", ctx, s2.sum, want);
                         check(cs.sum == want, "callback sum", ctx, cs.sum, want);
                         check(cs.bad == 0, "callback per-pair", ctx, cs.bad, 0);
-                        const uint64_t T = (40 + tile - 1) / tile;
+                        const uint64_t T = (300 + tile - 1) / tile;
                         check(cs.calls == T * (T + 1) / 2, "callback tiles", ctx, cs.calls, T * (T + 1) / 2);
                     }
                 }

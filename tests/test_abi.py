@@ -79,10 +79,72 @@ def test_cpp_mirror_header_compiles_and_links(tmp_path):
     exported by libstorm_gpu.so (no device touched)."""
     src = tmp_path / "t.cpp"
     src.write_text('#include "storm_allpairs_gpu.h"\n'
-                   "int main(){ storm::gpu::AllPairsStats s; (void)s;\n"
-                   "  auto f = &storm::gpu::allpairs_sum; auto g = &storm::gpu::allpairs_tiles;\n"
-                   "  (void)f; (void)g; return storm::gpu::name_of(storm::gpu::Policy::PerTile)[0] == 'p' ? 0 : 1; }\n")
+                   "using namespace storm::gpu;\n"
+                   "int main(){ AllPairsStats s; (void)s;\n"
+                   "  AllPairsStats (*f)(Engine&, const RowsCSR&, const storm_hip_cost_model*,"
+                   " Policy, uint32_t, bool, Pairing) = &allpairs_sum;\n"
+                   "  AllPairsStats (*g)(Engine&, storm_hip_corpus*, const storm_hip_cost_model*,"
+                   " Policy, TileCallback, void*, uint32_t) = &allpairs_tiles;\n"
+                   "  int (*h)(RowArenas&, uint32_t*) = &infer_universe;\n"
+                   "  Engine& (*e)() = &default_engine;\n"
+                   "  (void)f; (void)g; (void)h; (void)e;\n"
+                   "  return name_of(Policy::PerTile)[0] == 'p' ? 0 : 1; }\n")
     rc = os.system(f"g++ -std=c++17 -Wall -Werror -I{ROOT}/include {src} -o {tmp_path}/t "
                    f"-L{PKG} -lstorm_gpu -lstorm_hip -Wl,-rpath,{PKG}")
     assert rc == 0
     assert os.system(f"{tmp_path}/t") == 0
+
+
+def test_infer_universe_from_row_arenas(tmp_path):
+    """The reference-typed overloads recover the universe a Row vector was
+    built over from build_row's gates (storm_repr.cpp:139,188): zone map iff
+    card*512 > U, complement iff 2*card > U with U - card positions."""
+    src = tmp_path / "u.cpp"
+    src.write_text(r'''
+#include <array>
+#include <cstdio>
+#include "storm_allpairs_gpu.h"
+using namespace storm::gpu;
+// rows: (card, last, has_bitmap, has_occ, comp_len)
+static int run(std::initializer_list<std::array<uint32_t, 5>> rows, uint32_t nw, uint32_t* U,
+               bool* comp) {
+    RowArenas a;
+    a.n_rows = rows.size(); a.n_words = nw;
+    for (auto* o : {&a.list_off, &a.run_off, &a.ewah_off, &a.bitmap_off, &a.rank_off, &a.occ_off,
+                    &a.comp_off}) o->assign(1, 0);
+    for (auto r : rows) {
+        const uint32_t m[8] = {r[0], 1, nw, 1, r[0] ? 0u : 0xFFFFFFFFu, r[1], 0, r[3]};
+        a.meta.insert(a.meta.end(), m, m + 8);
+        a.list_off.push_back(0); a.run_off.push_back(0);
+        a.ewah_off.push_back(a.ewah_off.back() + (r[2] ? 1 : 0));
+        a.bitmap_off.push_back(a.bitmap_off.back() + (r[2] ? nw : 0));
+        a.rank_off.push_back(0);
+        a.occ_off.push_back(a.occ_off.back() + (r[3] ? 1 : 0));
+        a.comp_off.push_back(a.comp_off.back() + r[4]);
+    }
+    const int rc = infer_universe(a, U);
+    *comp = a.has_comp;
+    return rc;
+}
+int main() {
+    uint32_t U = 0; bool comp = false; int bad = 0;
+    // A zone-mapped row with its last bit at 100 over 2 words: U = 101.
+    bad += run({{3, 100, 1, 1, 0}}, 2, &U, &comp) != 0 || U != 101;
+    // A complement of 28 beside a 100-element row fixes U = 128.
+    bad += run({{3, 100, 1, 1, 0}, {100, 127, 1, 1, 28}}, 2, &U, &comp) != 0 || U != 128 || !comp;
+    // Complements disagreeing on U: no universe fits.
+    bad += run({{100, 127, 1, 1, 28}, {90, 127, 1, 1, 30}}, 2, &U, &comp) == 0;
+    // No zone map on a bitmap row: U >= card * 512 (4 * 512 over 32 words).
+    bad += run({{4, 2000, 1, 0, 0}}, 32, &U, &comp) != 0 || U != 2048;
+    // ... which cannot hold over 2 words.
+    bad += run({{4, 100, 1, 0, 0}}, 2, &U, &comp) == 0;
+    // A sparse-only row past half density drops every complement.
+    bad += run({{3, 100, 1, 1, 0}, {60, 110, 0, 0, 0}}, 2, &U, &comp) != 0 || U != 111 || comp;
+    std::printf("%d bad\n", bad);
+    return bad;
+}
+''')
+    rc = os.system(f"g++ -std=c++17 -Wall -Werror -I{ROOT}/include {src} -o {tmp_path}/u "
+                   f"-L{PKG} -lstorm_gpu -lstorm_hip -Wl,-rpath,{PKG}")
+    assert rc == 0
+    assert os.system(f"{tmp_path}/u") == 0

@@ -19,3 +19,20 @@ def test_cpp_allpairs_mirror():
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert " 0 failures" in r.stdout
+
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "test_allpairs_ref")
+
+
+def test_cpp_reference_typed_overloads():
+    """storm::gpu::allpairs_sum / allpairs_tiles with the reference's exact
+    parameter lists against the reference's own driver (built from its
+    sources beside the checker): equal sums, pairs and callback traces."""
+    if not os.path.exists(REF_BIN):
+        pytest.skip("reference-typed test program not built (the reference's sources are "
+                    "needed to build it)")
+    print("case cpp reference-typed overloads", flush=True)
+    r = subprocess.run([REF_BIN], capture_output=True, text=True, timeout=600)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert " 0 failures" in r.stdout

@@ -71,3 +71,56 @@ def test_storm_t_duplicates_count_once():
     h = _fill(G, [v, w])
     assert G.STORM_pairw_intersect_cardinality(h) == 3
     G.STORM_free(h)
+
+
+def _square_want(a, b):
+    ua = [np.unique(x) for x in a]
+    ub = [np.unique(y) for y in b]
+    return sum(len(np.intersect1d(x, y, assume_unique=True)) for x in ua for y in ub)
+
+
+def test_storm_t_square_cross_block_and_cache():
+    """The square form over the n1 x n2 block only: shapes 1 x N, N x 1 and
+    ragged, the same handle on both sides, repeated calls (cached corpus),
+    updates to either side, and a freed handle whose address is reused."""
+    rng = np.random.default_rng(11)
+
+    def vecs(n, u, k):
+        return [np.sort(rng.choice(u, size=int(rng.integers(0, k + 1)), replace=False))
+                .astype(np.uint32) for _ in range(n)]
+
+    G = gpu_storm()
+    a, b = vecs(1, 300000, 4000), vecs(2500, 300000, 900)
+    ha, hb = _fill(G, a), _fill(G, b)
+    print("case storm_t square 1 x N", flush=True)
+    want = _square_want(a, b)
+    assert G.STORM_intersect_cardinality_square(ha, hb) == want
+    assert G.STORM_intersect_cardinality_square(ha, hb) == want      # cached
+    assert G.STORM_intersect_cardinality_square(hb, ha) == want      # N x 1
+    assert G.STORM_intersect_cardinality_square(ha, ha) == _square_want(a, a)
+    # An update on either side is seen.
+    extra = np.arange(0, 300000, 97, dtype=np.uint32)
+    assert G.STORM_add(hb, extra.ctypes.data_as(C.c_void_p), len(extra)) == 1
+    b2 = b + [extra]
+    assert G.STORM_intersect_cardinality_square(ha, hb) == _square_want(a, b2)
+    assert G.STORM_add(ha, extra.ctypes.data_as(C.c_void_p), len(extra)) == 1
+    a2 = a + [extra]
+    assert G.STORM_intersect_cardinality_square(ha, hb) == _square_want(a2, b2)
+    # Free the second side and fill a new handle with other rows.
+    G.STORM_free(hb)
+    c = vecs(70, 300000, 3000)
+    hc = _fill(G, c)
+    print("case storm_t square after free", flush=True)
+    assert G.STORM_intersect_cardinality_square(ha, hc) == _square_want(a2, c)
+    if O.ref_available():
+        R = O.reference()
+        ra, rc = _fill(R, a2), _fill(R, c)
+        assert R.STORM_intersect_cardinality_square(ra, rc) == _square_want(a2, c)
+        R.STORM_free(ra)
+        R.STORM_free(rc)
+    # Ragged: 130 x 77 over a small universe (dense rows, complements).
+    d, e = vecs(130, 5000, 4000), vecs(77, 5000, 2500)
+    hd, he = _fill(G, d), _fill(G, e)
+    assert G.STORM_intersect_cardinality_square(hd, he) == _square_want(d, e)
+    for h in (ha, hc, hd, he):
+        G.STORM_free(h)
