@@ -20,8 +20,15 @@
  * kernels/storm_allpairs.cpp:316-323, with no permutation passed), row order
  * under every other policy.
  *
+ * Policies: AllBitmap, PerTile (analytic per tile), Fixed, Probe (measure
+ * then commit on the device), Refine (per-pair choice between a tile's two
+ * best cells on heterogeneous tiles) and PerPair / Oracle (every pair picks
+ * its own cell) -- storm_hip.h, STORM_HIP_POLICY_*.
+ *
  * Differences a caller sees:
- *   - policies PerPair and Oracle have no GPU form (status UNSUPPORTED);
+ *   - Probe measures tile throughput on sampled tiles and commits per class
+ *     of tiles, not per tile on 3 timed pairs (a GPU's cost is a tile's);
+ *   - Oracle is PerPair: the same model choice, made on the device;
  *   - the engine selects per tile of 64, 128 or 256 rows: a narrower `tile`
  *     (8, 16, 32) selects at 64 and only the callback granularity is `tile`;
  *   - a failure is reported in AllPairsStats::status (a storm_hip status

@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-#define STORM_HIP_ABI_VERSION 2
+#define STORM_HIP_ABI_VERSION 3
 
 /* Status codes. Negative = failure; the reference's C ABI uses the same
  * convention (storm.cpp:1160-1219: NULL handle -1, NULL values -2, bad_alloc -4). */
@@ -249,8 +249,31 @@ enum {
                                          analytic when features are out of
                                          the training range)                 */
     STORM_HIP_POLICY_FIXED      = 3,  /* one cell for every eligible pair     */
-    STORM_HIP_POLICY_EXHAUSTIVE = 4   /* time every eligible cell per tile    */
+    STORM_HIP_POLICY_EXHAUSTIVE = 4,  /* time every eligible cell per tile    */
+    STORM_HIP_POLICY_PROBE      = 5,  /* Policy::Probe: measure, then commit
+                                         (below)                             */
+    STORM_HIP_POLICY_REFINE     = 6,  /* Policy::Refine: per-pair choice
+                                         between a tile's two best cells     */
+    STORM_HIP_POLICY_PER_PAIR   = 7   /* Policy::PerPair / Oracle: every pair
+                                         selects its own cell (below)        */
 };
+/* PROBE (kernels/storm_allpairs.cpp:445-482 measures a tile's two best
+ * candidates on 3 of its pairs and commits the faster for the rest). Timing 3
+ * pairs means nothing on a GPU, whose cost is a tile's throughput, so the
+ * device measures at tile granularity and commits per class of tiles: in a
+ * first pass every tile with one candidate runs, and on a sample of the
+ * others (at least 4 per class, then 1 in 32) the best candidate runs for
+ * real and the runner-up beside it on a shadow copy that writes nothing, each
+ * work item timed; tiles are classed by (best, runner-up, predicted-ratio
+ * bucket); the second pass runs every pending tile with the candidate its
+ * class measured faster (in CU-time). Both passes are one call (capturable).
+ * REFINE (:563-588, :636-641): a tile whose rows spread by max/mean
+ * cardinality >= 8 on either side and whose runner-up is predicted within 3x
+ * lets each pair run the cheaper of the two as priced on the pair's own rows
+ * (one wavefront per pair); other tiles run the best candidate.
+ * PER_PAIR (:645-660): select_pairing on each pair's own rows over every cell
+ * the two rows can run (the reference's PerPair; its Oracle makes the same
+ * model choice off the clock), one wavefront per pair after the span test. */
 enum {
     STORM_HIP_OUT_SUM       = 0,  /* u64 sum only                            */
     STORM_HIP_OUT_UPPER_TRI = 1   /* u32 per pair, original row order,
@@ -296,6 +319,9 @@ typedef struct {
     double   ms_bb;                     /* HIP-event time of B x B launches   */
     double   ms_select;                 /* HIP-event time of selection        */
     uint64_t bb_launches;
+    uint64_t probe_tiles;               /* PROBE: tiles run with both candidates */
+    uint64_t probe_switched;            /* PROBE: pending tiles committed to the
+                                           runner-up                          */
 } storm_hip_stats;
 
 int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c,

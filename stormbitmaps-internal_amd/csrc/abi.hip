@@ -113,7 +113,7 @@ const char* storm_hip_strerror(int s) {
     return "unknown status";
 }
 
-const char* storm_hip_version(void) { return "storm_hip 0.1 gfx950 abi 1"; }
+const char* storm_hip_version(void) { return "storm_hip 0.1 gfx950 abi 3"; }
 
 const char* storm_hip_last_error(void) { return g_err; }
 
@@ -259,7 +259,7 @@ int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
     if (!ctx || !c || c->ctx != ctx) return STORM_HIP_ERR_INVALID;
     storm_hip_options o{};
     if (opt) o = *opt;
-    if (o.policy > STORM_HIP_POLICY_EXHAUSTIVE || o.out_mode > STORM_HIP_OUT_UPPER_TRI ||
+    if (o.policy > STORM_HIP_POLICY_PER_PAIR || o.out_mode > STORM_HIP_OUT_UPPER_TRI ||
         o.bb_kernel > STORM_HIP_BB_MFMA_FP4 || o.fixed_cell >= STORM_HIP_NCELL)
         return STORM_HIP_ERR_INVALID;
     storm_hip_stats local{};

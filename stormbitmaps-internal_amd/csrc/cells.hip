@@ -30,6 +30,7 @@
 
 #include "device_common.h"
 #include "general.h"
+#include "cost_model.h"
 
 namespace stormhip {
 
@@ -973,6 +974,8 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         const DevCorpus D = Ds[e.inst];
         const uint32_t n = D.n, tile = D.tile;
         uint64_t local = 0;
+        // A shadow item (probe runner-up) leaves the call's pair counts alone.
+        const uint64_t pairs0 = pairs, skipped0 = skipped;
         uint32_t i0 = e.bi * tile, j0 = e.bj * tile;
         uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
         if (e.sub) {
@@ -1102,6 +1105,7 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
                 }
             }
         }
+        if (D.shadow) { pairs = pairs0; skipped = skipped0; }
         local = wave_sum_u64(local);
         if (lane == 0 && local) atomicAdd(D.sum, (unsigned long long)local);
         __syncthreads();
@@ -1120,6 +1124,145 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
             atomicAdd(&ctr->skipped, (unsigned long long)skipped);
         }
     }
+}
+
+// One row's metadata as a one-row tile, for per-pair pricing.
+__device__ inline TileMeta row_tile(const DevCorpus& D, uint32_t r) {
+    TileMeta m;
+    m.card = meta(D, r, 0); m.runs = meta(D, r, 1); m.nw = meta(D, r, 2); m.nnz = meta(D, r, 3);
+    m.first = m.card ? meta(D, r, 4) : 0; m.last = meta(D, r, 5);
+    m.has_rank = meta(D, r, 6); m.has_occ = meta(D, r, 7);
+    m.all_bitmap = D.bm_row[r] != kNoBitmap;
+    m.all_comp = 2ull * m.card > D.universe;
+    m.any_empty = m.card == 0;
+    m.max_card = m.min_card = m.card; m.max_runs = m.runs;
+    m.max_ewah = (uint32_t)(D.mk_off[r + 1] - D.mk_off[r]);
+    m.rows = 1;
+    m.spread = 1.0;
+    return m;
+}
+
+// Lane `first` of `stride` of one pair's count under a runtime-chosen cell
+// (B x B: AND + popcount over the words; the rest as their kernels count).
+__device__ uint32_t pair_partial(int cell, const DevCorpus& D, const DevModel& M, uint32_t i,
+                                 uint32_t j, uint32_t first, uint32_t stride) {
+    switch (cell) {
+        case STORM_HIP_CELL_BB: {
+            const uint64_t* a = bitmap_of(D, i);
+            const uint64_t* b = bitmap_of(D, j);
+            uint32_t c = 0;
+            for (uint32_t k = first; k < D.nw; k += stride) c += __popcll(a[k] & b[k]);
+            return c;
+        }
+        case STORM_HIP_CELL_BS: return cell_partial<STORM_HIP_CELL_BS>(D, M, i, j, first, stride);
+        case STORM_HIP_CELL_BR: return cell_partial<STORM_HIP_CELL_BR>(D, M, i, j, first, stride);
+        case STORM_HIP_CELL_BW: return cell_partial<STORM_HIP_CELL_BW>(D, M, i, j, first, stride);
+        case STORM_HIP_CELL_SS: return cell_partial<STORM_HIP_CELL_SS>(D, M, i, j, first, stride);
+        case STORM_HIP_CELL_SR: return cell_partial<STORM_HIP_CELL_SR>(D, M, i, j, first, stride);
+        case STORM_HIP_CELL_SW: return cell_partial<STORM_HIP_CELL_SW>(D, M, i, j, first, stride);
+        case STORM_HIP_CELL_RR: return cell_partial<STORM_HIP_CELL_RR>(D, M, i, j, first, stride);
+        case STORM_HIP_CELL_RW: return cell_partial<STORM_HIP_CELL_RW>(D, M, i, j, first, stride);
+        default: return cell_partial<STORM_HIP_CELL_WW>(D, M, i, j, first, stride);
+    }
+}
+
+// Refined tiles (Policy::Refine, kernels/storm_allpairs.cpp:563-588 and
+// :636-641): per pair, the cheaper of the tile's two candidates as priced on
+// the pair's own rows (denser row first), one wavefront per pair, the span
+// test first. Per-pair tiles (Policy::PerPair / Oracle, :645-660, candidates
+// both kPerPairCell): select_pairing on the pair's rows over every cell the
+// two rows can run. Items are whole tiles or kSplitRows blocks; the item's
+// inst carries the candidates above kRefineShift.
+__global__ __launch_bounds__(256) void refine_kernel(const DevCorpus* __restrict__ Ds, DevModel M,
+                                                     const QItem* __restrict__ queue,
+                                                     DevCounters* __restrict__ ctr,
+                                                     unsigned long long* __restrict__ tile_cycles) {
+    __shared__ uint32_t s_t;
+    __shared__ unsigned long long s_pairs[kNumCells];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < kNumCells) s_pairs[threadIdx.x] = 0;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_t = atomicAdd(&ctr->qhead[kQueueRefine], 1u);
+        __syncthreads();
+        const uint32_t t = s_t;
+        if (t >= ctr->qcount[kQueueRefine]) break;
+        const uint64_t t0 = wall_clock64();
+        const QItem e = queue[t];
+        const uint32_t inst = e.inst & ((1u << kRefineShift) - 1);
+        const int first = (e.inst >> kRefineShift) & 15, second = (e.inst >> (kRefineShift + 4)) & 15;
+        const DevCorpus D = Ds[inst];
+        const uint32_t n = D.n, tile = D.tile;
+        uint32_t i0 = e.bi * tile, j0 = e.bj * tile;
+        uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
+        if (e.sub) {
+            const uint32_t sb = e.sub & 0xFFFFu;
+            const uint32_t oi = ((e.sub >> 16) & 0xFFu) * sb, oj = (e.sub >> 24) * sb;
+            ni = oi < ni ? min(ni - oi, sb) : 0u;
+            nj = oj < nj ? min(nj - oj, sb) : 0u;
+            i0 += oi;
+            j0 += oj;
+        }
+        const bool same_tile = e.bi == e.bj;
+        uint64_t local = 0;
+        for (uint32_t p = wave; p < ni * nj; p += blockDim.x / 64) {
+            const uint32_t a = i0 + p / nj, b = j0 + p % nj;
+            if (same_tile && b <= a) continue;
+            const uint32_t i = D.perm[a], j = D.perm[b];
+            uint32_t cnt = 0;
+            int cell = STORM_HIP_CELL_EMPTY;
+            if (!span_disjoint(D, i, j)) {
+                const bool i_dense = meta(D, i, 0) >= meta(D, j, 0);
+                const TileMeta md = row_tile(D, i_dense ? i : j), ms = row_tile(D, i_dense ? j : i);
+                if (first == kPerPairCell) {
+                    const uint32_t elig = eligible_mask(md, ms, D.reps) & 0x3FFu;
+                    double c;
+                    cell = select_pairing(M, md, ms, &c);
+                    if (cell == STORM_HIP_CELL_EMPTY || !(elig >> cell & 1))
+                        cell = cheapest_eligible(M, md, ms, elig);
+                } else {
+                    cell = predict(M, second, md, ms) < predict(M, first, md, ms) ? second : first;
+                }
+                cnt = wave_sum_u32(pair_partial(cell, D, M, i, j, lane, 64));
+            }
+            if (lane == 0) {
+                atomicAdd(&s_pairs[cell], 1ull);
+                local += cnt;
+                emit_pair(D, i, j, cnt);
+            }
+        }
+        local = wave_sum_u64(local);
+        if (lane == 0 && local) atomicAdd(D.sum, (unsigned long long)local);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint64_t now = wall_clock64();
+            if (tile_cycles) tile_cycles[t] = now - t0;
+            if (D.finish) atomicMax(D.finish, (unsigned long long)now);
+        }
+    }
+    // Pairs per cell run; span-test skips count as EMPTY and as skipped.
+    if (threadIdx.x < kNumCells && s_pairs[threadIdx.x]) {
+        atomicAdd(&ctr->cell_pairs[threadIdx.x], s_pairs[threadIdx.x]);
+        if (threadIdx.x == STORM_HIP_CELL_EMPTY) atomicAdd(&ctr->skipped, s_pairs[threadIdx.x]);
+    }
+}
+
+int launch_refine(hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
+                  const QItem* queue, DevCounters* ctr, unsigned long long* tile_cycles) {
+    hipLaunchKernelGGL(refine_kernel, dim3(grid), dim3(256), 0, s, Ds, M, queue, ctr, tile_cycles);
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
+uint32_t refine_blocks_per_cu() {
+    static std::atomic<int> cache{0};
+    int v = cache.load(std::memory_order_relaxed);
+    if (v > 0) return (uint32_t)v;
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, refine_kernel, 256, 0);
+    v = (e == hipSuccess && nb > 0) ? nb : 4;
+    cache.store(v, std::memory_order_relaxed);
+    return (uint32_t)v;
 }
 
 // Workgroups of a cell kernel resident on one CU at once (registers, LDS and

@@ -63,6 +63,11 @@ struct DevCorpus {
     uint32_t emit_kind, emit_cols, emit_split;
     uint32_t* emit_buf;
     const uint32_t* inv;
+    // Shadow instance of a probe call: the same corpus, no output (tri, hits
+    // and emit off, sum into a scratch counter, pair counts not recorded).
+    uint32_t shadow;
+    uint16_t* probe;                  // per-tile probe state of a probe call
+    uint32_t reps;                    // representations held (kRep* bits)
 };
 
 // Per-pair output of a general call: the packed triangle, or the pair itself
@@ -159,6 +164,7 @@ struct TileMeta {
     uint32_t has_rank, has_occ;                  // AND over rows
     uint32_t all_bitmap, all_comp, any_empty;    // AND / AND / OR over rows
     uint32_t max_card, min_card, max_runs, max_ewah, rows;
+    double spread;                               // max / mean cardinality
 };
 
 // B x S probes the lists of side A (else B) against the other side's bitmaps:
@@ -222,7 +228,38 @@ struct DevModel {
 
 // Internal policy of exploration calls: each non-empty tile pair runs a cell
 // drawn (seeded) from the cells the learned selector may rank on it.
-constexpr uint32_t kPolicyExplore = 5;
+constexpr uint32_t kPolicyExplore = 16;
+// The two selection passes of a PROBE call: pass 1 runs every tile that has
+// a single candidate and, on the sampled tiles, both of its two best
+// candidates (the second on the corpus's shadow instance, which writes no
+// output); pass 2 commits the pending tiles from the measured times.
+constexpr uint32_t kPolicyProbe1 = 17, kPolicyProbe2 = 18;
+// Probe classes: (first, second, predicted-ratio bucket) of a tile's two best
+// candidates; sampled tiles of a class decide for its pending tiles.
+constexpr uint32_t kProbeBuckets = 4;
+constexpr uint32_t kProbeClasses = 10 * 10 * kProbeBuckets;
+// Every class samples at least this many tiles, then one tile in 2^24 /
+// kProbeShare more.
+constexpr uint32_t kProbeMinSamples = 4;
+constexpr uint32_t kProbeShare = 1u << 19;   // 1 / 32
+// Per-tile probe state (u16): first | second << 4 | bucket << 8, plus
+// kProbeSampled or kProbePending; 0 for a tile settled in pass 1.
+constexpr uint16_t kProbeSampled = 1u << 12, kProbePending = 1u << 13;
+
+struct ProbeAcc {
+    unsigned long long ns[2][kProbeClasses];   // CU-time of first / second
+    unsigned int tiles[kProbeClasses];         // tiles seen per class
+    unsigned long long sampled, switched;      // tiles sampled; committed to second
+};
+
+// Workgroups per CU of each cell's kernel in this call (B x B: the launched
+// form), to turn an item's wall time into CU-time.
+struct ProbeBpc { float v[kNumCells]; };
+
+// Refine (kernels/storm_allpairs.cpp:563-588): a tile whose rows spread by at
+// least kRefineHet (max / mean cardinality) and whose two best candidates
+// are within kRefineRatio of each other lets each pair pick between them.
+constexpr double kRefineHet = 8.0, kRefineRatio = 3.0;
 
 // Learned selection ranks only cells predicted (analytically) within this
 // factor of the cheapest eligible cell on the tile.
@@ -233,7 +270,6 @@ constexpr double kLearnedPrune = 32.0;
 // below the calibrated analytic model's); a positive margin keeps the
 // analytic choice on near ties.
 constexpr float kLearnedMargin = 0.0f;
-__host__ __device__ double predict(const DevModel& m, int p, const TileMeta& a, const TileMeta& b);
 __host__ __device__ uint32_t learned_candidates(const DevModel& m, const TileMeta& a,
                                                 const TileMeta& b, uint32_t elig);
 
@@ -241,7 +277,13 @@ __host__ __device__ uint32_t learned_candidates(const DevModel& m, const TileMet
 // words or more) go to a queue of their own, run by the int8 matrix-core
 // kernel, so the other corpora of a batch keep the fp4 one.
 constexpr int kQueueBBWide = kNumCells;
-constexpr int kNumQueues = kNumCells + 1;
+// Refined tiles: each pair runs the cheaper (by its own rows' metadata) of
+// the tile's two candidates, named in the item's inst (kRefineShift).
+constexpr int kQueueRefine = kNumCells + 1;
+constexpr int kNumQueues = kNumCells + 2;
+constexpr uint32_t kRefineShift = 24;
+// Candidates of a per-pair item (PerPair / Oracle): every cell, per pair.
+constexpr int kPerPairCell = 15;
 
 // Counters written by the selection and cell kernels.
 struct DevCounters {
@@ -249,8 +291,14 @@ struct DevCounters {
     unsigned long long cell_tiles[kNumCells];
     unsigned long long skipped;
     unsigned long long fallback_tiles;
-    unsigned int qcount[kNumCells + 1];   // items queued per cell (+ wide B x B)
-    unsigned int qhead[kNumCells + 1];    // next item to take, per queue
+    unsigned int qcount[kNumQueues];   // items queued per queue
+    unsigned int qhead[kNumQueues];    // next item to take, per queue
 };
+
+// Probe: fold pass 1's timed items into their classes, then empty the
+// queues for pass 2.
+int launch_probe_reduce(hipStream_t s, const DevCorpus* Ds, uint32_t n_inst, const QItem* queue,
+                        uint64_t qstride, const DevCounters* ctr,
+                        const unsigned long long* ticks, const ProbeBpc& bpc, ProbeAcc* acc);
 
 }  // namespace stormhip

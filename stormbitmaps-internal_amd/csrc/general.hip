@@ -27,7 +27,11 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
                   bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
-                  const uint32_t* sel, DevCounters* ctr);
+                  const uint32_t* sel, DevCounters* ctr, uint16_t* probe, ProbeAcc* acc,
+                  uint32_t shadow_inst);
+int launch_refine(hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
+                  const QItem* queue, DevCounters* ctr, unsigned long long* tile_cycles);
+uint32_t refine_blocks_per_cu();
 int launch_compose_rowmap(hipStream_t s, const uint32_t* order, const uint32_t* bm_row,
                           uint32_t npad, uint32_t* rowmap);
 int launch_cell(int cell, hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
@@ -60,11 +64,12 @@ struct Work {
     uint32_t* rowmap = nullptr;
     float* learned_cost = nullptr;
     uint8_t* learned_ok = nullptr;
+    uint16_t* probe = nullptr;        // per-tile probe state (PROBE calls)
 };
 
 void free_work(Work* w) {
     if (!w) return;
-    void* p[] = {w->tm, w->decisions, w->rowmap, w->learned_cost, w->learned_ok};
+    void* p[] = {w->tm, w->decisions, w->rowmap, w->learned_cost, w->learned_ok, w->probe};
     for (void* x : p) if (x) (void)hipFree(x);
     delete w;
 }
@@ -83,6 +88,7 @@ int make_work(const storm_hip_corpus* c, uint32_t tile, Work** out) {
     SH_HIP(hipMalloc(&w->rowmap, sizeof(uint32_t) * w->npad));
     SH_HIP(hipMalloc(&w->learned_cost, sizeof(float) * P1 * 10));
     SH_HIP(hipMalloc(&w->learned_ok, P1));
+    SH_HIP(hipMalloc(&w->probe, sizeof(uint16_t) * P1));
     return STORM_HIP_OK;
 }
 
@@ -135,17 +141,23 @@ struct Plan {
     uint32_t sel_dev[4] = {0, 0, 0, 0};
     uint32_t* d_sel = nullptr;
     unsigned long long* ticks = nullptr;
+    // Probe calls: per-class measurements, and the shadow instances' sum
+    // (d_corpora holds each corpus's shadow at index k + n).
+    ProbeAcc* acc = nullptr;
+    unsigned long long* shadow_sum = nullptr;
 };
 
 void free_plan(Plan* p) {
     if (!p) return;
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
     for (Work* w : p->works) free_work(w);
-    void* q[] = {p->d_corpora, p->queue, p->ctr,   p->sums,
-                 p->finish,    p->thr_buf, p->ticks, p->d_sel};
+    void* q[] = {p->d_corpora, p->queue,   p->ctr,   p->sums,  p->finish,
+                 p->thr_buf,   p->ticks,   p->d_sel, p->acc,   p->shadow_sum};
     for (void* x : q) if (x) (void)hipFree(x);
     delete p;
 }
+
+uint32_t reps_of(const storm_hip_corpus* c);
 
 DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
                      unsigned long long* sum, unsigned long long* finish, const Thresh& th) {
@@ -165,7 +177,8 @@ DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
     D.emit_kind = th.kind; D.emit_cols = th.cols; D.emit_split = th.split;
     D.emit_buf = th.buf; D.inv = th.inv;
     D.slice = c->d_slice; D.slice_of = c->d_slice_of;
-    D.tm = w->tm; D.rowmap = w->rowmap; D.tile = w->tile;
+    D.tm = w->tm; D.rowmap = w->rowmap; D.tile = w->tile; D.probe = w->probe;
+    D.reps = reps_of(c);
     D.tri = tri; D.sum = sum; D.finish = finish;
     // K slices for wide 256-row B x B tiles (not in thresholded calls, whose
     // B x B tiles test each pair from the whole count).
@@ -226,18 +239,32 @@ int build_plan(Plan*& p, const std::vector<storm_hip_corpus*>& cs,
     for (storm_hip_corpus* c : cs)
         if (tile == 256 && !th.emits()) split = std::max<uint64_t>(split, bb_slices_for(c->n_words));
     p->qstride = P1 * split;
-    SH_HIP(hipMalloc(&p->d_corpora, sizeof(DevCorpus) * std::max<size_t>(n, 1)));
+    SH_HIP(hipMalloc(&p->d_corpora, sizeof(DevCorpus) * std::max<size_t>(2 * n, 1)));
     SH_HIP(hipMalloc(&p->queue, sizeof(QItem) * p->qstride * kNumQueues));
     SH_HIP(hipMalloc(&p->ctr, sizeof(DevCounters)));
     SH_HIP(hipMalloc(&p->d_sel, sizeof p->sel));
+    SH_HIP(hipMalloc(&p->acc, sizeof(ProbeAcc)));
+    SH_HIP(hipMalloc(&p->shadow_sum, 8));
     SH_HIP(hipMalloc(&p->sums, 8 * std::max<size_t>(n, 1)));
     SH_HIP(hipMalloc(&p->finish, 8 * (n + 1)));
     p->h_corpora.resize(n);
     for (size_t k = 0; k < n; ++k)
         p->h_corpora[k] =
             dev_corpus(cs[k], p->works[k], tris[k], p->sums + k, p->finish + k, th);
+    // Shadow instances: the same arrays, no output of any kind.
+    std::vector<DevCorpus> all(p->h_corpora);
+    for (size_t k = 0; k < n; ++k) {
+        DevCorpus D = p->h_corpora[k];
+        D.tri = nullptr;
+        D.hits = nullptr; D.n_hits = nullptr; D.hit_cap = 0;
+        D.emit_kind = 0; D.emit_buf = nullptr;
+        D.sum = p->shadow_sum;
+        D.finish = nullptr;
+        D.shadow = 1;
+        all.push_back(D);
+    }
     if (n)
-        SH_HIP(hipMemcpy(p->d_corpora, p->h_corpora.data(), sizeof(DevCorpus) * n,
+        SH_HIP(hipMemcpy(p->d_corpora, all.data(), sizeof(DevCorpus) * 2 * n,
                          hipMemcpyHostToDevice));
     return STORM_HIP_OK;
 }
@@ -290,57 +317,11 @@ bool bb_wide(const storm_hip_corpus* c, const storm_hip_options& o, uint32_t til
     return tile == 256 && !thr && o.bb_kernel != STORM_HIP_BB_MFMA && c->n_words >= (1u << 18);
 }
 
-// The kernel sequence of one selecting call on the context's stream.
-int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, bool timed) {
+// Dispatch the queued work items: every queue's persistent kernel, on three
+// streams, joined back into the context's stream.
+int dispatch_queues(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, const DevModel& M,
+                    bool fp4, bool timed, bool with_ticks) {
     hipStream_t s = ctx->stream;
-    const DevModel M = dev_model(ctx->model);
-    const size_t n = p->cs.size();
-    SH_TRY(zero_words(s, p->ctr, sizeof(DevCounters)));
-    SH_TRY(zero_words(s, p->sums, 8 * n));
-    SH_TRY(zero_words(s, p->finish, 8 * n));
-    if (p->th.n_hits) SH_TRY(zero_words(s, p->th.n_hits, 8));
-    if (p->th.kind) SH_TRY(zero_words(s, p->th.buf, p->th.buf_bytes));
-    hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, p->finish + n);
-    SH_HIP(hipGetLastError());
-    if (timed) SH_HIP(hipEventRecord(ctx->ev[4], s));
-    uint32_t max_words = 0;
-    for (size_t k = 0; k < n; ++k) {
-        storm_hip_corpus* c = p->cs[k];
-        Work* w = p->works[k];
-        SH_TRY(launch_tile_meta(s, p->h_corpora[k], w->tile, w->T, w->tm));
-        const float* lc = nullptr;
-        const uint8_t* lok = nullptr;
-        if (o.policy == STORM_HIP_POLICY_LEARNED) {
-            bool have = false;
-            SH_TRY(learned_scores(ctx, w->tm, w->T, w->P, c->universe, w->learned_cost,
-                                  w->learned_ok, &have));
-            if (have) { lc = w->learned_cost; lok = w->learned_ok; }
-        }
-        // Corpora whose B x B tiles run as K slices add into a zeroed triangle.
-        const uint32_t ks = p->h_corpora[k].bb_ksplit;
-        if (ks > 1 && p->h_corpora[k].tri)
-            SH_TRY(zero_words(s, p->h_corpora[k].tri, (uint64_t)c->n_rows * (c->n_rows - 1) / 2 * 4));
-        SH_TRY(launch_select(s, w->tm, w->T, M, reps_of(c), o.policy, o.fixed_cell, lc, lok,
-                             w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
-                             p->th.thr, ks, bb_wide(c, o, p->tile, thresholded_plan(p)),
-                             o.policy == kPolicyExplore ? p->explore_seed : 0, p->explore_share,
-                             p->d_sel, p->ctr));
-        SH_TRY(launch_compose_rowmap(s, p->h_corpora[k].perm, c->d_bm_row, w->npad, w->rowmap));
-        // Only corpora with bitmap rows can queue B x B tiles.
-        if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
-    }
-    // fp4 with f32 accumulation is exact below 2^18 words (counts < 2^24);
-    // 256-row B x B tiles run on the matrix cores, fp4 unless int8 is asked
-    // for, and the tiles of wider corpora in the int8 queue (in thresholded
-    // calls every corpus takes the one format the widest allows).
-    const bool thr_call = thresholded_plan(p);
-    const bool fp4 = o.bb_kernel != STORM_HIP_BB_MFMA && (!thr_call || max_words < (1u << 18));
-    if (p->tile == 256)
-        for (storm_hip_corpus* c : p->cs)
-            if (c->n_bitmap_rows)
-                SH_TRY(bb_mfma_check(fp4 && !bb_wide(c, o, p->tile, thr_call),
-                                     ctx->mfma.stage_words, c->n_words, c->ld));
-    if (timed) SH_HIP(hipEventRecord(ctx->ev[5], s));
     // Dispatch on three streams so the classes overlap: B x B first on side
     // stream 0 (its tiles are the longest, and one launch holds at most one
     // workgroup per CU), the cell classes alternating between the main stream
@@ -356,16 +337,15 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     const uint32_t gbb = (uint32_t)std::min<uint64_t>(
         p->qstride, (uint64_t)ctx->cu_count * bb_blocks_per_cu(p->tile, ctx->mfma));
     const bool thresholded = p->th.emits();
-    // Per-item times of an exploration call, per queue.
+    // Per-item times (exploration and probe calls), per queue.
     auto ticks_of = [&](int q) -> unsigned long long* {
-        return (o.policy == kPolicyExplore && p->ticks) ? p->ticks + (uint64_t)q * p->qstride
-                                                        : nullptr;
+        return (with_ticks && p->ticks) ? p->ticks + (uint64_t)q * p->qstride : nullptr;
     };
     if (p->tile == 256 && thresholded)
         rc = launch_bb_mfma_queue(fp4, ctx->mfma, sb, p->d_corpora, qbb,
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
-                                  &p->ctr->qhead[STORM_HIP_CELL_BB], p->thr_grid, nullptr,
-                                  p->thr_buf);
+                                  &p->ctr->qhead[STORM_HIP_CELL_BB], p->thr_grid,
+                                  ticks_of(STORM_HIP_CELL_BB), p->thr_buf);
     else if (p->tile == 256)
         rc = launch_bb_mfma_queue(fp4, ctx->mfma, sb, p->d_corpora, qbb,
                                   &p->ctr->qcount[STORM_HIP_CELL_BB],
@@ -402,6 +382,15 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
             rc = launch_cell(cell, side ? ctx->aux[1] : s, grid, p->d_corpora, M,
                              p->queue + (uint64_t)cell * p->qstride, p->ctr, ticks_of(cell));
         }
+    // Refined and per-pair tiles (REFINE and PER_PAIR calls only queue any).
+    if (rc == STORM_HIP_OK &&
+        (o.policy == STORM_HIP_POLICY_REFINE || o.policy == STORM_HIP_POLICY_PER_PAIR)) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(
+            p->qstride, (uint64_t)ctx->cu_count * refine_blocks_per_cu());
+        rc = launch_refine(s, grid, p->d_corpora, M,
+                           p->queue + (uint64_t)kQueueRefine * p->qstride, p->ctr,
+                           ticks_of(kQueueRefine));
+    }
     // Tiles settled empty (zeros only) go last.
     if (rc == STORM_HIP_OK) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(
@@ -421,10 +410,102 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
     return rc;
 }
 
+// Workgroups each class shares a CU with while its queue drains: the
+// divisor that turns an item's wall time into CU-time.
+ProbeBpc probe_bpc(storm_hip_ctx* ctx, const Plan* p) {
+    ProbeBpc b;
+    const uint32_t cu = (uint32_t)std::max(1, ctx->cu_count);
+    for (int c = 0; c < kNumCells; ++c) {
+        const uint32_t per_cu = c == STORM_HIP_CELL_BB ? bb_blocks_per_cu(p->tile, ctx->mfma)
+                                                       : cell_blocks_per_cu(c);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(p->qstride, (uint64_t)cu * per_cu);
+        b.v[c] = (float)std::max(1.0, std::min((double)per_cu, (double)grid / cu));
+    }
+    return b;
+}
+
+// The kernel sequence of one selecting call on the context's stream. PROBE
+// runs two select-and-dispatch passes with the measurement between them.
+int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, bool timed) {
+    hipStream_t s = ctx->stream;
+    const DevModel M = dev_model(ctx->model);
+    const size_t n = p->cs.size();
+    const bool probe = o.policy == STORM_HIP_POLICY_PROBE;
+    SH_TRY(zero_words(s, p->ctr, sizeof(DevCounters)));
+    SH_TRY(zero_words(s, p->sums, 8 * n));
+    SH_TRY(zero_words(s, p->finish, 8 * n));
+    if (p->th.n_hits) SH_TRY(zero_words(s, p->th.n_hits, 8));
+    if (p->th.kind) SH_TRY(zero_words(s, p->th.buf, p->th.buf_bytes));
+    if (probe) SH_TRY(zero_words(s, p->acc, sizeof(ProbeAcc)));
+    hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, s, p->finish + n);
+    SH_HIP(hipGetLastError());
+    if (timed) SH_HIP(hipEventRecord(ctx->ev[4], s));
+    const uint32_t sel_policy = probe ? kPolicyProbe1 : o.policy;
+    uint32_t max_words = 0;
+    for (size_t k = 0; k < n; ++k) {
+        storm_hip_corpus* c = p->cs[k];
+        Work* w = p->works[k];
+        SH_TRY(launch_tile_meta(s, p->h_corpora[k], w->tile, w->T, w->tm));
+        const float* lc = nullptr;
+        const uint8_t* lok = nullptr;
+        if (o.policy == STORM_HIP_POLICY_LEARNED) {
+            bool have = false;
+            SH_TRY(learned_scores(ctx, w->tm, w->T, w->P, c->universe, w->learned_cost,
+                                  w->learned_ok, &have));
+            if (have) { lc = w->learned_cost; lok = w->learned_ok; }
+        }
+        // Corpora whose B x B tiles run as K slices add into a zeroed triangle.
+        const uint32_t ks = p->h_corpora[k].bb_ksplit;
+        if (ks > 1 && p->h_corpora[k].tri)
+            SH_TRY(zero_words(s, p->h_corpora[k].tri, (uint64_t)c->n_rows * (c->n_rows - 1) / 2 * 4));
+        SH_TRY(launch_select(s, w->tm, w->T, M, reps_of(c), sel_policy, o.fixed_cell, lc, lok,
+                             w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
+                             p->th.thr, ks, bb_wide(c, o, p->tile, thresholded_plan(p)),
+                             o.policy == kPolicyExplore ? p->explore_seed
+                                                        : (probe ? kGolden : 0),
+                             p->explore_share, p->d_sel, p->ctr, probe ? w->probe : nullptr,
+                             p->acc, (uint32_t)(k + n)));
+        SH_TRY(launch_compose_rowmap(s, p->h_corpora[k].perm, c->d_bm_row, w->npad, w->rowmap));
+        // Only corpora with bitmap rows can queue B x B tiles.
+        if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
+    }
+    // fp4 with f32 accumulation is exact below 2^18 words (counts < 2^24);
+    // 256-row B x B tiles run on the matrix cores, fp4 unless int8 is asked
+    // for, and the tiles of wider corpora in the int8 queue (in thresholded
+    // calls every corpus takes the one format the widest allows).
+    const bool thr_call = thresholded_plan(p);
+    const bool fp4 = o.bb_kernel != STORM_HIP_BB_MFMA && (!thr_call || max_words < (1u << 18));
+    if (p->tile == 256)
+        for (storm_hip_corpus* c : p->cs)
+            if (c->n_bitmap_rows)
+                SH_TRY(bb_mfma_check(fp4 && !bb_wide(c, o, p->tile, thr_call),
+                                     ctx->mfma.stage_words, c->n_words, c->ld));
+    if (timed) SH_HIP(hipEventRecord(ctx->ev[5], s));
+    int rc = dispatch_queues(ctx, p, o, M, fp4, timed,
+                             o.policy == kPolicyExplore || (probe && p->ticks));
+    if (rc != STORM_HIP_OK || !probe) return rc;
+    // Probe: fold pass 1's times into the classes, then commit the pending
+    // tiles and run them.
+    SH_TRY(launch_probe_reduce(s, p->d_corpora, (uint32_t)n, p->queue, p->qstride, p->ctr,
+                               p->ticks, probe_bpc(ctx, p), p->acc));
+    for (size_t k = 0; k < n; ++k) {
+        storm_hip_corpus* c = p->cs[k];
+        Work* w = p->works[k];
+        SH_TRY(launch_select(s, w->tm, w->T, M, reps_of(c), kPolicyProbe2, o.fixed_cell, nullptr,
+                             nullptr, w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
+                             p->th.thr, p->h_corpora[k].bb_ksplit,
+                             bb_wide(c, o, p->tile, thresholded_plan(p)), 0, 0, p->d_sel, p->ctr,
+                             w->probe, p->acc, (uint32_t)(k + n)));
+    }
+    return dispatch_queues(ctx, p, o, M, fp4, false, false);
+}
+
 // Run the plan (graph or eager), then read sums, counters, decisions, latencies.
 int run_plan(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, storm_hip_stats* st,
              uint64_t* sums, double* latency_ms) {
     hipStream_t s = ctx->stream;
+    if (o.policy == STORM_HIP_POLICY_PROBE && !p->ticks)
+        SH_HIP(hipMalloc(&p->ticks, sizeof(unsigned long long) * p->qstride * kNumQueues));
     if (std::memcmp(p->sel, p->sel_dev, sizeof p->sel) != 0) {
         SH_HIP(hipMemcpyAsync(p->d_sel, p->sel, sizeof p->sel, hipMemcpyHostToDevice, s));
         std::memcpy(p->sel_dev, p->sel, sizeof p->sel);
@@ -454,7 +535,10 @@ int run_plan(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, storm_hip_
     const size_t n = p->cs.size();
     std::vector<unsigned long long> hs(n), hf(n + 1);
     DevCounters h;
+    unsigned long long probe_st[2] = {0, 0};
     SH_HIP(hipMemcpyAsync(&h, p->ctr, sizeof h, hipMemcpyDeviceToHost, s));
+    if (o.policy == STORM_HIP_POLICY_PROBE)
+        SH_HIP(hipMemcpyAsync(probe_st, &p->acc->sampled, sizeof probe_st, hipMemcpyDeviceToHost, s));
     if (n) SH_HIP(hipMemcpyAsync(hs.data(), p->sums, 8 * n, hipMemcpyDeviceToHost, s));
     SH_HIP(hipMemcpyAsync(hf.data(), p->finish, 8 * (n + 1), hipMemcpyDeviceToHost, s));
     for (size_t k = 0; k < n; ++k) {
@@ -481,6 +565,8 @@ int run_plan(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, storm_hip_
     st->fallback_tiles = h.fallback_tiles;
     st->decisions = decisions;
     st->bb_launches = (h.qcount[STORM_HIP_CELL_BB] ? 1 : 0) + (h.qcount[kQueueBBWide] ? 1 : 0);
+    st->probe_tiles = probe_st[0];
+    st->probe_switched = probe_st[1];
     if (!o.use_graph) {
         float ms = 0;
         hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
@@ -521,7 +607,7 @@ int explore_plan(storm_hip_ctx* ctx, Plan* p, uint32_t tile, uint64_t seed, doub
     const uint32_t cu = (uint32_t)std::max(1, ctx->cu_count);
     for (int q = 0; q < kNumQueues; ++q) {
         const uint32_t cnt = h.qcount[q];
-        if (!cnt || q == STORM_HIP_CELL_EMPTY) continue;
+        if (!cnt || q == STORM_HIP_CELL_EMPTY || q == kQueueRefine) continue;
         const int cell = q == kQueueBBWide ? STORM_HIP_CELL_BB : q;
         std::vector<QItem> items(cnt);
         std::vector<unsigned long long> ticks(cnt);
@@ -690,7 +776,7 @@ int range_call_options(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
                        storm_hip_options* o, uint32_t* tile) {
     *o = storm_hip_options{};
     if (opt) *o = *opt;
-    if (o->policy > STORM_HIP_POLICY_EXHAUSTIVE || o->bb_kernel > STORM_HIP_BB_MFMA_FP4 ||
+    if (o->policy > STORM_HIP_POLICY_PER_PAIR || o->bb_kernel > STORM_HIP_BB_MFMA_FP4 ||
         o->fixed_cell >= STORM_HIP_NCELL)
         return STORM_HIP_ERR_INVALID;
     if (c->dense) {
@@ -783,8 +869,11 @@ int storm_hip_batch_allpairs(storm_hip_ctx* ctx, storm_hip_batch* b, const storm
                              storm_hip_stats* st, uint64_t* sums, double* latency_ms) {
     if (!ctx || !b || b->ctx != ctx || !opt) return STORM_HIP_ERR_INVALID;
     const storm_hip_options& o = *opt;
+    if (o.policy > STORM_HIP_POLICY_PER_PAIR || o.bb_kernel > STORM_HIP_BB_MFMA_FP4 ||
+        o.fixed_cell >= STORM_HIP_NCELL)
+        return STORM_HIP_ERR_INVALID;
     if (o.policy == STORM_HIP_POLICY_ALL_BITMAP || o.policy == STORM_HIP_POLICY_EXHAUSTIVE) {
-        set_error("batched calls select per tile (analytic, learned or fixed)");
+        set_error("batched calls take a selecting policy (not all-bitmap or exhaustive)");
         return STORM_HIP_ERR_UNSUPPORTED;
     }
     const uint32_t tile = o.tile ? o.tile : 64;
@@ -1262,11 +1351,11 @@ int storm_hip_allpairs_threshold(storm_hip_ctx* ctx, storm_hip_corpus* c,
     }
     storm_hip_options o{};
     if (opt) o = *opt;
-    if (o.policy > STORM_HIP_POLICY_EXHAUSTIVE || o.bb_kernel > STORM_HIP_BB_MFMA_FP4 ||
+    if (o.policy > STORM_HIP_POLICY_PER_PAIR || o.bb_kernel > STORM_HIP_BB_MFMA_FP4 ||
         o.fixed_cell >= STORM_HIP_NCELL)
         return STORM_HIP_ERR_INVALID;
-    if (c->dense || (o.policy != STORM_HIP_POLICY_ANALYTIC &&
-                     o.policy != STORM_HIP_POLICY_LEARNED && o.policy != STORM_HIP_POLICY_FIXED)) {
+    if (c->dense || o.policy == STORM_HIP_POLICY_ALL_BITMAP ||
+        o.policy == STORM_HIP_POLICY_EXHAUSTIVE) {
         set_error("thresholded calls take a general corpus and a selecting policy");
         return STORM_HIP_ERR_UNSUPPORTED;
     }

@@ -3,6 +3,7 @@
 // representations actually resident, and the per-cell work queues.
 #include "device_common.h"
 #include "general.h"
+#include "cost_model.h"
 
 namespace stormhip {
 
@@ -70,100 +71,11 @@ __global__ __launch_bounds__(256) void tile_meta_kernel(DevCorpus D, uint32_t ti
         m.max_runs = s32[4][0];
         m.max_ewah = s32[5][0];
         m.rows = hi - lo;
+        // max / mean cardinality (the refine gate's spread, mean in double).
+        const double mean = hi > lo ? (double)s64[0][0] / (double)(hi - lo) : 0.0;
+        m.spread = mean > 0.0 ? (double)m.max_card / mean : 1.0;
         out[t] = m;
     }
-}
-
-// ---- the analytic model (kernels/storm_cost.cpp:195-465) -----------------
-__host__ __device__ inline uint32_t bit_length(uint32_t x) {   // 32 - clz(x)
-    uint32_t b = 0;
-    while (x) { ++b; x >>= 1; }
-    return b;
-}
-
-__host__ __device__ inline double bb_expected_bins(const TileMeta& a, const TileMeta& b) {
-    const double bins = fmax(1.0, (double)max(a.nw, b.nw) / 8.0);
-    const double fa = fmin(1.0, (double)a.nnz / (double)max(1u, a.nw));
-    const double fb = fmin(1.0, (double)b.nnz / (double)max(1u, b.nw));
-    const double pa = fmin(1.0, 8.0 * fa), pb = fmin(1.0, 8.0 * fb);
-    return fmax(1.0, bins * pa * pb * 8.0);
-}
-
-__host__ __device__ double work_units(int p, const TileMeta& a, const TileMeta& b, double margin) {
-    const double sa = a.card, sb = b.card, ra = a.runs, rb = b.runs;
-    const double wa = 2.0 * ra + 1.0 + (double)a.nnz, wb = 2.0 * rb + 1.0 + (double)b.nnz;
-    const bool a_sparse = sa <= sb;
-    const double s_card = a_sparse ? sa : sb;
-    const double d_runs = a_sparse ? rb : ra;
-    const double d_ewah = a_sparse ? wb : wa;
-    switch (p) {
-        case STORM_HIP_CELL_BB: return bb_expected_bins(a, b);
-        case STORM_HIP_CELL_BS: return s_card;
-        case STORM_HIP_CELL_BR: {
-            const double dr = a_sparse ? rb : ra, sr = a_sparse ? ra : rb;
-            return (margin > 0.0 && dr * margin <= sr) ? dr : sr;
-        }
-        case STORM_HIP_CELL_BW: {
-            const double dr = a_sparse ? rb : ra, sr = a_sparse ? ra : rb;
-            const bool sw = margin > 0.0 && dr * margin <= sr;
-            return sw ? (2.0 * dr + 1.0 + (a_sparse ? b.nnz : a.nnz))
-                      : (2.0 * sr + 1.0 + (a_sparse ? a.nnz : b.nnz));
-        }
-        case STORM_HIP_CELL_SS: {
-            const uint32_t mx = (uint32_t)fmax(2.0, fmax(sa, sb));
-            return fmin(sa, sb) * (double)bit_length(mx);
-        }
-        case STORM_HIP_CELL_SR: return s_card + d_runs;
-        case STORM_HIP_CELL_SW: return s_card + d_ewah;
-        case STORM_HIP_CELL_RR: return ra + rb;
-        case STORM_HIP_CELL_RW: return fmin(ra + wb, rb + wa);
-        case STORM_HIP_CELL_WW: return wa + wb;
-        default: return 0.0;
-    }
-}
-
-__host__ __device__ double predict(const DevModel& m, int p, const TileMeta& a, const TileMeta& b) {
-    if (p == STORM_HIP_CELL_EMPTY) return 0.0;
-    if (p == STORM_HIP_CELL_BB && (m.bb_zonemap == 0.0 || !(a.has_occ && b.has_occ)))
-        return m.ns_fixed + m.ns_per_unit[p] * (double)max(a.nw, b.nw);
-    double c = m.ns_fixed + m.ns_per_unit[p] * work_units(p, a, b, m.br_margin) *
-                                (p == STORM_HIP_CELL_RR ? m.rr_scale : 1.0);
-    if (p == STORM_HIP_CELL_BS && m.ns_probe_cold > m.ns_probe_hot) {
-        const double touches = a.card <= b.card ? (double)a.nnz : (double)b.nnz;
-        c += (m.ns_probe_cold - m.ns_probe_hot) * m.probe_scale * touches;
-    }
-    if (p == STORM_HIP_CELL_BB) c += m.ns_per_occ_word * fmax(1.0, (double)max(a.nw, b.nw) / 512.0);
-    return c;
-}
-
-// select_pairing over the reference's candidate set (BB seeded, uncalibrated
-// rates excluded), storm_cost.cpp:422-465. Returns EMPTY on its emptiness proof.
-__device__ int select_pairing(const DevModel& m, const TileMeta& a, const TileMeta& b, double* cost) {
-    if (a.card == 0 || b.card == 0 || a.last < b.first || b.last < a.first) {
-        *cost = 0.0;
-        return STORM_HIP_CELL_EMPTY;
-    }
-    int best = STORM_HIP_CELL_BB;
-    double bc = predict(m, best, a, b);
-    for (int p = 0; p < 10; ++p) {
-        if (m.ns_per_unit[p] <= 0.0) continue;
-        const double c = predict(m, p, a, b);
-        if (c < bc) { bc = c; best = p; }
-    }
-    *cost = bc;
-    return best;
-}
-
-__device__ inline int cheapest_eligible(const DevModel& m, const TileMeta& a, const TileMeta& b,
-                                        uint32_t elig) {
-    int best = -1;
-    double bc = 0.0;
-    for (int p = 0; p < 10; ++p) {
-        if (!(elig >> p & 1)) continue;
-        const double c = predict(m, p, a, b);
-        if (best < 0 || c < bc) { bc = c; best = p; }
-    }
-    return best < 0 ? STORM_HIP_CELL_SS : best;
 }
 
 // Cells the learned selector may rank on a tile: eligible cells whose
@@ -185,15 +97,110 @@ __host__ __device__ uint32_t learned_candidates(const DevModel& m, const TileMet
     return mask;
 }
 
+// The tile's two best eligible candidates by predicted cost (tile_top2,
+// kernels/storm_allpairs.cpp:330-347, over the cells the tile can run):
+// first is select_pairing's choice (EMPTY -> B x B, then the cheapest
+// eligible cell when that one is not resident), second the cheapest other
+// eligible cell with a calibrated rate (first when there is none); ratio =
+// predicted second / first.
+__device__ void tile_top2(const DevModel& M, const TileMeta& a, const TileMeta& b, uint32_t elig,
+                          int* first, int* second, double* ratio) {
+    double c1;
+    int p = select_pairing(M, a, b, &c1);
+    // The mean-based emptiness proof: B x B, no runner-up (tile_top2 returns
+    // Empty for both; the driver runs B x B and lets the span test act).
+    const bool proof = p == STORM_HIP_CELL_EMPTY;
+    if (proof) p = STORM_HIP_CELL_BB;
+    if (!(elig >> p & 1)) p = cheapest_eligible(M, a, b, elig);
+    c1 = predict(M, p, a, b);
+    int q = p;
+    double c2 = 0.0;
+    for (int k = 0; k < 10 && !proof; ++k) {
+        if (k == p || !(elig >> k & 1) || M.ns_per_unit[k] <= 0.0) continue;
+        const double c = predict(M, k, a, b);
+        if (q == p || c < c2) { q = k; c2 = c; }
+    }
+    *first = p;
+    *second = q;
+    *ratio = (q != p && c1 > 0.0) ? c2 / c1 : 1e300;
+}
+
+__device__ inline uint32_t probe_class(int first, int second, double ratio) {
+    const uint32_t bucket = ratio < 1.25 ? 0u : ratio < 2.0 ? 1u : ratio < 4.0 ? 2u : 3u;
+    return ((uint32_t)first * 10u + (uint32_t)second) * kProbeBuckets + bucket;
+}
+
+// Append the work items of tile (bi, bj) run by cell p for instance inst.
+__device__ void enqueue_tile(int p, uint32_t inst, uint32_t bi, uint32_t bj, const TileMeta& a,
+                             const TileMeta& b, QItem* __restrict__ queue, uint64_t qstride,
+                             bool split, uint32_t bb_ksplit, bool bb_wide,
+                             DevCounters* __restrict__ ctr) {
+    if (split && p == STORM_HIP_CELL_BS) {
+        // B x S: stripes of kSplitRows probed rows against the whole other
+        // side (read through its slices).
+        const bool pa = bs_probe_a(a, b);
+        const uint32_t ns = ((pa ? a.rows : b.rows) + kSplitRows - 1) / kSplitRows;
+        uint32_t slot = atomicAdd(&ctr->qcount[p], ns);
+        for (uint32_t x = 0; x < ns; ++x)
+            queue[(uint64_t)p * qstride + slot++] =
+                QItem{inst, bi, bj, pa ? make_sub(kSplitRows, x, kWholeSide)
+                                       : make_sub(kSplitRows, kWholeSide, x)};
+    } else if (split && p != STORM_HIP_CELL_BB && p != STORM_HIP_CELL_EMPTY) {
+        // Per-pair cells: the tile as kSplitRows x kSplitRows blocks (upper
+        // triangle of blocks on a diagonal tile), each its own work item.
+        const uint32_t ni = (a.rows + kSplitRows - 1) / kSplitRows;
+        const uint32_t nj = (b.rows + kSplitRows - 1) / kSplitRows;
+        const uint32_t count = bi == bj ? ni * (ni + 1) / 2 : ni * nj;
+        uint32_t slot = atomicAdd(&ctr->qcount[p], count);
+        for (uint32_t x = 0; x < ni; ++x)
+            for (uint32_t y = bi == bj ? x : 0; y < nj; ++y)
+                queue[(uint64_t)p * qstride + slot++] =
+                    QItem{inst, bi, bj, make_sub(kSplitRows, x, y)};
+    } else if (p == STORM_HIP_CELL_BB) {
+        const uint32_t q = bb_wide ? kQueueBBWide : STORM_HIP_CELL_BB;
+        const uint32_t ks = bb_ksplit > 1 ? bb_ksplit : 1;
+        uint32_t slot = atomicAdd(&ctr->qcount[q], ks);
+        for (uint32_t x = 0; x < ks; ++x)
+            queue[(uint64_t)q * qstride + slot++] =
+                QItem{inst, bi, bj, ks > 1 ? (x | (ks << 16)) : 0u};
+    } else {
+        const uint32_t slot = atomicAdd(&ctr->qcount[p], 1u);
+        queue[(uint64_t)p * qstride + slot] = QItem{inst, bi, bj, 0};
+    }
+}
+
+// Append tile (bi, bj) to the refine queue (blocks of kSplitRows when the
+// tile is split), its candidates in the items' inst.
+__device__ void enqueue_refine(int first, int second, uint32_t inst, uint32_t bi, uint32_t bj,
+                               const TileMeta& a, const TileMeta& b, QItem* __restrict__ queue,
+                               uint64_t qstride, bool split, DevCounters* __restrict__ ctr) {
+    const uint32_t ni = (a.rows + kSplitRows - 1) / kSplitRows;
+    const uint32_t nj = (b.rows + kSplitRows - 1) / kSplitRows;
+    const uint32_t blocks = split ? (bi == bj ? ni * (ni + 1) / 2 : ni * nj) : 1u;
+    const uint32_t tag = inst | ((uint32_t)first << kRefineShift) |
+                         ((uint32_t)second << (kRefineShift + 4));
+    uint32_t slot = atomicAdd(&ctr->qcount[kQueueRefine], blocks);
+    if (!split) {
+        queue[(uint64_t)kQueueRefine * qstride + slot] = QItem{tag, bi, bj, 0};
+        return;
+    }
+    for (uint32_t x = 0; x < ni; ++x)
+        for (uint32_t y = bi == bj ? x : 0; y < nj; ++y)
+            queue[(uint64_t)kQueueRefine * qstride + slot++] =
+                QItem{tag, bi, bj, make_sub(kSplitRows, x, y)};
+}
+
 // One thread per tile pair: decide, then append to that cell's queue.
-// policy: 1 analytic, 2 learned (scores given), 3 fixed.
+// policy: 1 analytic, 2 learned (scores given), 3 fixed, 6 refine, 7 per
+// pair, and the internal exploration and probe passes.
 __global__ __launch_bounds__(256) void select_kernel(
     const TileMeta* __restrict__ tm, uint32_t T, uint64_t P, DevModel M, uint32_t reps,
     uint32_t policy, uint32_t fixed_cell, const float* __restrict__ learned_cost,
     const uint8_t* __restrict__ learned_ok, uint8_t* __restrict__ decisions,
     QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, bool split, double thr,
     uint32_t bb_ksplit, bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
-    const uint32_t* __restrict__ sel, DevCounters* __restrict__ ctr) {
+    const uint32_t* __restrict__ sel, DevCounters* __restrict__ ctr, uint16_t* __restrict__ probe,
+    ProbeAcc* __restrict__ acc, uint32_t shadow_inst) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P) return;
     uint32_t bi, bj;
@@ -201,16 +208,77 @@ __global__ __launch_bounds__(256) void select_kernel(
     // Band and cross calls queue only the tile pairs of their range.
     if (bi < sel[0] || bi >= sel[1] || bj < sel[2] || bj >= sel[3]) {
         decisions[t] = 0xFF;
+        if (probe) probe[t] = 0;
+        return;
+    }
+    if (policy == kPolicyProbe2) {
+        // Commit a pending tile to the candidate its class measured faster
+        // (both ran on the class's sampled tiles).
+        const uint16_t st = probe[t];
+        if (!(st & kProbePending)) return;
+        const int first = st & 15, second = (st >> 4) & 15;
+        const uint32_t cls = ((uint32_t)first * 10u + (uint32_t)second) * kProbeBuckets +
+                             ((st >> 8) & 15);
+        const unsigned long long na = acc->ns[0][cls], nb = acc->ns[1][cls];
+        const int p = (nb > 0 && na > 0 && nb < na) ? second : first;
+        if (p == second) atomicAdd(&acc->switched, 1ull);
+        const TileMeta a = tm[bi], b = tm[bj];
+        decisions[t] = (uint8_t)p;
+        enqueue_tile(p, inst, bi, bj, a, b, queue, qstride, split, bb_ksplit, bb_wide, ctr);
+        atomicAdd(&ctr->cell_tiles[p], 1ull);
+        if (p == STORM_HIP_CELL_BB)
+            atomicAdd(&ctr->cell_pairs[p],
+                      (unsigned long long)(bi == bj ? (uint64_t)a.rows * (a.rows - 1) / 2
+                                                    : (uint64_t)a.rows * b.rows));
         return;
     }
     const TileMeta a = tm[bi], b = tm[bj];
     const uint32_t elig = eligible_mask(a, b, reps);
     int p;
     bool fallback = false;
+    if (probe) probe[t] = 0;
     // Thresholded calls: a tile whose length bound is below the threshold
     // holds no reportable pair.
     if (tile_empty(a, b) || (thr > 0.0 && length_bound(a, b) < thr)) {
         p = STORM_HIP_CELL_EMPTY;
+    } else if (policy == STORM_HIP_POLICY_PER_PAIR) {
+        // Every pair of the tile selects for itself.
+        decisions[t] = (uint8_t)kPerPairCell;
+        enqueue_refine(kPerPairCell, kPerPairCell, inst, bi, bj, a, b, queue, qstride, split, ctr);
+        return;
+    } else if (policy == kPolicyProbe1 || policy == STORM_HIP_POLICY_REFINE) {
+        int first, second;
+        double ratio;
+        tile_top2(M, a, b, elig, &first, &second, &ratio);
+        p = first;
+        if (policy == STORM_HIP_POLICY_REFINE) {
+            // Refine when the tile's rows are heterogeneous (max / mean
+            // cardinality, either side) and the runner-up is close.
+            const double het = fmax(a.spread, b.spread);
+            if (second != first && het >= kRefineHet && ratio <= kRefineRatio) {
+                decisions[t] = (uint8_t)first;
+                enqueue_refine(first, second, inst, bi, bj, a, b, queue, qstride, split, ctr);
+                atomicAdd(&ctr->cell_tiles[first], 1ull);
+                return;
+            }
+        } else if (second != first) {
+            const uint32_t cls = probe_class(first, second, ratio);
+            const uint32_t k = atomicAdd(&acc->tiles[cls], 1u);
+            const bool sampled =
+                k < kProbeMinSamples ||
+                (splitmix_mix(explore_seed ^ (t * 0xD1B54A32D192ED03ull)) >> 40) < kProbeShare;
+            const uint16_t st = (uint16_t)(first | (second << 4) | ((cls % kProbeBuckets) << 8));
+            if (!sampled) {
+                // Pending: pass 2 commits it.
+                probe[t] = st | kProbePending;
+                return;
+            }
+            probe[t] = st | kProbeSampled;
+            atomicAdd(&acc->sampled, 1ull);
+            // The runner-up runs on the shadow instance: timed, no output.
+            enqueue_tile(second, shadow_inst, bi, bj, a, b, queue, qstride, split, bb_ksplit,
+                         bb_wide, ctr);
+        }
     } else if (policy == kPolicyExplore &&
                (splitmix_mix(explore_seed ^ (t * 0xD1B54A32D192ED03ull)) >> 40) <
                    (uint64_t)explore_share) {
@@ -260,38 +328,7 @@ __global__ __launch_bounds__(256) void select_kernel(
         if (!(elig >> p & 1)) p = cheapest_eligible(M, a, b, elig);
     }
     decisions[t] = (uint8_t)p;
-    if (split && p == STORM_HIP_CELL_BS) {
-        // B x S: stripes of kSplitRows probed rows against the whole other
-        // side (read through its slices).
-        const bool pa = bs_probe_a(a, b);
-        const uint32_t ns = ((pa ? a.rows : b.rows) + kSplitRows - 1) / kSplitRows;
-        uint32_t slot = atomicAdd(&ctr->qcount[p], ns);
-        for (uint32_t x = 0; x < ns; ++x)
-            queue[(uint64_t)p * qstride + slot++] =
-                QItem{inst, bi, bj, pa ? make_sub(kSplitRows, x, kWholeSide)
-                                       : make_sub(kSplitRows, kWholeSide, x)};
-    } else if (split && p != STORM_HIP_CELL_BB && p != STORM_HIP_CELL_EMPTY) {
-        // Per-pair cells: the tile as kSplitRows x kSplitRows blocks (upper
-        // triangle of blocks on a diagonal tile), each its own work item.
-        const uint32_t ni = (a.rows + kSplitRows - 1) / kSplitRows;
-        const uint32_t nj = (b.rows + kSplitRows - 1) / kSplitRows;
-        const uint32_t count = bi == bj ? ni * (ni + 1) / 2 : ni * nj;
-        uint32_t slot = atomicAdd(&ctr->qcount[p], count);
-        for (uint32_t x = 0; x < ni; ++x)
-            for (uint32_t y = bi == bj ? x : 0; y < nj; ++y)
-                queue[(uint64_t)p * qstride + slot++] =
-                    QItem{inst, bi, bj, make_sub(kSplitRows, x, y)};
-    } else if (p == STORM_HIP_CELL_BB) {
-        const uint32_t q = bb_wide ? kQueueBBWide : STORM_HIP_CELL_BB;
-        const uint32_t ks = bb_ksplit > 1 ? bb_ksplit : 1;
-        uint32_t slot = atomicAdd(&ctr->qcount[q], ks);
-        for (uint32_t x = 0; x < ks; ++x)
-            queue[(uint64_t)q * qstride + slot++] =
-                QItem{inst, bi, bj, ks > 1 ? (x | (ks << 16)) : 0u};
-    } else {
-        const uint32_t slot = atomicAdd(&ctr->qcount[p], 1u);
-        queue[(uint64_t)p * qstride + slot] = QItem{inst, bi, bj, 0};
-    }
+    enqueue_tile(p, inst, bi, bj, a, b, queue, qstride, split, bb_ksplit, bb_wide, ctr);
     atomicAdd(&ctr->cell_tiles[p], 1ull);
     if (p == STORM_HIP_CELL_BB) {
         // B x B tiles skip the per-pair span test (same cost either way);
@@ -300,6 +337,55 @@ __global__ __launch_bounds__(256) void select_kernel(
         atomicAdd(&ctr->cell_pairs[p], (unsigned long long)(bi == bj ? ni * (ni - 1) / 2 : ni * nj));
     }
     if (fallback) atomicAdd(&ctr->fallback_tiles, 1ull);
+}
+
+// Probe: add each timed item of pass 1 to its tile's class, as CU-time (wall
+// time over the class kernel's workgroups per CU, 10 ns per tick), first or
+// second candidate by the queue it ran in. Grid: (x, queue).
+__global__ __launch_bounds__(256) void probe_reduce_kernel(
+    const DevCorpus* __restrict__ Ds, uint32_t n_inst, const QItem* __restrict__ queue,
+    uint64_t qstride, const DevCounters* __restrict__ ctr,
+    const unsigned long long* __restrict__ ticks, ProbeBpc bpc, ProbeAcc* __restrict__ acc) {
+    const uint32_t q = blockIdx.y;
+    const int cell = q == kQueueBBWide ? STORM_HIP_CELL_BB : (int)q;
+    if (cell >= 10) return;   // EMPTY, C x B and refine never probe
+    const uint32_t count = ctr->qcount[q];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count;
+         k += gridDim.x * blockDim.x) {
+        const QItem e = queue[(uint64_t)q * qstride + k];
+        const uint32_t inst = e.inst >= n_inst ? e.inst - n_inst : e.inst;
+        const DevCorpus& D = Ds[inst];
+        const uint32_t T = (D.n + D.tile - 1) / D.tile;
+        const uint64_t t = (uint64_t)e.bi * (2ull * T - e.bi + 1) / 2 + (e.bj - e.bi);
+        const uint16_t st = D.probe[t];
+        if (!(st & kProbeSampled)) continue;
+        const int first = st & 15, second = (st >> 4) & 15;
+        const uint32_t cls = ((uint32_t)first * 10u + (uint32_t)second) * kProbeBuckets +
+                             ((st >> 8) & 15);
+        const int which = cell == first ? 0 : (cell == second ? 1 : -1);
+        if (which < 0) continue;
+        const double ns = (double)ticks[(uint64_t)q * qstride + k] * 10.0 / bpc.v[cell];
+        atomicAdd(&acc->ns[which][cls], (unsigned long long)ns);
+    }
+}
+
+// Pass 2 starts its queues from empty; the counters of pass 1 stay.
+__global__ void probe_reset_queues_kernel(DevCounters* ctr) {
+    const int q = threadIdx.x;
+    if (q < kNumQueues) { ctr->qcount[q] = 0; ctr->qhead[q] = 0; }
+}
+
+int launch_probe_reduce(hipStream_t s, const DevCorpus* Ds, uint32_t n_inst, const QItem* queue,
+                        uint64_t qstride, const DevCounters* ctr,
+                        const unsigned long long* ticks, const ProbeBpc& bpc, ProbeAcc* acc) {
+    const uint32_t gx = (uint32_t)std::min<uint64_t>((qstride + 255) / 256, 1024);
+    hipLaunchKernelGGL(probe_reduce_kernel, dim3(std::max(gx, 1u), kNumQueues), dim3(256), 0, s,
+                       Ds, n_inst, queue, qstride, ctr, ticks, bpc, acc);
+    SH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(probe_reset_queues_kernel, dim3(1), dim3(64), 0, s,
+                       const_cast<DevCounters*>(ctr));
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
 }
 
 // Host view of the work function, for calibration.
@@ -329,13 +415,14 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   const uint8_t* learned_ok, uint8_t* decisions, QItem* queue, uint64_t qstride,
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
                   bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
-                  const uint32_t* sel, DevCounters* ctr) {
+                  const uint32_t* sel, DevCounters* ctr, uint16_t* probe, ProbeAcc* acc,
+                  uint32_t shadow_inst) {
     const uint64_t P = (uint64_t)T * (T + 1) / 2;
     if (P == 0) return STORM_HIP_OK;
     hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, s, tm, T, P,
                        M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions, queue,
                        qstride, inst, tile > kSplitRows, thr, bb_ksplit, bb_wide, explore_seed,
-                       explore_share, sel, ctr);
+                       explore_share, sel, ctr, probe, acc, shadow_inst);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

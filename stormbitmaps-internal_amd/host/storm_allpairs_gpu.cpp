@@ -24,7 +24,8 @@ uint64_t now_ns() {
 // anything else 64 (the callback granularity stays the caller's).
 uint32_t engine_tile(uint32_t tile) { return (tile == 128 || tile == 256) ? tile : 64; }
 
-// Policy -> options; UNSUPPORTED for the per-pair policies.
+// Policy -> options. Oracle makes the same per-pair model choice as PerPair
+// (storm_allpairs.cpp:645-660 precomputes it off the clock).
 int options_for(Policy policy, uint32_t tile, Pairing fixed_cell, storm_hip_options* o) {
     *o = storm_hip_options{};
     o->tile = engine_tile(tile);
@@ -36,8 +37,10 @@ int options_for(Policy policy, uint32_t tile, Pairing fixed_cell, storm_hip_opti
             o->policy = STORM_HIP_POLICY_FIXED;
             o->fixed_cell = (uint32_t)fixed_cell;
             break;
-        case Policy::Probe:
-        case Policy::Refine: o->policy = STORM_HIP_POLICY_LEARNED; break;
+        case Policy::Probe: o->policy = STORM_HIP_POLICY_PROBE; break;
+        case Policy::Refine: o->policy = STORM_HIP_POLICY_REFINE; break;
+        case Policy::PerPair:
+        case Policy::Oracle: o->policy = STORM_HIP_POLICY_PER_PAIR; break;
         default: return STORM_HIP_ERR_UNSUPPORTED;
     }
     return STORM_HIP_OK;

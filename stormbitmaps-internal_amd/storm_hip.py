@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libstorm_hip.so")
 GPU_LIB_PATH = os.path.join(HERE, "libstorm_gpu.so")
 
-POLICY = {"all_bitmap": 0, "analytic": 1, "learned": 2, "fixed": 3, "exhaustive": 4}
+POLICY = {"all_bitmap": 0, "analytic": 1, "learned": 2, "fixed": 3, "exhaustive": 4,
+          "probe": 5, "refine": 6, "per_pair": 7}
 OUT = {"sum": 0, "tri": 1}
 BB = {"auto": 0, "valu": 1, "mfma": 2, "mfma_fp4": 3}
 CELLS = ["BB", "BS", "BR", "BW", "SS", "SR", "SW", "RR", "RW", "WW", "EMPTY", "CB"]
@@ -46,7 +47,8 @@ class Stats(C.Structure):
                 ("decisions", C.c_uint64), ("cell_pairs", C.c_uint64 * NCELL),
                 ("cell_tiles", C.c_uint64 * NCELL), ("fallback_tiles", C.c_uint64),
                 ("ms_total", C.c_double), ("ms_device", C.c_double), ("ms_bb", C.c_double),
-                ("ms_select", C.c_double), ("bb_launches", C.c_uint64)]
+                ("ms_select", C.c_double), ("bb_launches", C.c_uint64),
+                ("probe_tiles", C.c_uint64), ("probe_switched", C.c_uint64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_

@@ -160,10 +160,12 @@ int main() {
                         check(cs.calls == T * (T + 1) / 2, "callback tiles", ctx, cs.calls, T * (T + 1) / 2);
                     }
                 }
-                // The per-pair policies have no GPU form.
-                const AllPairsStats sp_ = allpairs_sum(eng, R, nullptr, Policy::PerPair);
-                check(sp_.status == STORM_HIP_ERR_UNSUPPORTED, "per-pair unsupported", "",
-                      (uint64_t)-sp_.status, (uint64_t)-STORM_HIP_ERR_UNSUPPORTED);
+                // The per-pair policies.
+                for (Policy pol : {Policy::PerPair, Policy::Oracle, Policy::Refine}) {
+                    const AllPairsStats sp_ = allpairs_sum(eng, R, nullptr, pol);
+                    check(sp_.status == STORM_HIP_OK && sp_.sum == want, "per-pair policies",
+                          name_of(pol), sp_.sum, want);
+                }
             }
     // Fewer than two rows: an empty result, as the reference returns.
     RowsCSR one;
