@@ -255,10 +255,11 @@ int upload_bitmaps(storm_hip_corpus* c, hipStream_t s) {
     return STORM_HIP_OK;
 }
 
-// Bit-sliced bitmaps: a workgroup per (slab, 8 words), its wave v on group
-// 4q + v. Lane k holds 8 words of sorted row 64g + k; for each of the 512
-// positions a ballot gathers the 64 rows' bits and lane t keeps the mask of
-// position 64w + t. The four waves fill the four masks of the same positions.
+// Bit-sliced bitmaps: a workgroup per (slab, 8 words), its wave v on groups
+// 8q + v and 8q + 4 + v. Lane k holds 8 words of sorted row 64g + k; for each
+// of the 512 positions a ballot gathers the 64 rows' bits and lane t keeps the
+// mask of position 64w + t. The waves fill the eight masks of the same
+// positions (slab layout: position-major, kSlabGroups masks per position).
 __global__ __launch_bounds__(256) void slice_bits_kernel(const uint32_t* __restrict__ perm,
                                                          const uint32_t* __restrict__ bm_row,
                                                          const uint64_t* __restrict__ bitmap,
@@ -268,48 +269,51 @@ __global__ __launch_bounds__(256) void slice_bits_kernel(const uint32_t* __restr
     const uint32_t lane = threadIdx.x & 63, v = threadIdx.x >> 6;
     const uint32_t w0 = blockIdx.x * 8;
     if (w0 >= ld) return;
-    for (uint32_t q = blockIdx.y; 4 * q < groups; q += gridDim.y) {
-        const uint32_t g = 4 * q + v;
-        const uint32_t sl = g < groups ? slice_of[g] : kNoBitmap;
-        if (sl == kNoBitmap) continue;
-        // Padding positions of the density order map to the zero row.
-        const uint64_t* src = bitmap + (uint64_t)bm_row[perm[64 * g + lane]] * ld + w0;
-        uint64_t x[8];
+    for (uint32_t q = blockIdx.y; kSlabGroups * q < groups; q += gridDim.y) {
+        for (uint32_t h = 0; h < kSlabGroups; h += 4) {
+            const uint32_t g = kSlabGroups * q + h + v;
+            const uint32_t sl = g < groups ? slice_of[g] : kNoBitmap;
+            if (sl == kNoBitmap) continue;
+            // Padding positions of the density order map to the zero row.
+            const uint64_t* src = bitmap + (uint64_t)bm_row[perm[64 * g + lane]] * ld + w0;
+            uint64_t x[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) x[k] = src[k];
-        uint64_t* dst = slice + ((uint64_t)(sl >> 2) * ld * 64 + (uint64_t)w0 * 64) * 4 + (sl & 3);
+            for (int k = 0; k < 8; ++k) x[k] = src[k];
+            uint64_t* dst = slice + ((uint64_t)(sl / kSlabGroups) * ld * 64 + (uint64_t)w0 * 64) *
+                                        kSlabGroups + (sl % kSlabGroups);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            uint64_t mine = 0;
-            for (uint32_t t = 0; t < 64; ++t) {
-                const uint64_t m = __ballot((x[k] >> t) & 1);
-                if (lane == t) mine = m;
+            for (int k = 0; k < 8; ++k) {
+                uint64_t mine = 0;
+                for (uint32_t t = 0; t < 64; ++t) {
+                    const uint64_t m = __ballot((x[k] >> t) & 1);
+                    if (lane == t) mine = m;
+                }
+                dst[((uint64_t)k * 64 + lane) * kSlabGroups] = mine;
             }
-            dst[((uint64_t)k * 64 + lane) * 4] = mine;
         }
     }
 }
 
 // (Re)builds the slices of every 64-row group of the density order whose rows
-// all hold bitmaps (rows past n are the zero row); groups 4q..4q+3 share slab
+// all hold bitmaps (rows past n are the zero row); groups 8q..8q+7 share slab
 // q when any of them has slices.
 int build_slices(storm_hip_corpus* c, hipStream_t s) {
     if (c->d_slice) { SH_HIP(hipFree(c->d_slice)); c->d_slice = nullptr; }
     if (c->d_slice_of) { SH_HIP(hipFree(c->d_slice_of)); c->d_slice_of = nullptr; }
-    const uint64_t slab_bytes = (uint64_t)c->ld * 64 * 4 * 8;
+    const uint64_t slab_bytes = (uint64_t)c->ld * 64 * kSlabGroups * 8;
     c->device_bytes -= (uint64_t)c->n_slices * slab_bytes;
     c->n_slices = 0;
     if (!c->want_slices || c->n_rows == 0) return STORM_HIP_OK;
     const uint32_t groups = (c->n_rows + 63) / 64;
     std::vector<uint32_t> of(groups, kNoBitmap);
     uint32_t ns = 0;
-    for (uint32_t q = 0; 4 * q < groups; ++q) {
+    for (uint32_t q = 0; kSlabGroups * q < groups; ++q) {
         bool any = false;
-        for (uint32_t g = 4 * q; g < std::min(groups, 4 * q + 4); ++g) {
+        for (uint32_t g = kSlabGroups * q; g < std::min(groups, kSlabGroups * (q + 1)); ++g) {
             bool all = true;
             for (uint32_t k = 64 * g; k < std::min(c->n_rows, 64 * g + 64) && all; ++k)
                 all = c->h_bm_row[c->h_perm[k]] != kNoBitmap;
-            if (all) { of[g] = 4 * ns + (g & 3); any = true; }
+            if (all) { of[g] = kSlabGroups * ns + g % kSlabGroups; any = true; }
         }
         if (any) ++ns;
     }
@@ -319,7 +323,7 @@ int build_slices(storm_hip_corpus* c, hipStream_t s) {
     SH_HIP(hipMemsetAsync(c->d_slice, 0, (size_t)ns * slab_bytes, s));
     c->n_slices = ns;
     c->device_bytes += (uint64_t)ns * slab_bytes;
-    const dim3 grid(c->ld / 8, std::min<uint32_t>((groups + 3) / 4, 65535));
+    const dim3 grid(c->ld / 8, std::min<uint32_t>((groups + kSlabGroups - 1) / kSlabGroups, 65535));
     hipLaunchKernelGGL(slice_bits_kernel, grid, dim3(256), 0, s, c->d_perm, c->d_bm_row,
                        c->d_bitmap, c->d_slice_of, c->d_slice, c->ld, groups);
     SH_HIP(hipGetLastError());

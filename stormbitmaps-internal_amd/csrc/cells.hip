@@ -477,21 +477,27 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
 // ---- B x S through bit-sliced bitmaps --------------------------------------
 //
 // A 64-row group of bitmap rows has a slice: per position, the mask of the
-// group's rows that contain it; four consecutive groups keep their masks of a
-// position side by side (32 bytes). A B x S item is a stripe: up to 64 rows of
-// the probed side (lists) against up to 256 rows of the other side, whose
-// groups all have slices. A wave takes one probed row and streams its list
-// once; its lanes load 64 elements' masks (32 bytes for four groups) and add 1
-// to the wave's LDS counter of every slab row set in them. One load per list
-// element serves up to 256 pairs (the per-pair cell probes the bitmap once per
-// element per pair); the counters are private to the wave, so no barrier.
-constexpr int kSliceFetch = 4;
+// group's rows that contain it; the eight groups of a slab (512 rows) keep
+// their masks of a position side by side (64 bytes). A B x S item is a
+// stripe: up to 64 rows of the probed side (lists) against up to 256 rows of
+// the other side (one tile, 32 bytes per position) or 512 (a wide stripe over
+// two tiles, the whole 64 bytes), whose groups all have slices. A wave takes
+// one probed row and streams its list once; its lanes load 64 elements'
+// masks and add 1 to the wave's LDS counter of every slab row set in them.
+// One load per list element serves up to 512 pairs (the per-pair cell probes
+// the bitmap once per element per pair); the counters are private to the
+// wave, so no barrier. The loads are random and small: their number, not
+// their bytes, bounds the stripes (a 64-byte load costs about 1.15x a 32-byte
+// one), so the wide stripe halves the loads per pair.
 
 // Orders one wave's LDS accesses across its lanes (no cross-wave effect).
 __device__ inline void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
-}   // 64-element batches a wave loads before counting
+}
+
+// Counters per wave: one per slab row of the widest stripe.
+constexpr uint32_t kStripeRows = 64 * kSlabGroups;
 
 // Probe rows at sorted positions [p0, p0 + np), slab rows [s0, s0 + ns) (s0 a
 // multiple of 64, G = ceil(ns / 64) groups, all with slices). Pairs are (A
@@ -501,39 +507,50 @@ template <int G>
 __device__ void bs_stripe(const DevCorpus& D, uint32_t* __restrict__ wave_cnt, uint32_t p0,
                           uint32_t np, uint32_t s0, uint32_t ns, bool probe_a, bool same_tile,
                           uint64_t& local, uint64_t& pairs, uint64_t& skipped) {
+    // 64-element batches a wave loads before counting (same bytes in flight
+    // whatever the width).
+    constexpr int F = G > 4 ? 2 : 4;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     const uint32_t g0 = s0 >> 6;
     const uint64_t* base[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const uint32_t sl = D.slice_of[g0 + g];
-        base[g] = D.slice + (uint64_t)(sl >> 2) * D.ld * 256 + (sl & 3);
+        base[g] = D.slice + (uint64_t)(sl / kSlabGroups) * D.ld * 64 * kSlabGroups +
+                  sl % kSlabGroups;
     }
-    const bool packed = G == 4 && (g0 & 3) == 0;   // the four masks of a position at once
+    // The masks of a position in one or two 16-byte pairs: four groups of a
+    // tile (32 bytes) or the eight of a slab (64 bytes), all in one slab.
+    constexpr int V = G == 8 ? 4 : (G == 4 ? 2 : 0);
+    const bool packed = V > 0 && (g0 % G) == 0;
     // This wave's counters, one per slab row: each lane adds 1 for every set
     // bit of its elements' masks (LDS atomics, a few per element at most
     // densities), then the row's counts are read back and cleared.
-    uint32_t* cnt = wave_cnt + wave * 256;
-    for (uint32_t q = lane; q < 256; q += 64) cnt[q] = 0;
+    uint32_t* cnt = wave_cnt + wave * kStripeRows;
+    for (uint32_t q = lane; q < 64 * G; q += 64) cnt[q] = 0;
     // Lanes hand counters to each other through LDS (every lane adds to any
     // counter, lane k reads back counters k, k+64, ...): a wave-scope fence
     // and barrier order the clears, the adds and the read-back.
     wave_lds_sync();
     for (uint32_t r = wave; r < np; r += nwaves) {
         const Span32 L = list_of(D, D.perm[p0 + r]);
-        for (uint32_t k0 = 0; k0 < L.n; k0 += 64 * kSliceFetch) {
-            uint64_t m[kSliceFetch][G];
+        for (uint32_t k0 = 0; k0 < L.n; k0 += 64 * F) {
+            uint64_t m[F][G];
 #pragma unroll
-            for (int u = 0; u < kSliceFetch; ++u) {
+            for (int u = 0; u < F; ++u) {
                 const uint32_t k = k0 + u * 64 + lane;
 #pragma unroll
                 for (int g = 0; g < G; ++g) m[u][g] = 0;
                 if (k < L.n) {
-                    const uint64_t x = (uint64_t)L.p[k] * 4;
+                    const uint64_t x = (uint64_t)L.p[k] * kSlabGroups;
                     if (packed) {
                         const ulonglong2* q = reinterpret_cast<const ulonglong2*>(base[0] + x);
-                        const ulonglong2 lo = q[0], hi = q[1];
-                        m[u][0] = lo.x; m[u][1 % G] = lo.y; m[u][2 % G] = hi.x; m[u][3 % G] = hi.y;
+#pragma unroll
+                        for (int v = 0; v < V; ++v) {
+                            const ulonglong2 w = q[v];
+                            m[u][(2 * v) % G] = w.x;
+                            m[u][(2 * v + 1) % G] = w.y;
+                        }
                     } else {
 #pragma unroll
                         for (int g = 0; g < G; ++g) m[u][g] = base[g][x];
@@ -541,7 +558,7 @@ __device__ void bs_stripe(const DevCorpus& D, uint32_t* __restrict__ wave_cnt, u
                 }
             }
 #pragma unroll
-            for (int u = 0; u < kSliceFetch; ++u)
+            for (int u = 0; u < F; ++u)
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     unsigned long long x = m[u][g];
@@ -978,9 +995,14 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
         const uint64_t pairs0 = pairs, skipped0 = skipped;
         uint32_t i0 = e.bi * tile, j0 = e.bj * tile;
         uint32_t ni = min(n, i0 + tile) - i0, nj = min(n, j0 + tile) - j0;
+        const bool wide = e.sub && (e.sub >> 24) == kWideSide;
+        if (wide) {
+            // Both bitmap tiles bj, bj + 1 (full, so 512 rows from j0).
+            nj = 2 * tile;
+        }
         if (e.sub) {
             // One block of the tile (empty when it lies past the last row), or
-            // a stripe: one side whole.
+            // a stripe: one side whole (or two tiles whole: wide).
             const uint32_t sb = e.sub & 0xFFFFu;
             const uint32_t xi = (e.sub >> 16) & 0xFFu, xj = e.sub >> 24;
             if (xi != kWholeSide) {
@@ -988,7 +1010,7 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
                 ni = oi < ni ? min(ni - oi, sb) : 0u;
                 i0 += oi;
             }
-            if (xj != kWholeSide) {
+            if (xj != kWholeSide && xj != kWideSide) {
                 const uint32_t oj = xj * sb;
                 nj = oj < nj ? min(nj - oj, sb) : 0u;
                 j0 += oj;
@@ -1008,25 +1030,28 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
             // A stripe names its probed side; a whole tile or block probes the
             // side bs_probe_a picks.
             bool probe_a = bs_probe_a(A, B);
-            if (e.sub && (e.sub >> 24) == kWholeSide) probe_a = true;
+            if (e.sub && (e.sub >> 24) >= kWideSide) probe_a = true;
             else if (e.sub && ((e.sub >> 16) & 0xFFu) == kWholeSide) probe_a = false;
             const uint32_t s0 = probe_a ? j0 : i0, ns = probe_a ? nj : ni;
             const uint32_t p0 = probe_a ? i0 : j0, np_ = probe_a ? ni : nj;
-            bool ok = D.slice != nullptr && (s0 & 63) == 0 && ns > 0 && ns <= 256 && np_ > 0;
+            // A wide stripe covers 512 rows, every other stripe one tile.
+            bool ok = D.slice != nullptr && (s0 & 63) == 0 && ns > 0 && np_ > 0 &&
+                      (wide ? ns == 512 : ns <= 256);
             for (uint32_t g = 0; ok && g < (ns + 63) / 64; ++g)
                 ok = D.slice_of[(s0 >> 6) + g] != kNoBitmap;
-            // Probing the denser side costs its elements once per stripe; the
-            // per-pair cell probes the sparser side's once per pair.
             const uint32_t probe_card = probe_a ? A.card : B.card;
-            if (ok && (double)probe_card <= 64.0 * min(A.card, B.card) + 64.0) {
+            // The select kernel queues wide stripes only where both halves
+            // pass this test.
+            if (ok && (wide || bs_stripes_pay(probe_card, A, B))) {
                 joined = true;
                 const bool same = e.bi == e.bj;
-                __shared__ uint32_t wave_cnt[4 * 256];   // 256 counters per wave (4 waves)
+                __shared__ uint32_t wave_cnt[4 * kStripeRows];   // per wave (4 waves)
                 switch ((ns + 63) / 64) {
                     case 1: bs_stripe<1>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
                     case 2: bs_stripe<2>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
                     case 3: bs_stripe<3>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
-                    default: bs_stripe<4>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                    case 4: bs_stripe<4>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
+                    default: bs_stripe<8>(D, wave_cnt, p0, np_, s0, ns, probe_a, same, local, pairs, skipped); break;
                 }
             }
         }

@@ -29,6 +29,11 @@ __host__ __device__ inline void tri_tile(uint32_t t, uint32_t T, uint32_t& bi, u
     bj = (uint32_t)(b + (t - start(b)));
 }
 
+// (bi, bj), bi <= bj < T -> the row-major id that tri_tile inverts.
+__host__ __device__ inline uint64_t tri_tile_id(uint32_t bi, uint32_t bj, uint32_t T) {
+    return (uint64_t)bi * (2ull * T - bi + 1) / 2 + (bj - bi);
+}
+
 // Grid block -> position in a tile order. Workgroups are dispatched to the
 // 8 XCDs round-robin (block b on XCD b % 8), so block b takes position
 // (b % 8) * (G8 / 8) + b / 8 of the first G8 = 8 * floor(G / 8) positions:

@@ -34,10 +34,10 @@ struct DevCorpus {
     const uint32_t* occ_row;  const uint64_t* occ;  uint32_t occ_ld;
     const uint32_t* meta;     // 8 per original row (RowMeta + has_rank, has_occ)
     const uint32_t* perm;     // sorted position -> original row id
-    // Bit-sliced bitmaps of 64-row groups of the density order, four groups
-    // (256 rows) per slab: with s = slice_of[g] (kNoBitmap: none), word
-    // slice[((s >> 2) * ld * 64 + p) * 4 + (s & 3)] has bit k set when sorted
-    // row 64g + k holds position p; groups 4q..4q+3 share slab q, lane g & 3.
+    // Bit-sliced bitmaps of 64-row groups of the density order, eight groups
+    // (512 rows) per slab: with s = slice_of[g] (kNoBitmap: none), word
+    // slice[((s / 8) * ld * 64 + p) * 8 + s % 8] has bit k set when sorted
+    // row 64g + k holds position p; groups 8q..8q+7 share slab q, lane g % 8.
     const uint64_t* slice;    const uint32_t* slice_of;
     // Per-call state of this corpus inside a (possibly multi-corpus) call.
     const struct TileMeta* tm;        // its row tiles' metadata
@@ -139,6 +139,16 @@ __host__ __device__ inline uint32_t bb_slices_for(uint32_t n_words) {
 // every row of the other side, whose bitmaps are read through their slices.
 // On a diagonal tile a stripe keeps only the pairs above the diagonal.
 constexpr uint32_t kWholeSide = 0xFF;
+// kWideSide in place of the column block: a wide stripe, kSplitRows list rows
+// of tile bi against both bitmap tiles bj and bj + 1 (bj even, both full and
+// sliced, bi < bj): one 64-byte slice load per list element answers for 512
+// rows. The select kernel queues one when both tile pairs chose B x S.
+constexpr uint32_t kWideSide = 0xFE;
+
+// Bit-sliced bitmaps come in slabs of kSlabGroups 64-row groups (the two
+// 256-row tiles 2k, 2k+1): per universe position the slab keeps its groups'
+// masks side by side, kSlabGroups x 8 bytes.
+constexpr uint32_t kSlabGroups = 8;
 
 // Thresholded calls (the corpora carry a Jaccard threshold and pair buffer,
 // their tri is null) use the B x B instances that report pairs; the matrix-core
@@ -171,6 +181,25 @@ struct TileMeta {
 // A when B is all bitmaps and A is the sparser side (or not all bitmaps).
 __host__ __device__ inline bool bs_probe_a(const TileMeta& a, const TileMeta& b) {
     return b.all_bitmap && (!a.all_bitmap || a.card <= b.card);
+}
+
+// The stripes serve a B x S item when the probed side's mean cardinality is
+// at most 64x the sparser side's (probing the denser side costs its elements
+// once per stripe; the per-pair cell probes the sparser side's once per pair).
+__host__ __device__ inline bool bs_stripes_pay(uint32_t probe_card, const TileMeta& a,
+                                               const TileMeta& b) {
+    return (double)probe_card <= 64.0 * (double)(a.card < b.card ? a.card : b.card) + 64.0;
+}
+
+// Tile pairs (bi, bj) and (bi, bj + 1) that both chose B x S run as wide
+// stripes: bj even, bi < bj, both bitmap tiles full and all bitmaps (so their
+// eight groups have slices), the list side bi probed against each.
+__host__ __device__ inline bool bs_wide_ok(uint32_t bi, uint32_t bj, uint32_t tile,
+                                           const TileMeta& a, const TileMeta& b0,
+                                           const TileMeta& b1) {
+    return tile == 256 && (bj & 1) == 0 && bi < bj && b0.rows == 256 && b1.rows == 256 &&
+           b0.all_bitmap && b1.all_bitmap && bs_probe_a(a, b0) && bs_probe_a(a, b1) &&
+           bs_stripes_pay(a.card, a, b0) && bs_stripes_pay(a.card, a, b1);
 }
 
 // Representations held for every row of a corpus (reps bits).

@@ -28,7 +28,7 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
                   bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
                   const uint32_t* sel, DevCounters* ctr, uint16_t* probe, ProbeAcc* acc,
-                  uint32_t shadow_inst);
+                  uint32_t shadow_inst, bool wide_bs);
 int launch_refine(hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
                   const QItem* queue, DevCounters* ctr, unsigned long long* tile_cycles);
 uint32_t refine_blocks_per_cu();
@@ -317,6 +317,15 @@ bool bb_wide(const storm_hip_corpus* c, const storm_hip_options& o, uint32_t til
     return tile == 256 && !thr && o.bb_kernel != STORM_HIP_BB_MFMA && c->n_words >= (1u << 18);
 }
 
+// B x S tile pairs may be queued as wide stripes (two bitmap tiles per
+// stripe): 256-row tiles of a corpus with slices, under a policy that decides
+// every tile in one pass (not Probe, Refine or PerPair).
+bool bs_wide_call(const storm_hip_corpus* c, uint32_t policy, uint32_t tile) {
+    return tile == 256 && c->d_slice != nullptr &&
+           (policy == STORM_HIP_POLICY_ANALYTIC || policy == STORM_HIP_POLICY_LEARNED ||
+            policy == STORM_HIP_POLICY_FIXED || policy == kPolicyExplore);
+}
+
 // Dispatch the queued work items: every queue's persistent kernel, on three
 // streams, joined back into the context's stream.
 int dispatch_queues(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, const DevModel& M,
@@ -464,7 +473,7 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
                              o.policy == kPolicyExplore ? p->explore_seed
                                                         : (probe ? kGolden : 0),
                              p->explore_share, p->d_sel, p->ctr, probe ? w->probe : nullptr,
-                             p->acc, (uint32_t)(k + n)));
+                             p->acc, (uint32_t)(k + n), bs_wide_call(c, sel_policy, p->tile)));
         SH_TRY(launch_compose_rowmap(s, p->h_corpora[k].perm, c->d_bm_row, w->npad, w->rowmap));
         // Only corpora with bitmap rows can queue B x B tiles.
         if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
@@ -495,7 +504,7 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
                              nullptr, w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
                              p->th.thr, p->h_corpora[k].bb_ksplit,
                              bb_wide(c, o, p->tile, thresholded_plan(p)), 0, 0, p->d_sel, p->ctr,
-                             w->probe, p->acc, (uint32_t)(k + n)));
+                             w->probe, p->acc, (uint32_t)(k + n), false));
     }
     return dispatch_queues(ctx, p, o, M, fp4, false, false);
 }
@@ -622,10 +631,15 @@ int explore_plan(storm_hip_ctx* ctx, Plan* p, uint32_t tile, uint64_t seed, doub
         for (uint32_t e = 0; e < cnt; ++e) {
             const QItem& it = items[e];
             const uint32_t T = p->works[it.inst]->T;
-            const uint64_t t = base[it.inst] + (uint64_t)it.bi * T -
-                               (uint64_t)it.bi * (it.bi - 1) / 2 + (it.bj - it.bi);
-            cost[t] += (double)ticks[e] * 10.0 / sharing;   // 100 MHz clock -> ns
-            cell_of[t] = (int8_t)cell;
+            // A wide stripe serves tile pairs (bi, bj) and (bi, bj + 1): half
+            // of its time to each.
+            const bool wide = it.sub && (it.sub >> 24) == kWideSide;
+            const double ns = (double)ticks[e] * 10.0 / sharing;   // 100 MHz clock -> ns
+            for (uint32_t h = 0; h < (wide ? 2u : 1u); ++h) {
+                const uint64_t t = base[it.inst] + tri_tile_id(it.bi, it.bj + h, T);
+                cost[t] += wide ? ns / 2 : ns;
+                cell_of[t] = (int8_t)cell;
+            }
         }
     }
     uint32_t m = 0;

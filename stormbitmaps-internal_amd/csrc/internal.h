@@ -85,8 +85,8 @@ struct storm_hip_corpus {
     bool has_runs = false, has_ewah = false, has_comp = false;
     // Bit-sliced bitmaps (STORM_HIP_BUILD_SLICES): for each group of 64
     // density-ordered rows that all hold bitmaps, per position p a mask whose
-    // bit k says row 64g + k holds p; the groups of 256 consecutive rows share
-    // a slab of ld * 64 positions x 4 masks. d_slice_of[g] = 4 * slab + (g & 3)
+    // bit k says row 64g + k holds p; the groups of 512 consecutive rows share
+    // a slab of ld * 64 positions x 8 masks. d_slice_of[g] = 8 * slab + g % 8
     // or kNoBitmap.
     bool want_slices = false;
     uint64_t* d_slice = nullptr;

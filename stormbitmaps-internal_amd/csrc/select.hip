@@ -190,21 +190,19 @@ __device__ void enqueue_refine(int first, int second, uint32_t inst, uint32_t bi
                 QItem{tag, bi, bj, make_sub(kSplitRows, x, y)};
 }
 
-// One thread per tile pair: decide, then append to that cell's queue.
-// policy: 1 analytic, 2 learned (scores given), 3 fixed, 6 refine, 7 per
-// pair, and the internal exploration and probe passes.
-__global__ __launch_bounds__(256) void select_kernel(
-    const TileMeta* __restrict__ tm, uint32_t T, uint64_t P, DevModel M, uint32_t reps,
-    uint32_t policy, uint32_t fixed_cell, const float* __restrict__ learned_cost,
+// Decide tile pair (bi, bj) = row-major id t and append it to its cell's
+// queue. policy: 1 analytic, 2 learned (scores given), 3 fixed, 6 refine, 7
+// per pair, and the internal exploration and probe passes. With defer_bs a
+// split B x S choice is recorded and counted but left to the caller to queue
+// (*deferred set), which may pair it with its neighbour into wide stripes.
+__device__ void select_tile(
+    uint64_t t, uint32_t bi, uint32_t bj, const TileMeta* __restrict__ tm, const DevModel& M,
+    uint32_t reps, uint32_t policy, uint32_t fixed_cell, const float* __restrict__ learned_cost,
     const uint8_t* __restrict__ learned_ok, uint8_t* __restrict__ decisions,
     QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, bool split, double thr,
     uint32_t bb_ksplit, bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
     const uint32_t* __restrict__ sel, DevCounters* __restrict__ ctr, uint16_t* __restrict__ probe,
-    ProbeAcc* __restrict__ acc, uint32_t shadow_inst) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P) return;
-    uint32_t bi, bj;
-    tri_tile((uint32_t)t, T, bi, bj);
+    ProbeAcc* __restrict__ acc, uint32_t shadow_inst, bool defer_bs, bool* deferred) {
     // Band and cross calls queue only the tile pairs of their range.
     if (bi < sel[0] || bi >= sel[1] || bj < sel[2] || bj >= sel[3]) {
         decisions[t] = 0xFF;
@@ -328,7 +326,10 @@ __global__ __launch_bounds__(256) void select_kernel(
         if (!(elig >> p & 1)) p = cheapest_eligible(M, a, b, elig);
     }
     decisions[t] = (uint8_t)p;
-    enqueue_tile(p, inst, bi, bj, a, b, queue, qstride, split, bb_ksplit, bb_wide, ctr);
+    if (defer_bs && split && p == STORM_HIP_CELL_BS)
+        *deferred = true;
+    else
+        enqueue_tile(p, inst, bi, bj, a, b, queue, qstride, split, bb_ksplit, bb_wide, ctr);
     atomicAdd(&ctr->cell_tiles[p], 1ull);
     if (p == STORM_HIP_CELL_BB) {
         // B x B tiles skip the per-pair span test (same cost either way);
@@ -337,6 +338,48 @@ __global__ __launch_bounds__(256) void select_kernel(
         atomicAdd(&ctr->cell_pairs[p], (unsigned long long)(bi == bj ? ni * (ni - 1) / 2 : ni * nj));
     }
     if (fallback) atomicAdd(&ctr->fallback_tiles, 1ull);
+}
+
+// One thread per row bi and pair of tile columns (2k, 2k + 1): decides both
+// tile pairs, then queues their B x S choices, as wide stripes when both chose
+// B x S and bs_wide_ok holds (wide_bs: the call may pair them).
+__global__ __launch_bounds__(256) void select_kernel(
+    const TileMeta* __restrict__ tm, uint32_t T, DevModel M, uint32_t reps, uint32_t policy,
+    uint32_t fixed_cell, const float* __restrict__ learned_cost,
+    const uint8_t* __restrict__ learned_ok, uint8_t* __restrict__ decisions,
+    QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, uint32_t tile, bool split,
+    double thr, uint32_t bb_ksplit, bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
+    const uint32_t* __restrict__ sel, DevCounters* __restrict__ ctr, uint16_t* __restrict__ probe,
+    ProbeAcc* __restrict__ acc, uint32_t shadow_inst, bool wide_bs) {
+    const uint32_t KT = (T + 1) / 2;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (uint64_t)T * KT) return;
+    const uint32_t bi = (uint32_t)(g / KT), bj0 = 2 * (uint32_t)(g % KT);
+    if (bj0 + 1 < bi) return;   // both columns below the diagonal
+    const bool defer = wide_bs && bi < bj0;
+    bool d[2] = {false, false};
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t bj = bj0 + h;
+        if (bj < bi || bj >= T) continue;
+        select_tile(tri_tile_id(bi, bj, T), bi, bj, tm, M, reps, policy, fixed_cell,
+                    learned_cost, learned_ok, decisions, queue, qstride, inst, split, thr,
+                    bb_ksplit, bb_wide, explore_seed, explore_share, sel, ctr, probe, acc,
+                    shadow_inst, defer, &d[h]);
+    }
+    if (!d[0] && !d[1]) return;
+    const TileMeta a = tm[bi];
+    if (d[0] && d[1] && bs_wide_ok(bi, bj0, tile, a, tm[bj0], tm[bj0 + 1])) {
+        const uint32_t ns = (a.rows + kSplitRows - 1) / kSplitRows;
+        uint32_t slot = atomicAdd(&ctr->qcount[STORM_HIP_CELL_BS], ns);
+        for (uint32_t x = 0; x < ns; ++x)
+            queue[(uint64_t)STORM_HIP_CELL_BS * qstride + slot++] =
+                QItem{inst, bi, bj0, make_sub(kSplitRows, x, kWideSide)};
+        return;
+    }
+    for (uint32_t h = 0; h < 2; ++h)
+        if (d[h])
+            enqueue_tile(STORM_HIP_CELL_BS, inst, bi, bj0 + h, a, tm[bj0 + h], queue, qstride,
+                         split, bb_ksplit, bb_wide, ctr);
 }
 
 // Probe: add each timed item of pass 1 to its tile's class, as CU-time (wall
@@ -416,13 +459,13 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
                   bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
                   const uint32_t* sel, DevCounters* ctr, uint16_t* probe, ProbeAcc* acc,
-                  uint32_t shadow_inst) {
-    const uint64_t P = (uint64_t)T * (T + 1) / 2;
-    if (P == 0) return STORM_HIP_OK;
-    hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((P + 255) / 256)), dim3(256), 0, s, tm, T, P,
-                       M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions, queue,
-                       qstride, inst, tile > kSplitRows, thr, bb_ksplit, bb_wide, explore_seed,
-                       explore_share, sel, ctr, probe, acc, shadow_inst);
+                  uint32_t shadow_inst, bool wide_bs) {
+    const uint64_t threads = (uint64_t)T * ((T + 1) / 2);
+    if (threads == 0) return STORM_HIP_OK;
+    hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s,
+                       tm, T, M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions,
+                       queue, qstride, inst, tile, tile > kSplitRows, thr, bb_ksplit, bb_wide,
+                       explore_seed, explore_share, sel, ctr, probe, acc, shadow_inst, wide_bs);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

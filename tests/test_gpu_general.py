@@ -240,6 +240,50 @@ def test_autotune_general_keeps_counts(ctx, cases):
     ctx.set_bb_config(8, 4, 2, 8)
 
 
+def test_bs_wide_stripes(ctx):
+    """Wide B x S stripes: 1,300 rows (tiles 0..4 full, tile 5 of 20 rows) with
+    cardinalities spanning 1..20,000 over a universe of 60,001, every row a
+    bitmap, so the list tiles 0 and 1 run against the full bitmap tile pairs
+    (2, 3) as wide stripes (one 64-byte slice load per element for 512 rows)
+    and against (4, 5) as ordinary ones (tile 5 partial); also with bitmaps for
+    the dense rows only (no wide stripe where a tile has lists). Exact per pair
+    under the forced cell, the analytic policy, thresholded calls and an
+    exploration call (its labels split a wide stripe's time over its two tile
+    pairs)."""
+    rng = np.random.default_rng(17)
+    u = 60_001
+    rows = [corpora.uniform_row(rng, u, int(np.exp(rng.random() * np.log(20_000))))
+            for _ in range(1300)]
+    rows[3] = np.zeros(0, np.uint32)
+    offs, pos = corpora.csr(rows)
+    want_sum, want = O.allpairs_lists(offs, pos)
+    card = np.diff(offs.astype(np.int64)).astype(np.float64)
+    I, J = np.triu_indices(len(rows), 1)
+    v = want.astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        jac = v / (card[I] + card[J] - v)
+    for bitmaps in (1, 0):
+        print("case bs wide stripes bitmaps", bitmaps, flush=True)
+        with ctx.from_lists(offs, pos, u, bitmap_policy=bitmaps | BUILD_ALL | 64) as c:
+            for pol in ("fixed", "analytic"):
+                for graph in (False, True):
+                    st = c.allpairs(policy=pol, fixed_cell=CELLS.index("BS"), out="tri",
+                                    tile=256, use_graph=graph)
+                    assert st["sum"] == want_sum, (bitmaps, pol, graph)
+                    assert np.array_equal(c.tri(), want), (bitmaps, pol, graph)
+                    assert sum(st["cell_pairs"].values()) == st["pairs"], (bitmaps, pol, st)
+            st, hits = c.allpairs_threshold(0.05, policy="fixed", fixed_cell=CELLS.index("BS"),
+                                            tile=256)
+            ok = (want > 0) & (jac >= 0.05)
+            exp = np.stack([I[ok], J[ok], want[ok]], 1).astype(np.uint32)
+            assert np.array_equal(hits, exp), bitmaps
+            X, Y, M, idx, s = ctx.selector_explore(c, tile=256, seed=3, fraction=1.0)
+            assert s == want_sum, bitmaps
+            T = (len(rows) + 255) // 256
+            assert len(X) == T * (T + 1) // 2, (bitmaps, len(X))
+            assert np.all(M.sum(axis=1) == 1), bitmaps
+
+
 @pytest.mark.parametrize("tile", [64, 128, 256])
 def test_bs_slices(ctx, tile):
     """B x S through bit-sliced bitmap groups: 300 rows whose cardinalities span
