@@ -125,13 +125,24 @@ int  storm_hip_corpus_upload(storm_hip_ctx* ctx, const storm_hip_arena* arena,
  * the bitmaps of every 64 density-ordered rows that all hold one (one 64-bit
  * row mask per position; as much device memory as those bitmaps), which lets
  * B x S count a 64-row block against a list with one load per list element;
- * it has no counterpart in the reference's representations. */
+ * it has no counterpart in the reference's representations.
+ * STORM_HIP_BUILD_ZONES adds a coarse positional zone map per row (the Occ of
+ * bench/bench_tile.cpp:52-62 at 16,384 buckets: bucket = position >> shift,
+ * shift the smallest with 2^shift >= ceil(universe / 16384), at most 257
+ * words per row), built on the device from whichever representation the row
+ * holds. Calls then settle as empty every tile pair whose tiles share no
+ * bucket (the OR of their rows' maps; the empty-tile skip of
+ * bench_tile.cpp:807-827) and, where a tile pair shares under half of the
+ * buckets, skip in the per-pair cells every pair whose rows share none (the
+ * gated per-pair filter of bench_tile.cpp:180-190). Counts are unchanged;
+ * pairs skipped this way count as skipped, like the span test's. */
 enum {
     STORM_HIP_BITMAP_DENSE = 0, STORM_HIP_BITMAP_ALL = 1, STORM_HIP_BITMAP_NONE = 2,
     STORM_HIP_BUILD_RUNS = 4, STORM_HIP_BUILD_EWAH = 8, STORM_HIP_BUILD_COMP = 16,
     STORM_HIP_BUILD_ALL = 4 | 8 | 16,
     STORM_HIP_BUILD_ON_HOST = 32,
-    STORM_HIP_BUILD_SLICES = 64
+    STORM_HIP_BUILD_SLICES = 64,
+    STORM_HIP_BUILD_ZONES = 128
 };
 /* Seeded synthetic corpora as host CSR lists (storm::generate's role,
  * kernels/storm_gen.cpp:173-232, with the spectra the benchmark workloads

@@ -330,6 +330,81 @@ int build_slices(storm_hip_corpus* c, hipStream_t s) {
     return STORM_HIP_OK;
 }
 
+// Coarse zone map of one row per workgroup (STORM_HIP_BUILD_ZONES): bucket
+// p >> shift of every position p the row holds, gathered in LDS from its list,
+// else its runs, else its bitmap, then written out (zld words).
+constexpr uint32_t kZoneBuckets = 16384, kZoneMaxWords = kZoneBuckets / 64 + 1;
+
+__global__ __launch_bounds__(256) void zone_build_kernel(
+    const uint64_t* __restrict__ list_off, const uint32_t* __restrict__ list,
+    const uint64_t* __restrict__ run_off, const uint32_t* __restrict__ run_start,
+    const uint32_t* __restrict__ run_end, const uint32_t* __restrict__ bm_row,
+    const uint64_t* __restrict__ bitmap, uint32_t ld, uint32_t nw, uint32_t shift,
+    uint32_t zld, uint64_t* __restrict__ zone) {
+    __shared__ unsigned long long z[kZoneMaxWords];
+    const uint32_t r = blockIdx.x, tid = threadIdx.x;
+    for (uint32_t w = tid; w < zld; w += blockDim.x) z[w] = 0;
+    __syncthreads();
+    auto mark = [&](uint32_t b) { atomicOr(&z[b >> 6], 1ull << (b & 63)); };
+    if (list) {
+        for (uint64_t k = list_off[r] + tid; k < list_off[r + 1]; k += blockDim.x)
+            mark(list[k] >> shift);
+    } else if (run_off) {
+        // A run [s, e) covers buckets s >> shift .. (e - 1) >> shift.
+        for (uint64_t k = run_off[r] + tid; k < run_off[r + 1]; k += blockDim.x)
+            for (uint32_t b = run_start[k] >> shift; b <= (run_end[k] - 1) >> shift; ++b) mark(b);
+    } else if (bm_row[r] != kNoBitmap) {
+        const uint64_t* bm = bitmap + (uint64_t)bm_row[r] * ld;
+        for (uint32_t w = tid; w < nw; w += blockDim.x) {
+            uint64_t v = bm[w];
+            if (!v) continue;
+            if (shift >= 6) {
+                mark((w * 64u) >> shift);
+            } else {
+                while (v) {
+                    mark((w * 64u + (uint32_t)__ffsll((unsigned long long)v) - 1) >> shift);
+                    v &= v - 1;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t w = tid; w < zld; w += blockDim.x) zone[(uint64_t)r * zld + w] = z[w];
+}
+
+// (Re)builds every row's zone map when the corpus asks for them.
+int build_zones(storm_hip_corpus* c, hipStream_t s) {
+    if (c->d_zone) {
+        SH_HIP(hipFree(c->d_zone));
+        c->d_zone = nullptr;
+        c->device_bytes -= (uint64_t)c->n_rows * c->zone_ld * 8;
+    }
+    if (!c->want_zones || c->n_rows == 0 || c->dense) return STORM_HIP_OK;
+    if (!c->d_list && !c->d_run_off && !c->d_bitmap) {
+        set_error("zone maps: the rows hold no list, runs or bitmap");
+        return STORM_HIP_ERR_INVALID;
+    }
+    // bench/bench_tile.cpp Occ::init: width = ceil(universe / buckets), shift
+    // the smallest with 2^shift >= width, ((universe >> shift) + 1 + 63) / 64 words.
+    const uint64_t width = ((uint64_t)c->universe + kZoneBuckets - 1) / kZoneBuckets;
+    uint32_t shift = 0;
+    while ((1ull << shift) < width) ++shift;
+    c->zone_shift = shift;
+    c->zone_ld = (uint32_t)((((uint64_t)c->universe >> shift) + 1 + 63) / 64);
+    const uint64_t bytes = (uint64_t)c->n_rows * c->zone_ld * 8;
+    SH_HIP(hipMalloc(&c->d_zone, bytes));
+    c->device_bytes += bytes;
+    const bool lists = c->d_list != nullptr && c->has_lists;
+    const bool runs = !lists && c->d_run_off != nullptr;
+    hipLaunchKernelGGL(zone_build_kernel, dim3(c->n_rows), dim3(256), 0, s,
+                       lists ? c->d_list_off : nullptr, lists ? c->d_list : nullptr,
+                       runs ? c->d_run_off : nullptr, c->d_run_start, c->d_run_end,
+                       c->d_bm_row, c->d_bitmap, c->ld, c->n_words, shift, c->zone_ld,
+                       c->d_zone);
+    SH_HIP(hipGetLastError());
+    return STORM_HIP_OK;
+}
+
 int upload_host_arena(storm_hip_ctx* ctx, const HostArena& A, uint32_t flags,
                       storm_hip_corpus** out) {
     storm_hip_corpus* c = corpus_new(ctx, A.n, A.universe, flags);
@@ -369,6 +444,7 @@ storm_hip_corpus* corpus_new(storm_hip_ctx* ctx, uint32_t n, uint32_t universe, 
     c->has_ewah = flags & STORM_HIP_BUILD_EWAH;
     c->has_comp = flags & STORM_HIP_BUILD_COMP;
     c->want_slices = flags & STORM_HIP_BUILD_SLICES;
+    c->want_zones = flags & STORM_HIP_BUILD_ZONES;
     return c;
 }
 
@@ -413,6 +489,7 @@ int corpus_finish_store(storm_hip_ctx* ctx, storm_hip_corpus* c) {
     SH_TRY(upload_orders(c, s));
     SH_TRY(build_indexes(c, s));
     SH_TRY(build_slices(c, s));
+    SH_TRY(build_zones(c, s));
     SH_HIP(hipStreamSynchronize(s));
     return STORM_HIP_OK;
 }
@@ -439,6 +516,7 @@ int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uin
     B += ((size_t)nb + 1) * c->ld * 8;
     SH_TRY(build_indexes(c, s));
     SH_TRY(build_slices(c, s));
+    SH_TRY(build_zones(c, s));
     SH_HIP(hipStreamSynchronize(s));
     return STORM_HIP_OK;
 }
@@ -469,7 +547,8 @@ void general_free_arenas(storm_hip_corpus* c) {
                     c->d_ewah_off, c->d_ewah, c->d_mk_off, c->d_mk_word, c->d_mk_pos,
                     c->d_comp_off, c->d_comp, c->d_meta, c->d_bm_row, c->d_bitmap,
                     c->d_perm, c->d_ident, c->d_inv, c->d_occ,
-                    c->d_slice, c->d_slice_of, c->d_rank, c->d_rank_row, c->d_occ_row};
+                    c->d_slice, c->d_slice_of, c->d_rank, c->d_rank_row, c->d_occ_row,
+                    c->d_zone};
     for (void* p : ptrs) if (p) (void)hipFree(p);
 }
 

@@ -330,6 +330,29 @@ __device__ inline bool span_disjoint(const DevCorpus& D, uint32_t i, uint32_t j)
     return D.thr > 0.0 && (double)min(ci, cj) / (double)max(ci, cj) < D.thr;
 }
 
+// Coarse zone maps (bench/bench_tile.cpp:180-190, the per-pair filter in front
+// of the cell): rows i and j share no bucket, so no position. Called by a
+// whole wave; every lane gets the answer.
+__device__ inline bool zone_disjoint(const DevCorpus& D, uint32_t i, uint32_t j, uint32_t lane) {
+    const uint64_t* zi = D.zone + (uint64_t)i * D.zone_ld;
+    const uint64_t* zj = D.zone + (uint64_t)j * D.zone_ld;
+    uint64_t any = 0;
+    for (uint32_t w = lane; w < D.zone_ld; w += 64) any |= zi[w] & zj[w];
+    return __ballot(any != 0) == 0;
+}
+
+// The per-pair zone test pays where the tile pair's tiles share under half of
+// the buckets (the gate); elsewhere nearly every pair shares one. Called by a
+// whole wave.
+__device__ inline bool zone_gate(const DevCorpus& D, uint32_t bi, uint32_t bj, uint32_t lane) {
+    if (!D.zone || !D.tzone) return false;
+    const uint64_t* a = D.tzone + (uint64_t)bi * D.zone_ld;
+    const uint64_t* b = D.tzone + (uint64_t)bj * D.zone_ld;
+    uint32_t c = 0;
+    for (uint32_t w = lane; w < D.zone_ld; w += 64) c += __popcll(a[w] & b[w]);
+    return 2u * wave_sum_u32(c) < D.zone_ld * 64u;
+}
+
 // ---- S x S at tile granularity: a hash join in LDS -------------------------
 //
 // For a block of up to 64 x 64 pairs of list rows, every pair's count is the
@@ -1113,12 +1136,13 @@ __global__ __launch_bounds__(256) void cell_kernel(const DevCorpus* __restrict__
                 emit_pair(D, i, j, cnt);
             }
         } else {
+            const bool gate = zone_gate(D, e.bi, e.bj, lane);
             for (uint32_t p = wave; p < np; p += blockDim.x / 64) {
                 const uint32_t a = i0 + p / nj, b = j0 + p % nj;
                 if (same_tile && b <= a) continue;
                 const uint32_t i = D.perm[a], j = D.perm[b];
                 uint32_t cnt = 0;
-                const bool dis = span_disjoint(D, i, j);
+                const bool dis = span_disjoint(D, i, j) || (gate && zone_disjoint(D, i, j, lane));
                 if (lane == 0) { skipped += dis ? 1u : 0u; pairs += dis ? 0u : 1u; }
                 if (!dis) {
                     cnt = wave_sum_u32(cell_partial<CELL>(D, M, i, j, lane, 64));
@@ -1167,18 +1191,49 @@ __device__ inline TileMeta row_tile(const DevCorpus& D, uint32_t r) {
     return m;
 }
 
+// One pair's B x B with the zone-map plan (bb_occ_sel, kernels/cell_bb.cpp:
+// 537-550): when both rows carry a zone map (one bit per 8-word bin), the
+// bins live on both sides are counted first; none settles the pair at 0,
+// fewer than a third of all bins (kOccPlanSelectivity, :522) visits only
+// those bins, else every word. Lane `first` of `stride` (a whole wave:
+// first = lane, stride = 64, when the plan is taken).
+__device__ uint32_t bb_pair_partial(const DevCorpus& D, uint32_t i, uint32_t j, uint32_t first,
+                                    uint32_t stride) {
+    const uint64_t* a = bitmap_of(D, i);
+    const uint64_t* b = bitmap_of(D, j);
+    const uint64_t* oa = occ_of(D, i);
+    const uint64_t* ob = occ_of(D, j);
+    uint32_t c = 0;
+    if (oa && ob && stride == 64) {
+        const uint32_t bins = (D.nw + 7) / 8;
+        uint32_t live = 0;
+        for (uint32_t k = first; k < D.occ_ld; k += 64) live += __popcll(oa[k] & ob[k]);
+        live = wave_sum_u32(live);
+        if (live == 0) return 0;
+        if ((double)live < 0.33 * (double)bins) {
+            // Each lane takes the live bins of its own zone-map words.
+            for (uint32_t k = first; k < D.occ_ld; k += 64) {
+                uint64_t m = oa[k] & ob[k];
+                while (m) {
+                    const uint32_t bin = k * 64 + (uint32_t)__ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const uint32_t w0 = bin * 8, w1 = min(w0 + 8, D.nw);
+                    for (uint32_t w = w0; w < w1; ++w) c += __popcll(a[w] & b[w]);
+                }
+            }
+            return c;
+        }
+    }
+    for (uint32_t k = first; k < D.nw; k += stride) c += __popcll(a[k] & b[k]);
+    return c;
+}
+
 // Lane `first` of `stride` of one pair's count under a runtime-chosen cell
 // (B x B: AND + popcount over the words; the rest as their kernels count).
 __device__ uint32_t pair_partial(int cell, const DevCorpus& D, const DevModel& M, uint32_t i,
                                  uint32_t j, uint32_t first, uint32_t stride) {
     switch (cell) {
-        case STORM_HIP_CELL_BB: {
-            const uint64_t* a = bitmap_of(D, i);
-            const uint64_t* b = bitmap_of(D, j);
-            uint32_t c = 0;
-            for (uint32_t k = first; k < D.nw; k += stride) c += __popcll(a[k] & b[k]);
-            return c;
-        }
+        case STORM_HIP_CELL_BB: return bb_pair_partial(D, i, j, first, stride);
         case STORM_HIP_CELL_BS: return cell_partial<STORM_HIP_CELL_BS>(D, M, i, j, first, stride);
         case STORM_HIP_CELL_BR: return cell_partial<STORM_HIP_CELL_BR>(D, M, i, j, first, stride);
         case STORM_HIP_CELL_BW: return cell_partial<STORM_HIP_CELL_BW>(D, M, i, j, first, stride);

@@ -39,8 +39,12 @@ struct DevCorpus {
     // slice[((s / 8) * ld * 64 + p) * 8 + s % 8] has bit k set when sorted
     // row 64g + k holds position p; groups 8q..8q+7 share slab q, lane g % 8.
     const uint64_t* slice;    const uint32_t* slice_of;
+    // Coarse zone maps (null: none): row r's zone_ld words at zone + r *
+    // zone_ld, bit b for positions p with p >> zone_shift == b.
+    const uint64_t* zone;     uint32_t zone_ld, zone_shift;
     // Per-call state of this corpus inside a (possibly multi-corpus) call.
     const struct TileMeta* tm;        // its row tiles' metadata
+    uint64_t* tzone;                  // per row tile: OR of its rows' zone maps
     const uint32_t* rowmap;           // sorted position -> bitmap row
     uint32_t* tri;                    // per-pair output or null
     unsigned long long* sum;          // its sum
@@ -176,6 +180,17 @@ struct TileMeta {
     uint32_t max_card, min_card, max_runs, max_ewah, rows;
     double spread;                               // max / mean cardinality
 };
+
+// Buckets shared by the zone maps of row tiles bi and bj (tz: one map of
+// zone_ld words per tile): 0 proves every pair of the tile pair disjoint.
+__device__ inline uint32_t tile_zone_shared(const uint64_t* __restrict__ tz, uint32_t zone_ld,
+                                            uint32_t bi, uint32_t bj) {
+    const uint64_t* a = tz + (uint64_t)bi * zone_ld;
+    const uint64_t* b = tz + (uint64_t)bj * zone_ld;
+    uint32_t c = 0;
+    for (uint32_t w = 0; w < zone_ld; ++w) c += __popcll(a[w] & b[w]);
+    return c;
+}
 
 // B x S probes the lists of side A (else B) against the other side's bitmaps:
 // A when B is all bitmaps and A is the sparser side (or not all bitmaps).

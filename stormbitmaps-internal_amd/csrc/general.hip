@@ -28,7 +28,8 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
                   bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
                   const uint32_t* sel, DevCounters* ctr, uint16_t* probe, ProbeAcc* acc,
-                  uint32_t shadow_inst, bool wide_bs);
+                  uint32_t shadow_inst, bool wide_bs, const uint64_t* tzone,
+                  uint32_t zone_ld);
 int launch_refine(hipStream_t s, uint32_t grid, const DevCorpus* Ds, const DevModel& M,
                   const QItem* queue, DevCounters* ctr, unsigned long long* tile_cycles);
 uint32_t refine_blocks_per_cu();
@@ -65,11 +66,13 @@ struct Work {
     float* learned_cost = nullptr;
     uint8_t* learned_ok = nullptr;
     uint16_t* probe = nullptr;        // per-tile probe state (PROBE calls)
+    uint64_t* tzone = nullptr;        // per row tile: OR of its rows' zone maps
 };
 
 void free_work(Work* w) {
     if (!w) return;
-    void* p[] = {w->tm, w->decisions, w->rowmap, w->learned_cost, w->learned_ok, w->probe};
+    void* p[] = {w->tm, w->decisions, w->rowmap, w->learned_cost, w->learned_ok, w->probe,
+                 w->tzone};
     for (void* x : p) if (x) (void)hipFree(x);
     delete w;
 }
@@ -89,6 +92,8 @@ int make_work(const storm_hip_corpus* c, uint32_t tile, Work** out) {
     SH_HIP(hipMalloc(&w->learned_cost, sizeof(float) * P1 * 10));
     SH_HIP(hipMalloc(&w->learned_ok, P1));
     SH_HIP(hipMalloc(&w->probe, sizeof(uint16_t) * P1));
+    if (c->d_zone)
+        SH_HIP(hipMalloc(&w->tzone, sizeof(uint64_t) * c->zone_ld * (w->T ? w->T : 1)));
     return STORM_HIP_OK;
 }
 
@@ -177,6 +182,8 @@ DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
     D.emit_kind = th.kind; D.emit_cols = th.cols; D.emit_split = th.split;
     D.emit_buf = th.buf; D.inv = th.inv;
     D.slice = c->d_slice; D.slice_of = c->d_slice_of;
+    D.zone = c->d_zone; D.zone_ld = c->zone_ld; D.zone_shift = c->zone_shift;
+    D.tzone = w->tzone;
     D.tm = w->tm; D.rowmap = w->rowmap; D.tile = w->tile; D.probe = w->probe;
     D.reps = reps_of(c);
     D.tri = tri; D.sum = sum; D.finish = finish;
@@ -473,7 +480,8 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
                              o.policy == kPolicyExplore ? p->explore_seed
                                                         : (probe ? kGolden : 0),
                              p->explore_share, p->d_sel, p->ctr, probe ? w->probe : nullptr,
-                             p->acc, (uint32_t)(k + n), bs_wide_call(c, sel_policy, p->tile)));
+                             p->acc, (uint32_t)(k + n), bs_wide_call(c, sel_policy, p->tile),
+                             w->tzone, c->zone_ld));
         SH_TRY(launch_compose_rowmap(s, p->h_corpora[k].perm, c->d_bm_row, w->npad, w->rowmap));
         // Only corpora with bitmap rows can queue B x B tiles.
         if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
@@ -504,7 +512,8 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
                              nullptr, w->decisions, p->queue, p->qstride, (uint32_t)k, p->tile,
                              p->th.thr, p->h_corpora[k].bb_ksplit,
                              bb_wide(c, o, p->tile, thresholded_plan(p)), 0, 0, p->d_sel, p->ctr,
-                             w->probe, p->acc, (uint32_t)(k + n), false));
+                             w->probe, p->acc, (uint32_t)(k + n), false, w->tzone,
+                             c->zone_ld));
     }
     return dispatch_queues(ctx, p, o, M, fp4, false, false);
 }

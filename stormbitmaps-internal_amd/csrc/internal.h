@@ -92,6 +92,12 @@ struct storm_hip_corpus {
     uint64_t* d_slice = nullptr;
     uint32_t* d_slice_of = nullptr;
     uint32_t n_slices = 0;
+    // Coarse zone maps (STORM_HIP_BUILD_ZONES): row r's map is zone_ld words
+    // at d_zone + r * zone_ld, bit b set when the row holds a position p with
+    // p >> zone_shift == b.
+    bool want_zones = false;
+    uint64_t* d_zone = nullptr;
+    uint32_t zone_ld = 0, zone_shift = 0;
     // Dense layout: n_rows_pad x ld words.
     uint64_t* d_words = nullptr;
     uint32_t ld = 0;

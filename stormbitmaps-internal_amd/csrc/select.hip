@@ -76,6 +76,13 @@ __global__ __launch_bounds__(256) void tile_meta_kernel(DevCorpus D, uint32_t ti
         m.spread = mean > 0.0 ? (double)m.max_card / mean : 1.0;
         out[t] = m;
     }
+    // The tile's zone map: the OR of its rows' maps.
+    if (D.zone && D.tzone)
+        for (uint32_t w = threadIdx.x; w < D.zone_ld; w += blockDim.x) {
+            uint64_t v = 0;
+            for (uint32_t k = lo; k < hi; ++k) v |= D.zone[(uint64_t)D.perm[k] * D.zone_ld + w];
+            D.tzone[(uint64_t)t * D.zone_ld + w] = v;
+        }
 }
 
 // Cells the learned selector may rank on a tile: eligible cells whose
@@ -202,7 +209,8 @@ __device__ void select_tile(
     QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, bool split, double thr,
     uint32_t bb_ksplit, bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
     const uint32_t* __restrict__ sel, DevCounters* __restrict__ ctr, uint16_t* __restrict__ probe,
-    ProbeAcc* __restrict__ acc, uint32_t shadow_inst, bool defer_bs, bool* deferred) {
+    ProbeAcc* __restrict__ acc, uint32_t shadow_inst, bool defer_bs, bool* deferred,
+    const uint64_t* __restrict__ tzone, uint32_t zone_ld) {
     // Band and cross calls queue only the tile pairs of their range.
     if (bi < sel[0] || bi >= sel[1] || bj < sel[2] || bj >= sel[3]) {
         decisions[t] = 0xFF;
@@ -237,7 +245,8 @@ __device__ void select_tile(
     if (probe) probe[t] = 0;
     // Thresholded calls: a tile whose length bound is below the threshold
     // holds no reportable pair.
-    if (tile_empty(a, b) || (thr > 0.0 && length_bound(a, b) < thr)) {
+    if (tile_empty(a, b) || (thr > 0.0 && length_bound(a, b) < thr) ||
+        (tzone && tile_zone_shared(tzone, zone_ld, bi, bj) == 0)) {
         p = STORM_HIP_CELL_EMPTY;
     } else if (policy == STORM_HIP_POLICY_PER_PAIR) {
         // Every pair of the tile selects for itself.
@@ -350,7 +359,8 @@ __global__ __launch_bounds__(256) void select_kernel(
     QItem* __restrict__ queue, uint64_t qstride, uint32_t inst, uint32_t tile, bool split,
     double thr, uint32_t bb_ksplit, bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
     const uint32_t* __restrict__ sel, DevCounters* __restrict__ ctr, uint16_t* __restrict__ probe,
-    ProbeAcc* __restrict__ acc, uint32_t shadow_inst, bool wide_bs) {
+    ProbeAcc* __restrict__ acc, uint32_t shadow_inst, bool wide_bs,
+    const uint64_t* __restrict__ tzone, uint32_t zone_ld) {
     const uint32_t KT = (T + 1) / 2;
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (uint64_t)T * KT) return;
@@ -364,7 +374,7 @@ __global__ __launch_bounds__(256) void select_kernel(
         select_tile(tri_tile_id(bi, bj, T), bi, bj, tm, M, reps, policy, fixed_cell,
                     learned_cost, learned_ok, decisions, queue, qstride, inst, split, thr,
                     bb_ksplit, bb_wide, explore_seed, explore_share, sel, ctr, probe, acc,
-                    shadow_inst, defer, &d[h]);
+                    shadow_inst, defer, &d[h], tzone, zone_ld);
     }
     if (!d[0] && !d[1]) return;
     const TileMeta a = tm[bi];
@@ -459,13 +469,15 @@ int launch_select(hipStream_t s, const TileMeta* tm, uint32_t T, const DevModel&
                   uint32_t inst, uint32_t tile, double thr, uint32_t bb_ksplit,
                   bool bb_wide, uint64_t explore_seed, uint32_t explore_share,
                   const uint32_t* sel, DevCounters* ctr, uint16_t* probe, ProbeAcc* acc,
-                  uint32_t shadow_inst, bool wide_bs) {
+                  uint32_t shadow_inst, bool wide_bs, const uint64_t* tzone,
+                  uint32_t zone_ld) {
     const uint64_t threads = (uint64_t)T * ((T + 1) / 2);
     if (threads == 0) return STORM_HIP_OK;
     hipLaunchKernelGGL(select_kernel, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, s,
                        tm, T, M, reps, policy, fixed_cell, learned_cost, learned_ok, decisions,
                        queue, qstride, inst, tile, tile > kSplitRows, thr, bb_ksplit, bb_wide,
-                       explore_seed, explore_share, sel, ctr, probe, acc, shadow_inst, wide_bs);
+                       explore_seed, explore_share, sel, ctr, probe, acc, shadow_inst, wide_bs,
+                       tzone, zone_ld);
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
 }

@@ -317,3 +317,86 @@ def test_bs_slices(ctx, tile):
                 ok = (want > 0) & (v / (card[I] + card[J] - v) >= 0.2)
             exp = np.stack([I[ok], J[ok], want[ok]], 1).astype(np.uint32)
             assert np.array_equal(hits, exp), bitmaps
+
+
+def _phase_rows(rng, u, shift, phases, card):
+    """Positions whose bucket (p >> shift) falls in one of `phases` mod 6."""
+    nb = (u >> shift) + 1
+    buckets = np.array([b for b in range(nb) if b % 6 in phases], np.int64)
+    pick = rng.choice(len(buckets), size=card, replace=True)
+    p = buckets[pick] * (1 << shift) + rng.integers(0, 1 << shift, size=card)
+    return np.unique(p[p < u]).astype(np.uint32)
+
+
+def test_zone_maps(ctx):
+    """Coarse zone maps (16,384 buckets; here bucket = position >> 6): twelve
+    64-row groups with disjoint cardinality bands, so each 64-row tile of the
+    density order is one group; groups 0..5 hold positions only in buckets of
+    one residue mod 6, groups 6..11 in two residues. Tile pairs of disjoint
+    residue sets are settled empty before any cell runs; tile pairs that
+    share under half the buckets test every pair (rows of different residues
+    are skipped). Exact per pair under the analytic policy and every per-pair
+    cell, with more skipped pairs and empty tiles than without zone maps."""
+    rng = np.random.default_rng(5)
+    u = 1_000_003
+    rows = []
+    for g in range(12):
+        ph = {g % 6} if g < 6 else {g % 6, (g + 1) % 6}
+        for _ in range(64):
+            rows.append(_phase_rows(rng, u, 6, ph, 40 + 100 * g + int(rng.integers(0, 50))))
+    # bands are disjoint after de-duplication too: check the grouping holds
+    cards = [len(r) for r in rows]
+    assert all(max(cards[64 * g:64 * g + 64]) < min(cards[64 * g + 64:64 * g + 128])
+               for g in range(11))
+    offs, pos = corpora.csr(rows)
+    want_sum, want = O.allpairs_lists(offs, pos)
+    skipped = {}
+    for zones in (0, 128):
+        for bitmaps in (1, 0):
+            print("case zone maps zones", zones, "bitmaps", bitmaps, flush=True)
+            with ctx.from_lists(offs, pos, u, bitmap_policy=bitmaps | BUILD_ALL | zones) as c:
+                for tile in (64, 256):
+                    st = c.allpairs(policy="analytic", out="tri", tile=tile)
+                    assert st["sum"] == want_sum, (zones, bitmaps, tile)
+                    assert np.array_equal(c.tri(), want), (zones, bitmaps, tile)
+                    if tile == 64 and bitmaps == 1:
+                        empties = st["cell_tiles"].get("EMPTY", 0)
+                        if zones:
+                            assert empties > 0, st["cell_tiles"]
+                        else:
+                            assert empties == 0, st["cell_tiles"]
+                cells = ["SS", "SR", "SW", "RR", "RW", "WW"] + (["BS", "BR", "BW"] if bitmaps else [])
+                for cell in cells:
+                    st = c.allpairs(policy="fixed", fixed_cell=CELLS.index(cell), out="tri",
+                                    tile=64)
+                    assert st["sum"] == want_sum, (zones, bitmaps, cell)
+                    assert np.array_equal(c.tri(), want), (zones, bitmaps, cell)
+                    assert sum(st["cell_pairs"].values()) == st["pairs"], (cell, st)
+                    skipped[(zones, bitmaps, cell)] = st["skipped"]
+    for (zones, bitmaps, cell), sk in skipped.items():
+        if zones:
+            assert sk > skipped[(0, bitmaps, cell)], (bitmaps, cell, sk)
+
+
+def test_bb_zone_map_plan_per_pair(ctx):
+    """bb_occ_sel for one pair: 300 dense rows (1,000-1,400 positions, so
+    every row carries a zone map of 8-word bins) clustered in three windows of
+    4,096 positions each over a universe of 300,007; under PerPair most pairs
+    choose B x B, priced by their live bins, and count only the bins live on
+    both sides (or none). Exact against the oracle."""
+    rng = np.random.default_rng(23)
+    u = 300_007
+    rows = []
+    for r in range(300):
+        starts = rng.integers(0, u - 4096, size=3)
+        k = int(rng.integers(1000, 1400))
+        p = np.concatenate([s + rng.integers(0, 4096, size=k // 3 + 1) for s in starts])
+        rows.append(np.unique(p).astype(np.uint32))
+    offs, pos = corpora.csr(rows)
+    want_sum, want = O.allpairs_lists(offs, pos)
+    with ctx.from_lists(offs, pos, u, bitmap_policy=1 | BUILD_ALL) as c:
+        for tile in (64, 256):
+            st = c.allpairs(policy="per_pair", out="tri", tile=tile)
+            assert st["sum"] == want_sum, tile
+            assert np.array_equal(c.tri(), want), tile
+            assert st["cell_pairs"].get("BB", 0) > 0, st["cell_pairs"]
