@@ -16,7 +16,9 @@ same rows.
 
 Other workloads (--config): dense1 (N=2,048 x M=262,144, p=0.01), zipf3 and
 runs4 (mixed representations: analytic vs calibrated vs learned selection vs
-all-bitmap), batch5 (many independent corpora in one set of launches).
+all-bitmap), batch5 (many independent corpora in one set of launches). The
+default run also reports the thresholded mode on sparse skewed corpora
+(prefix-filtered candidates vs the exact thresholded engine).
 """
 import argparse
 import importlib.util
@@ -407,6 +409,64 @@ def sparse_bench(sh, args):
     return out
 
 
+def skewed_sparse_rows(n, u, s, seed):
+    """n rows of 5..40 positions drawn with popularity ~ 1 / rank^s over u
+    positions (popular ids scattered by a seeded permutation): a sparse corpus
+    with skewed document frequencies, the regime prefix filtering is for."""
+    rng = np.random.default_rng(seed)
+    w = 1.0 / np.arange(1, u + 1) ** s
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    lens = rng.integers(5, 41, size=n)
+    draws = np.searchsorted(cdf, rng.random(int(lens.sum())))
+    ids = rng.permutation(u)
+    rows, k = [], 0
+    for L in lens:
+        rows.append(np.unique(ids[draws[k:k + L]]).astype(np.uint32))
+        k += L
+    offs = np.zeros(n + 1, np.uint64)
+    offs[1:] = np.cumsum([len(r) for r in rows])
+    return offs, np.concatenate(rows).astype(np.uint32)
+
+
+def threshold_bench(sh, steps=3):
+    """Thresholded mode (pairs with Jaccard >= t, exact counts) on sparse
+    skewed corpora: prefix-filtered candidates against the exact thresholded
+    engine, the gate's choice, and the pairs each reports (identical sets)."""
+    out = []
+    ctx = sh.Context(0)
+    for n, u, s in ((100_000, 2_000_000, 0.9), (400_000, 4_000_000, 0.6)):
+        offs, pos = skewed_sparse_rows(n, u, s, 7)
+        with ctx.from_lists(offs, pos, u, bitmap_policy=4 | 8 | 16) as c:
+            for t in (0.1, 0.5):
+                r = {"rows": n, "universe": u, "popularity_exponent": s, "t": t}
+                for mode in ("exact", "prefix", "gated"):
+                    ts, res = [], None
+                    for k in range(1 + steps):
+                        t0 = time.perf_counter()
+                        res = c.allpairs_threshold_prefix(t, mode=mode)
+                        if k:
+                            ts.append(time.perf_counter() - t0)
+                    st, hits, info = res
+                    r[mode] = {"ms": 1000 * float(np.median(ts)), "pairs_reported": len(hits),
+                               "candidates": info["candidates"],
+                               "unique_candidates": info["unique_candidates"],
+                               "est_ratio": info["est_ratio"],
+                               "used_prefix": bool(info["used_prefix"])}
+                    if mode == "exact":
+                        want = hits
+                    else:
+                        r[mode]["same_pairs_as_exact"] = bool(np.array_equal(hits, want))
+                r["prefix_over_exact"] = r["exact"]["ms"] / r["prefix"]["ms"]
+                r["gated_over_exact"] = r["exact"]["ms"] / r["gated"]["ms"]
+                progress(f"threshold n={n} t={t}: exact {r['exact']['ms']:.1f} ms, prefix "
+                         f"{r['prefix']['ms']:.1f} ms, gate -> "
+                         f"{'prefix' if r['gated']['used_prefix'] else 'exact'}")
+                out.append(r)
+    ctx.close()
+    return out
+
+
 def build_timing(sh, ctx, offs, pos, u):
     """Representation building (runs, EWAH + markers, complements, meta,
     bitmaps of dense rows) three ways: from lists already in HBM on the device,
@@ -746,6 +806,10 @@ def main():
         out = dense_bench(sh, args)
     if args.config == "dense2" and not args.no_extra:
         out["other_configs"] = other_configs(sh, args)
+        try:
+            out["other_configs"]["threshold_sparse"] = threshold_bench(sh)
+        except Exception as e:   # the exact-count lines above stand on their own
+            out["other_configs"]["threshold_sparse"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 

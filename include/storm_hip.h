@@ -346,6 +346,40 @@ int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c,
 int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                               uint64_t* row_sum);
 
+/* Thresholded mode by prefix-filtered candidate generation
+ * (bench/bench_threshold.cpp:210-290; Bayardo, Ma & Srikant, WWW 2007): the
+ * same contract and the same reported pairs as storm_hip_allpairs_threshold
+ * (exact counts of every pair with |A n B| > 0 and Jaccard >= t, read back
+ * with storm_hip_result_copy_pairs), but the pairs examined are only those
+ * sharing an element of their prefixes (p(X) = |X| - ceil(t|X|) + 1 elements,
+ * rarest document frequency first), generated from a postings index of the
+ * prefixes on the device and verified exactly. mode:
+ *   STORM_HIP_PREFIX_GATED   the reference's gate (bench_threshold.cpp:170-200):
+ *                            prefix filtering when its estimated work (2 sum p
+ *                            + sum_e L_e(L_e-1)/2 x mean |X|) is below 0.04 of
+ *                            the exact scan's (pairs x mean |X|) -- the cut
+ *                            measured for the device's two paths (the
+ *                            reference's CPU cut is 5) -- else, or when the
+ *                            candidates outgrow one sort (2^31), the exact
+ *                            thresholded engine
+ *   STORM_HIP_PREFIX_ALWAYS  prefix filtering (the rows must hold lists)
+ *   STORM_HIP_PREFIX_EXACT   the exact thresholded engine
+ * exact_opt: options of the exact engine when it runs (NULL: analytic, 256-row
+ * tiles). info (optional) reports the gate's estimate and the candidate counts;
+ * under prefix filtering stats->sum is the sum of the reported pairs' counts. */
+enum { STORM_HIP_PREFIX_GATED = 0, STORM_HIP_PREFIX_ALWAYS = 1, STORM_HIP_PREFIX_EXACT = 2 };
+typedef struct storm_hip_prefix_info {
+    double   est_ratio;          /* estimated prefix work / exact-scan work      */
+    uint64_t prefix_elements;    /* sum over rows of p(X)                        */
+    uint64_t candidates;         /* (row, earlier row) emissions, duplicates in  */
+    uint64_t unique_candidates;  /* distinct pairs verified                      */
+    uint32_t used_prefix;        /* 1: prefix filtering ran; 0: the exact engine */
+} storm_hip_prefix_info;
+int storm_hip_allpairs_threshold_prefix(storm_hip_ctx* ctx, storm_hip_corpus* c, double t,
+                                        uint32_t mode, const storm_hip_options* exact_opt,
+                                        storm_hip_stats* stats, uint64_t* n_pairs,
+                                        storm_hip_prefix_info* info);
+
 /* ---- streamed tile rows and cross blocks (general corpora) ---------------
  *
  * storm::allpairs_tiles (kernels/storm_allpairs.cpp:677-732) visits the upper

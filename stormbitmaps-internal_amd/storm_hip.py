@@ -60,6 +60,12 @@ class Stats(C.Structure):
         return d
 
 
+class PrefixInfo(C.Structure):
+    _fields_ = [("est_ratio", C.c_double), ("prefix_elements", C.c_uint64),
+                ("candidates", C.c_uint64), ("unique_candidates", C.c_uint64),
+                ("used_prefix", C.c_uint32)]
+
+
 class Arena(C.Structure):
     P64 = C.POINTER(C.c_uint64)
     P32 = C.POINTER(C.c_uint32)
@@ -191,6 +197,9 @@ def lib():
         "storm_hip_allpairs_cross": (i32, [vp, vp, C.POINTER(Options), u32, vp,
                                            C.POINTER(Stats)]),
         "storm_hip_corpus_order": (i32, [vp, vp, vp]),
+        "storm_hip_allpairs_threshold_prefix": (i32, [vp, vp, C.c_double, u32,
+                                                      C.POINTER(Options), C.POINTER(Stats),
+                                                      C.POINTER(u64), C.POINTER(PrefixInfo)]),
     }
     for name, (res, args) in sig.items():
         if not hasattr(L, name):
@@ -568,6 +577,25 @@ class Corpus:
         check(lib().storm_hip_result_copy_pairs(self.ctx.h, self.h, 0, n.value, _ptr(out)),
               "copy_pairs")
         return st.as_dict(), out
+
+    def allpairs_threshold_prefix(self, t, mode="gated", policy="analytic", tile=256):
+        """Thresholded mode by prefix-filtered candidates
+        (storm_hip_allpairs_threshold_prefix): (stats, pairs, info) with pairs
+        as allpairs_threshold's and info the gate's estimate and candidate
+        counts; mode "gated", "prefix" or "exact"."""
+        o = Options()
+        o.policy, o.out_mode, o.bb_kernel = POLICY[policy], OUT["sum"], BB["auto"]
+        o.tile = tile
+        st = Stats()
+        info = PrefixInfo()
+        n = C.c_uint64(0)
+        check(lib().storm_hip_allpairs_threshold_prefix(
+            self.ctx.h, self.h, t, {"gated": 0, "prefix": 1, "exact": 2}[mode], C.byref(o),
+            C.byref(st), C.byref(n), C.byref(info)), "allpairs_threshold_prefix")
+        out = np.zeros((n.value, 3), dtype=np.uint32)
+        check(lib().storm_hip_result_copy_pairs(self.ctx.h, self.h, 0, n.value, _ptr(out)),
+              "copy_pairs")
+        return st.as_dict(), out, {k: getattr(info, k) for k, _ in info._fields_}
 
     def allpairs_band(self, band, policy="analytic", order="density", bb="auto", tile=64,
                       fixed_cell=0, use_graph=False):

@@ -115,3 +115,108 @@ def test_rejects_bad_calls(ctx, sh):
     with ctx.gen_bernoulli(300, 5_000, 0.05, 7) as d:
         with pytest.raises(sh.StormHipError):
             d.allpairs_threshold(0.1)
+
+
+@pytest.mark.parametrize("t", [0.01, 0.2, 0.75, 1.0])
+def test_prefix_filter_matches_exact(ctx, cases, t):
+    """Prefix-filtered candidates (every mode: forced, gated, exact) report
+    exactly the oracle's pairs with their counts."""
+    for name, offs, pos, u in cases:
+        want = expected(offs, pos, t)
+        print("case prefix", name, "t", t, "hits", len(want), flush=True)
+        with ctx.from_lists(offs, pos, u, bitmap_policy=0 | BUILD_ALL) as c:
+            for mode in ("prefix", "gated", "exact"):
+                st, got, info = c.allpairs_threshold_prefix(t, mode=mode)
+                assert np.array_equal(got, want), (name, t, mode, len(got), len(want))
+                assert info["used_prefix"] == (mode == "prefix" or
+                                               (mode == "gated" and info["est_ratio"] < 0.04))
+                if info["used_prefix"]:
+                    assert info["unique_candidates"] <= info["candidates"]
+                    assert st["sum"] == (int(want[:, 2].sum(dtype=np.uint64)) if len(want) else 0)
+
+
+def prefix_estimate(rows, t):
+    """The gate's estimate restated on the CPU: document frequencies, each
+    row's prefix of p = |X| - ceil(t|X|) + 1 elements in (floor(log2 df), id)
+    order, posting lengths L_e, then (2 sum p + sum_e L_e(L_e-1)/2 mean|X|) /
+    (pairs mean|X|) (bench/bench_threshold.cpp:175-196)."""
+    allp = np.concatenate(rows).astype(np.int64)
+    df = np.bincount(allp)
+    b = np.floor(np.log2(np.maximum(df, 1))).astype(np.int64)
+    L = np.zeros(len(df), np.int64)
+    sump = 0
+    for r in rows:
+        r = r.astype(np.int64)
+        ci = len(r)
+        p = min(ci, max(0, ci - int(np.ceil(t * ci)) + 1))
+        pre = r[np.argsort(b[r] * (1 << 32) + r, kind="stable")[:p]]
+        np.add.at(L, pre, 1)
+        sump += p
+    n = len(rows)
+    mean_s = len(allp) / n
+    return (2 * sump + (L * (L - 1) / 2).sum() * mean_s) / (n * (n - 1) / 2 * mean_s)
+
+
+def test_prefix_filter_sparse_skewed_corpus(ctx):
+    """A sparse corpus with skewed element popularity (the regime the prefix
+    filter is for): 6,000 rows of 5-40 positions drawn with popularity ~ 1 /
+    rank^0.6 over 1,000,000 positions, plus near-duplicate row pairs so that
+    high thresholds report pairs. The device's gate estimate equals its CPU
+    restatement (which pins the prefixes and postings), the gate picks prefix
+    filtering, far fewer candidates than pairs are verified, and the reported
+    pairs equal the oracle's at every threshold."""
+    rng = np.random.default_rng(31)
+    u = 1_000_000
+    w = 1.0 / np.arange(1, u + 1) ** 0.6
+    w /= w.sum()
+    rows = []
+    for r in range(6000):
+        if r % 10 == 9:
+            base = rows[r - 1]
+            extra = rng.choice(u, size=2, p=w)
+            rows.append(np.unique(np.concatenate([base[: max(1, len(base) - 2)], extra])).astype(np.uint32))
+        else:
+            rows.append(np.unique(rng.choice(u, size=int(rng.integers(5, 41)), p=w)).astype(np.uint32))
+    offs, pos = corpora.csr(rows)
+    with ctx.from_lists(offs, pos, u, bitmap_policy=0 | BUILD_ALL) as c:
+        for t in (0.1, 0.3, 0.6):
+            want = expected(offs, pos, t)
+            print("case sparse skewed t", t, "hits", len(want), flush=True)
+            st, got, info = c.allpairs_threshold_prefix(t, mode="gated")
+            assert abs(info["est_ratio"] - prefix_estimate(rows, t)) <= 1e-9 * info["est_ratio"], info
+            assert info["used_prefix"] == 1, info
+            assert info["unique_candidates"] < 6000 * 5999 // 2 // 100, info
+            assert np.array_equal(got, want), (t, len(got), len(want))
+
+
+def _skewed_rows(n, u, s, seed):
+    rng = np.random.default_rng(seed)
+    w = 1.0 / np.arange(1, u + 1) ** s
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    lens = rng.integers(5, 41, size=n)
+    draws = np.searchsorted(cdf, rng.random(int(lens.sum())))
+    ids = rng.permutation(u)
+    offs = np.zeros(n + 1, np.uint64)
+    rows, k = [], 0
+    for L in lens:
+        rows.append(np.unique(ids[draws[k:k + L]]).astype(np.uint32))
+        k += L
+    offs[1:] = np.cumsum([len(r) for r in rows])
+    return offs, np.concatenate(rows).astype(np.uint32)
+
+
+def test_prefix_filter_many_candidates(ctx):
+    """100,000 sparse rows whose popular elements make about 4.4e8 candidate
+    emissions (more candidates than one launch has work-items when each takes
+    a wavefront): forced prefix filtering reports exactly the pairs and
+    counts of the exact thresholded engine, and the gate keeps the exact
+    engine here (its estimate is above the cut)."""
+    offs, pos = _skewed_rows(100_000, 2_000_000, 0.9, 7)
+    with ctx.from_lists(offs, pos, 2_000_000, bitmap_policy=0 | BUILD_ALL) as c:
+        _, want, _ = c.allpairs_threshold_prefix(0.1, mode="exact")
+        st, got, info = c.allpairs_threshold_prefix(0.1, mode="prefix")
+        assert info["candidates"] > (1 << 32) // 64, info
+        assert np.array_equal(got, want), (len(got), len(want))
+        _, _, ginfo = c.allpairs_threshold_prefix(0.1, mode="gated")
+        assert ginfo["used_prefix"] == 0 and ginfo["est_ratio"] >= 0.04, ginfo
