@@ -476,7 +476,7 @@ int storm_hip_result_copy_pairs(storm_hip_ctx* ctx, const storm_hip_corpus* c,
 /* ---- launch shapes and the autotuner -------------------------------------
  * No reference counterpart (the reference's kernels have fixed shapes).
  * The matrix-core B x B kernel's launch shape: packed words per LDS stage
- * (4 or 8), tile order (0 = row-major upper triangle, g > 0 = bands of g tile
+ * (4, 8 or 16; 16 with 8 waves only, 144 KB of LDS), tile order (0 = row-major upper triangle, g > 0 = bands of g tile
  * rows walked column by column, each XCD taking a contiguous range) and the
  * K-split target (blocks per CU when a corpus has fewer 256 x 256 tiles than
  * that) and the waves per workgroup (8: two per SIMD, 128 x 64 pairs each;

@@ -13,7 +13,7 @@
 // Structure: a workgroup of 8 waves owns a 256 x 256 block of pairs; wave w
 // owns rows 128*(w/4) .. +128 and columns 64*(w%4) .. +64 of it (4 x 2 MFMA
 // blocks of 32 x 32, 128 accumulator registers, two waves per SIMD). Per stage
-// of SW words (SW = 4 or 8), every thread stages one row's packed bits in LDS
+// of SW words (SW = 4, 8 or 16), every thread stages one row's packed bits in LDS
 // (double buffered, one barrier per stage) exactly as loaded, and every wave
 // expands the packed bits of its own fragments into 0/1 elements in registers
 // right before its MFMAs. The next stage's global loads are issued before the
@@ -139,12 +139,13 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
 
     // The next stage's words, in named registers (an array here would be
     // placed in memory by the compiler).
-    u32x4 g0, g1, g2, g3, h0, h1, h2, h3;
+    u32x4 g0, g1, g2, g3, g4, g5, g6, g7, h0, h1, h2, h3;
     auto gload = [&](uint32_t s) {
         global_u32x4* p = src + (size_t)L * s;
         g0 = p[0];
         g1 = p[1];
-        if constexpr (L == 4) { g2 = p[2]; g3 = p[3]; }
+        if constexpr (L >= 4) { g2 = p[2]; g3 = p[3]; }
+        if constexpr (L == 8) { g4 = p[4]; g5 = p[5]; g6 = p[6]; g7 = p[7]; }
         if constexpr (THREADS < 512) {
             global_u32x4* p2 = src2 + (size_t)L * s;
             h0 = p2[0];
@@ -156,7 +157,8 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
         u32x4* d = reinterpret_cast<u32x4*>(&sm.st[buf][t][0]);
         d[0] = g0;
         d[1] = g1;
-        if constexpr (L == 4) { d[2] = g2; d[3] = g3; }
+        if constexpr (L >= 4) { d[2] = g2; d[3] = g3; }
+        if constexpr (L == 8) { d[4] = g4; d[5] = g5; d[6] = g6; d[7] = g7; }
         if constexpr (THREADS < 512) {
             u32x4* d2 = reinterpret_cast<u32x4*>(&sm.st[buf][t + THREADS][0]);
             d2[0] = h0;
@@ -173,61 +175,73 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
     for (uint32_t s = 0; s < n_stages; ++s) {
         const int buf = s & 1;
         // This wave's packed fragments of the stage: 4 A rows, NB B rows, each
-        // SW chunks of this lane group's 32 bits.
-        uint4 pa[4][R], pb[NB][R];
+        // SW chunks of this lane group's 32 bits. Stages of 16 words read them
+        // one group of four chunks at a time (JIT), right before their MFMAs,
+        // so the fragments of a whole stage never hold registers together.
+        constexpr bool kJit = SW > 8;
+        constexpr int RU = kJit ? 1 : R;
+        uint4 pa[4][RU], pb[NB][RU];
+        auto read_frags = [&](int r0) {
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+            for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                pa[m][r] = *reinterpret_cast<const uint4*>(
-                    &sm.st[buf][wr * 128 + m * 32 + lr][lg * SW + 4 * r]);
+                for (int r = 0; r < RU; ++r)
+                    pa[m][r] = *reinterpret_cast<const uint4*>(
+                        &sm.st[buf][wr * 128 + m * 32 + lr][lg * SW + 4 * (r0 + r)]);
 #pragma unroll
-        for (int q = 0; q < NB; ++q)
+            for (int q = 0; q < NB; ++q)
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                pb[q][r] = *reinterpret_cast<const uint4*>(
-                    &sm.st[buf][kTile + wc * 32 * NB + q * 32 + lr][lg * SW + 4 * r]);
+                for (int r = 0; r < RU; ++r)
+                    pb[q][r] = *reinterpret_cast<const uint4*>(
+                        &sm.st[buf][kTile + wc * 32 * NB + q * 32 + lr][lg * SW + 4 * (r0 + r)]);
+        };
+        if constexpr (!kJit) read_frags(0);
         // Stage the next words while this stage computes.
         if (s + 1 < n_stages) {
             stage_store(buf ^ 1);
             if (s + 2 < n_stages) gload(s + 2);
         }
 #pragma unroll
-        for (int c = 0; c < SW; ++c) {
-            uint32_t wa[4], wb[NB];
+        for (int r = 0; r < R; ++r) {
+            if constexpr (kJit) read_frags(r);
+            const int rr = kJit ? 0 : r;   // fragment index of chunk group r
 #pragma unroll
-            for (int m = 0; m < 4; ++m) wa[m] = comp(pa[m][c >> 2], c & 3);
+            for (int d = 0; d < 4; ++d) {
+                if constexpr (FP4) {
+                    // Step c = 4r + d: bit slice d of the four chunks 4r..4r+3.
+                    const int sc = kUnitScale + 1 - (d < 2 ? d : 2);
+                    i32x8 fb[NB];
 #pragma unroll
-            for (int q = 0; q < NB; ++q) wb[q] = comp(pb[q][c >> 2], c & 3);
-            if constexpr (FP4) {
-                // Step c = 4r + d: bit slice d of the four chunks 4r..4r+3.
-                const int r = c >> 2, d = c & 3;
-                const int sc = kUnitScale + 1 - (d < 2 ? d : 2);
-                i32x8 fb[NB];
-#pragma unroll
-                for (int q = 0; q < NB; ++q) fb[q] = slice_fp4(pb[q][r], d);
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const i32x8 fa = slice_fp4(pa[m][r], d);
-#pragma unroll
-                    for (int q = 0; q < NB; ++q)
-                        acc(m, q) = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                            fa, fb[q], acc(m, q), 4, 4, 0, sc, 0, sc);
-                }
-            } else {
-                // 32 bits per lane group per chunk = two K=32 steps (h = 0, 1
-                // select which nibble half of each byte position the lane takes).
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    i8x16 fb[NB];
-#pragma unroll
-                    for (int q = 0; q < NB; ++q) fb[q] = expand_i8(wb[q], h);
+                    for (int q = 0; q < NB; ++q) fb[q] = slice_fp4(pb[q][rr], d);
 #pragma unroll
                     for (int m = 0; m < 4; ++m) {
-                        const i8x16 fa = expand_i8(wa[m], h);
+                        const i32x8 fa = slice_fp4(pa[m][rr], d);
 #pragma unroll
                         for (int q = 0; q < NB; ++q)
-                            acc(m, q) = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb[q], acc(m, q), 0, 0, 0);
+                            acc(m, q) = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                                fa, fb[q], acc(m, q), 4, 4, 0, sc, 0, sc);
+                    }
+                } else {
+                    // Chunk c = 4r + d: 32 bits per lane group = two K=32 steps
+                    // (h = 0, 1 select which nibble half of each byte position
+                    // the lane takes).
+                    uint32_t wa[4], wb[NB];
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) wa[m] = comp(pa[m][rr], d);
+#pragma unroll
+                    for (int q = 0; q < NB; ++q) wb[q] = comp(pb[q][rr], d);
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        i8x16 fb[NB];
+#pragma unroll
+                        for (int q = 0; q < NB; ++q) fb[q] = expand_i8(wb[q], h);
+#pragma unroll
+                        for (int m = 0; m < 4; ++m) {
+                            const i8x16 fa = expand_i8(wa[m], h);
+#pragma unroll
+                            for (int q = 0; q < NB; ++q)
+                                acc(m, q) = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb[q], acc(m, q), 0, 0, 0);
+                        }
                     }
                 }
             }
@@ -381,8 +395,8 @@ __global__ __launch_bounds__(512, 1) void bb_mfma_rect_kernel(
 // Exactness and layout checks of one bitmap matrix (n_words per row, rows
 // ld words apart) for the matrix-core kernel with this stage width.
 int bb_mfma_check(bool fp4, uint32_t sw, uint32_t n_words, uint32_t ld) {
-    if (sw != 4 && sw != 8) {
-        set_error("bb_mfma: stage width %u words (4 or 8)", sw);
+    if (sw != 4 && sw != 8 && sw != 16) {
+        set_error("bb_mfma: stage width %u words (4, 8 or 16)", sw);
         return STORM_HIP_ERR_INVALID;
     }
     const uint32_t n_stages = (n_words + sw - 1) / sw;
@@ -448,6 +462,8 @@ struct Occupancy {
 template <class F>
 int with_shape(bool fp4, const MfmaConfig& cfg, F&& f) {
     const uint32_t sw = cfg.stage_words, nw = cfg.waves;
+    if (fp4 && sw == 16 && nw == 8) { f.template operator()<true, 16, 8>(); return STORM_HIP_OK; }
+    if (!fp4 && sw == 16 && nw == 8) { f.template operator()<false, 16, 8>(); return STORM_HIP_OK; }
     if (fp4 && sw == 8 && nw == 8)  { f.template operator()<true, 8, 8>(); return STORM_HIP_OK; }
     if (fp4 && sw == 4 && nw == 8)  { f.template operator()<true, 4, 8>(); return STORM_HIP_OK; }
     if (fp4 && sw == 8 && nw == 4)  { f.template operator()<true, 8, 4>(); return STORM_HIP_OK; }

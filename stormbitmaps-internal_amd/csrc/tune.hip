@@ -10,8 +10,9 @@ using namespace stormhip;
 namespace {
 
 bool valid(const storm_hip_bb_config& c) {
-    return (c.stage_words == 4 || c.stage_words == 8) && c.band <= 64 && c.blocks_per_cu >= 1 &&
-           c.blocks_per_cu <= 16 && (c.waves == 4 || c.waves == 8);
+    return (c.stage_words == 4 || c.stage_words == 8 || (c.stage_words == 16 && c.waves == 8)) &&
+           c.band <= 64 && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 16 &&
+           (c.waves == 4 || c.waves == 8);
 }
 
 MfmaConfig to_internal(const storm_hip_bb_config& c) {
@@ -51,7 +52,8 @@ extern "C" {
 
 int storm_hip_set_bb_config(storm_hip_ctx* ctx, const storm_hip_bb_config* cfg) {
     if (!ctx || !cfg || !valid(*cfg)) {
-        set_error("bb config: stage_words 4 or 8, band <= 64, blocks_per_cu in [1, 16], waves 4 or 8");
+        set_error("bb config: stage_words 4, 8 or 16 (16: 8 waves), band <= 64, "
+                  "blocks_per_cu in [1, 16], waves 4 or 8");
         return STORM_HIP_ERR_INVALID;
     }
     ctx->mfma = to_internal(*cfg);
@@ -117,9 +119,10 @@ int storm_hip_autotune(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
     bool have = false;
     std::vector<MfmaConfig> cands;
     for (uint32_t waves : {8u, 4u})
-        for (uint32_t sw : {8u, 4u})
+        for (uint32_t sw : {16u, 8u, 4u})
             for (uint32_t band : {0u, 2u, 4u, 8u})
                 for (uint32_t bpc : bpcs) {
+                    if (sw == 16 && waves != 8) continue;   // 16-word stages: 8 waves only
                     MfmaConfig m;
                     m.stage_words = sw;
                     m.band = band;

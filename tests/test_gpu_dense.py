@@ -77,7 +77,8 @@ def test_config1_known_sum(ctx, bb):
 
 
 SHAPE_SET = [(8, 0, 2, 8), (4, 0, 1, 8), (8, 4, 2, 8), (4, 2, 4, 8), (8, 3, 1, 8), (8, 8, 16, 8),
-             (4, 64, 2, 8), (8, 4, 2, 4), (4, 0, 1, 4), (8, 2, 4, 4)]
+             (4, 64, 2, 8), (8, 4, 2, 4), (4, 0, 1, 4), (8, 2, 4, 4), (16, 4, 2, 8), (16, 0, 1, 8),
+             (16, 8, 4, 8)]
 DEFAULT_SHAPE = {"stage_words": 8, "band": 4, "blocks_per_cu": 2, "waves": 8}
 
 
@@ -105,7 +106,8 @@ def test_every_launch_shape_exact(sh, ctx, cfg):
 
 
 def test_bb_config_rejects_bad_shapes(sh, ctx):
-    for bad in [(6, 0, 2, 8), (8, 65, 2, 8), (8, 4, 0, 8), (8, 4, 17, 8), (8, 4, 2, 6)]:
+    for bad in [(6, 0, 2, 8), (8, 65, 2, 8), (8, 4, 0, 8), (8, 4, 17, 8), (8, 4, 2, 6),
+                (16, 4, 2, 4), (32, 4, 2, 8)]:
         with pytest.raises(sh.StormHipError):
             ctx.set_bb_config(*bad)
     assert ctx.get_bb_config() == DEFAULT_SHAPE
@@ -118,9 +120,9 @@ def test_autotune_dense_keeps_counts(ctx):
     want_sum, want = O.allpairs_words(words)
     with ctx.upload_contig(words) as c:
         r = ctx.autotune(c, bb="auto")
-        # 2 wave counts x 2 stage widths x 4 tile orders x 3 K-split targets
-        assert r["n_tried"] == 48 and r["tile"] == 0
-        assert r["bb"]["stage_words"] in (4, 8) and 0 < r["ms_best"] <= r["ms_default"] * 1.0001
+        # (8 waves x 3 stage widths + 4 waves x 2) x 4 tile orders x 3 K-split targets
+        assert r["n_tried"] == 60 and r["tile"] == 0
+        assert r["bb"]["stage_words"] in (4, 8, 16) and 0 < r["ms_best"] <= r["ms_default"] * 1.0001
         assert ctx.get_bb_config() == r["bb"]
         st = c.allpairs(out="tri", bb="auto")
         assert st["sum"] == want_sum
