@@ -196,14 +196,23 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
                         &sm.st[buf][kTile + wc * 32 * NB + q * 32 + lr][lg * SW + 4 * (r0 + r)]);
         };
         if constexpr (!kJit) read_frags(0);
-        // Stage the next words while this stage computes.
-        if (s + 1 < n_stages) {
+        // Stage the next words while this stage computes: before the MFMAs
+        // (4- and 8-word stages) or, with 16-word stages, right after the
+        // first chunk group's MFMAs are issued, so the stage opens on matrix
+        // work and the LDS stores overlap it (config 2: 150.9 -> 146 ms; after
+        // the second group no gain, after the last the reload comes too late
+        // and the kernel halves its speed).
+        if (!kJit && s + 1 < n_stages) {
             stage_store(buf ^ 1);
             if (s + 2 < n_stages) gload(s + 2);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if constexpr (kJit) read_frags(r);
+            if (kJit && r == 1 && s + 1 < n_stages) {
+                stage_store(buf ^ 1);
+                if (s + 2 < n_stages) gload(s + 2);
+            }
             const int rr = kJit ? 0 : r;   // fragment index of chunk group r
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
