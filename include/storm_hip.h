@@ -480,7 +480,7 @@ int storm_hip_result_copy_pairs(storm_hip_ctx* ctx, const storm_hip_corpus* c,
  * rows walked column by column, each XCD taking a contiguous range) and the
  * K-split target (blocks per CU when a corpus has fewer 256 x 256 tiles than
  * that) and the waves per workgroup (8: two per SIMD, 128 x 64 pairs each;
- * 4: one per SIMD, 128 x 128 pairs each). A context starts from {8, 4, 2, 8}. */
+ * 4: one per SIMD, 128 x 128 pairs each). A context starts from {16, 4, 2, 8}. */
 typedef struct {
     uint32_t stage_words;
     uint32_t band;

@@ -598,12 +598,19 @@ int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, co
         return launch_bb_valu(s, 128, X, ld, n_words, rowmap, nullptr, n, tri, sum);
     const bool fp4 = bb_kernel == STORM_HIP_BB_MFMA_FP4 ||
                      (bb_kernel == STORM_HIP_BB_AUTO && n_words < (1u << 18));
+    // A caller's row stride shorter than two stages of the configured width
+    // (rows of a few words, laid out tightly) takes the widest stage it holds.
+    MfmaConfig shape = cfg;
+    auto fits = [&](uint32_t sw) {
+        return (uint64_t)std::max((n_words + sw - 1) / sw, 2u) * sw <= ld;
+    };
+    while (shape.stage_words > 4 && !fits(shape.stage_words)) shape.stage_words /= 2;
     // Too few 256 x 256 blocks to fill the chip: split K so that about
     // blocks_per_cu blocks per CU run, each stage range at least 4 stages long.
     const uint32_t T = (n + kTile - 1) / kTile;
     const uint64_t tiles = (uint64_t)T * (T + 1) / 2;
-    const uint32_t n_stages = (n_words + cfg.stage_words - 1) / cfg.stage_words;
-    const uint64_t want = (uint64_t)std::max(1u, cfg.blocks_per_cu) * cu_count;
+    const uint32_t n_stages = (n_words + shape.stage_words - 1) / shape.stage_words;
+    const uint64_t want = (uint64_t)std::max(1u, shape.blocks_per_cu) * cu_count;
     uint32_t k_split = 1;
     if (tiles && n_stages && tiles < want) {
         k_split = (uint32_t)std::min<uint64_t>((want + tiles - 1) / tiles,
@@ -619,7 +626,7 @@ int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, co
                            dim3(256), 0, s, tri, count);
         SH_HIP(hipGetLastError());
     }
-    return launch_bb_mfma_fmt(fp4, cfg, s, X, ld, n_words, rowmap, nullptr, n, tri, sum, k_split);
+    return launch_bb_mfma_fmt(fp4, shape, s, X, ld, n_words, rowmap, nullptr, n, tri, sum, k_split);
 }
 
 }  // namespace stormhip

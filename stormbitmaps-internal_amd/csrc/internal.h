@@ -46,7 +46,7 @@ struct DeviceCostModel {
 
 // Launch shape of the matrix-core B x B kernel (chosen by the autotuner).
 struct MfmaConfig {
-    uint32_t stage_words = 8;     // packed words per LDS stage: 4 or 8
+    uint32_t stage_words = 16;    // packed words per LDS stage: 4, 8 or 16
     uint32_t band = 4;            // tile order: 0 row-major, else bands of this many tile rows
     uint32_t blocks_per_cu = 2;   // K-split target when there are fewer tiles than this per CU
     uint32_t waves = 8;           // 8 (two per SIMD, 128 x 64 each) or 4 (one per SIMD, 128 x 128)

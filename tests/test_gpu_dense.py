@@ -79,7 +79,7 @@ def test_config1_known_sum(ctx, bb):
 SHAPE_SET = [(8, 0, 2, 8), (4, 0, 1, 8), (8, 4, 2, 8), (4, 2, 4, 8), (8, 3, 1, 8), (8, 8, 16, 8),
              (4, 64, 2, 8), (8, 4, 2, 4), (4, 0, 1, 4), (8, 2, 4, 4), (16, 4, 2, 8), (16, 0, 1, 8),
              (16, 8, 4, 8)]
-DEFAULT_SHAPE = {"stage_words": 8, "band": 4, "blocks_per_cu": 2, "waves": 8}
+DEFAULT_SHAPE = {"stage_words": 16, "band": 4, "blocks_per_cu": 2, "waves": 8}
 
 
 @pytest.mark.parametrize("cfg", SHAPE_SET)

@@ -230,14 +230,14 @@ def test_autotune_general_keeps_counts(ctx, cases):
     with ctx.from_lists(offs, pos, u, bitmap_policy=1 | BUILD_ALL) as c:
         r = ctx.autotune(c, policy="analytic", tile=64)
         assert r["tile"] in (64, 128, 256)
-        # 3 tile widths, then at 256 rows the launch shapes: waves {8, 4} x
-        # stage words {8, 4} x band {0, 2, 4, 8} x K-split target {1, 2, 4}
-        # (one target when the corpus has at least 4 tiles per CU)
-        assert r["n_tried"] in (3, 3 + 16, 3 + 48)
+        # 3 tile widths, then at 256 rows the launch shapes: (8 waves x stage
+        # words {16, 8, 4} + 4 waves x {8, 4}) x band {0, 2, 4, 8} x K-split
+        # target {1, 2, 4} (one target when the corpus has at least 4 tiles per CU)
+        assert r["n_tried"] in (3, 3 + 20, 3 + 60)
         st = c.allpairs(policy="analytic", out="tri", tile=r["tile"], use_graph=True)
         assert st["sum"] == want_sum
         assert np.array_equal(c.tri(), want)
-    ctx.set_bb_config(8, 4, 2, 8)
+    ctx.set_bb_config(16, 4, 2, 8)
 
 
 def test_bs_wide_stripes(ctx):
