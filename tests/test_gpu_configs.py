@@ -161,3 +161,43 @@ def test_batched_instances_subset(sh, ctx):
             assert (lat > 0).sum() == sum(1 for x in chosen if x["n"] >= 2)
     for c in corpora:
         c.free()
+
+
+def test_batched_instances_all_512(sh, ctx):
+    """The batched workload in full: all 512 instances, generated on the device
+    in their stored representations, run in batches of about 150 GiB
+    (estimated); every instance's sum equals its column closed form (computed
+    on the device from the rows, no cell). The analytic model prices only
+    B x B, S x S and R x R here (the cells the device-calibrated model picks
+    on this workload), so the whole run takes about a minute."""
+    inst = sh.batched_instances(512, 5)
+    budget = 150 << 30
+    batches, cur, used = [], [], 0
+    for x in inst:
+        if cur and used + x["est_bytes"] > budget:
+            batches.append(cur)
+            cur, used = [], 0
+        cur.append(x)
+        used += x["est_bytes"]
+    if cur:
+        batches.append(cur)
+    saved = ctx.get_cost_model()
+    ctx.set_cost_model([0.116, 0, 0, 0, 0.22, 0, 0, 0.30, 0, 0])
+    checked = 0
+    try:
+        for bi, group in enumerate(batches):
+            print("case batched instances batch", bi, "of", len(batches), ":", len(group),
+                  "instances", flush=True)
+            corpora = [sh.gen_instance(ctx, x) for x in group]
+            wants = [int(c.closed_form(row_sums=False)[0]) for c in corpora]
+            with ctx.batch(corpora) as b:
+                st, sums, lat = b.allpairs(policy="analytic", out="sum", tile=256)
+                for x, s_, w in zip(group, sums, wants):
+                    assert int(s_) == w, (x["k"], int(s_), w)
+                    checked += 1
+            for c in corpora:
+                c.free()
+            print("  done, cells", st["cell_tiles"], flush=True)
+    finally:
+        ctx.put_cost_model(saved)
+    assert checked == 512
