@@ -520,6 +520,8 @@ def batch_bench(sh, args):
     progress(f"{len(inst)} instances in {len(batches)} batches "
              f"(~{sum(x['est_bytes'] for x in inst) / 2**30:.0f} GiB stored)")
     ctx = sh.Context(0)
+    if args.shape:
+        ctx.set_bb_config(*(int(x) for x in args.shape.split(",")))
     ref_model = ctx.get_cost_model()
     cal_model = None
     pol_names = ["analytic_reference_rates", "analytic", "learned"]
@@ -750,8 +752,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="dense configs: keep the default B x B launch shape")
     ap.add_argument("--shape", default="",
-                    help="dense configs: pin the B x B launch shape "
-                         "stage_words,band,blocks_per_cu,waves (no tuning)")
+                    help="pin the B x B launch shape stage_words,band,blocks_per_cu,waves "
+                         "(dense configs: no tuning; batch configs: every call)")
     ap.add_argument("--rows", type=int, default=0, help="sparse configs: first N rows only")
     ap.add_argument("--tile", type=int, default=256,
                     help="rows per selection tile (sparse and batch configs)")

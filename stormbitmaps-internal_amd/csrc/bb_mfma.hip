@@ -403,7 +403,8 @@ __global__ __launch_bounds__(512, 1) void bb_mfma_rect_kernel(
 
 // Exactness and layout checks of one bitmap matrix (n_words per row, rows
 // ld words apart) for the matrix-core kernel with this stage width.
-int bb_mfma_check(bool fp4, uint32_t sw, uint32_t n_words, uint32_t ld) {
+int bb_mfma_check(bool fp4, uint32_t sw, uint32_t n_words, uint32_t ld, uint32_t acc_words) {
+    if (acc_words == 0) acc_words = n_words;
     if (sw != 4 && sw != 8 && sw != 16) {
         set_error("bb_mfma: stage width %u words (4, 8 or 16)", sw);
         return STORM_HIP_ERR_INVALID;
@@ -414,9 +415,10 @@ int bb_mfma_check(bool fp4, uint32_t sw, uint32_t n_words, uint32_t ld) {
         set_error("bb_mfma: %u stages of %u words exceed row stride %u", n_stages, sw, ld);
         return STORM_HIP_ERR_INVALID;
     }
-    // Exactness bounds: i32 accumulators below 2^31 bits; f32 below 2^24.
-    if (n_words >= (fp4 ? (1u << 18) : (1u << 25))) {
-        set_error("bb_mfma: universe of %u words too large for exact %s accumulation", n_words,
+    // Exactness bounds on what one item accumulates: i32 accumulators below
+    // 2^31 bits; f32 below 2^24.
+    if (acc_words >= (fp4 ? (1u << 18) : (1u << 25))) {
+        set_error("bb_mfma: %u words per item too many for exact %s accumulation", acc_words,
                   fp4 ? "f32" : "i32");
         return STORM_HIP_ERR_RANGE;
     }

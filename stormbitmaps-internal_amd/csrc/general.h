@@ -137,6 +137,13 @@ __host__ __device__ inline uint32_t bb_slices_for(uint32_t n_words) {
     const uint32_t k = (n_words + kBBSliceWords - 1) / kBBSliceWords;
     return k < 1 ? 1 : (k > kBBMaxSlices ? kBBMaxSlices : k);
 }
+// A bound on the words one of ks K slices of a 256-row B x B tile covers
+// (whole stages of up to 16 words each): the f32 counts of an fp4 item are
+// exact while this stays below 2^18, whatever the universe, because the
+// slices add into the output as integers.
+__host__ __device__ inline uint32_t bb_item_words(uint32_t n_words, uint32_t ks) {
+    return (n_words + ks - 1) / (ks ? ks : 1) + 32;
+}
 
 // A block index of kWholeSide in a sub means "the whole side of the tile":
 // B x S items are stripes, kSplitRows rows of the probed (list) side against
@@ -164,7 +171,10 @@ int launch_bb_mfma_queue(bool fp4, const MfmaConfig& cfg, hipStream_t s, const D
                          const QItem* queue, const unsigned int* qcount, unsigned int* qhead,
                          uint32_t grid, unsigned long long* tile_ticks,
                          uint32_t* thr_buf = nullptr);
-int bb_mfma_check(bool fp4, uint32_t stage_words, uint32_t n_words, uint32_t ld);
+// acc_words: the most words one work item accumulates in its registers (a K
+// slice of a wide tile; 0: n_words) -- fp4's f32 counts are exact below 2^18.
+int bb_mfma_check(bool fp4, uint32_t stage_words, uint32_t n_words, uint32_t ld,
+                  uint32_t acc_words = 0);
 // Workgroups resident per CU: a cell kernel; the B x B kernel of a tile width
 // (256 = matrix cores with this launch shape, else VALU).
 uint32_t cell_blocks_per_cu(int cell);

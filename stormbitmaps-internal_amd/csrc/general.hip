@@ -321,7 +321,8 @@ bool thresholded_plan(const Plan* p) { return p->th.emits(); }
 // 2^24 (2^18 words or more). Thresholded calls and explicit int8 use one
 // queue for every corpus.
 bool bb_wide(const storm_hip_corpus* c, const storm_hip_options& o, uint32_t tile, bool thr) {
-    return tile == 256 && !thr && o.bb_kernel != STORM_HIP_BB_MFMA && c->n_words >= (1u << 18);
+    return tile == 256 && !thr && o.bb_kernel != STORM_HIP_BB_MFMA && c->n_words >= (1u << 18) &&
+           bb_item_words(c->n_words, bb_slices_for(c->n_words)) >= (1u << 18);
 }
 
 // B x S tile pairs may be queued as wide stripes (two bitmap tiles per
@@ -373,8 +374,12 @@ int dispatch_queues(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, con
                                   &p->ctr->qhead[STORM_HIP_CELL_BB], gbb,
                                   ticks_of(STORM_HIP_CELL_BB), thresholded);
     if (timed) (void)hipEventRecord(ctx->ev[3], sb);
-    // Wide corpora's B x B tiles (int8) on side stream 1, ahead of the joins.
-    if (rc == STORM_HIP_OK && p->tile == 256 && !thresholded)
+    // Wide corpora's B x B tiles (int8) on side stream 1, ahead of the joins,
+    // launched only when some corpus of the plan has them: an empty launch
+    // still waits for room on the CUs and would hold the joins behind it.
+    bool any_wide = false;
+    for (storm_hip_corpus* c : p->cs) any_wide = any_wide || (c->n_bitmap_rows && bb_wide(c, o, p->tile, thresholded));
+    if (rc == STORM_HIP_OK && p->tile == 256 && !thresholded && any_wide)
         rc = launch_bb_mfma_queue(false, ctx->mfma, ctx->aux[1], p->d_corpora,
                                   p->queue + (uint64_t)kQueueBBWide * p->qstride,
                                   &p->ctr->qcount[kQueueBBWide], &p->ctr->qhead[kQueueBBWide],
@@ -486,17 +491,23 @@ int enqueue_selecting(storm_hip_ctx* ctx, Plan* p, const storm_hip_options& o, b
         // Only corpora with bitmap rows can queue B x B tiles.
         if (c->n_bitmap_rows) max_words = std::max(max_words, c->n_words);
     }
-    // fp4 with f32 accumulation is exact below 2^18 words (counts < 2^24);
-    // 256-row B x B tiles run on the matrix cores, fp4 unless int8 is asked
-    // for, and the tiles of wider corpora in the int8 queue (in thresholded
-    // calls every corpus takes the one format the widest allows).
+    // fp4 with f32 accumulation is exact while one item covers fewer than
+    // 2^18 words (counts < 2^24): 256-row B x B tiles run on the matrix cores,
+    // fp4 unless int8 is asked for; wide universes split their tiles into K
+    // slices that add into the output as integers, so they stay fp4 up to
+    // kBBMaxSlices slices of 2^18 words; only tiles beyond that go to the int8
+    // queue (in thresholded calls, which do not split K, every corpus takes
+    // the one format the widest allows).
     const bool thr_call = thresholded_plan(p);
     const bool fp4 = o.bb_kernel != STORM_HIP_BB_MFMA && (!thr_call || max_words < (1u << 18));
     if (p->tile == 256)
-        for (storm_hip_corpus* c : p->cs)
+        for (size_t k = 0; k < n; ++k) {
+            storm_hip_corpus* c = p->cs[k];
             if (c->n_bitmap_rows)
                 SH_TRY(bb_mfma_check(fp4 && !bb_wide(c, o, p->tile, thr_call),
-                                     ctx->mfma.stage_words, c->n_words, c->ld));
+                                     ctx->mfma.stage_words, c->n_words, c->ld,
+                                     bb_item_words(c->n_words, p->h_corpora[k].bb_ksplit)));
+        }
     if (timed) SH_HIP(hipEventRecord(ctx->ev[5], s));
     int rc = dispatch_queues(ctx, p, o, M, fp4, timed,
                              o.policy == kPolicyExplore || (probe && p->ticks));
@@ -1050,9 +1061,13 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
     std::vector<unsigned long long> ticks(S);
     std::vector<double> best((size_t)S * 10, 0.0);
     std::vector<uint8_t> has((size_t)S * 10, 0);
-    const bool fp4 = c->n_words < (1u << 18);   // counts < 2^24: exact in f32
+    // B x B items as production queues them: whole 256-row tiles, or their K
+    // slices on wide universes (fp4 while each slice stays exact in f32).
+    const uint32_t bb_ks = tile == 256 ? std::max(1u, p->h_corpora[0].bb_ksplit) : 1u;
+    const bool fp4 = bb_item_words(c->n_words, bb_ks) < (1u << 18);
     if (tile == 256 && c->n_bitmap_rows)
-        SH_TRY(bb_mfma_check(fp4, ctx->mfma.stage_words, c->n_words, c->ld));
+        SH_TRY(bb_mfma_check(fp4, ctx->mfma.stage_words, c->n_words, c->ld,
+                             bb_item_words(c->n_words, bb_ks)));
     const uint32_t cu = (uint32_t)std::max(1, ctx->cu_count);
     // The block-join cells (S x S, R x R) and the B x S stripes are timed on
     // every block or stripe of the tile, queued as the selector queues them:
@@ -1094,11 +1109,17 @@ int storm_hip_selector_sample(storm_hip_ctx* ctx, storm_hip_corpus* c, uint32_t 
                             factor.push_back(1.0);
                         }
                 }
+            } else if (cell == STORM_HIP_CELL_BB) {
+                // Whole tiles, or all K slices of the tile (their times add).
+                for (uint32_t x = 0; x < bb_ks; ++x) {
+                    q.push_back(QItem{single.inst, bi, bj, bb_ks > 1 ? (x | (bb_ks << 16)) : 0u});
+                    slot_of.push_back(k);
+                    factor.push_back(1.0);
+                }
             } else {
                 q.push_back(single);
-                if (cell == STORM_HIP_CELL_BB) q.back().sub = 0;   // whole tiles
                 slot_of.push_back(k);
-                factor.push_back(cell == STORM_HIP_CELL_BB ? 1.0 : single_factor);
+                factor.push_back(single_factor);
             }
         };
         for (uint32_t k = 0; k < S; ++k)

@@ -385,10 +385,13 @@ def test_wide_bb_tiles_in_k_slices(sh, ctx):
                 assert np.array_equal(c2.tri(), t2)
 
 
-def test_wide_corpus_bb_int8_queue_beside_fp4(sh, ctx):
-    """A corpus of 2^18 words or more per row (counts may pass fp4's exact
-    range) runs its B x B tiles in the int8 queue, in K slices, while a
-    narrow corpus of the same batch keeps the fp4 queue: both exact."""
+def test_wide_corpus_bb_fp4_k_slices_and_int8_queue(sh, ctx):
+    """Rows of 2^18 words or more (whole-row counts may pass fp4's exact f32
+    range): their 256-row B x B tiles run as K slices, each below 2^18 words,
+    which stay exact in fp4 and add into the output as integers -- one fp4
+    queue beside a narrow corpus of the same batch. Rows past 64 slices of
+    2^18 words (here 2^30 + 12,345 bits) keep the int8 queue beside the fp4
+    one. All exact."""
     n, u, p, seed = 100, (1 << 24) + 4097, 0.04, 53   # p >= 1/32: rows held as bitmaps
     words = O.bernoulli_rows(seed, p, u, 0, n)
     want_sum, want = O.allpairs_words(words)
@@ -405,5 +408,22 @@ def test_wide_corpus_bb_int8_queue_beside_fp4(sh, ctx):
                                            use_graph=graph)
                 assert [int(x) for x in sums] == [want_sum, s2]
                 assert np.array_equal(c.tri(), want)
+                assert np.array_equal(c2.tri(), t2)
+                assert st["bb_launches"] == 1
+        # Rows of 2^30 + 12,345 bits: 64 K slices of more than 2^18 words.
+        rng = np.random.default_rng(55)
+        uw = (1 << 30) + 12_345
+        pool = np.unique(rng.integers(0, uw, size=60_000)).astype(np.uint32)
+        rows = [np.sort(rng.choice(pool, size=20_000, replace=False)).astype(np.uint32)
+                for _ in range(6)]
+        offs = np.zeros(7, np.uint64)
+        offs[1:] = np.cumsum([len(r) for r in rows])
+        pos = np.concatenate(rows)
+        ws, wt = O.allpairs_lists(offs, pos)
+        with ctx.from_lists(offs, pos, uw, bitmap_policy=1 | 4 | 8 | 16) as c3:
+            with ctx.batch([c3, c2]) as b:
+                st, sums, lat = b.allpairs(policy="fixed", fixed_cell=0, out="tri", tile=256)
+                assert [int(x) for x in sums] == [ws, s2]
+                assert np.array_equal(c3.tri(), wt)
                 assert np.array_equal(c2.tri(), t2)
                 assert st["bb_launches"] == 2
