@@ -16,7 +16,9 @@ same rows.
 
 Other workloads (--config): dense1 (N=2,048 x M=262,144, p=0.01), zipf3 and
 runs4 (mixed representations: analytic vs calibrated vs learned selection vs
-all-bitmap), batch5 (many independent corpora in one set of launches). The
+all-bitmap), batch5 (many independent corpora in one set of launches); their
+per-pair output is the upper triangle over density-order positions by default
+(--out; the learned call is also timed with the other triangle order). The
 default run also reports the thresholded mode on sparse skewed corpora
 (prefix-filtered candidates vs the exact thresholded engine).
 """
@@ -33,6 +35,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "stormbitmaps-internal_amd")
 METRIC = "exact pairs/sec on 1 MI355X, learned vs analytic selection; % of roofline"
+# Per-pair outputs of the sparse and batched workloads.
+OUTPUTS = {
+    "tri_density": "u32 upper triangle over density-order positions (the coordinates of "
+                   "storm::allpairs_tiles under PerTile; corpus.order() maps them to rows)",
+    "tri": "u32 upper triangle in original row order",
+    "sum": "u64 sum only",
+}
 
 CONFIGS = {
     # name: (n_rows, n_bits, p, seed, description)
@@ -273,10 +282,11 @@ def sparse_bench(sh, args):
     pairs = n * (n - 1) // 2
     res = {}
 
-    def timed(policy, **kw):
+    def timed(policy, out=None, **kw):
+        out = out or args.out
         progress(f"policy {policy}: warm-up")
         t1 = time.perf_counter()
-        st = corpus.allpairs(policy=policy, out="tri", tile=args.tile, **kw)
+        st = corpus.allpairs(policy=policy, out=out, tile=args.tile, **kw)
         ctx.sync()
         warm = time.perf_counter() - t1
         ts = []
@@ -284,7 +294,7 @@ def sparse_bench(sh, args):
         # alone (graph capture included), and says so.
         for k in range(args.steps if warm < args.slow_s else 0):
             t1 = time.perf_counter()
-            st = corpus.allpairs(policy=policy, out="tri", tile=args.tile, **kw)
+            st = corpus.allpairs(policy=policy, out=out, tile=args.tile, **kw)
             ts.append(time.perf_counter() - t1)
             progress(f"policy {policy}: step {k} {1000 * ts[-1]:.1f} ms")
         single = not ts
@@ -306,7 +316,7 @@ def sparse_bench(sh, args):
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
                "data": "synthetic (seeded host generator)",
                "config": {"workload": desc, "n_rows": n, "universe": u, "pairs": pairs,
-                          "tile": args.tile},
+                          "tile": args.tile, "output": OUTPUTS[args.out]},
                "policy": name, "exact": r["sum"] == want, "result": r}
         corpus.free()
         ctx.close()
@@ -355,6 +365,10 @@ def sparse_bench(sh, args):
     t_train = time.perf_counter() - t0
     res["learned"] = timed("learned", use_graph=True)
     decisions["learned"] = corpus.last_decisions(T * (T + 1) // 2)
+    # The same learned call writing the triangle in original row order
+    # (scattered 4-byte writes), for comparison.
+    other_out = "tri" if args.out != "tri" else "tri_density"
+    learned_other = timed("learned", out=other_out, use_graph=True)
     regret = selection_regret(ctx, X[n_fit:], Y[n_fit:], M[n_fit:], idx[n_fit:], decisions)
     if args.dump_samples:
         np.savez(args.dump_samples, X=X, Y=Y, M=M, idx=idx, n_fit=n_fit,
@@ -384,13 +398,17 @@ def sparse_bench(sh, args):
         "steps": args.steps, "warmup": 1, "ms_per_step": res["learned"]["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic (seeded host generator)",
-        "config": {"workload": desc, "n_rows": n, "universe": u, "pairs": pairs, "tile": args.tile},
+        "config": {"workload": desc, "n_rows": n, "universe": u, "pairs": pairs, "tile": args.tile,
+                   "output": OUTPUTS[args.out]},
         "exact_sums_agree": len(sums) == 1 and want in sums,
         "sum": sums.pop() if len(sums) == 1 else None, "column_closed_form_sum": want,
         "policies": res,
         "learned_over_analytic": res["learned"]["pairs_per_s"] / res["analytic"]["pairs_per_s"],
         "learned_over_analytic_reference_rates":
             res["learned"]["pairs_per_s"] / res["analytic_reference_rates"]["pairs_per_s"],
+        "learned_other_output": {"output": OUTPUTS[other_out], "ms": learned_other["ms"],
+                                 "pairs_per_s": learned_other["pairs_per_s"],
+                                 "sum_agrees": learned_other["sum"] == want},
         "selector": {"sampler": args.sampler, "labelled_tiles": int(len(X)),
                      "train_loss": loss, "sample_s": t_sample,
                      "train_s": t_train, "held_out_tiles": int(n_hold),
@@ -533,7 +551,7 @@ def batch_bench(sh, args):
     def timed(batch, name, wants):
         policy = "learned" if name == "learned" else "analytic"
         t1 = time.perf_counter()
-        st, sums, lat = batch.allpairs(policy=policy, out="tri", tile=args.tile, use_graph=True)
+        st, sums, lat = batch.allpairs(policy=policy, out=args.out, tile=args.tile, use_graph=True)
         warm = time.perf_counter() - t1
         progress(f"  {name}: warm-up {1000 * warm:.1f} ms, cells {st['cell_tiles']}")
         ts, lats = [], []
@@ -541,7 +559,7 @@ def batch_bench(sh, args):
         # alone (graph capture included), and says so.
         for k in range(args.steps if warm < args.slow_s else 0):
             t1 = time.perf_counter()
-            st, sums, lat = batch.allpairs(policy=policy, out="tri", tile=args.tile,
+            st, sums, lat = batch.allpairs(policy=policy, out=args.out, tile=args.tile,
                                            use_graph=True)
             ts.append(time.perf_counter() - t1)
             lats.append(lat)
@@ -627,7 +645,7 @@ def batch_bench(sh, args):
         "data": "synthetic (seeded device generators)",
         "config": {"workload": desc, "instances_run": n_done, "instances_total": count,
                    "batches": len(batches), "pairs": pairs, "tile": args.tile,
-                   "batch_budget_gib": args.budget_gib,
+                   "output": OUTPUTS[args.out], "batch_budget_gib": args.budget_gib,
                    "instances_with_bitmaps_added": sum(1 for x in inst if x["bitmaps_added"]),
                    "stored_gib_estimate": sum(x["est_bytes"] for x in inst) / 2**30},
         "exact_all": all(p["exact_all_instances"] for p in policies.values()),
@@ -755,6 +773,9 @@ def main():
                     help="pin the B x B launch shape stage_words,band,blocks_per_cu,waves "
                          "(dense configs: no tuning; batch configs: every call)")
     ap.add_argument("--rows", type=int, default=0, help="sparse configs: first N rows only")
+    ap.add_argument("--out", default="tri_density", choices=sorted(OUTPUTS),
+                    help="sparse and batch configs: per-pair output (the triangle packed over "
+                         "density-order positions, over original rows, or the sum only)")
     ap.add_argument("--tile", type=int, default=256,
                     help="rows per selection tile (sparse and batch configs)")
     ap.add_argument("--slow-s", type=float, default=30.0,

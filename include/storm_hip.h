@@ -287,8 +287,20 @@ enum {
  * model choice off the clock), one wavefront per pair after the span test. */
 enum {
     STORM_HIP_OUT_SUM       = 0,  /* u64 sum only                            */
-    STORM_HIP_OUT_UPPER_TRI = 1   /* u32 per pair, original row order,
+    STORM_HIP_OUT_UPPER_TRI = 1,  /* u32 per pair, original row order,
                                      idx(i,j) = i*N - i(i+1)/2 + (j-i-1)     */
+    STORM_HIP_OUT_UPPER_TRI_DENSITY = 2
+                                  /* u32 per pair, the same packing over the
+                                     positions a < b of the density order
+                                     (storm_hip_corpus_order: position a holds
+                                     row perm[a]) -- the coordinates in which
+                                     storm::allpairs_tiles reports counts under
+                                     PerTile (kernels/storm_allpairs.cpp:
+                                     689-714). The engine works in this order,
+                                     so every block of counts lands in whole
+                                     row segments (original order scatters
+                                     them). Dense corpora: identity order, the
+                                     same as UPPER_TRI.                      */
 };
 enum {
     STORM_HIP_BB_AUTO = 0,       /* fastest exact form for the universe       */
@@ -338,11 +350,13 @@ typedef struct {
 int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c,
                        const storm_hip_options* opt, storm_hip_stats* stats);
 /* Copy `count` packed upper-triangle counts starting at `offset` from the
- * last UPPER_TRI call on this corpus. */
+ * last UPPER_TRI or UPPER_TRI_DENSITY call on this corpus (in the order that
+ * call wrote). */
 int storm_hip_result_copy_tri(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                               uint64_t offset, uint64_t count, uint32_t* out);
-/* Row sums of the last UPPER_TRI result, computed on the device:
- * row_sum[i] = sum_{j != i} |Xi n Xj|. */
+/* Row sums of the last UPPER_TRI or UPPER_TRI_DENSITY result, computed on
+ * the device, by original row either way: row_sum[i] = sum_{j != i}
+ * |Xi n Xj|. */
 int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c,
                               uint64_t* row_sum);
 

@@ -51,6 +51,7 @@ __global__ void gen_bernoulli_kernel(uint64_t* __restrict__ X, uint32_t ld, uint
 
 // Row sums of a packed upper-triangle result: one block per row.
 __global__ void tri_row_sums_kernel(const uint32_t* __restrict__ tri, uint32_t n,
+                                    const uint32_t* __restrict__ perm,
                                     unsigned long long* __restrict__ out) {
     const uint32_t i = blockIdx.x;
     uint64_t s = 0;
@@ -64,7 +65,7 @@ __global__ void tri_row_sums_kernel(const uint32_t* __restrict__ tri, uint32_t n
     if (threadIdx.x == 0) {
         uint64_t t = 0;
         for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += part[w];
-        out[i] = t;
+        out[perm ? perm[i] : i] = t;   // a position-packed triangle: row perm[i]
     }
 }
 
@@ -259,7 +260,7 @@ int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
     if (!ctx || !c || c->ctx != ctx) return STORM_HIP_ERR_INVALID;
     storm_hip_options o{};
     if (opt) o = *opt;
-    if (o.policy > STORM_HIP_POLICY_PER_PAIR || o.out_mode > STORM_HIP_OUT_UPPER_TRI ||
+    if (o.policy > STORM_HIP_POLICY_PER_PAIR || o.out_mode > STORM_HIP_OUT_UPPER_TRI_DENSITY ||
         o.bb_kernel > STORM_HIP_BB_MFMA_FP4 || o.fixed_cell >= STORM_HIP_NCELL)
         return STORM_HIP_ERR_INVALID;
     storm_hip_stats local{};
@@ -274,7 +275,7 @@ int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
     SH_HIP(hipMallocAsync((void**)&d_sum, 8, ctx->stream));
     SH_HIP(hipMemsetAsync(d_sum, 0, 8, ctx->stream));
     uint32_t* tri = nullptr;
-    if (o.out_mode == STORM_HIP_OUT_UPPER_TRI) {
+    if (o.out_mode != STORM_HIP_OUT_SUM) {
         if (c->tri_count != s.pairs) {
             if (c->d_tri) { SH_HIP(hipFree(c->d_tri)); c->d_tri = nullptr; c->tri_count = 0; }
             if (s.pairs) SH_HIP(hipMalloc(&c->d_tri, s.pairs * 4));
@@ -312,6 +313,7 @@ int storm_hip_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_
     }
     s.sum = h_sum;
     c->tri_valid = (tri != nullptr);
+    c->tri_density = tri && !c->dense && o.out_mode == STORM_HIP_OUT_UPPER_TRI_DENSITY;
     s.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return STORM_HIP_OK;
 }
@@ -387,7 +389,7 @@ int storm_hip_result_row_sums(storm_hip_ctx* ctx, const storm_hip_corpus* c, uin
     unsigned long long* d = nullptr;
     SH_HIP(hipMalloc(&d, (size_t)c->n_rows * 8));
     hipLaunchKernelGGL(tri_row_sums_kernel, dim3(c->n_rows), dim3(256), 0, ctx->stream, c->d_tri,
-                       c->n_rows, d);
+                       c->n_rows, c->tri_density ? c->d_perm : nullptr, d);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipMemcpyAsync(row_sum, d, (size_t)c->n_rows * 8, hipMemcpyDeviceToHost, ctx->stream);

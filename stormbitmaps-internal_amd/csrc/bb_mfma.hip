@@ -357,8 +357,11 @@ __global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
                 const uint32_t per = (ns + ks - 1) / ks;
                 const uint32_t s0 = min(slice * per, ns);
                 const uint32_t cnt = min(per, ns - s0);
+                // A position-packed triangle takes the tile's positions as
+                // they are (no map to original rows).
+                const uint32_t* out_perm = D.tri_pos ? nullptr : D.perm;
                 part = cnt ? bb_mfma_tile<FP4, SW, NW>(sm, D.bitmap, D.ld, s0, cnt, D.rowmap,
-                                                       D.perm, D.n, e.bi, e.bj, D.tri, ks > 1)
+                                                       out_perm, D.n, e.bi, e.bj, D.tri, ks > 1)
                            : 0;
             }
             part = wave_sum_u64(part);

@@ -147,8 +147,10 @@ __global__ __launch_bounds__(256, 2) void bb_valu_kernel(
         const QItem e = queue[t];
         const DevCorpus D = Ds[e.inst];
         const uint64_t t0 = wall_clock64();
+        // A position-packed triangle takes the tile's positions as they are.
+        const uint32_t* out_perm = (!THR && D.tri_pos) ? nullptr : D.perm;
         uint64_t local = bb_valu_tile<BM, THR>(sm, D.bitmap, D.ld, (D.nw + 15) / 16 * 16,
-                                               D.rowmap, D.perm, D.n, e.bi, e.bj, D.tri,
+                                               D.rowmap, out_perm, D.n, e.bi, e.bj, D.tri,
                                                &Ds[e.inst]);
         local = wave_sum_u64(local);
         if ((threadIdx.x & 63) == 0 && local) atomicAdd(D.sum, (unsigned long long)local);

@@ -479,6 +479,17 @@ int upload_orders(storm_hip_corpus* c, hipStream_t s) {
     SH_TRY(upload_vec(c->d_perm, perm, s, B));
     SH_TRY(upload_vec(c->d_ident, ident, s, B));
     SH_TRY(upload_vec(c->d_inv, inv, s, B));
+    // Cardinality, first and last position of the row at each position of
+    // the density order (padding: empty), for the span tests of kernels that
+    // walk positions.
+    std::vector<uint32_t> span((size_t)npad * 4, 0);
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t* m = &c->h_meta[(size_t)c->h_perm[k] * 8];
+        span[(size_t)k * 4] = m[0];
+        span[(size_t)k * 4 + 1] = m[4];
+        span[(size_t)k * 4 + 2] = m[5];
+    }
+    SH_TRY(upload_vec(c->d_pspan, span, s, B));
     return STORM_HIP_OK;
 }
 
@@ -546,7 +557,7 @@ void general_free_arenas(storm_hip_corpus* c) {
     void* ptrs[] = {c->d_list_off, c->d_list, c->d_run_off, c->d_run_start, c->d_run_end,
                     c->d_ewah_off, c->d_ewah, c->d_mk_off, c->d_mk_word, c->d_mk_pos,
                     c->d_comp_off, c->d_comp, c->d_meta, c->d_bm_row, c->d_bitmap,
-                    c->d_perm, c->d_ident, c->d_inv, c->d_occ,
+                    c->d_perm, c->d_ident, c->d_inv, c->d_pspan, c->d_occ,
                     c->d_slice, c->d_slice_of, c->d_rank, c->d_rank_row, c->d_occ_row,
                     c->d_zone};
     for (void* p : ptrs) if (p) (void)hipFree(p);

@@ -329,6 +329,17 @@ __device__ inline bool span_disjoint(const DevCorpus& D, uint32_t i, uint32_t j)
         return true;
     return D.thr > 0.0 && (double)min(ci, cj) / (double)max(ci, cj) < D.thr;
 }
+// The same test on two rows' (cardinality, first, last) records.
+__device__ inline bool span_disjoint(const DevCorpus& D, uint4 a, uint4 b) {
+    if (a.x == 0 || b.x == 0 || a.z < b.y || b.z < a.y) return true;
+    return D.thr > 0.0 && (double)min(a.x, b.x) / (double)max(a.x, b.x) < D.thr;
+}
+// The test for the rows i = perm[pa], j = perm[pb], read by position where
+// the call's order has position records.
+__device__ inline bool span_disjoint_at(const DevCorpus& D, uint32_t pa, uint32_t pb, uint32_t i,
+                                        uint32_t j) {
+    return D.pspan ? span_disjoint(D, D.pspan[pa], D.pspan[pb]) : span_disjoint(D, i, j);
+}
 
 // Coarse zone maps (bench/bench_tile.cpp:180-190, the per-pair filter in front
 // of the cell): rows i and j share no bucket, so no position. Called by a
@@ -487,12 +498,12 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
         {
             // Counted as values, not as a choice between two counters (which
             // the compiler turns into a pointer into scratch memory).
-            const uint64_t dis = span_disjoint(D, i, j) ? 1u : 0u;
+            const uint64_t dis = span_disjoint_at(D, ia + a, ja + b, i, j) ? 1u : 0u;
             skipped += dis;
             pairs += 1u - dis;
         }
         local += v;
-        emit_pair(D, i, j, v);
+        emit_at(D, ia + a, ja + b, i, j, v);
     }
     __syncthreads();
 }
@@ -593,6 +604,9 @@ __device__ void bs_stripe(const DevCorpus& D, uint32_t* __restrict__ wave_cnt, u
                 }
         }
         wave_lds_sync();
+        // The probe row's span record (one for the wave) and each slab row's,
+        // read by position: 16 contiguous bytes per lane.
+        const uint4 sp_r = D.pspan ? D.pspan[p0 + r] : uint4{0, 0, 0, 0};
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const uint32_t sa = 64 * g + lane;
@@ -603,12 +617,14 @@ __device__ void bs_stripe(const DevCorpus& D, uint32_t* __restrict__ wave_cnt, u
             if (same_tile && pb <= pa) continue;
             const uint32_t i = D.perm[pa], j = D.perm[pb];
             {
-                const uint64_t dis = span_disjoint(D, i, j) ? 1u : 0u;
+                const bool d = D.pspan ? span_disjoint(D, sp_r, D.pspan[s0 + sa])
+                                       : span_disjoint(D, i, j);
+                const uint64_t dis = d ? 1u : 0u;
                 skipped += dis;
                 pairs += 1u - dis;
             }
             local += v;
-            emit_pair(D, i, j, v);
+            emit_at(D, pa, pb, i, j, v);
         }
         wave_lds_sync();
     }
@@ -831,12 +847,12 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
         {
             // Counted as values, not as a choice between two counters (which
             // the compiler turns into a pointer into scratch memory).
-            const uint64_t dis = span_disjoint(D, i, j) ? 1u : 0u;
+            const uint64_t dis = span_disjoint_at(D, ia + a, ja + b, i, j) ? 1u : 0u;
             skipped += dis;
             pairs += 1u - dis;
         }
         local += v;
-        emit_pair(D, i, j, v);
+        emit_at(D, ia + a, ja + b, i, j, v);
     }
     __syncthreads();
 }

@@ -34,6 +34,9 @@ struct DevCorpus {
     const uint32_t* occ_row;  const uint64_t* occ;  uint32_t occ_ld;
     const uint32_t* meta;     // 8 per original row (RowMeta + has_rank, has_occ)
     const uint32_t* perm;     // sorted position -> original row id
+    // Per position of perm's order: the row's cardinality, first and last
+    // position (null when perm is the identity order: read meta instead).
+    const uint4* pspan;
     // Bit-sliced bitmaps of 64-row groups of the density order, eight groups
     // (512 rows) per slab: with s = slice_of[g] (kNoBitmap: none), word
     // slice[((s / 8) * ld * 64 + p) * 8 + s % 8] has bit k set when sorted
@@ -47,6 +50,9 @@ struct DevCorpus {
     uint64_t* tzone;                  // per row tile: OR of its rows' zone maps
     const uint32_t* rowmap;           // sorted position -> bitmap row
     uint32_t* tri;                    // per-pair output or null
+    // 1: tri is packed over density positions (inv maps a row to its
+    // position); 0: over original rows.
+    uint32_t tri_pos;
     unsigned long long* sum;          // its sum
     unsigned long long* finish;       // wall clock of its last finished tile (max)
     uint32_t tile;                    // rows per tile
@@ -61,7 +67,8 @@ struct DevCorpus {
     unsigned long long* n_hits;
     uint64_t hit_cap;
     // Band and cross calls (emit_kind 1 / 2, tri null): counts go to emit_buf
-    // by position in the call's order (inv: original row -> position):
+    // by position in the call's order (inv: original row -> position; also
+    // the map of a position-packed triangle):
     //   1  band of one tile row:  emit_buf[(lo % tile) * n + hi]
     //   2  cross rows [0, split) x [split, n): emit_buf[lo * emit_cols + hi - split]
     uint32_t emit_kind, emit_cols, emit_split;
@@ -80,6 +87,7 @@ struct DevCorpus {
 // |A n B| in double, reported when |A n B| > 0 and |A n B| / u >= thr.
 __device__ inline void emit_pair(const DevCorpus& D, uint32_t i, uint32_t j, uint32_t v) {
     if (D.tri) {
+        if (D.tri_pos) { i = D.inv[i]; j = D.inv[j]; }
         D.tri[i < j ? tri_index(i, j, D.n) : tri_index(j, i, D.n)] = v;
         return;
     }
@@ -110,6 +118,17 @@ __device__ inline void emit_pair(const DevCorpus& D, uint32_t i, uint32_t j, uin
             }
         }
     }
+}
+
+// emit_pair for a pair whose density positions a and b are at hand (rows
+// i = perm[a], j = perm[b]): a position-packed triangle takes them as they are.
+__device__ inline void emit_at(const DevCorpus& D, uint32_t a, uint32_t b, uint32_t i, uint32_t j,
+                               uint32_t v) {
+    if (D.tri && D.tri_pos) {
+        D.tri[a < b ? tri_index(a, b, D.n) : tri_index(b, a, D.n)] = v;
+        return;
+    }
+    emit_pair(D, i, j, v);
 }
 
 // One queued tile pair: corpus index, tile row, tile column, and `sub`: 0 for

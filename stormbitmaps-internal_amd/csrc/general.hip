@@ -110,6 +110,8 @@ struct Thresh {
     uint64_t buf_bytes = 0;
     const uint32_t* inv = nullptr;
     bool identity = false;    // identity order instead of the density order
+    // Unthresholded calls: the triangle is packed over density positions.
+    bool tri_density = false;
     bool emits() const { return hits != nullptr || kind != 0; }
 };
 
@@ -179,6 +181,7 @@ DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
     D.occ_row = c->d_occ_row; D.occ = c->d_occ; D.occ_ld = c->occ_ld;
     D.meta = c->d_meta;
     D.perm = th.identity ? c->d_ident : c->d_perm;
+    D.pspan = th.identity ? nullptr : reinterpret_cast<const uint4*>(c->d_pspan);
     D.emit_kind = th.kind; D.emit_cols = th.cols; D.emit_split = th.split;
     D.emit_buf = th.buf; D.inv = th.inv;
     D.slice = c->d_slice; D.slice_of = c->d_slice_of;
@@ -187,6 +190,7 @@ DevCorpus dev_corpus(const storm_hip_corpus* c, const Work* w, uint32_t* tri,
     D.tm = w->tm; D.rowmap = w->rowmap; D.tile = w->tile; D.probe = w->probe;
     D.reps = reps_of(c);
     D.tri = tri; D.sum = sum; D.finish = finish;
+    if (tri && th.tri_density) { D.tri_pos = 1; D.inv = c->d_inv; }
     // K slices for wide 256-row B x B tiles (not in thresholded calls, whose
     // B x B tiles test each pair from the whole count).
     D.bb_ksplit = (w->tile == 256 && !th.emits()) ? bb_slices_for(c->n_words) : 1;
@@ -725,16 +729,20 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
     const uint32_t tile = o.tile ? o.tile : 64;
     SH_TRY(check_tile(tile));
     hipStream_t s = ctx->stream;
-    uint32_t* tri = (o.out_mode == STORM_HIP_OUT_UPPER_TRI) ? c->d_tri : nullptr;
+    const bool by_position = o.out_mode == STORM_HIP_OUT_UPPER_TRI_DENSITY;
+    uint32_t* tri = (o.out_mode != STORM_HIP_OUT_SUM) ? c->d_tri : nullptr;
     if (c->n_rows < 2) return STORM_HIP_OK;
 
     if (o.policy == STORM_HIP_POLICY_ALL_BITMAP) {
-        // Every pair B x B in identity order, no span test (storm_allpairs.cpp:
-        // AllBitmap), bitmaps materialised on the device where missing.
+        // Every pair B x B, no span test (storm_allpairs.cpp: AllBitmap),
+        // bitmaps materialised on the device where missing; rows in identity
+        // order, or in density order for a position-packed triangle (the
+        // kernel packs its output by the order it walks).
         SH_TRY(general_materialize(ctx, c));
         Work* w = nullptr;
         SH_TRY(ensure_bitmap_rowmap(c, &w));
-        SH_TRY(launch_compose_rowmap(s, c->d_ident, c->d_bm_row, w->npad, w->rowmap));
+        SH_TRY(launch_compose_rowmap(s, by_position ? c->d_perm : c->d_ident, c->d_bm_row,
+                                     w->npad, w->rowmap));
         SH_HIP(hipEventRecord(ctx->ev[2], s));
         SH_TRY(launch_bb_dense(o.bb_kernel, ctx->mfma, s, c->d_bitmap, c->ld, c->n_words, w->rowmap,
                                c->n_rows, tri, d_sum, (uint32_t)ctx->cu_count));
@@ -753,7 +761,9 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
         return STORM_HIP_ERR_UNSUPPORTED;
     }
     Plan* p = static_cast<Plan*>(c->plan);
-    const int rc = ensure_plan(p, {c}, {tri}, tile);
+    Thresh out;
+    out.tri_density = by_position;
+    const int rc = ensure_plan(p, {c}, {tri}, tile, out);
     c->plan = p;
     SH_TRY(rc);
     uint64_t sum = 0;
@@ -904,7 +914,7 @@ int storm_hip_batch_allpairs(storm_hip_ctx* ctx, storm_hip_batch* b, const storm
     if (!ctx || !b || b->ctx != ctx || !opt) return STORM_HIP_ERR_INVALID;
     const storm_hip_options& o = *opt;
     if (o.policy > STORM_HIP_POLICY_PER_PAIR || o.bb_kernel > STORM_HIP_BB_MFMA_FP4 ||
-        o.fixed_cell >= STORM_HIP_NCELL)
+        o.fixed_cell >= STORM_HIP_NCELL || o.out_mode > STORM_HIP_OUT_UPPER_TRI_DENSITY)
         return STORM_HIP_ERR_INVALID;
     if (o.policy == STORM_HIP_POLICY_ALL_BITMAP || o.policy == STORM_HIP_POLICY_EXHAUSTIVE) {
         set_error("batched calls take a selecting policy (not all-bitmap or exhaustive)");
@@ -922,7 +932,7 @@ int storm_hip_batch_allpairs(storm_hip_ctx* ctx, storm_hip_batch* b, const storm
         const uint64_t n = c->n_rows, pairs = n ? n * (n - 1) / 2 : 0;
         s.pairs += pairs;
         c->tri_valid = false;
-        if (o.out_mode == STORM_HIP_OUT_UPPER_TRI) {
+        if (o.out_mode != STORM_HIP_OUT_SUM) {
             if (c->tri_count != pairs) {
                 if (c->d_tri) { SH_HIP(hipFree(c->d_tri)); c->d_tri = nullptr; }
                 if (pairs) SH_HIP(hipMalloc(&c->d_tri, pairs * 4));
@@ -932,14 +942,19 @@ int storm_hip_batch_allpairs(storm_hip_ctx* ctx, storm_hip_batch* b, const storm
         }
     }
     SH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-    SH_TRY(ensure_plan(b->plan, b->cs, tris, tile));
+    Thresh out;
+    out.tri_density = o.out_mode == STORM_HIP_OUT_UPPER_TRI_DENSITY;
+    SH_TRY(ensure_plan(b->plan, b->cs, tris, tile, out));
     SH_TRY(run_plan(ctx, b->plan, o, &s, sums, latency_ms));
     SH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
     SH_HIP(hipEventSynchronize(ctx->ev[1]));
     float ms = 0;
     hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
     s.ms_device = ms;
-    for (size_t k = 0; k < b->cs.size(); ++k) b->cs[k]->tri_valid = tris[k] != nullptr;
+    for (size_t k = 0; k < b->cs.size(); ++k) {
+        b->cs[k]->tri_valid = tris[k] != nullptr;
+        b->cs[k]->tri_density = out.tri_density && tris[k] && !b->cs[k]->dense;
+    }
     return STORM_HIP_OK;
 }
 

@@ -132,16 +132,21 @@ struct storm_hip_corpus {
     // padded with the zero-row sentinel mapping; d_inv: row -> its position
     // in the density order.
     uint32_t *d_perm = nullptr, *d_ident = nullptr, *d_inv = nullptr;
+    // Per density position: the row's cardinality, first and last position
+    // (4 u32 each, padding positions empty).
+    uint32_t* d_pspan = nullptr;
     std::vector<uint32_t> h_perm;
     uint64_t device_bytes = 0;
     std::vector<uint32_t> h_meta;    // host copy of meta (8 per row)
     bool bitmaps_complete = false;
     void* work = nullptr;            // per-corpus selection state of the general path
     void* plan = nullptr;            // cached launch plan of its single-corpus calls
-    // Result of the last UPPER_TRI call.
+    // Result of the last UPPER_TRI / UPPER_TRI_DENSITY call (tri_density:
+    // packed over density positions).
     uint32_t* d_tri = nullptr;
     uint64_t tri_count = 0;
     bool tri_valid = false;
+    bool tri_density = false;
     // Decisions of the last selecting call.
     std::vector<uint8_t> last_cells;
     // Thresholded calls: device pair buffer (capacity in pairs) and counter,

@@ -15,7 +15,7 @@ GPU_LIB_PATH = os.path.join(HERE, "libstorm_gpu.so")
 
 POLICY = {"all_bitmap": 0, "analytic": 1, "learned": 2, "fixed": 3, "exhaustive": 4,
           "probe": 5, "refine": 6, "per_pair": 7}
-OUT = {"sum": 0, "tri": 1}
+OUT = {"sum": 0, "tri": 1, "tri_density": 2}
 BB = {"auto": 0, "valu": 1, "mfma": 2, "mfma_fp4": 3}
 CELLS = ["BB", "BS", "BR", "BW", "SS", "SR", "SW", "RR", "RW", "WW", "EMPTY", "CB"]
 NCELL = len(CELLS)
@@ -637,6 +637,19 @@ class Corpus:
         out = np.empty(count, dtype=np.uint32)
         check(lib().storm_hip_result_copy_tri(self.ctx.h, self.h, offset, count, _ptr(out)),
               "copy_tri")
+        return out
+
+    def tri_to_original(self, t):
+        """A density-packed triangle (out="tri_density") rearranged into the
+        original row order of out="tri" (host-side, for checking)."""
+        n = self.n
+        perm = self.order().astype(np.int64)
+        iu, ju = np.triu_indices(n, 1)
+        a, b = perm[iu], perm[ju]
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        dst = lo * n - lo * (lo + 1) // 2 + (hi - lo - 1)
+        out = np.empty_like(t)
+        out[dst] = t
         return out
 
     def row_sums(self):

@@ -56,6 +56,12 @@ def test_allpairs_tri_matches_oracle(ctx, shape, bb):
             assert np.array_equal(c.row_sums(), rs)
         st2 = c.allpairs(out="sum", bb=bb)
         assert st2["sum"] == want_sum
+        # A dense corpus is in identity order: the density-packed triangle is
+        # the same array.
+        st3 = c.allpairs(out="tri_density", bb=bb)
+        assert st3["sum"] == want_sum
+        if n >= 2:
+            assert np.array_equal(c.tri(), want_tri)
 
 
 @pytest.mark.parametrize("bb", BB_KERNELS)

@@ -400,3 +400,75 @@ def test_bb_zone_map_plan_per_pair(ctx):
             assert st["sum"] == want_sum, tile
             assert np.array_equal(c.tri(), want), tile
             assert st["cell_pairs"].get("BB", 0) > 0, st["cell_pairs"]
+
+
+def _row_sums(n, tri):
+    I, J = np.triu_indices(n, 1)
+    rs = np.zeros(n, np.uint64)
+    np.add.at(rs, I, tri.astype(np.uint64))
+    np.add.at(rs, J, tri.astype(np.uint64))
+    return rs
+
+
+def test_density_order_triangle(ctx, cases):
+    """out="tri_density": the triangle packed over positions of the density
+    order (the coordinates of the reference's allpairs_tiles under PerTile).
+    Rearranged by the corpus's order it equals the original-order triangle of
+    the oracle, for the selecting policies (B x S stripes, wide ones
+    included, joins and B x B tiles writing by position), forced cells, the
+    all-bitmap path and batches; row sums come back by original row."""
+    for name, u, offs, pos, want_sum, want in cases[::6]:
+        print("case density tri", name, flush=True)
+        n = len(offs) - 1
+        rs_want = _row_sums(n, want) if n > 1 else None
+        with ctx.from_lists(offs, pos, u, bitmap_policy=1 | BUILD_ALL | 64) as c:
+            runs = [("analytic", 0, 64, False), ("analytic", 0, 256, True),
+                    ("fixed", CELLS.index("BS"), 256, False), ("fixed", CELLS.index("SS"), 64, False),
+                    ("fixed", CELLS.index("RR"), 64, True), ("fixed", CELLS.index("BB"), 256, False),
+                    ("all_bitmap", 0, 64, False)]
+            for pol, cell, tile, graph in runs:
+                st = c.allpairs(policy=pol, fixed_cell=cell, out="tri_density", tile=tile,
+                                use_graph=graph)
+                assert st["sum"] == want_sum, (name, pol, cell, tile)
+                got = c.tri()
+                assert np.array_equal(c.tri_to_original(got), want), (name, pol, cell, tile)
+                if n > 1:
+                    assert np.array_equal(c.row_sums(), rs_want), (name, pol, cell, tile)
+            # The original-order call after a density-order one (the cached
+            # plan is rebuilt for the other packing).
+            st = c.allpairs(policy="analytic", out="tri", tile=256, use_graph=True)
+            assert np.array_equal(c.tri(), want), name
+
+
+def test_density_order_triangle_wide_stripes_and_batch(ctx, cases):
+    """The density-packed triangle through wide B x S stripes (1,300 rows of
+    cardinality 1..20,000, every row a bitmap) and through a batch."""
+    rng = np.random.default_rng(17)
+    u = 60_001
+    rows = [corpora.uniform_row(rng, u, int(np.exp(rng.random() * np.log(20_000))))
+            for _ in range(1300)]
+    offs, pos = corpora.csr(rows)
+    want_sum, want = O.allpairs_lists(offs, pos)
+    print("case density tri wide stripes", flush=True)
+    with ctx.from_lists(offs, pos, u, bitmap_policy=1 | BUILD_ALL | 64) as c:
+        for pol in ("fixed", "analytic"):
+            st = c.allpairs(policy=pol, fixed_cell=CELLS.index("BS"), out="tri_density",
+                            tile=256, use_graph=True)
+            assert st["sum"] == want_sum, pol
+            assert np.array_equal(c.tri_to_original(c.tri()), want), pol
+        assert np.array_equal(c.row_sums(), _row_sums(len(rows), want))
+    picks = [cases[k] for k in (0, 5, 14, 27, 41, 60)]
+    print("case density tri batch", [p[0] for p in picks], flush=True)
+    cs = [ctx.from_lists(offs_, pos_, u_, bitmap_policy=(k % 3) | BUILD_ALL | 64)
+          for k, (name, u_, offs_, pos_, _, _) in enumerate(picks)]
+    try:
+        with ctx.batch(cs) as b:
+            for tile, graph in ((64, False), (256, True)):
+                st, sums, _ = b.allpairs(policy="analytic", out="tri_density", tile=tile,
+                                         use_graph=graph)
+                for c, (name, u_, offs_, pos_, want_sum_, want_) in zip(cs, picks):
+                    assert np.array_equal(c.tri_to_original(c.tri()), want_), (name, tile)
+                assert [int(s) for s in sums] == [p[4] for p in picks]
+    finally:
+        for c in cs:
+            c.free()
