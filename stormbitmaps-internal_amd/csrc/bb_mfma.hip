@@ -598,9 +598,10 @@ int launch_bb_rect(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uint64_
 
 int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
                     uint32_t ld, uint32_t n_words, const uint32_t* rowmap, uint32_t n,
-                    uint32_t* tri, unsigned long long* sum, uint32_t cu_count) {
+                    uint32_t* tri, unsigned long long* sum, uint32_t cu_count,
+                    const uint32_t* perm) {
     if (bb_kernel == STORM_HIP_BB_VALU)
-        return launch_bb_valu(s, 128, X, ld, n_words, rowmap, nullptr, n, tri, sum);
+        return launch_bb_valu(s, 128, X, ld, n_words, rowmap, perm, n, tri, sum);
     const bool fp4 = bb_kernel == STORM_HIP_BB_MFMA_FP4 ||
                      (bb_kernel == STORM_HIP_BB_AUTO && n_words < (1u << 18));
     // A caller's row stride shorter than two stages of the configured width
@@ -631,7 +632,7 @@ int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, co
                            dim3(256), 0, s, tri, count);
         SH_HIP(hipGetLastError());
     }
-    return launch_bb_mfma_fmt(fp4, shape, s, X, ld, n_words, rowmap, nullptr, n, tri, sum, k_split);
+    return launch_bb_mfma_fmt(fp4, shape, s, X, ld, n_words, rowmap, perm, n, tri, sum, k_split);
 }
 
 }  // namespace stormhip

@@ -32,7 +32,8 @@ int upload_vec(T*& d, const std::vector<T>& h, hipStream_t s, uint64_t& bytes) {
 // whose arenas and host meta are in place; synchronises the stream.
 int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uint8_t>& want_bitmap);
 // The same for a corpus whose bitmap store is already complete (one slot per
-// row, c->h_bm_row set): orders, indexes and slices only.
+// row, slot = row): orders, the store rearranged into density order (bm_row
+// updated), indexes and slices.
 int corpus_finish_store(storm_hip_ctx* ctx, storm_hip_corpus* c);
 
 // Builds runs, EWAH (+ marker tables), complements and meta on the device

@@ -495,9 +495,53 @@ int upload_orders(storm_hip_corpus* c, hipStream_t s) {
 
 }  // namespace
 
+// A store filled with slot = row (rows generated or copied in row order) is
+// rearranged in place into density order (slot = position, as corpus_finish
+// lays stores out), following the permutation's cycles with one spare row.
+static int store_to_density_order(storm_hip_corpus* c, hipStream_t s) {
+    const uint32_t n = c->n_rows;
+    if (n < 2 || !c->d_bitmap) return STORM_HIP_OK;
+    const size_t row_bytes = (size_t)c->ld * 8;
+    uint64_t* spare = nullptr;
+    SH_HIP(hipMalloc((void**)&spare, row_bytes));
+    uint64_t* X = c->d_bitmap;
+    auto slot = [&](uint32_t k) { return X + (uint64_t)k * c->ld; };
+    std::vector<uint8_t> done(n, 0);
+    hipError_t e = hipSuccess;
+    // Slot k receives row perm[k], which sits in slot perm[k] until moved.
+    for (uint32_t k = 0; k < n && e == hipSuccess; ++k) {
+        if (done[k]) continue;
+        done[k] = 1;
+        if (c->h_perm[k] == k) continue;
+        e = hipMemcpyAsync(spare, slot(k), row_bytes, hipMemcpyDeviceToDevice, s);
+        uint32_t cur = k;
+        while (e == hipSuccess) {
+            const uint32_t src = c->h_perm[cur];
+            if (src == k) {
+                e = hipMemcpyAsync(slot(cur), spare, row_bytes, hipMemcpyDeviceToDevice, s);
+                break;
+            }
+            e = hipMemcpyAsync(slot(cur), slot(src), row_bytes, hipMemcpyDeviceToDevice, s);
+            done[src] = 1;
+            cur = src;
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(spare);
+    if (e != hipSuccess) {
+        set_error("bitmap store reorder: %s", hipGetErrorString(e));
+        return STORM_HIP_ERR_DEVICE;
+    }
+    for (uint32_t k = 0; k < n; ++k) c->h_bm_row[c->h_perm[k]] = k;
+    SH_HIP(hipMemcpyAsync(c->d_bm_row, c->h_bm_row.data(), (size_t)n * 4, hipMemcpyHostToDevice,
+                          s));
+    return STORM_HIP_OK;
+}
+
 int corpus_finish_store(storm_hip_ctx* ctx, storm_hip_corpus* c) {
     hipStream_t s = ctx->stream;
     SH_TRY(upload_orders(c, s));
+    SH_TRY(store_to_density_order(c, s));
     SH_TRY(build_indexes(c, s));
     SH_TRY(build_slices(c, s));
     SH_TRY(build_zones(c, s));
@@ -515,8 +559,13 @@ int corpus_finish(storm_hip_ctx* ctx, storm_hip_corpus* c, const std::vector<uin
     // Bitmap rows; the sentinel row id n maps to the zero row.
     c->h_bm_row.assign((size_t)n + 1, kNoBitmap);
     uint32_t nb = 0;
-    for (uint32_t r = 0; r < n; ++r)
+    // The store is laid out in density order: the rows of a tile of that
+    // order (which the B x B kernels read as 256-row panels) are one
+    // contiguous run of the store, not rows scattered over all of it.
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t r = c->h_perm[k];
         if (want_bitmap[r]) c->h_bm_row[r] = nb++;
+    }
     c->n_bitmap_rows = nb;
     c->h_bm_row[n] = nb;   // zero row
     c->bitmaps_complete = (nb == n);
@@ -539,7 +588,8 @@ int general_materialize(storm_hip_ctx* ctx, storm_hip_corpus* c) {
         return STORM_HIP_ERR_INVALID;
     }
     c->device_bytes -= ((size_t)c->n_bitmap_rows + 1) * c->ld * 8;
-    for (uint32_t r = 0; r < c->n_rows; ++r) c->h_bm_row[r] = r;
+    // Every row, the store in density order (slot = position).
+    for (uint32_t k = 0; k < c->n_rows; ++k) c->h_bm_row[c->h_perm[k]] = k;
     c->n_bitmap_rows = c->n_rows;
     c->h_bm_row[c->n_rows] = c->n_rows;
     SH_TRY(upload_bitmaps(c, ctx->stream));

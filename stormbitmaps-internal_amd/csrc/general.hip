@@ -735,17 +735,19 @@ int general_allpairs(storm_hip_ctx* ctx, storm_hip_corpus* c, const storm_hip_op
 
     if (o.policy == STORM_HIP_POLICY_ALL_BITMAP) {
         // Every pair B x B, no span test (storm_allpairs.cpp: AllBitmap),
-        // bitmaps materialised on the device where missing; rows in identity
-        // order, or in density order for a position-packed triangle (the
-        // kernel packs its output by the order it walks).
+        // bitmaps materialised on the device where missing. The walk takes
+        // the density order, in which the bitmap store is laid out (a tile's
+        // rows are one contiguous run of the store); the counts are the same
+        // in any order, written for rows perm[i], perm[j] (original order)
+        // or at the positions themselves (position-packed triangle).
         SH_TRY(general_materialize(ctx, c));
         Work* w = nullptr;
         SH_TRY(ensure_bitmap_rowmap(c, &w));
-        SH_TRY(launch_compose_rowmap(s, by_position ? c->d_perm : c->d_ident, c->d_bm_row,
-                                     w->npad, w->rowmap));
+        SH_TRY(launch_compose_rowmap(s, c->d_perm, c->d_bm_row, w->npad, w->rowmap));
         SH_HIP(hipEventRecord(ctx->ev[2], s));
         SH_TRY(launch_bb_dense(o.bb_kernel, ctx->mfma, s, c->d_bitmap, c->ld, c->n_words, w->rowmap,
-                               c->n_rows, tri, d_sum, (uint32_t)ctx->cu_count));
+                               c->n_rows, tri, d_sum, (uint32_t)ctx->cu_count,
+                               by_position ? nullptr : c->d_perm));
         SH_HIP(hipEventRecord(ctx->ev[3], s));
         st->cell_pairs[STORM_HIP_CELL_BB] = st->pairs;
         st->cell_tiles[STORM_HIP_CELL_BB] = w->P;

@@ -100,7 +100,7 @@ def _instance_rows(c, inst):
     else:
         nw = (u + 63) // 64
         ld = len(a["bitmap"]) // n
-        bm = a["bitmap"].reshape(n, ld)[:, :nw]
+        bm = a["bitmap"].reshape(n, ld)[a["bm_row"].astype(np.int64), :nw]
         rows = [np.flatnonzero(np.unpackbits(w.view(np.uint8), bitorder="little")[:u])
                 .astype(np.uint32) for w in bm]
     offs = np.zeros(n + 1, np.uint64)

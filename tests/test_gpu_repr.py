@@ -122,8 +122,11 @@ def test_device_dense_bernoulli_matches_stream(ctx, case):
         a = c.arenas()
         nw = (u + 63) // 64
         ld = len(a["bitmap"]) // max(n, 1)
-        assert np.array_equal(a["bm_row"], np.arange(n, dtype=np.uint32))
-        got = a["bitmap"].reshape(n, ld)[:, :nw]
+        # The store is laid out in density order: row r in slot bm_row[r],
+        # which is its position in the corpus's order.
+        assert np.array_equal(np.sort(a["bm_row"]), np.arange(n, dtype=np.uint32))
+        assert np.array_equal(a["bm_row"][c.order()], np.arange(n, dtype=np.uint32))
+        got = a["bitmap"].reshape(n, ld)[a["bm_row"].astype(np.int64), :nw]
         assert np.array_equal(got, words)
         for r in range(n):
             m = O.build_row(rows[r], u)
