@@ -864,7 +864,8 @@ def other_configs(sh, args):
             keep = ("value", "unit", "ms_per_step", "dtype", "config", "roofline", "exact_sum",
                     "exact_sums_agree", "column_closed_form_sum", "cpu_baseline",
                     "learned_over_analytic_reference_rates", "learned_over_analytic",
-                    "learned_over_all_bitmap", "selector", "launch_shape")
+                    "learned_over_all_bitmap", "learned_other_output", "selector",
+                    "launch_shape")
             r2 = {k: r[k] for k in keep if k in r}
             if "policies" in r:
                 r2["policies"] = {k: {q: v[q] for q in ("pairs_per_s", "ms", "cell_tiles",
