@@ -35,6 +35,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "stormbitmaps-internal_amd")
 METRIC = "exact pairs/sec on 1 MI355X, learned vs analytic selection; % of roofline"
+CELL_NAMES = ["BB", "BS", "BR", "BW", "SS", "SR", "SW", "RR", "RW", "WW", "EMPTY", "CB"]
 # Per-pair outputs of the sparse and batched workloads.
 OUTPUTS = {
     "tri_density": "u32 upper triangle over density-order positions (the coordinates of "
@@ -309,8 +310,12 @@ def sparse_bench(sh, args):
     if args.only_policy:
         # One policy alone (for profiling its kernels without the others').
         name = args.only_policy
-        r = timed("analytic" if name == "analytic_reference_rates" else name,
-                  use_graph=name != "all_bitmap")
+        if name.startswith("fixed_"):
+            # Every tile pair forced to one cell (where the tile pair can run it).
+            r = timed("fixed", fixed_cell=CELL_NAMES.index(name[6:].upper()), use_graph=True)
+        else:
+            r = timed("analytic" if name == "analytic_reference_rates" else name,
+                      use_graph=name != "all_bitmap")
         out = {"metric": METRIC, "value": r["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
                "steps": args.steps, "warmup": 1, "ms_per_step": r["ms"],
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
@@ -784,7 +789,8 @@ def main():
                     help="sparse configs: keep a bitmap of every row when they fit this")
     ap.add_argument("--sample-tiles", type=int, default=1024)
     ap.add_argument("--only-policy", default="",
-                    choices=["", "analytic_reference_rates", "all_bitmap"],
+                    choices=["", "analytic_reference_rates", "all_bitmap", "fixed_bb",
+                             "fixed_bs", "fixed_rr", "fixed_ss"],
                     help="sparse configs: time this one policy only (no sampling or training)")
     ap.add_argument("--dump-samples", default="",
                     help="sparse configs: write the selector's sampled tiles (features, "

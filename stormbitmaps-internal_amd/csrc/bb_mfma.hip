@@ -611,16 +611,19 @@ int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, co
         return (uint64_t)std::max((n_words + sw - 1) / sw, 2u) * sw <= ld;
     };
     while (shape.stage_words > 4 && !fits(shape.stage_words)) shape.stage_words /= 2;
-    // Too few 256 x 256 blocks to fill the chip: split K so that about
-    // blocks_per_cu blocks per CU run, each stage range at least 4 stages long.
+    // Too few 256 x 256 blocks to fill the chip: split K so that the blocks
+    // fit in one round of the workgroup slots (blocks_per_cu per CU, at most
+    // what the shape's LDS lets reside), each stage range at least 4 stages
+    // long. Rounding the split up instead leaves a second round of a few
+    // blocks that costs as much as the first.
     const uint32_t T = (n + kTile - 1) / kTile;
     const uint64_t tiles = (uint64_t)T * (T + 1) / 2;
     const uint32_t n_stages = (n_words + shape.stage_words - 1) / shape.stage_words;
-    const uint64_t want = (uint64_t)std::max(1u, shape.blocks_per_cu) * cu_count;
+    const uint32_t resident = bb_blocks_per_cu(kTile, shape);
+    const uint64_t want = (uint64_t)std::max(1u, std::min(shape.blocks_per_cu, resident)) * cu_count;
     uint32_t k_split = 1;
     if (tiles && n_stages && tiles < want) {
-        k_split = (uint32_t)std::min<uint64_t>((want + tiles - 1) / tiles,
-                                               std::max(1u, n_stages / 4));
+        k_split = (uint32_t)std::min<uint64_t>(want / tiles, std::max(1u, n_stages / 4));
         k_split = std::max(1u, k_split);
         // No empty slices: as many slices as ceil(n_stages / per) needs.
         const uint32_t per = (n_stages + k_split - 1) / k_split;

@@ -94,6 +94,21 @@ __device__ inline uint32_t lower_bound(const uint32_t* a, uint32_t n, uint32_t x
     }
     return lo;
 }
+// lower_bound for an x expected near the front of a: probes 1, 2, 4, ...
+// elements ahead, then bisects the last step, so the reads stay near a[0]
+// (the join kernels' chunk boundaries advance a few dozen elements at a time).
+__device__ inline uint32_t gallop_lower_bound(const uint32_t* a, uint32_t n, uint32_t x) {
+    if (n == 0 || a[0] >= x) return 0;
+    uint32_t lo = 0, step = 1;   // a[lo] < x throughout
+    while (lo + step < n && a[lo + step] < x) { lo += step; step <<= 1; }
+    uint32_t hi = min(n, lo + step);   // a[hi] >= x, or hi == n
+    ++lo;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
 // First index with a[idx] > x.
 __device__ inline uint32_t upper_bound(const uint32_t* a, uint32_t n, uint32_t x) {
     uint32_t lo = 0, hi = n;
@@ -417,18 +432,20 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
     const uint64_t target = kJoinSize / 4;
     uint64_t width = S.total_i <= target ? U : max<uint64_t>(1, U * target / S.total_i);
     uint64_t lo = 0;
+    // Threads 0..63 follow the I rows, 64..127 the J rows: st = the row's first
+    // element at or past lo (the previous chunk's end), en = past the chunk.
+    uint32_t st = 0, en = 0;
     while (lo < U) {
         const uint64_t hi = min(U, lo + width);
-        // Chunk spans of every row: threads 0..63 take I rows, 64..127 J rows.
+        // Chunk spans of every row.
         if (tid < 128) {
             const uint32_t side = tid >> 6, r = tid & 63;
             const uint32_t nr = side ? nj : ni;
-            uint32_t st = 0, c = 0;
+            uint32_t c = 0;
             if (r < nr) {
                 const uint32_t row = D.perm[(side ? ja : ia) + r];
                 const Span32 L = list_of(D, row);
-                st = lower_bound(L.p, L.n, (uint32_t)lo);
-                const uint32_t en = hi >= U ? L.n : lower_bound(L.p, L.n, (uint32_t)hi);
+                en = hi >= U ? L.n : st + gallop_lower_bound(L.p + st, L.n - st, (uint32_t)hi);
                 c = en - st;
             }
             S.start[side][r] = st;
@@ -488,6 +505,7 @@ __device__ void ss_tile_join(const DevCorpus& D, JoinSmem& S, uint32_t ia, uint3
         }
         __syncthreads();
         lo = hi;
+        st = en;
         if (ei < kJoinSize / 8) width = min<uint64_t>(U, width * 2);
     }
     for (uint32_t p = tid; p < ni * nj; p += blockDim.x) {
@@ -717,7 +735,7 @@ __device__ void rr_tile_join(const DevCorpus& D, RunJoinSmem& S, uint32_t ia, ui
         if (tid < 128) {
             uint32_t c = 0;
             if (follows) {
-                en = hi >= U ? own.n : st + lower_bound(own.s + st, own.n - st, (uint32_t)hi);
+                en = hi >= U ? own.n : st + gallop_lower_bound(own.s + st, own.n - st, (uint32_t)hi);
                 S.first[side][r_own] = own_off + st;
                 c = en - st;
             }
