@@ -338,11 +338,12 @@ constexpr double kRefineHet = 8.0, kRefineRatio = 3.0;
 // factor of the cheapest eligible cell on the tile.
 constexpr double kLearnedPrune = 32.0;
 // ... and replaces the analytic choice when it predicts its own pick faster
-// by more than this many log2 units. 0: the network's ranking decides (with
-// costs sampled at production occupancy its held-out regret is about 1.002,
-// below the calibrated analytic model's); a positive margin keeps the
-// analytic choice on near ties.
-constexpr float kLearnedMargin = 0.0f;
+// by more than this many log2 units (a trust region around the analytic
+// model). At 0.5 (a predicted gain of sqrt(2)) the network's errors on near
+// ties stay out: on the skewed and run-heavy workloads, margins 0 / 0.25 /
+// 0.5 / 1.0 gave 1.00 / 1.09 / 1.13 / 1.14x and 0.91 / 1.03 / 0.99 / 0.97x
+// over the better analytic model (about 4% run-to-run noise).
+constexpr float kLearnedMargin = 0.5f;
 __host__ __device__ uint32_t learned_candidates(const DevModel& m, const TileMeta& a,
                                                 const TileMeta& b, uint32_t elig);
 
