@@ -102,7 +102,8 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
                                                  const uint32_t* __restrict__ perm, uint32_t n,
                                                  uint32_t bi, uint32_t bj,
                                                  uint32_t* __restrict__ tri, bool accumulate,
-                                                 const RectArgs* __restrict__ Rx = nullptr) {
+                                                 const RectArgs* __restrict__ Rx = nullptr,
+                                                 bool block_local = false) {
     using Acc = typename std::conditional<FP4, f32x16, i32x16>::type;
     constexpr int L = SW / 2;        // 16-byte loads per row per stage
     constexpr int R = SW / 4;        // 16-byte LDS reads per fragment row per stage
@@ -284,7 +285,7 @@ __device__ __forceinline__ uint64_t bb_mfma_tile(BBMfmaSmem<SW>& sm, const uint6
                         if (accumulate) { if (val) atomicAdd(dst, val); }
                         else            *dst = val;
                     }
-                } else if constexpr (THR) {
+                } else if (THR || block_local) {
                     tri[(i - i0) * kTile + (j - j0)] = val;
                 } else if (tri) {
                     const uint32_t oj = perm ? perm[j] : j;
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
     uint32_t n, uint32_t T, const DevCorpus* __restrict__ Ds, const QItem* __restrict__ queue,
     const unsigned int* __restrict__ qcount, unsigned int* __restrict__ qhead,
     uint32_t* __restrict__ tri, unsigned long long* __restrict__ sum,
-    unsigned long long* __restrict__ tile_ticks) {
+    unsigned long long* __restrict__ tile_ticks, uint32_t* __restrict__ partial) {
     __shared__ __attribute__((aligned(16))) BBMfmaSmem<SW> sm;
     __shared__ uint32_t s_t;
     uint64_t local = 0;
@@ -321,9 +322,18 @@ __global__ __launch_bounds__(64 * NW, 1) void bb_mfma_kernel(
         // Clamp first so the stage count never goes through a wrapped difference.
         const uint32_t s0 = min(slice * per, n_stages);
         const uint32_t cnt = min(per, n_stages - s0);
-        if (cnt)
+        if (cnt && partial) {
+            // K slices with a partial buffer: slice `slice` stores its block's
+            // counts at (slice, tile) in the block's own layout, and a pass
+            // after the launch adds the slices (plain stores, no atomics).
+            const uint64_t tiles = (uint64_t)T * (T + 1) / 2;
+            uint32_t* buf = partial + ((uint64_t)slice * tiles + tri_tile_id(bi, bj, T)) * kTile * kTile;
+            local = bb_mfma_tile<FP4, SW, NW>(sm, X, ld, s0, cnt, rowmap, perm, n, bi, bj, buf,
+                                              false, nullptr, true);
+        } else if (cnt) {
             local = bb_mfma_tile<FP4, SW, NW>(sm, X, ld, s0, cnt, rowmap, perm, n, bi, bj, tri,
-                                          k_split > 1);
+                                              k_split > 1);
+        }
     } else {
         for (;;) {
             __syncthreads();
@@ -439,11 +449,12 @@ struct GridLaunch {
     uint32_t n, T;
     uint32_t* tri;
     unsigned long long* sum;
+    uint32_t* partial;
     template <bool FP4, int SW, int NW>
     void operator()() const {
         hipLaunchKernelGGL((bb_mfma_kernel<FP4, SW, NW>), dim3(g), dim3(64 * NW), 0, s, X, ld, ns,
                            k_split, band, rowmap, perm, n, T, nullptr, nullptr, nullptr, nullptr,
-                           tri, sum, nullptr);
+                           tri, sum, nullptr, partial);
     }
 };
 
@@ -459,7 +470,7 @@ struct QueueLaunch {
     void operator()() const {
         hipLaunchKernelGGL((bb_mfma_kernel<FP4, SW, NW>), dim3(grid), dim3(64 * NW), 0, s, nullptr,
                            0u, 1u, 1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue, qcount, qhead,
-                           nullptr, nullptr, tile_ticks);
+                           nullptr, nullptr, tile_ticks, nullptr);
     }
 };
 
@@ -503,7 +514,8 @@ __global__ void zero_u32_kernel(uint32_t* p, uint64_t count) {
 // blocks that atomically add into `tri`, which the caller has zeroed.
 int launch_bb_mfma_fmt(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
                        uint32_t ld, uint32_t n_words, const uint32_t* rowmap, const uint32_t* perm,
-                       uint32_t n, uint32_t* tri, unsigned long long* sum, uint32_t k_split) {
+                       uint32_t n, uint32_t* tri, unsigned long long* sum, uint32_t k_split,
+                       uint32_t* partial) {
     const uint32_t sw = cfg.stage_words;
     SH_TRY(bb_mfma_check(fp4, sw, n_words, ld));
     if (k_split == 0) k_split = 1;
@@ -516,7 +528,8 @@ int launch_bb_mfma_fmt(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uin
         return STORM_HIP_ERR_RANGE;
     }
     const uint32_t ns = n_stages ? n_stages : 1;
-    const GridLaunch L{s, (uint32_t)g, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum};
+    const GridLaunch L{s, (uint32_t)g, X, ld, ns, k_split, cfg.band, rowmap, perm, n, T, tri, sum,
+                       k_split > 1 ? partial : nullptr};
     SH_TRY(with_shape(fp4, cfg, L));
     SH_HIP(hipGetLastError());
     return STORM_HIP_OK;
@@ -544,11 +557,11 @@ int launch_bb_mfma_queue(bool fp4, const MfmaConfig& cfg, hipStream_t s, const D
         if (fp4)
             hipLaunchKernelGGL((bb_mfma_kernel<true, 8, 8, true>), dim3(grid), dim3(512), 0, s,
                                nullptr, 0u, 1u, 1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue,
-                               qcount, qhead, thr_buf, nullptr, tile_ticks);
+                               qcount, qhead, thr_buf, nullptr, tile_ticks, nullptr);
         else
             hipLaunchKernelGGL((bb_mfma_kernel<false, 8, 8, true>), dim3(grid), dim3(512), 0, s,
                                nullptr, 0u, 1u, 1u, 0u, nullptr, nullptr, 0u, 0u, Ds, queue,
-                               qcount, qhead, thr_buf, nullptr, tile_ticks);
+                               qcount, qhead, thr_buf, nullptr, tile_ticks, nullptr);
     } else {
         SH_TRY(with_shape(fp4, cfg, QueueLaunch{s, grid, Ds, queue, qcount, qhead, tile_ticks}));
     }
@@ -596,10 +609,31 @@ int launch_bb_rect(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uint64_
     return STORM_HIP_OK;
 }
 
+// Sums the K slices' block-local counts of every pair into the triangle: a
+// workgroup per (tile, tile row), a thread per column; slices are read
+// 256 columns at a time and each row segment of the triangle is written once.
+__global__ __launch_bounds__(256) void ksplit_sum_kernel(const uint32_t* __restrict__ partial,
+                                                         uint32_t k_split, uint32_t T, uint32_t n,
+                                                         const uint32_t* __restrict__ perm,
+                                                         uint32_t* __restrict__ tri) {
+    const uint64_t tiles = (uint64_t)T * (T + 1) / 2;
+    const uint64_t t = blockIdx.x / kTile;
+    const uint32_t r = blockIdx.x % kTile, c = threadIdx.x;
+    uint32_t bi, bj;
+    tri_tile(t, T, bi, bj);
+    const uint32_t i = bi * kTile + r, j = bj * kTile + c;
+    if (i >= n || j >= n || j <= i) return;
+    const uint32_t* src = partial + t * kTile * kTile + (uint64_t)r * kTile + c;
+    uint32_t v = 0;
+    for (uint32_t sl = 0; sl < k_split; ++sl) v += src[(uint64_t)sl * tiles * kTile * kTile];
+    const uint32_t oi = perm ? perm[i] : i, oj = perm ? perm[j] : j;
+    tri[oi < oj ? tri_index(oi, oj, n) : tri_index(oj, oi, n)] = v;
+}
+
 int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
                     uint32_t ld, uint32_t n_words, const uint32_t* rowmap, uint32_t n,
                     uint32_t* tri, unsigned long long* sum, uint32_t cu_count,
-                    const uint32_t* perm) {
+                    const uint32_t* perm, bool may_allocate) {
     if (bb_kernel == STORM_HIP_BB_VALU)
         return launch_bb_valu(s, 128, X, ld, n_words, rowmap, perm, n, tri, sum);
     const bool fp4 = bb_kernel == STORM_HIP_BB_MFMA_FP4 ||
@@ -628,6 +662,26 @@ int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, co
         // No empty slices: as many slices as ceil(n_stages / per) needs.
         const uint32_t per = (n_stages + k_split - 1) / k_split;
         k_split = (n_stages + per - 1) / per;
+    }
+    if (k_split > 1 && tri && may_allocate) {
+        // The slices store their blocks (k_split x tiles x 256 KB, at most a
+        // few hundred MB: only small corpora split K) and one pass adds them:
+        // no atomic per pair per slice into a zeroed triangle.
+        uint32_t* partial = nullptr;
+        const size_t bytes = (size_t)k_split * tiles * kTile * kTile * 4;
+        SH_HIP(hipMallocAsync((void**)&partial, bytes, s));
+        int rc = launch_bb_mfma_fmt(fp4, shape, s, X, ld, n_words, rowmap, perm, n, tri, sum,
+                                    k_split, partial);
+        if (rc == STORM_HIP_OK) {
+            hipLaunchKernelGGL(ksplit_sum_kernel, dim3((uint32_t)(tiles * kTile)), dim3(kTile), 0,
+                               s, partial, k_split, T, n, perm, tri);
+            if (hipGetLastError() != hipSuccess) {
+                set_error("ksplit_sum_kernel launch failed");
+                rc = STORM_HIP_ERR_DEVICE;
+            }
+        }
+        SH_HIP(hipFreeAsync(partial, s));
+        return rc;
     }
     if (k_split > 1 && tri) {
         const uint64_t count = (uint64_t)n * (n - 1) / 2;

@@ -175,18 +175,21 @@ int launch_bb_valu(hipStream_t s, int bm, const uint64_t* X, uint32_t ld, uint32
                    unsigned long long* sum);
 int launch_bb_mfma_fmt(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
                        uint32_t ld, uint32_t n_words, const uint32_t* rowmap, const uint32_t* perm,
-                       uint32_t n, uint32_t* tri, unsigned long long* sum, uint32_t k_split = 1);
+                       uint32_t n, uint32_t* tri, unsigned long long* sum, uint32_t k_split = 1,
+                       uint32_t* partial = nullptr);
 // Queue launches over tiles of several corpora (declared with the device
 // corpus view in general.h).
 // The B x B form for a dense launch: VALU tile of 128, or MFMA (int8 / fp4).
 // AUTO takes fp4 where its exactness bound holds, else int8. Small corpora
 // split K across blocks so the launch fills `cu_count` CUs.
 // perm (optional): the triangle is written for rows perm[i], perm[j] of the
-// walk's positions i, j (rowmap gives their bitmap rows).
+// walk's positions i, j (rowmap gives their bitmap rows). may_allocate: a
+// K-split launch may take a stream-ordered scratch buffer for the slices'
+// partial counts (otherwise the slices add into the zeroed triangle).
 int launch_bb_dense(uint32_t bb_kernel, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X,
                     uint32_t ld, uint32_t n_words, const uint32_t* rowmap, uint32_t n,
                     uint32_t* tri, unsigned long long* sum, uint32_t cu_count,
-                    const uint32_t* perm = nullptr);
+                    const uint32_t* perm = nullptr, bool may_allocate = true);
 // Rectangular X . Y^T on the matrix cores (rows of both padded to 256, same
 // stride); out: nx * ny u32 row-major, or null for the sum only.
 int launch_bb_rect(bool fp4, const MfmaConfig& cfg, hipStream_t s, const uint64_t* X, uint32_t nx,

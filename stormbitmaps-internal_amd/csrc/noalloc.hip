@@ -104,8 +104,10 @@ int storm_hip_allpairs_contig_noalloc(storm_hip_ctx* ctx, const uint64_t* d_word
     hipLaunchKernelGGL(noalloc_rowmap_kernel, dim3((npad + 255) / 256), dim3(256), 0, s, rowmap,
                        n_rows, npad);
     SH_HIP(hipGetLastError());
+    // No scratch allocation: K slices (small corpora) add into the zeroed
+    // triangle.
     return launch_bb_dense(bb_kernel, ctx->mfma, s, d_words, ld, n_words, rowmap, n_rows, d_tri,
-                           d_sum, (uint32_t)ctx->cu_count);
+                           d_sum, (uint32_t)ctx->cu_count, nullptr, false);
 }
 
 }  // extern "C"
