@@ -168,7 +168,8 @@ def cpu_baseline(sh, corpus, n_bits, target_s=12.0, single_target_s=6.0):
     return out
 
 
-def cpu_baseline_sparse(offs, pos, universe, max_rows=2048, tile=64, target_s=8.0):
+def cpu_baseline_sparse(offs, pos, universe, max_rows=2048, tile=64, target_s=8.0,
+                        all_bitmap=True):
     """The reference as shipped on one thread (leg (i)) over a stride sample
     of the corpus's rows (every k-th row, so the sample spans the density
     spectrum): allpairs_sum under PerTile with the model calibrate() fits on
@@ -200,7 +201,8 @@ def cpu_baseline_sparse(offs, pos, universe, max_rows=2048, tile=64, target_s=8.
         return k, len(rows), secs.value, int(got) == want
 
     res = {}
-    for name, policy, cal in (("per_tile_calibrated", 2, 1), ("all_bitmap", 0, 0)):
+    legs = (("per_tile_calibrated", 2, 1), ("all_bitmap", 0, 0))
+    for name, policy, cal in legs if all_bitmap else legs[:1]:
         _, r0, t0, _ = run(policy, cal, 128)
         rate = (r0 * (r0 - 1) / 2) / max(t0, 1e-6)
         want_rows = int(min(max_rows, max(128, (2 * rate * target_s) ** 0.5)))
@@ -662,6 +664,54 @@ def batch_bench(sh, args):
         "all_bitmap": "not run: every instance as bitmaps is about 2.7 TB",
     }
     ctx.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(sh, inst)
+    return out
+
+
+def cpu_baseline_batch(sh, inst, per_kind=2, target_s=3.0, max_elements=1 << 26):
+    """The reference as shipped on one thread over a bounded sample of the
+    batched workload: the first instances of each kind (i.i.d. and run rows)
+    with at least 512 rows and at most max_elements positions, built by the
+    host generator (the device generator's rows, bit for bit), each timed as
+    cpu_baseline_sparse times a corpus (PerTile + calibrate() over stride
+    samples of rows, every sum against the sample's closed form;
+    the AllBitmap leg is left out here: on these universes a 128-row sample
+    alone takes up to 15 s); the aggregate is the sampled pairs over the
+    summed seconds."""
+    picks = []
+    for kind in ("bernoulli", "runs"):
+        for x in inst:
+            if x["kind"] != kind or x["n"] < 512:
+                continue
+            if x["n"] * x["universe"] * x["density"] * 1.05 > max_elements:
+                continue
+            picks.append(x)
+            if sum(1 for p in picks if p["kind"] == kind) == per_kind:
+                break
+    parts = []
+    for x in picks:
+        kind, n, u, p1, p2, p3, seed, _ = x["gen"]
+        offs, pos = sh.gen_corpus(kind, n, u, p1, p2, p3, seed)
+        progress(f"cpu reference on instance {x['k']} ({kind}, {n} rows, {len(pos)} positions)")
+        r = cpu_baseline_sparse(offs, pos, u, target_s=target_s, all_bitmap=False)
+        if r is None:
+            return None
+        parts.append({"instance": x["k"], "kind": kind, "rows": n, "universe": u,
+                      "positions": int(len(pos)), "legs": r["legs"]})
+    if not parts:
+        return None
+    legs = [p["legs"]["per_tile_calibrated"] for p in parts]
+    pairs = sum(l["rows"] * (l["rows"] - 1) // 2 for l in legs)
+    secs = sum(l["seconds"] for l in legs)
+    O = _oracle_mod()
+    out = {"value": pairs / secs, "unit": "pairs/s", "cores": 1, "kind": "reference",
+           "sample": f"{len(parts)} instances ({', '.join(str(p['instance']) for p in parts)}), "
+                     f"stride samples of their rows, reference allpairs_sum PerTile + calibrate(), "
+                     f"one thread, tile 64; {pairs} pairs in {secs:.1f} s",
+           "exact": all(l["exact"] for p in parts for l in p["legs"].values()),
+           "instances": parts}
+    out.update(host_info(O))
     return out
 
 
