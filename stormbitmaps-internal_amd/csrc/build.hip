@@ -500,7 +500,10 @@ int upload_orders(storm_hip_corpus* c, hipStream_t s) {
 // lays stores out), following the permutation's cycles with one spare row.
 static int store_to_density_order(storm_hip_corpus* c, hipStream_t s) {
     const uint32_t n = c->n_rows;
-    if (n < 2 || !c->d_bitmap) return STORM_HIP_OK;
+    if (n < 2 || !c->d_bitmap || c->h_bm_row.size() < n) return STORM_HIP_OK;
+    // Only a store still in row order (a store already laid out is left as it is).
+    for (uint32_t r = 0; r < n; ++r)
+        if (c->h_bm_row[r] != r) return STORM_HIP_OK;
     const size_t row_bytes = (size_t)c->ld * 8;
     uint64_t* spare = nullptr;
     SH_HIP(hipMalloc((void**)&spare, row_bytes));
