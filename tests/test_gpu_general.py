@@ -415,8 +415,10 @@ def test_density_order_triangle(ctx, cases):
     order (the coordinates of the reference's allpairs_tiles under PerTile).
     Rearranged by the corpus's order it equals the original-order triangle of
     the oracle, for the selecting policies (B x S stripes, wide ones
-    included, joins and B x B tiles writing by position), forced cells, the
-    all-bitmap path and batches; row sums come back by original row."""
+    included, joins and B x B tiles writing by position; probe calls with
+    their shadow instances, refined and per-pair tiles; the learned policy,
+    as analytic where no selector is trained), forced cells, the all-bitmap
+    path and batches; row sums come back by original row."""
     for name, u, offs, pos, want_sum, want in cases[::6]:
         print("case density tri", name, flush=True)
         n = len(offs) - 1
@@ -425,7 +427,8 @@ def test_density_order_triangle(ctx, cases):
             runs = [("analytic", 0, 64, False), ("analytic", 0, 256, True),
                     ("fixed", CELLS.index("BS"), 256, False), ("fixed", CELLS.index("SS"), 64, False),
                     ("fixed", CELLS.index("RR"), 64, True), ("fixed", CELLS.index("BB"), 256, False),
-                    ("all_bitmap", 0, 64, False)]
+                    ("all_bitmap", 0, 64, False), ("probe", 0, 64, False), ("refine", 0, 64, False),
+                    ("per_pair", 0, 64, False), ("learned", 0, 256, False)]
             for pol, cell, tile, graph in runs:
                 st = c.allpairs(policy=pol, fixed_cell=cell, out="tri_density", tile=tile,
                                 use_graph=graph)
